@@ -1,0 +1,174 @@
+/* mrp.h - C ABI of the MI355X (gfx950) MRPNN rendering hot path.
+ *
+ * The reference exposes this path as the C++ classes `VolumeRender`
+ * (core/volume.hpp:45-146) and `Camera` (core/camera.hpp:10-41), backed by
+ * process-global CUDA textures and `__device__` symbols.  This library keeps
+ * the same operations but puts all state in an opaque handle and passes every
+ * per-call setting in a plain struct, so that several volumes can live in one
+ * process and nothing crosses the boundary except pointers, sizes and PODs.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative MRP_E* code on
+ *     failure; mrp_last_error() returns the message of the last failure on the
+ *     calling thread.  (The reference prints CUDA errors and continues,
+ *     core/volume.cu:15.)
+ *   - Calls are synchronous unless documented otherwise: work is queued on the
+ *     volume's own HIP stream and the call waits for it, as the reference
+ *     synchronises before returning (core/volume.cu:455,539).
+ *   - Grids are FP32, index (x*res + y)*res + z (z fastest), voxel centres at
+ *     (i+0.5)/res, the layout of `VolumeRender::datas` (core/volume.cu:938-940).
+ *   - Positions are in the box [-0.5, 0.5]^3.
+ *   - "dev" pointers are device (HBM) pointers; "host" pointers are host memory.
+ */
+#ifndef MRP_H_
+#define MRP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRP_OK 0
+#define MRP_E_INVALID -1   /* bad argument */
+#define MRP_E_HIP -2       /* HIP runtime error */
+#define MRP_E_STATE -3     /* call order violated (e.g. render before update) */
+#define MRP_E_NOMEM -4     /* allocation failed */
+#define MRP_E_IO -5        /* file error */
+
+typedef struct mrp_vec3 {
+    float x, y, z;
+} mrp_vec3; /* layout of CUDA float3 (12 bytes) */
+
+/* Per-pixel histogram of the progressive renderer: 3 x 10 bins of r/(r+1),
+ * sample count, running mean of l^2 and of l = r+g+b (compressed).
+ * Same 132-byte POD as `Histogram` (core/volume.hpp:12-19). */
+#define MRP_HISTO_SIZE 10
+typedef struct mrp_histogram {
+    float bin[MRP_HISTO_SIZE * 3];
+    float total_sample_num;
+    float x2;
+    float x;
+} mrp_histogram;
+
+/* `VolumeRender::RenderType` (core/volume.hpp:88-90). RPNN is not provided. */
+#define MRP_RENDER_PT 0
+#define MRP_RENDER_MRPNN 2
+
+/* Everything one offline render needs.  Fields mirror the arguments of
+ * `VolumeRender::Render(int2 size, float3 ori, float3 up, float3 right,
+ * float3 lightDir, RenderType rt, float g, float alpha, float3 lightColor,
+ * int multiScatter, int sampleNum)` (core/volume.hpp:141) plus the device
+ * globals its setters write (`scatter_rate`, `tr_scale`, `enviroment_exp`,
+ * core/render.cuh:14-16). */
+typedef struct mrp_render_params {
+    int width, height;          /* image size (the reference is square-only) */
+    mrp_vec3 ori, up, right;    /* camera: dir = normalize(-ori + right(2u-1) + up(2v-1)) */
+    mrp_vec3 light_dir;         /* direction towards the light (normalised inside) */
+    mrp_vec3 light_color;
+    mrp_vec3 sky;               /* constant environment radiance returned for escaping rays */
+    mrp_vec3 scatter_rate;      /* per-bounce albedo multiplier (SetScatterRate) */
+    float g;                    /* Henyey-Greenstein asymmetry */
+    float alpha;                /* extinction scale: sigma = alpha * density */
+    float tr_scale;             /* step scale of the shadow-ray tracker (SetTrScale), 1 = unbiased */
+    int multi_scatter;          /* maximum number of scattering events per path */
+    int spp_begin;              /* first sample (frame) index */
+    int spp_count;              /* number of samples (frames) */
+    uint64_t seed;              /* Philox key */
+    int render_type;            /* MRP_RENDER_PT or MRP_RENDER_MRPNN */
+    int mlp_precision;          /* MRPNN only: 0 = FP32, 1 = BF16 MFMA, 2 = FP8 MFMA */
+} mrp_render_params;
+
+/* Counters and timings of the last trace (filled when a pointer is given). */
+typedef struct mrp_trace_stats {
+    uint64_t lookups;       /* trilinear density lookups (8 FP32 corners each) */
+    uint64_t paths;         /* (pixel, sample) paths traced */
+    uint64_t scatters;      /* scattering events */
+    double kernel_ms;       /* summed device time of the tracing kernel(s), HIP events */
+    double reduce_ms;       /* summed device time of the accumulation kernel(s) */
+    double total_ms;        /* device time of the whole call */
+    int launches;           /* tracing kernel launches */
+    int grid_blocks;        /* persistent workgroups per launch */
+} mrp_trace_stats;
+
+/* ------------------------------------------------------------------ errors */
+const char* mrp_last_error(void);
+int mrp_device_count(int* count);
+
+/* ------------------------------------------------ device memory plumbing
+ * For hosts without their own HIP allocator (the Python tests and bench). */
+int mrp_malloc(void** dev, size_t bytes);
+int mrp_free(void* dev);
+int mrp_memcpy_h2d(void* dev_dst, const void* host_src, size_t bytes);
+int mrp_memcpy_d2h(void* host_dst, const void* dev_src, size_t bytes);
+int mrp_memset_zero(void* dev, size_t bytes);
+
+/* ------------------------------------------------------------------ volume */
+typedef struct mrp_volume mrp_volume;
+
+/* `VolumeRender(int resolution)` + `MallocMemory` (core/volume.cu:748-773,831-833). */
+int mrp_volume_create(int resolution, mrp_volume** out);
+/* `~VolumeRender` (core/volume.cu:913-936). */
+int mrp_volume_destroy(mrp_volume* vol);
+int mrp_volume_resolution(const mrp_volume* vol, int* res);
+/* Copy res^3 FP32 densities in; the result of `SetDatas`/`SetData`
+ * (core/volume.cu:938-944).  `grid_is_device` selects host or HBM source. */
+int mrp_volume_set_data(mrp_volume* vol, const float* grid, int grid_is_device);
+/* `VolumeRender::Update` (core/volume.cu:1203-1316): upload, majorant
+ * (max density, floor 1e-5) and the 9-level density mip pyramid 256^3..1^3,
+ * built on the GPU with the intended semantics (level 0 = trilinear resample
+ * of the grid at 256^3 voxel centres, level m+1 = 2x2x2 mean of level m). */
+int mrp_volume_update(mrp_volume* vol);
+/* `max_density` after Update (core/volume.cu:1219-1226). */
+int mrp_volume_max_density(const mrp_volume* vol, float* out);
+/* Copy density mip `mip` ((256>>mip)^3 FP32, z fastest) to host memory. */
+int mrp_volume_get_mip(const mrp_volume* vol, int mip, float* host_out);
+/* `VolumeRender::Update_TR` (core/volume.cu:1320-1424), GPU path: 8
+ * transmittance mips (128>>m)^3 marched 128 steps toward the light through
+ * density mip m+1 with extinction alpha/1.73^(m+1). */
+int mrp_volume_update_tr(mrp_volume* vol, mrp_vec3 light_dir, float alpha);
+int mrp_volume_get_tr_mip(const mrp_volume* vol, int mip, float* host_out);
+/* `VolumeRender::UpdateHGLut` + `Fill_Hg` (core/volume.cu:1082-1183): 512x512
+ * cone-averaged HG table, 1024 scrambled-Halton samples per texel. */
+int mrp_volume_update_hg_lut(mrp_volume* vol, float g, uint64_t seed);
+int mrp_volume_get_hg_lut(const mrp_volume* vol, float* host_out /* 512*512 */);
+
+/* ------------------------------------------------------- synthetic inputs */
+/* Seeded Perlin-fBm cloud (this project's own generator; the reference's
+ * procedural volume, genall.cu:32-67, is a non-compiling stub):
+ * density = clamp((v - threshold)/(1 - threshold), 0, 1) * sphere falloff,
+ * v = (fBm + 1)/2 at voxel centres.  Writes res^3 FP32 (z fastest) to `out`
+ * (HBM if out_is_device, else host memory).  Same bits as the CPU oracle. */
+typedef struct mrp_cloud_params {
+    uint64_t seed;
+    int octaves;
+    float base_freq, lacunarity, gain, threshold;
+} mrp_cloud_params;
+int mrp_generate_cloud(const mrp_cloud_params* p, int res, float* out, int out_is_device);
+
+/* ------------------------------------------------------------------ render */
+/* Offline render to a host image: `VolumeRender::Render(int2 size, ...)`
+ * (core/volume.cu:410-467).  rgb_host receives width*height*3 FP32, row-major
+ * (row 0 first), the running mean over samples [spp_begin, spp_begin+spp_count).
+ * stats may be NULL. */
+int mrp_render_offline(mrp_volume* vol, const mrp_render_params* p, float* rgb_host, mrp_trace_stats* stats);
+
+/* Same computation, everything resident in HBM.
+ *   target_dev : width*height*3 FP32 running mean, updated in place (must hold
+ *                the mean of samples [0, spp_begin) on entry; zeros if spp_begin = 0)
+ *   sum64_dev  : optional width*height*3 FP64 per-pixel sums, accumulated
+ *                (+=) over the samples of this call (the high-spp oracle image)
+ * This is the hot path the throughput benchmark times. */
+int mrp_render_offline_dev(mrp_volume* vol, const mrp_render_params* p, float* target_dev, double* sum64_dev,
+                           mrp_trace_stats* stats);
+
+/* Device time of the last call's phases is in stats; this resets the
+ * per-volume peak sample-buffer budget (bytes, default 4 GiB). */
+int mrp_set_sample_buffer_budget(mrp_volume* vol, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRP_H_ */

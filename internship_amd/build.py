@@ -1,0 +1,94 @@
+"""Build the gfx950 product library and the CPU oracle in-tree.
+
+  internship_amd/libmrp.so   hipcc --offload-arch=gfx950 (HIP kernels + C ABI)
+  oracle/_build/liboracle.so g++ (CPU restatement; test infrastructure)
+
+Both are compiled with -ffp-contract=off so that the IEEE-only shared math
+headers produce identical bits on the GPU and the CPU.
+
+Usage: python -m internship_amd.build [--lib-only|--oracle-only] [-j N]
+"""
+import argparse
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "internship_amd", "csrc")
+OBJ = os.path.join(ROOT, "internship_amd", "_build")
+LIB = os.path.join(ROOT, "internship_amd", "libmrp.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+HIP_SOURCES = ["mrp_api.hip", "mrp_volume.hip", "mrp_tracer.hip"]
+ORACLE_SOURCES = ["oracle.cpp"]
+
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-ffp-contract=off",
+    # keep hipcc from forming packed FP32 (v_pk_*_f32) instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+    "-I" + os.path.join(ROOT, "include"),
+]
+GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+             "-I" + os.path.join(ROOT, "include")]
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and "not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build_lib(jobs: int = 4) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    cc = hipcc()
+
+    def compile_one(src):
+        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        run([cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
+
+    with ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(compile_one, HIP_SOURCES))
+    tmp = LIB + ".tmp"
+    run([cc, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle() -> str:
+    os.makedirs(os.path.dirname(ORACLE_LIB), exist_ok=True)
+    gxx = shutil.which("g++") or "g++"
+    tmp = ORACLE_LIB + ".tmp"
+    run([gxx, *GXX_FLAGS, "-shared", *[os.path.join(ORACLE_DIR, s) for s in ORACLE_SOURCES], "-o", tmp])
+    os.replace(tmp, ORACLE_LIB)
+    return ORACLE_LIB
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--lib-only", action="store_true")
+    ap.add_argument("--oracle-only", action="store_true")
+    ap.add_argument("-j", type=int, default=4)
+    a = ap.parse_args(argv)
+    if not a.oracle_only:
+        print("built", build_lib(a.j))
+    if not a.lib_only:
+        print("built", build_oracle())
+
+
+if __name__ == "__main__":
+    main(sys.argv[1:])

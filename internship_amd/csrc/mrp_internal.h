@@ -1,0 +1,69 @@
+// mrp_internal.h - state behind the opaque mrp_volume handle and shared host helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/mrp.h"
+#include "mrp_medium.h"
+
+namespace mrp {
+
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define MRP_HIP(call)                                           \
+    do {                                                        \
+        hipError_t e_ = (call);                                 \
+        if (e_ != hipSuccess) return ::mrp::hip_fail(e_, #call); \
+    } while (0)
+
+constexpr int kNumMips = 9;      // density mips 256^3 .. 1^3
+constexpr int kNumTrMips = 8;    // transmittance mips 128^3 .. 1^3
+constexpr int kLutSize = 512;    // HG lookup table is kLutSize x kLutSize
+constexpr int kNumQueues = 8;    // one work queue per XCD
+
+struct Volume {
+    int res = 0;
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    float* grid = nullptr;           // padded (res+2)^3, border zeros
+    float max_density = 0.00001f;
+    bool has_data = false;
+    bool updated = false;
+    float* mips[kNumMips] = {};      // padded (256>>m + 2)^3
+    float* tr_mips[kNumTrMips] = {}; // unpadded (128>>m)^3
+    bool tr_valid = false;
+    mrp_vec3 tr_light{0, 0, 0};
+    float tr_alpha = -1.0f;
+    float* hg_lut = nullptr;         // kLutSize^2, x (cos) fastest
+    bool lut_valid = false;
+    float lut_g = -100.0f;
+    uint64_t lut_seed = 0;
+    // scratch
+    uint32_t* queues = nullptr;              // kNumQueues counters
+    unsigned long long* counters = nullptr;  // [lookups, paths, scatters, spare]
+    void* samples = nullptr;                 // per-(sample, pixel) float4 results
+    size_t samples_bytes = 0;
+    size_t sample_budget = (size_t)4 << 30;
+};
+
+inline int mip_res(int m) { return 256 >> m; }
+inline int tr_mip_res(int m) { return 128 >> m; }
+
+// Ensure the per-volume sample buffer holds at least `bytes`.
+int ensure_samples(Volume* v, size_t bytes);
+
+// Kernel entry points implemented in the .hip files (host launchers).
+int launch_update(Volume* v);
+int launch_update_tr(Volume* v, vec3 light_dir, float alpha);
+int launch_update_hg_lut(Volume* v, float g, uint64_t seed);
+
+}  // namespace mrp
+
+struct mrp_volume {
+    mrp::Volume v;
+};

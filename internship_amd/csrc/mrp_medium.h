@@ -1,0 +1,217 @@
+// mrp_medium.h - leaf functions of the delta-tracking medium model, shared by
+// the gfx950 kernels and the CPU oracle (same source, same bits).
+//
+// Each function states the reference function whose arithmetic it follows.
+#pragma once
+
+#include "mrp_math.h"
+
+namespace mrp {
+
+// Density grids live in HBM as FP32, z fastest, with a one-voxel zero border
+// on every face: voxel (x,y,z) of an N^3 grid is stored at
+// ((x+1)*(N+2) + (y+1))*(N+2) + (z+1).  The border makes the eight corner
+// reads of a trilinear lookup branch-free for every cell index in [-1, N-1],
+// which is exactly the set of cells a border-addressed texture can touch with
+// a non-zero corner.
+struct Grid {
+    const float* data;
+    int res;
+};
+MRP_HD size_t padded_index(int res, int x, int y, int z) {
+    const size_t p = (size_t)res + 2;
+    return ((size_t)(x + 1) * p + (size_t)(y + 1)) * p + (size_t)(z + 1);
+}
+MRP_HD size_t padded_count(int res) {
+    const size_t p = (size_t)res + 2;
+    return p * p * p;
+}
+
+// Trilinear fetch at texture coordinate uv in [0,1]^3 with texture-border
+// (zero) addressing and exact FP32 weights.  Follows the host sampler
+// `Sample(float*, int, float3 uv)` (core/volume.cu:685-703): texel coordinate
+// uv*res - 0.5, lerp along z, then y, then x.  Returns 0 outside the support.
+MRP_HD float trilinear_border_uv(const float* data, int res, float ux, float uy, float uz) {
+    const float fr = (float)res;
+    const float tx = ux * fr - 0.5f;
+    const float ty = uy * fr - 0.5f;
+    const float tz = uz * fr - 0.5f;
+    if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) return 0.0f;
+    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
+    const int x = (int)fx, y = (int)fy, z = (int)fz;
+    const size_t P = (size_t)res + 2;
+    const float* c = data + padded_index(res, x, y, z);
+    const float c000 = c[0], c001 = c[1];
+    const float c010 = c[P], c011 = c[P + 1];
+    const float c100 = c[P * P], c101 = c[P * P + 1];
+    const float c110 = c[P * P + P], c111 = c[P * P + P + 1];
+    return lerp(lerp(lerp(c000, c001, wz), lerp(c010, c011, wz), wy),
+                lerp(lerp(c100, c101, wz), lerp(c110, c111, wz), wy), wx);
+}
+
+// Density at box position p in [-0.5,0.5]^3: uv = p + 0.5 as in `Density`
+// (core/render.cuh:67-71) and `MipDensityDynamic` (core/renderResources.cu:42-67).
+MRP_HD float trilinear_border(const float* data, int res, vec3 p) {
+    return trilinear_border_uv(data, res, p.x + 0.5f, p.y + 0.5f, p.z + 0.5f);
+}
+
+// Trilinear fetch with clamp-to-edge addressing on an unpadded N^3 grid (z
+// fastest).  Follows `SampleClamp` (core/volume.cu:704-722), the CPU twin of
+// the clamp-addressed transmittance textures (core/volume.cu:1412-1417).
+MRP_HD float trilinear_clamp(const float* data, int res, vec3 uv) {
+    const float fr = (float)res;
+    float tx = uv.x * fr - 0.5f, ty = uv.y * fr - 0.5f, tz = uv.z * fr - 0.5f;
+    // Clamp the coordinate first so that the integer conversion is defined for
+    // any input (NaN maps to the low edge); inside [-1, res] this changes no value.
+    tx = tx >= -1.0f ? fmin_(tx, fr) : -1.0f;
+    ty = ty >= -1.0f ? fmin_(ty, fr) : -1.0f;
+    tz = tz >= -1.0f ? fmin_(tz, fr) : -1.0f;
+    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
+    const int x = (int)fx, y = (int)fy, z = (int)fz;
+    const int hi = res - 1;
+    const int x0 = x < 0 ? 0 : (x > hi ? hi : x), x1 = x + 1 < 0 ? 0 : (x + 1 > hi ? hi : x + 1);
+    const int y0 = y < 0 ? 0 : (y > hi ? hi : y), y1 = y + 1 < 0 ? 0 : (y + 1 > hi ? hi : y + 1);
+    const int z0 = z < 0 ? 0 : (z > hi ? hi : z), z1 = z + 1 < 0 ? 0 : (z + 1 > hi ? hi : z + 1);
+    const size_t R = (size_t)res;
+#define MRP_AT(a, b, c) data[((size_t)(a) * R + (size_t)(b)) * R + (size_t)(c)]
+    const float v = lerp(lerp(lerp(MRP_AT(x0, y0, z0), MRP_AT(x0, y0, z1), wz),
+                              lerp(MRP_AT(x0, y1, z0), MRP_AT(x0, y1, z1), wz), wy),
+                         lerp(lerp(MRP_AT(x1, y0, z0), MRP_AT(x1, y0, z1), wz),
+                              lerp(MRP_AT(x1, y1, z0), MRP_AT(x1, y1, z1), wz), wy),
+                         wx);
+#undef MRP_AT
+    return v;
+}
+
+// Bilinear fetch with clamp addressing on a W x H table stored row-major
+// (x fastest), normalised coordinates: the HG lookup table read by the
+// descriptor gather (`tex2D(_HGLut, u, v)`, core/radiancePredict.cu:353-354,
+// texture state core/volume.cu:1173-1178).
+MRP_HD float bilinear_clamp(const float* data, int w, int h, float u, float v) {
+    float tx = u * (float)w - 0.5f, ty = v * (float)h - 0.5f;
+    tx = tx >= -1.0f ? fmin_(tx, (float)w) : -1.0f;
+    ty = ty >= -1.0f ? fmin_(ty, (float)h) : -1.0f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float wx = tx - fx, wy = ty - fy;
+    const int x = (int)fx, y = (int)fy;
+    const int x0 = x < 0 ? 0 : (x > w - 1 ? w - 1 : x), x1 = x + 1 < 0 ? 0 : (x + 1 > w - 1 ? w - 1 : x + 1);
+    const int y0 = y < 0 ? 0 : (y > h - 1 ? h - 1 : y), y1 = y + 1 < 0 ? 0 : (y + 1 > h - 1 ? h - 1 : y + 1);
+    return lerp(lerp(data[y0 * w + x0], data[y0 * w + x1], wx), lerp(data[y1 * w + x0], data[y1 * w + x1], wx), wy);
+}
+
+// 1/x per component with 0 -> 1e9: `inv` (core/vector.cu:19).
+MRP_HD vec3 inv3(vec3 a) {
+    return v3(a.x == 0.0f ? 999999999.0f : 1.0f / a.x, a.y == 0.0f ? 999999999.0f : 1.0f / a.y,
+              a.z == 0.0f ? 999999999.0f : 1.0f / a.z);
+}
+
+// Entry distance into the slab box of half-size 0.4999, -1 on a miss:
+// `RayBoxOffset` (core/vector.cu:41-58).
+MRP_HD float ray_box_offset(vec3 p, vec3 dir) {
+    const vec3 id = inv3(dir);
+    const vec3 bmax = v3(0.4999f, 0.4999f, 0.4999f);
+    const vec3 to_axis = (-p) * id;
+    const vec3 to_face = bmax * id;
+    const vec3 d0 = to_axis + to_face, d1 = to_axis - to_face;
+    const vec3 tmin = vmin(d0, d1), tmax = vmax(d0, d1);
+    const float tmi = fmax_(tmin.x, fmax_(tmin.y, tmin.z));
+    const float tma = fmin_(tmax.x, fmin_(tmax.y, tmax.z));
+    return tma >= tmi ? fmax_(tmi, 0.0f) : -1.0f;
+}
+
+// Exit distance from the unit box: `RayBoxDistance` (core/vector.cu:60-75).
+MRP_HD float ray_box_distance(vec3 p, vec3 dir) {
+    const vec3 id = inv3(dir);
+    const vec3 bmax = v3(0.5f, 0.5f, 0.5f);
+    const vec3 to_axis = (-p) * id;
+    const vec3 to_face = bmax * id;
+    const vec3 d0 = to_axis + to_face, d1 = to_axis - to_face;
+    const vec3 tmax = vmax(d0, d1);
+    return fmin_(tmax.x, fmin_(tmax.y, tmax.z));
+}
+
+// Henyey-Greenstein phase value without the 1/pi factor, as the reference
+// defines it: (1-g^2) / (1+g^2-2g cos)^1.5 / 4 (core/vector.cu:205-209).
+// d^1.5 is formed as d*sqrt(d).
+MRP_HD float hg_phase(float cos_t, float g) {
+    const float g2 = g * g;
+    const float d = (1.0f + g2) - 2.0f * g * cos_t;
+    return ((1.0f - g2) / (d * sqrtf(d))) * 0.25f;
+}
+
+// Inverse-CDF HG cosine: `SampleHeneyGreenstein` (core/vector.cu:211-218).
+MRP_HD float hg_sample_cos(float s, float g) {
+    if (fabs_(g) < 0.0001f) return s * 2.0f - 1.0f;
+    const float g2 = g * g;
+    const float t0 = (1.0f - g2) / ((1.0f - g) + 2.0f * g * s);
+    return ((1.0f + g2) - t0 * t0) / (2.0f * g);
+}
+
+// HG direction about v from two uniforms (e0 -> cos, e1 -> azimuth):
+// `SampleHenyeyGreenstein` (core/vector.cu:220-237).
+MRP_HD vec3 hg_sample_dir(float e0, float e1, vec3 v, float g) {
+    const float cos_t = hg_sample_cos(e0, g);
+    const float phi = 6.28318548f * e1;
+    const float sin_t = sqrtf(fmax_(0.0f, 1.0f - cos_t * cos_t));
+    vec3 v2;
+    if (fabs_(v.z) > 0.707106769f) {
+        v2 = normalize(cross(v, v3(0.0f, 1.0f, 0.0f)));
+    } else {
+        v2 = normalize(cross(v, v3(0.0f, 0.0f, 1.0f)));
+    }
+    const vec3 w3 = cross(v2, v);
+    float sp, cp;
+    sincosf_(phi, &sp, &cp);
+    return (w3 * (sin_t * cp)) + (v2 * (sin_t * sp)) + (v * cos_t);
+}
+
+// Uniform direction on the sphere: `UniformSampleSphere` (core/vector.cu:190-203).
+MRP_HD vec3 uniform_sphere(float ex, float ey) {
+    const float phi = 6.28318548f * ex;
+    const float cos_t = 1.0f - 2.0f * ey;
+    const float sin_t = sqrtf(1.0f - cos_t * cos_t);
+    float sp, cp;
+    sincosf_(phi, &sp, &cp);
+    return v3(sin_t * cp, sin_t * sp, cos_t);
+}
+
+// Pinhole ray through pixel (row, col) with jitter (xi_u, xi_v), the offline
+// camera of `RenderCamera` (core/volume.cu:158-163): u = 1-(col+xi)/W,
+// v = 1-(row+xi)/H, dir = normalize(forward + right(2u-1) + up(2v-1)).
+MRP_HD vec3 camera_dir(vec3 forward, vec3 right, vec3 up, int row, int col, int w, int h, float xi_u, float xi_v) {
+    const float u = 1.0f - ((float)col + xi_u) / (float)w;
+    const float v = 1.0f - ((float)row + xi_v) / (float)h;
+    const vec3 d = forward + (right * (u * 2.0f - 1.0f)) + (up * (v * 2.0f - 1.0f));
+    return normalize(d);
+}
+
+// Random orthonormal frame whose first axis is v1: `GetMatrixFromNormal`
+// (core/render.cuh:355-369).  The rejection loop is bounded (64 tries, then a
+// fixed helper axis) so that degenerate input cannot hang a wavefront.
+template <class RNG>
+MRP_HD void frame_from_normal(RNG* rng, vec3 v1, vec3* a1, vec3* a2, vec3* a3) {
+    v1 = normalize(v1);
+    vec3 v2 = v3(0.0f, 0.0f, 0.0f), v3_ = v3(0.0f, 0.0f, 0.0f);
+    bool ok = false;
+    for (int it = 0; it < 64 && !ok; ++it) {
+        const float ex = rng_next(rng);
+        const float ey = rng_next(rng);
+        const vec3 r = uniform_sphere(ex, ey);
+        if (fabs_(dot(r, v1)) < 0.01f) continue;
+        v2 = normalize(cross(v1, r));
+        v3_ = cross(v2, v1);
+        ok = true;
+    }
+    if (!ok) {
+        const vec3 h = fabs_(v1.x) < 0.9f ? v3(1.0f, 0.0f, 0.0f) : v3(0.0f, 1.0f, 0.0f);
+        v2 = normalize(cross(v1, h));
+        v3_ = cross(v2, v1);
+    }
+    *a1 = v1;
+    *a2 = v2;
+    *a3 = v3_;
+}
+
+}  // namespace mrp

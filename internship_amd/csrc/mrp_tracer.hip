@@ -1,0 +1,489 @@
+// mrp_tracer.hip - delta-tracking volume path tracer for gfx950.
+//
+// Reference algorithm: `CalculateRadiance` / `DeterminateNextVertex` / `Tr`
+// (core/render.cuh:93-258) driven by the offline `RenderCamera<false>` kernel
+// and `VolumeRender::Render` (core/volume.cu:147-179, 410-467).
+//
+// Structure (MI355X-first, not a translation of the reference's
+// one-thread-per-pixel launch-per-sample loop):
+//   * One persistent launch covers every (pixel, sample) path of a batch.
+//     Workgroup b reads work queue b % 8, i.e. the queue of the XCD the
+//     workgroup is dispatched to, and each queue owns one horizontal band of
+//     the image (8x8-pixel tiles, sample-major), so rays that share an XCD's
+//     L2 also share voxels.  A wave that drains its queue steals from the next.
+//   * Lanes fetch items in wave-batched atomics (one atomicAdd per wave) and
+//     regenerate a new path as soon as theirs ends, so lanes stay busy despite
+//     path lengths that differ by orders of magnitude.
+//   * Each path is a state machine (INIT, FF = free flight, TR1 = ratio
+//     tracking, TR2 = delta tracking toward the light).  Every loop iteration
+//     advances every live lane by exactly one tracking step, and all lanes
+//     reach the single density-lookup site together, so the eight corner reads
+//     issue as one wide memory instruction stream instead of three divergent ones.
+//   * Random numbers: Philox4x32-10 per (pixel, sample); an 8-word ring is
+//     topped up at one site per iteration, so the cipher code is not
+//     replicated at every draw site.
+//   * Results go to a per-(sample, pixel) float4 buffer; a second kernel folds
+//     them into the running mean in sample order, which keeps the image
+//     bit-identical to a sequential per-pixel loop (the CPU oracle).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mrp_internal.h"
+
+namespace mrp {
+
+enum : int { ST_IDLE = 0, ST_INIT = 1, ST_FF = 2, ST_TR1 = 3, ST_TR2 = 4 };
+enum : int { EV_NONE = 0, EV_ACCEPT = 1, EV_ESCAPE = 2, EV_TRDONE = 3, EV_DONE = 4 };
+
+struct TraceArgs {
+    const float* grid;
+    int res;
+    float smax, alpha, g, tr_scale;
+    vec3 ori, forward, right, up, light_dir, light_color, sky, scatter_rate;
+    int width, height, multi_scatter;
+    uint32_t seed_lo, seed_hi;
+    int spp_begin, spp_count;
+    int band_rows, tiles_x, tiles_per_band;
+    uint32_t items_per_queue;
+    uint32_t* queues;
+    unsigned long long* counters;
+    float4* out;
+};
+
+// Ring of 8 Philox words.  `n` is the next draw index, `hi` the number of
+// words generated so far; words of draw i live in slot i & 7.
+struct Ring {
+    uint32_t w0, w1, w2, w3, w4, w5, w6, w7;
+    uint32_t n, hi;
+};
+
+__device__ __forceinline__ uint32_t ring_get(const Ring& r, uint32_t i) {
+    const uint32_t s = i & 7u;
+    const uint32_t a = (s & 1u) ? r.w1 : r.w0;
+    const uint32_t b = (s & 1u) ? r.w3 : r.w2;
+    const uint32_t c = (s & 1u) ? r.w5 : r.w4;
+    const uint32_t d = (s & 1u) ? r.w7 : r.w6;
+    const uint32_t ab = (s & 2u) ? b : a;
+    const uint32_t cd = (s & 2u) ? d : c;
+    return (s & 4u) ? cd : ab;
+}
+
+__device__ __forceinline__ float ring_take(Ring& r) {
+    const uint32_t x = ring_get(r, r.n);
+    r.n += 1u;
+    return u01_(x);
+}
+
+__device__ __forceinline__ float fetch_density(const TraceArgs& a, vec3 p) {
+    return trilinear_border(a.grid, a.res, p);
+}
+
+__global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
+    const uint32_t lane = __lane_id();
+    int q = (int)(blockIdx.x & 7u);
+    int qtries = 0;
+
+    int state = ST_IDLE;
+    int pix = 0, sidx = 0;
+    Ring rng;
+    rng.w0 = rng.w1 = rng.w2 = rng.w3 = rng.w4 = rng.w5 = rng.w6 = rng.w7 = 0u;
+    rng.n = rng.hi = 0u;
+    vec3 pos = v3(0, 0, 0), rdir = v3(0, 0, 0), ndir = v3(0, 0, 0);
+    vec3 csr = v3(1, 1, 1), res = v3(0, 0, 0);
+    float t = 0.0f, dis = 0.0f, tfirst = -1.0f, tr = 1.0f, light_phase = 0.0f;
+    int scatter_num = 0, loop_num = 0;
+    uint32_t n_lookups = 0, n_paths = 0, n_scatters = 0;
+
+    const int P = a.width * a.height;
+
+    for (;;) {
+        // ------------------------------------------------ refill idle lanes
+        const bool want = (state == ST_IDLE);
+        const unsigned long long mask = __ballot(want);
+        if (mask != 0ull && qtries < kNumQueues) {
+            const int cnt = __popcll(mask);
+            const int leader = __ffsll((long long)mask) - 1;
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            uint32_t base = 0u;
+            if ((int)lane == leader) base = atomicAdd(&a.queues[q], (uint32_t)cnt);
+            base = __shfl(base, leader);
+            const int qf = q;
+            if (base + (uint32_t)cnt > a.items_per_queue) {  // this queue is exhausted: move on
+                q = (q + 1) & 7;
+                qtries += 1;
+            }
+            if (want) {
+                const uint32_t k = base + (uint32_t)rank;
+                if (k < a.items_per_queue) {
+                    const uint32_t per_sample = (uint32_t)a.tiles_per_band * 64u;
+                    const uint32_t s = k / per_sample;
+                    const uint32_t r = k - s * per_sample;
+                    const uint32_t tile = r >> 6, w = r & 63u;
+                    const int ty = (int)(tile / (uint32_t)a.tiles_x), tx = (int)(tile % (uint32_t)a.tiles_x);
+                    const int col = tx * 8 + (int)(w & 7u);
+                    const int row0 = qf * a.band_rows;
+                    const int row = row0 + ty * 8 + (int)(w >> 3);
+                    const int row_end = min(a.height, row0 + a.band_rows);
+                    if (col < a.width && row < row_end) {
+                        pix = row * a.width + col;
+                        sidx = (int)s;
+                        state = ST_INIT;
+                        rng.n = 0u;
+                        rng.hi = 0u;
+                    }
+                }
+            }
+        }
+        if (__ballot(state != ST_IDLE) == 0ull) {
+            if (qtries >= kNumQueues) break;
+            continue;
+        }
+
+        // ------------------------------------------------ RNG top-up (one site)
+        if (state != ST_IDLE && rng.hi - rng.n < 4u) {
+            u32x4 c;
+            c.v[0] = (uint32_t)pix;
+            c.v[1] = (uint32_t)(a.spp_begin + sidx);
+            c.v[2] = rng.hi >> 2;
+            c.v[3] = 0u;
+            const u32x4 o = philox4x32_10(c, a.seed_lo, a.seed_hi);
+            if (rng.hi & 4u) {
+                rng.w4 = o.v[0]; rng.w5 = o.v[1]; rng.w6 = o.v[2]; rng.w7 = o.v[3];
+            } else {
+                rng.w0 = o.v[0]; rng.w1 = o.v[1]; rng.w2 = o.v[2]; rng.w3 = o.v[3];
+            }
+            rng.hi += 4u;
+        }
+
+        int ev = EV_NONE;
+        bool lookup = false;
+        vec3 lp = v3(0, 0, 0);
+        float rk2 = 0.0f;
+
+        if (state == ST_INIT) {
+            // camera ray (core/volume.cu:158-163) then CalculateRadiance entry (core/render.cuh:205-226)
+            const float xu = ring_take(rng);
+            const float xv = ring_take(rng);
+            const int row = pix / a.width, col = pix - row * a.width;
+            vec3 dir = camera_dir(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv);
+            dir = normalize(dir);
+            const float off = ray_box_offset(a.ori, dir);
+            tfirst = -1.0f;
+            if (off < 0.0f) {
+                res = a.sky;
+                ev = EV_DONE;
+            } else {
+                pos = a.ori + dir * off;
+                rdir = dir;
+                csr = v3(1, 1, 1);
+                res = v3(0, 0, 0);
+                scatter_num = 0;
+                if (a.multi_scatter <= 0) {
+                    ev = EV_DONE;
+                } else {
+                    dis = ray_box_distance(pos, rdir);
+                    t = 0.0f;
+                    loop_num = 0;
+                    state = ST_FF;
+                }
+            }
+        } else if (state != ST_IDLE) {
+            // loop-condition checks of the reference while loops
+            if (state == ST_FF) {
+                const bool c = loop_num < 10000;
+                loop_num++;
+                if (!c) ev = EV_ACCEPT;
+            } else {
+                if (state == ST_TR1) {
+                    const bool c = loop_num < 10000;
+                    loop_num++;
+                    if (!(c && tr > 0.5f)) state = ST_TR2;
+                }
+                if (state == ST_TR2) {
+                    const bool c = loop_num < 10000;
+                    loop_num++;
+                    if (!(c && tr > 0.001f)) ev = EV_TRDONE;
+                }
+            }
+            if (ev == EV_NONE) {
+                const float rk = ring_take(rng);
+                const float scale = (state == ST_FF) ? 1.0f : a.tr_scale;
+                t -= logf_(1.0f - rk) / a.smax * scale;
+                if (t > dis) {
+                    if (state == ST_FF) {
+                        ev = EV_ESCAPE;
+                    } else {
+                        if (state == ST_TR1) {
+                            // the delta-tracking loop still runs once: its check, one
+                            // distance draw, then the same overshoot breaks it
+                            const bool c = loop_num < 10000;
+                            loop_num++;
+                            if (c && tr > 0.001f) (void)ring_take(rng);
+                        }
+                        ev = EV_TRDONE;
+                    }
+                } else {
+                    lookup = true;
+                    lp = pos + ((state == ST_FF ? rdir : a.light_dir) * t);
+                    if (state == ST_FF) rk2 = ring_take(rng);
+                }
+            }
+        }
+
+        // ------------------------------------------------ the one lookup site
+        float d = 0.0f;
+        if (lookup) {
+            d = fetch_density(a, lp);
+            n_lookups++;
+            const float ratio = (d * a.alpha) / a.smax;
+            if (state == ST_FF) {
+                if (ratio > rk2) ev = EV_ACCEPT;
+                else if (d < 0.0f) t -= d;
+            } else if (state == ST_TR1) {
+                tr *= 1.0f - fmax_(0.0f, ratio);
+                if (d < 0.0f) t -= d;
+            } else {
+                const float rk3 = ring_take(rng);
+                if (rk3 < fmax_(0.0f, ratio)) {
+                    tr = 0.0f;
+                    ev = EV_TRDONE;
+                } else if (d < 0.0f) {
+                    t -= d;
+                }
+            }
+        }
+
+        // ------------------------------------------------ events
+        if (ev == EV_ACCEPT) {
+            const float e0 = ring_take(rng);
+            const float e1 = ring_take(rng);
+            ndir = hg_sample_dir(e0, e1, rdir, a.g);
+            const vec3 npos = (rdir * t) + pos;
+            n_scatters++;
+            if (scatter_num == 0) tfirst = distance(a.ori, npos);
+            pos = npos;
+            csr = csr * a.scatter_rate;
+            dis = ray_box_distance(pos, a.light_dir);
+            light_phase = hg_phase(dot(rdir, a.light_dir), a.g);
+            tr = 1.0f;
+            t = 0.0f;
+            loop_num = 0;
+            state = ST_TR1;
+        } else if (ev == EV_ESCAPE) {
+            res = res + a.sky * csr;
+            ev = EV_DONE;
+        } else if (ev == EV_TRDONE) {
+            res = res + a.light_color * csr * (tr * light_phase);
+            rdir = ndir;
+            scatter_num++;
+            if (scatter_num < a.multi_scatter) {
+                dis = ray_box_distance(pos, rdir);
+                t = 0.0f;
+                loop_num = 0;
+                state = ST_FF;
+            } else {
+                ev = EV_DONE;
+            }
+        }
+        if (ev == EV_DONE) {
+            a.out[(size_t)sidx * (size_t)P + (size_t)pix] = make_float4(res.x, res.y, res.z, tfirst);
+            n_paths++;
+            state = ST_IDLE;
+        }
+    }
+
+    if (n_lookups) atomicAdd(&a.counters[0], (unsigned long long)n_lookups);
+    if (n_paths) atomicAdd(&a.counters[1], (unsigned long long)n_paths);
+    if (n_scatters) atomicAdd(&a.counters[2], (unsigned long long)n_scatters);
+}
+
+// Fold per-sample results into the running mean in sample order
+// (`target = lerp(target, max(0,r), 1/(1+s))`, core/volume.cu:173-178) and
+// optionally into FP64 sums.
+__global__ __launch_bounds__(256) void accumulate_kernel(const float4* __restrict__ samples, int P, int spp_begin,
+                                                         int spp_count, float* __restrict__ target,
+                                                         double* __restrict__ sum64) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    vec3 tg = v3(target[3 * i], target[3 * i + 1], target[3 * i + 2]);
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int k = 0; k < spp_count; ++k) {
+        const float4 r = samples[(size_t)k * (size_t)P + (size_t)i];
+        const vec3 c = vmax(v3(0, 0, 0), v3(r.x, r.y, r.z));
+        const float rate = 1.0f / (float)(1 + spp_begin + k);
+        tg = lerp(tg, c, rate);
+        s0 += (double)c.x;
+        s1 += (double)c.y;
+        s2 += (double)c.z;
+    }
+    target[3 * i] = tg.x;
+    target[3 * i + 1] = tg.y;
+    target[3 * i + 2] = tg.z;
+    if (sum64) {
+        sum64[3 * i] += s0;
+        sum64[3 * i + 1] += s1;
+        sum64[3 * i + 2] += s2;
+    }
+}
+
+__global__ void clear_u32_kernel(uint32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+__global__ void clear_u64_kernel(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
+
+static int check_params(const Volume* v, const mrp_render_params* p) {
+    if (!p) return fail(MRP_E_INVALID, "params is NULL");
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() must run before rendering");
+    if (p->width <= 0 || p->height <= 0 || (long long)p->width * p->height > (1ll << 28))
+        return fail(MRP_E_INVALID, "image size out of range");
+    if (p->spp_begin < 0 || p->spp_count <= 0 || p->spp_begin > (1 << 24) || p->spp_count > (1 << 24))
+        return fail(MRP_E_INVALID, "sample range out of range");
+    if (!(p->alpha > 0.0f)) return fail(MRP_E_INVALID, "alpha must be > 0");
+    if (p->multi_scatter < 0) return fail(MRP_E_INVALID, "multi_scatter must be >= 0");
+    if (p->render_type != MRP_RENDER_PT) return fail(MRP_E_INVALID, "render_type: only MRP_RENDER_PT here");
+    return MRP_OK;
+}
+
+// Trace samples [spp_begin, spp_begin+spp_count) into target/sum64 (device).
+static int trace_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
+    int rc = check_params(v, p);
+    if (rc) return rc;
+    const int W = p->width, H = p->height;
+    const size_t P = (size_t)W * H;
+    const size_t per_sample_bytes = P * sizeof(float4);
+    int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / per_sample_bytes, (size_t)p->spp_count));
+    const int band_rows = (H + kNumQueues - 1) / kNumQueues;
+    const int tiles_x = (W + 7) / 8;
+    const int tiles_per_band = tiles_x * ((band_rows + 7) / 8);
+    const uint64_t per_sample_items = (uint64_t)tiles_per_band * 64u;
+    batch = (int)std::min<uint64_t>((uint64_t)batch, (uint64_t)(((1ull << 32) - (1ull << 26)) / per_sample_items));
+    if (batch < 1) return fail(MRP_E_INVALID, "image too large for one work queue");
+    rc = ensure_samples(v, (size_t)batch * per_sample_bytes);
+    if (rc) return rc;
+
+    TraceArgs a;
+    a.grid = v->grid;
+    a.res = v->res;
+    a.alpha = p->alpha;
+    a.smax = v->max_density * p->alpha;
+    a.g = p->g;
+    a.tr_scale = p->tr_scale;
+    a.ori = V(p->ori);
+    a.forward = normalize(-a.ori);
+    a.right = V(p->right);
+    a.up = V(p->up);
+    a.light_dir = normalize(V(p->light_dir));
+    a.light_color = V(p->light_color);
+    a.sky = V(p->sky);
+    a.scatter_rate = V(p->scatter_rate);
+    a.width = W;
+    a.height = H;
+    a.multi_scatter = p->multi_scatter;
+    a.seed_lo = (uint32_t)p->seed;
+    a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.band_rows = band_rows;
+    a.tiles_x = tiles_x;
+    a.tiles_per_band = tiles_per_band;
+    a.queues = v->queues;
+    a.counters = v->counters;
+    a.out = (float4*)v->samples;
+
+    int occ = 0;
+    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel, 256, 0));
+    occ = std::max(1, std::min(occ, 8));
+    const int blocks = std::max(kNumQueues, v->num_cus * occ);
+
+    hipEvent_t e0, e1, e2, e3;
+    MRP_HIP(hipEventCreate(&e0));
+    MRP_HIP(hipEventCreate(&e1));
+    MRP_HIP(hipEventCreate(&e2));
+    MRP_HIP(hipEventCreate(&e3));
+    double kernel_ms = 0.0, reduce_ms = 0.0;
+    int launches = 0;
+    MRP_HIP(hipEventRecord(e0, v->stream));
+    clear_u64_kernel<<<1, 64, 0, v->stream>>>(v->counters, 4);
+    for (int s0 = 0; s0 < p->spp_count; s0 += batch) {
+        const int n = std::min(batch, p->spp_count - s0);
+        a.spp_begin = p->spp_begin + s0;
+        a.spp_count = n;
+        a.items_per_queue = (uint32_t)(per_sample_items * (uint64_t)n);
+        clear_u32_kernel<<<1, 64, 0, v->stream>>>(v->queues, kNumQueues);
+        MRP_HIP(hipEventRecord(e1, v->stream));
+        trace_kernel<<<blocks, 256, 0, v->stream>>>(a);
+        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventRecord(e2, v->stream));
+        accumulate_kernel<<<(unsigned)((P + 255) / 256), 256, 0, v->stream>>>((const float4*)v->samples, (int)P,
+                                                                           a.spp_begin, n, target, sum64);
+        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventRecord(e3, v->stream));
+        MRP_HIP(hipEventSynchronize(e3));
+        float ms = 0.0f;
+        MRP_HIP(hipEventElapsedTime(&ms, e1, e2));
+        kernel_ms += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, e2, e3));
+        reduce_ms += ms;
+        launches++;
+    }
+    float total = 0.0f;
+    MRP_HIP(hipEventElapsedTime(&total, e0, e3));
+    unsigned long long cnt[4] = {0, 0, 0, 0};
+    MRP_HIP(hipMemcpyAsync(cnt, v->counters, sizeof(cnt), hipMemcpyDeviceToHost, v->stream));
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(e2);
+    hipEventDestroy(e3);
+    if (st) {
+        st->lookups = cnt[0];
+        st->paths = cnt[1];
+        st->scatters = cnt[2];
+        st->kernel_ms = kernel_ms;
+        st->reduce_ms = reduce_ms;
+        st->total_ms = total;
+        st->launches = launches;
+        st->grid_blocks = blocks;
+    }
+    return MRP_OK;
+}
+
+}  // namespace mrp
+
+using namespace mrp;
+
+extern "C" int mrp_render_offline_dev(mrp_volume* vol, const mrp_render_params* p, float* target_dev,
+                                      double* sum64_dev, mrp_trace_stats* stats) {
+    if (!vol || !target_dev) return fail(MRP_E_INVALID, "volume/target is NULL");
+    MRP_HIP(hipSetDevice(vol->v.device));
+    return trace_dev(&vol->v, p, target_dev, sum64_dev, stats);
+}
+
+extern "C" int mrp_render_offline(mrp_volume* vol, const mrp_render_params* p, float* rgb_host,
+                                  mrp_trace_stats* stats) {
+    if (!vol || !rgb_host || !p) return fail(MRP_E_INVALID, "volume/params/output is NULL");
+    if (p->width <= 0 || p->height <= 0) return fail(MRP_E_INVALID, "image size out of range");
+    Volume* v = &vol->v;
+    MRP_HIP(hipSetDevice(v->device));
+    const size_t bytes = (size_t)p->width * p->height * 3 * sizeof(float);
+    float* target = nullptr;
+    MRP_HIP(hipMalloc(&target, bytes));
+    int rc = MRP_OK;
+    if (p->spp_begin > 0) {
+        rc = hipMemcpy(target, rgb_host, bytes, hipMemcpyHostToDevice) == hipSuccess ? MRP_OK
+                                                                                     : fail(MRP_E_HIP, "upload target");
+    } else {
+        rc = hipMemsetAsync(target, 0, bytes, v->stream) == hipSuccess ? MRP_OK : fail(MRP_E_HIP, "clear target");
+    }
+    if (rc == MRP_OK) rc = trace_dev(v, p, target, nullptr, stats);
+    if (rc == MRP_OK && hipMemcpy(rgb_host, target, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(MRP_E_HIP, "download target");
+    hipFree(target);
+    return rc;
+}

@@ -1,0 +1,393 @@
+// mrp_volume.hip - per-volume and per-light precompute on the GPU.
+//
+//   set_data   : host/HBM grid -> padded HBM grid (zero border)
+//   update     : majorant (max density) + 9-level density mip pyramid
+//                (`VolumeRender::Update`, core/volume.cu:1203-1316, with the
+//                intended semantics of its broken mip loop)
+//   update_tr  : 8 transmittance mips (`Fill_TR`/`Update_TR`, core/volume.cu:1320-1424)
+//   hg lut     : cone-averaged HG table (`Fill_Hg`/`UpdateHGLut`, core/volume.cu:1082-1183)
+//
+// All of these are HBM-streaming kernels with one thread per output voxel; the
+// grid and mips stay resident in HBM (a 512^3 grid and its pyramid take about
+// 0.6 GB of the 288 GB) so no host round trip happens between stages.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <vector>
+
+#include "mrp_hglut.h"
+#include "mrp_internal.h"
+#include "mrp_procedural.h"
+
+namespace mrp {
+
+__global__ void fill_zero_kernel(float* p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = i; k < n; k += stride) p[k] = 0.0f;
+}
+
+// src: res^3 unpadded (z fastest) -> dst: padded interior.
+__global__ void pad_kernel(const float* __restrict__ src, int res, float* __restrict__ dst) {
+    const size_t n = (size_t)res * res * res;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % res);
+        const int y = (int)((i / res) % res);
+        const int x = (int)(i / ((size_t)res * res));
+        dst[padded_index(res, x, y, z)] = src[i];
+    }
+}
+
+// padded -> unpadded res^3
+__global__ void unpad_kernel(const float* __restrict__ src, int res, float* __restrict__ dst) {
+    const size_t n = (size_t)res * res * res;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % res);
+        const int y = (int)((i / res) % res);
+        const int x = (int)(i / ((size_t)res * res));
+        dst[i] = src[padded_index(res, x, y, z)];
+    }
+}
+
+// max over the interior of a padded grid; result as float bits in *out
+// (initialised to bits(1e-5f) > 0, so unsigned ordering equals float ordering).
+__global__ __launch_bounds__(256) void max_kernel(const float* __restrict__ src, int res, unsigned* out) {
+    __shared__ float red[256];
+    const size_t n = (size_t)res * res * res;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    float m = 0.00001f;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % res);
+        const int y = (int)((i / res) % res);
+        const int x = (int)(i / ((size_t)res * res));
+        m = fmax_(m, src[padded_index(res, x, y, z)]);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmax_(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, f2u(red[0]));
+}
+
+// dst (res r, padded) voxel (x,y,z) = trilinear border sample of src (res s,
+// padded) at the voxel centre uv = (2i+1)/(2r): `Sample(source, source_res,
+// float3{u,v,w})` as used by the mip loop (core/volume.cu:1238-1241).
+__global__ void resample_kernel(const float* __restrict__ src, int s, float* __restrict__ dst, int r) {
+    const size_t n = (size_t)r * r * r;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const float inv = 1.0f / (float)(2 * r);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % r);
+        const int y = (int)((i / r) % r);
+        const int x = (int)(i / ((size_t)r * r));
+        const float u = (float)(2 * x + 1) * inv, v = (float)(2 * y + 1) * inv, w = (float)(2 * z + 1) * inv;
+        dst[padded_index(r, x, y, z)] = trilinear_border_uv(src, s, u, v, w);
+    }
+}
+
+// dst (res r) = mean of the 2x2x2 block of src (res 2r), summed dx, dy, dz
+// nested as in core/volume.cu:1259-1275, divided by 8.
+__global__ void box_kernel(const float* __restrict__ src, float* __restrict__ dst, int r) {
+    const size_t n = (size_t)r * r * r;
+    const int s = 2 * r;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % r);
+        const int y = (int)((i / r) % r);
+        const int x = (int)(i / ((size_t)r * r));
+        float sum = 0.0f;
+        for (int dx = 0; dx < 2; dx++)
+            for (int dy = 0; dy < 2; dy++)
+                for (int dz = 0; dz < 2; dz++) sum += src[padded_index(s, 2 * x + dx, 2 * y + dy, 2 * z + dz)];
+        dst[padded_index(r, x, y, z)] = sum / 8.0f;
+    }
+}
+
+// TR mip voxel: march 128 steps from the voxel centre to the box exit along
+// the light through density mip (same resolution), T = exp(-sum * a * step).
+__global__ void fill_tr_kernel(const float* __restrict__ mip, int r, float* __restrict__ tr, vec3 light,
+                               float alpha_m) {
+    const size_t n = (size_t)r * r * r;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % r);
+        const int y = (int)((i / r) % r);
+        const int x = (int)(i / ((size_t)r * r));
+        const vec3 uv = v3(((float)x + 0.5f) / (float)r, ((float)y + 0.5f) / (float)r, ((float)z + 0.5f) / (float)r);
+        const vec3 ori = uv - 0.5f;
+        const float dis = ray_box_distance(ori, light);
+        const float step = dis / 128.0f;
+        vec3 lp = ori;
+        float shadow = 0.0f;
+        for (int k = 0; k < 128; k++) {
+            lp = lp + light * step;
+            shadow = shadow + trilinear_border(mip, r, lp);
+        }
+        tr[i] = expf_(-shadow * alpha_m * step);
+    }
+}
+
+__global__ __launch_bounds__(64) void fill_hg_kernel(float* __restrict__ lut, const float* __restrict__ tables,
+                                                     float g, float blend, uint32_t seed_lo, uint32_t seed_hi) {
+    const int i = blockIdx.x * 8 + (threadIdx.x & 7);
+    const int j = blockIdx.y * 8 + (threadIdx.x >> 3);
+    if (i >= kHgLutSize || j >= kHgLutSize) return;
+    const float* halton = tables;
+    const float angle = tables[4 * kHgLutSamples + j];
+    u32x4 c;
+    c.v[0] = (uint32_t)(j * kHgLutSize + i);
+    c.v[1] = 0u;
+    c.v[2] = 0u;
+    c.v[3] = 2u;
+    const float bn = u01_(philox4x32_10(c, seed_lo, seed_hi).v[0]);
+    lut[j * kHgLutSize + i] = hg_lut_texel(i, angle, g, blend, bn, halton);
+}
+
+__global__ void cloud_kernel(const int* __restrict__ perm, CloudParams c, int res, float* __restrict__ out) {
+    __shared__ int sp[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) sp[i] = perm[i];
+    __syncthreads();
+    const size_t n = (size_t)res * res * res;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % res);
+        const int y = (int)((i / res) % res);
+        const int x = (int)(i / ((size_t)res * res));
+        out[i] = cloud_density(sp, c, res, x, y, z);
+    }
+}
+
+static unsigned blocks_for(size_t n) {
+    size_t b = (n + 255) / 256;
+    if (b > 65536) b = 65536;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+int launch_update(Volume* v) {
+    const int res = v->res;
+    // majorant
+    unsigned* dmax = nullptr;
+    MRP_HIP(hipMalloc(&dmax, sizeof(unsigned)));
+    const unsigned init = f2u(0.00001f);
+    MRP_HIP(hipMemcpyAsync(dmax, &init, sizeof(unsigned), hipMemcpyHostToDevice, v->stream));
+    max_kernel<<<blocks_for((size_t)res * res * res) > 4096 ? 4096 : blocks_for((size_t)res * res * res), 256, 0,
+                 v->stream>>>(v->grid, res, dmax);
+    unsigned bits = 0;
+    MRP_HIP(hipMemcpyAsync(&bits, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, v->stream));
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    hipFree(dmax);
+    v->max_density = u2f(bits);
+
+    // source chain: halve (by trilinear resampling) while above 2x256
+    int r = 256;
+    while (r * 2 < res) r <<= 1;
+    const float* src = v->grid;
+    int src_res = res;
+    std::vector<float*> temps;
+    for (; r > 256; r >>= 1) {
+        float* tmp = nullptr;
+        MRP_HIP(hipMalloc(&tmp, padded_count(r) * sizeof(float)));
+        temps.push_back(tmp);
+        fill_zero_kernel<<<blocks_for(padded_count(r)), 256, 0, v->stream>>>(tmp, padded_count(r));
+        resample_kernel<<<blocks_for((size_t)r * r * r), 256, 0, v->stream>>>(src, src_res, tmp, r);
+        src = tmp;
+        src_res = r;
+    }
+    for (int m = 0; m < kNumMips; ++m)
+        fill_zero_kernel<<<blocks_for(padded_count(mip_res(m))), 256, 0, v->stream>>>(v->mips[m],
+                                                                                      padded_count(mip_res(m)));
+    resample_kernel<<<blocks_for((size_t)256 * 256 * 256), 256, 0, v->stream>>>(src, src_res, v->mips[0], 256);
+    for (int m = 1; m < kNumMips; ++m) {
+        const int rm = mip_res(m);
+        box_kernel<<<blocks_for((size_t)rm * rm * rm), 256, 0, v->stream>>>(v->mips[m - 1], v->mips[m], rm);
+    }
+    MRP_HIP(hipGetLastError());
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    for (float* t : temps) hipFree(t);
+    v->updated = true;
+    v->tr_valid = false;
+    return MRP_OK;
+}
+
+// 1.73^(m+1) by repeated FP32 multiplication (the reference uses powf).
+static float tr_alpha_scale(int m) {
+    float a = 1.73f;
+    for (int k = 0; k < m; ++k) a *= 1.73f;
+    return a;
+}
+
+int launch_update_tr(Volume* v, vec3 light, float alpha) {
+    for (int m = 0; m < kNumTrMips; ++m) {
+        const int r = tr_mip_res(m);
+        const float am = alpha / tr_alpha_scale(m);
+        fill_tr_kernel<<<blocks_for((size_t)r * r * r), 256, 0, v->stream>>>(v->mips[m + 1], r, v->tr_mips[m], light,
+                                                                             am);
+    }
+    MRP_HIP(hipGetLastError());
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    return MRP_OK;
+}
+
+int launch_update_hg_lut(Volume* v, float g, uint64_t seed) {
+    std::vector<float> tables(4 * kHgLutSamples + kHgLutSize);
+    halton_table(tables.data());
+    for (int j = 0; j < kHgLutSize; ++j) {
+        const float aa = ((float)j + 0.5f) / (float)kHgLutSize;
+        tables[4 * kHgLutSamples + j] = (float)((double)aa * 3.1415926535 * 60.0 / 180.0);
+    }
+    float* dt = nullptr;
+    MRP_HIP(hipMalloc(&dt, tables.size() * sizeof(float)));
+    MRP_HIP(hipMemcpyAsync(dt, tables.data(), tables.size() * sizeof(float), hipMemcpyHostToDevice, v->stream));
+    dim3 grid(kHgLutSize / 8, kHgLutSize / 8);
+    fill_hg_kernel<<<grid, 64, 0, v->stream>>>(v->hg_lut, dt, g, hg_lut_blend(g), (uint32_t)seed,
+                                               (uint32_t)(seed >> 32));
+    MRP_HIP(hipGetLastError());
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    hipFree(dt);
+    return MRP_OK;
+}
+
+}  // namespace mrp
+
+using namespace mrp;
+
+extern "C" {
+
+int mrp_generate_cloud(const mrp_cloud_params* p, int res, float* out, int out_is_device) {
+    if (!p || !out) return fail(MRP_E_INVALID, "NULL argument");
+    if (res < 1 || res > 2048) return fail(MRP_E_INVALID, "resolution out of range");
+    if (p->octaves < 0 || p->octaves > 16 || !(p->threshold < 1.0f)) return fail(MRP_E_INVALID, "bad cloud params");
+    int perm[512];
+    cloud_permutation(p->seed, perm);
+    CloudParams c{p->seed, p->octaves, p->base_freq, p->lacunarity, p->gain, p->threshold};
+    const size_t n = (size_t)res * res * res;
+    int* dperm = nullptr;
+    float* dst = out;
+    float* tmp = nullptr;
+    MRP_HIP(hipMalloc(&dperm, sizeof(perm)));
+    hipError_t e = hipMemcpy(dperm, perm, sizeof(perm), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !out_is_device) {
+        e = hipMalloc(&tmp, n * sizeof(float));
+        dst = tmp;
+    }
+    if (e == hipSuccess) {
+        cloud_kernel<<<blocks_for(n), 256>>>(dperm, c, res, dst);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess && !out_is_device) e = hipMemcpy(out, tmp, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(dperm);
+    if (tmp) hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "generate_cloud");
+    return MRP_OK;
+}
+
+int mrp_volume_set_data(mrp_volume* h, const float* grid, int grid_is_device) {
+    if (!h || !grid) return fail(MRP_E_INVALID, "NULL argument");
+    Volume* v = &h->v;
+    MRP_HIP(hipSetDevice(v->device));
+    const size_t n = (size_t)v->res * v->res * v->res;
+    const float* src = grid;
+    float* tmp = nullptr;
+    if (!grid_is_device) {
+        MRP_HIP(hipMalloc(&tmp, n * sizeof(float)));
+        MRP_HIP(hipMemcpyAsync(tmp, grid, n * sizeof(float), hipMemcpyHostToDevice, v->stream));
+        src = tmp;
+    }
+    fill_zero_kernel<<<blocks_for(padded_count(v->res)), 256, 0, v->stream>>>(v->grid, padded_count(v->res));
+    pad_kernel<<<blocks_for(n), 256, 0, v->stream>>>(src, v->res, v->grid);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(v->stream);
+    if (tmp) hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "set_data");
+    v->has_data = true;
+    v->updated = false;
+    v->tr_valid = false;
+    return MRP_OK;
+}
+
+int mrp_volume_update(mrp_volume* h) {
+    if (!h) return fail(MRP_E_INVALID, "NULL volume");
+    if (!h->v.has_data) return fail(MRP_E_STATE, "mrp_volume_set_data() has not run");
+    MRP_HIP(hipSetDevice(h->v.device));
+    return launch_update(&h->v);
+}
+
+int mrp_volume_get_mip(const mrp_volume* h, int mip, float* host_out) {
+    if (!h || !host_out) return fail(MRP_E_INVALID, "NULL argument");
+    if (mip < 0 || mip >= kNumMips) return fail(MRP_E_INVALID, "mip out of range");
+    const Volume* v = &h->v;
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
+    MRP_HIP(hipSetDevice(v->device));
+    const int r = mip_res(mip);
+    const size_t n = (size_t)r * r * r;
+    float* tmp = nullptr;
+    MRP_HIP(hipMalloc(&tmp, n * sizeof(float)));
+    unpad_kernel<<<blocks_for(n), 256, 0, v->stream>>>(v->mips[mip], r, tmp);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(host_out, tmp, n * sizeof(float), hipMemcpyDeviceToHost, v->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(v->stream);
+    hipFree(tmp);
+    if (e != hipSuccess) return hip_fail(e, "get_mip");
+    return MRP_OK;
+}
+
+int mrp_volume_update_tr(mrp_volume* h, mrp_vec3 light_dir, float alpha) {
+    if (!h) return fail(MRP_E_INVALID, "NULL volume");
+    Volume* v = &h->v;
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
+    if (!(alpha >= 0.0f)) return fail(MRP_E_INVALID, "alpha must be >= 0");
+    const vec3 l = v3(light_dir.x, light_dir.y, light_dir.z);
+    if (!(dot(l, l) > 0.0f)) return fail(MRP_E_INVALID, "light_dir must be non-zero");
+    if (v->tr_valid && v->tr_light.x == light_dir.x && v->tr_light.y == light_dir.y && v->tr_light.z == light_dir.z &&
+        v->tr_alpha == alpha)
+        return MRP_OK;
+    MRP_HIP(hipSetDevice(v->device));
+    const int rc = launch_update_tr(v, normalize(l), alpha);
+    if (rc) return rc;
+    v->tr_valid = true;
+    v->tr_light = light_dir;
+    v->tr_alpha = alpha;
+    return MRP_OK;
+}
+
+int mrp_volume_get_tr_mip(const mrp_volume* h, int mip, float* host_out) {
+    if (!h || !host_out) return fail(MRP_E_INVALID, "NULL argument");
+    if (mip < 0 || mip >= kNumTrMips) return fail(MRP_E_INVALID, "mip out of range");
+    const Volume* v = &h->v;
+    if (!v->tr_valid) return fail(MRP_E_STATE, "mrp_volume_update_tr() has not run");
+    MRP_HIP(hipSetDevice(v->device));
+    const size_t r = (size_t)tr_mip_res(mip);
+    MRP_HIP(hipMemcpy(host_out, v->tr_mips[mip], r * r * r * sizeof(float), hipMemcpyDeviceToHost));
+    return MRP_OK;
+}
+
+int mrp_volume_update_hg_lut(mrp_volume* h, float g, uint64_t seed) {
+    if (!h) return fail(MRP_E_INVALID, "NULL volume");
+    Volume* v = &h->v;
+    if (!(g > -1.0f && g < 1.0f)) return fail(MRP_E_INVALID, "g must be in (-1, 1)");
+    if (v->lut_valid && v->lut_g == g && v->lut_seed == seed) return MRP_OK;
+    MRP_HIP(hipSetDevice(v->device));
+    const int rc = launch_update_hg_lut(v, g, seed);
+    if (rc) return rc;
+    v->lut_valid = true;
+    v->lut_g = g;
+    v->lut_seed = seed;
+    return MRP_OK;
+}
+
+int mrp_volume_get_hg_lut(const mrp_volume* h, float* host_out) {
+    if (!h || !host_out) return fail(MRP_E_INVALID, "NULL argument");
+    const Volume* v = &h->v;
+    if (!v->lut_valid) return fail(MRP_E_STATE, "mrp_volume_update_hg_lut() has not run");
+    MRP_HIP(hipSetDevice(v->device));
+    MRP_HIP(hipMemcpy(host_out, v->hg_lut, (size_t)kLutSize * kLutSize * sizeof(float), hipMemcpyDeviceToHost));
+    return MRP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,113 @@
+"""ctypes loader of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("oracle not built (python -m internship_amd.build --oracle-only): %s" % LIB_PATH)
+    _lib = ctypes.CDLL(LIB_PATH)
+    return _lib
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    load().oracle_philox(_ptr(c), _ptr(k), _ptr(out))
+    return out
+
+
+def uniforms(pid, sample, tag, seed, n):
+    out = np.zeros(n, dtype=np.float32)
+    load().oracle_uniforms(ctypes.c_uint32(pid), ctypes.c_uint32(sample), ctypes.c_uint32(tag),
+                           ctypes.c_uint64(seed), ctypes.c_int(n), _ptr(out))
+    return out
+
+
+MATH_FN = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "atan": 4, "acos": 5, "hg": 6, "hg_cos": 7}
+
+
+def math(fn, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b if b is not None else np.zeros_like(a), dtype=np.float32)
+    out = np.zeros_like(a)
+    rc = load().oracle_math(MATH_FN[fn], _ptr(a), _ptr(b), _ptr(out), ctypes.c_int(a.size))
+    assert rc == 0
+    return out
+
+
+def ray_box(p, d):
+    p = np.ascontiguousarray(p, dtype=np.float32)
+    d = np.ascontiguousarray(d, dtype=np.float32)
+    n = p.shape[0]
+    off = np.zeros(n, dtype=np.float32)
+    dist = np.zeros(n, dtype=np.float32)
+    load().oracle_ray_box(_ptr(p), _ptr(d), _ptr(off), _ptr(dist), ctypes.c_int(n))
+    return off, dist
+
+
+def hg_sample_dir(e0, e1, v, g):
+    e0 = np.ascontiguousarray(e0, dtype=np.float32)
+    e1 = np.ascontiguousarray(e1, dtype=np.float32)
+    v = np.ascontiguousarray(v, dtype=np.float32)
+    out = np.zeros((e0.size, 3), dtype=np.float32)
+    load().oracle_hg_sample_dir(_ptr(e0), _ptr(e1), _ptr(v), ctypes.c_float(g), _ptr(out), ctypes.c_int(e0.size))
+    return out
+
+
+def density(grid, pos):
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    pos = np.ascontiguousarray(pos, dtype=np.float32)
+    out = np.zeros(pos.shape[0], dtype=np.float32)
+    load().oracle_density(_ptr(grid), ctypes.c_int(grid.shape[0]), _ptr(pos), _ptr(out), ctypes.c_int(pos.shape[0]))
+    return out
+
+
+def max_density(grid):
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    out = ctypes.c_float()
+    load().oracle_max_density(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.byref(out))
+    return out.value
+
+
+def generate_cloud(res, seed, threshold=0.4, octaves=5, base_freq=4.0, lacunarity=2.0, gain=0.5):
+    from internship_amd._lib import CloudParams
+    p = CloudParams(seed, octaves, base_freq, lacunarity, gain, threshold)
+    out = np.empty((res, res, res), dtype=np.float32)
+    load().oracle_generate_cloud(ctypes.byref(p), ctypes.c_int(res), _ptr(out))
+    return out
+
+
+def render(grid, max_dens, params, target=None, sum64=None, threads=0, rows=None):
+    """Offline render on the CPU. Returns (target (h,w,3), lookups, scatters)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    h, w = params.height, params.width
+    if target is None:
+        target = np.zeros((h, w, 3), dtype=np.float32)
+    lk = ctypes.c_uint64()
+    sc = ctypes.c_uint64()
+    r0, r1 = rows if rows is not None else (0, h)
+    rc = load().oracle_render(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.c_float(max_dens),
+                              ctypes.byref(params), _ptr(target), _ptr(sum64), ctypes.byref(lk), ctypes.byref(sc),
+                              ctypes.c_int(threads), ctypes.c_int(r0), ctypes.c_int(r1))
+    assert rc == 0
+    return target, lk.value, sc.value
