@@ -77,6 +77,8 @@ def main(argv=None):
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=512)
+    ap.add_argument("--spp-per-step", type=int, default=1,
+                    help="1-spp frames traced per launch (a step renders that many frames)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -99,7 +101,7 @@ def main(argv=None):
     st = ia.TraceStats()
 
     def step(i):
-        p = config2_params(ia, W, H, i, 1, a.bounces)
+        p = config2_params(ia, W, H, i * a.spp_per_step, a.spp_per_step, a.bounces)
         vol.render_dev(p, target, None, st)
         return st.kernel_ms, st.lookups, st.paths, st.scatters
 
@@ -118,7 +120,7 @@ def main(argv=None):
     t1 = time.perf_counter()
     elapsed = t1 - t0
     ms_per_step = 1000.0 * elapsed / a.steps
-    fps = a.steps / elapsed
+    fps = a.steps * a.spp_per_step / elapsed
     avg_kernel_ms = kernel_ms / a.steps
     achieved = (lookups / a.steps) * BYTES_PER_LOOKUP / (avg_kernel_ms * 1e-3) / 1e9
     log("%.2f ms/frame, trace kernel %.2f ms, %.3g lookups/frame, %.1f GB/s algorithmic"
@@ -147,16 +149,16 @@ def main(argv=None):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "plain delta-tracking tracer, %d^3 Perlin-fBm cloud (seed 1), %dx%d, 1 spp/frame, "
-                               "%d bounces, alpha 256, albedo 0.999, g 0.857" % (res, W, H, a.bounces),
-                   "width": W, "height": H, "grid": res, "spp_per_step": 1, "bounces": a.bounces},
+        "config": {"workload": "plain delta-tracking tracer, %d^3 Perlin-fBm cloud (seed 1), %dx%d, 1 spp/frame, %d frames/launch, "
+                               "%d bounces, alpha 256, albedo 0.999, g 0.857" % (res, W, H, a.spp_per_step, a.bounces),
+                   "width": W, "height": H, "grid": res, "spp_per_step": a.spp_per_step, "bounces": a.bounces},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "kernel": "trace_kernel", "kernel_ms": avg_kernel_ms,
                      "lookups_per_launch": lookups / a.steps, "bytes_per_lookup": BYTES_PER_LOOKUP},
         "cpu_baseline": base,
         "extra": {"paths_per_frame": paths / a.steps, "scatters_per_frame": scatters / a.steps,
-                  "glookups_per_s": lookups / (kernel_ms * 1e-3) / 1e9,
+                  "glookups_per_s": lookups / (kernel_ms * 1e-3) / 1e9, "max_path_steps": st.max_path_steps,
                   "image_mean": float(img.mean()), "grid_blocks": st.grid_blocks},
     }
     print(json.dumps(out), flush=True)

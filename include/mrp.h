@@ -85,9 +85,11 @@ typedef struct mrp_trace_stats {
     uint64_t lookups;       /* trilinear density lookups (8 FP32 corners each) */
     uint64_t paths;         /* (pixel, sample) paths traced */
     uint64_t scatters;      /* scattering events */
-    double kernel_ms;       /* summed device time of the tracing kernel(s), HIP events */
-    double reduce_ms;       /* summed device time of the accumulation kernel(s) */
+    uint64_t max_path_steps; /* loop iterations of the longest path (latency tail) */
+    double kernel_ms;       /* device time of the tracing kernel(s) (MRPNN: of the network), HIP events */
+    double reduce_ms;       /* device time of the accumulation kernel(s) (MRPNN: of the descriptor gather) */
     double total_ms;        /* device time of the whole call */
+    double first_vertex_ms; /* MRPNN only: device time of the first-vertex stage */
     int launches;           /* tracing kernel launches */
     int grid_blocks;        /* persistent workgroups per launch */
 } mrp_trace_stats;
@@ -163,9 +165,37 @@ int mrp_render_offline(mrp_volume* vol, const mrp_render_params* p, float* rgb_h
 int mrp_render_offline_dev(mrp_volume* vol, const mrp_render_params* p, float* target_dev, double* sum64_dev,
                            mrp_trace_stats* stats);
 
-/* Device time of the last call's phases is in stats; this resets the
- * per-volume peak sample-buffer budget (bytes, default 4 GiB). */
+/* Cap (bytes, default 4 GiB) of the per-volume buffer that holds the
+ * per-(sample, pixel) results of one batch; larger sample counts are split
+ * into batches. */
 int mrp_set_sample_buffer_budget(mrp_volume* vol, size_t bytes);
+
+/* ---------------------------------------------------------------- network */
+/* Trained MRPNN weights: 49,673 FP32 values, the 146 arrays of
+ * core/NNWeight.cuh concatenated in header order (row-major [out][in]).
+ * The handle keeps an FP32 copy and a BF16 MFMA-operand packing in HBM.
+ * Rendering with render_type = MRP_RENDER_MRPNN uses the network set on the
+ * volume; the reference compiles its weights in (`#include "NNWeight.cuh"`,
+ * core/radiancePredict.cu:305). */
+typedef struct mrp_network mrp_network;
+int mrp_network_create(const float* weights_host, size_t count, mrp_network** out);
+int mrp_network_destroy(mrp_network* net);
+int mrp_volume_set_network(mrp_volume* vol, const mrp_network* net);
+
+/* Descriptor gather on caller-given points (the gather of `RadiancePredict`,
+ * core/radiancePredict.cu:319-368), all pointers in HBM:
+ *   pos_act  n x float4 (position, active flag)   frame_* n x float4 (X, Y, Z axes in .xyz)
+ *   feat     n x 576 FP32 [log-density 192 | transmittance 192 | log-phase 192]
+ *   gamma    n FP32, acos(X . LX)
+ * Builds/refreshes the transmittance mips (light_dir, alpha) and the HG table
+ * (g, seed) first, as `Render` does (core/volume.cu:434-437). */
+int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
+                       const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
+                       uint64_t seed, float* feat_dev, float* gamma_dev);
+/* The network on n descriptors (HBM): out[n][3] = max(exp(P_c)-1, 0)*srp_c,
+ * srp = (scatter_rate/1.001)^4 per channel.  precision 0 = FP32, 1 = BF16 MFMA. */
+int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
+                    mrp_vec3 srp, int precision, float* out_dev);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,64 @@ def generate_cloud(res, seed, threshold=0.4, octaves=5, base_freq=4.0, lacunarit
     return out
 
 
+DATA_DIR = __import__("os").path.join(__import__("os").path.dirname(__import__("os").path.abspath(__file__)), "data")
+
+
+def load_reference_weights() -> np.ndarray:
+    """The trained MRPNN weights of core/NNWeight.cuh (49,673 FP32, header
+    order), as committed in data/mrpnn_weights.bin; checked against the
+    manifest's sha256."""
+    import hashlib
+    import json
+    import os
+    raw = open(os.path.join(DATA_DIR, "mrpnn_weights.bin"), "rb").read()
+    man = json.load(open(os.path.join(DATA_DIR, "mrpnn_weights.json")))
+    if hashlib.sha256(raw).hexdigest() != man["sha256"]:
+        raise MrpError("mrpnn_weights.bin does not match its manifest checksum")
+    return np.frombuffer(raw, dtype="<f4").copy()
+
+
+def weight_manifest():
+    import json
+    import os
+    return json.load(open(os.path.join(DATA_DIR, "mrpnn_weights.json")))
+
+
+class Network:
+    """Trained MRPNN weights in HBM (FP32 + BF16 MFMA packing)."""
+
+    def __init__(self, weights: np.ndarray = None):
+        self.lib = load()
+        w = load_reference_weights() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        self.weights = w
+        self.h = ctypes.c_void_p()
+        check(self.lib.mrp_network_create(_ptr(w), ctypes.c_size_t(w.size), ctypes.byref(self.h)))
+
+    def forward(self, feat: "DeviceBuffer", gamma: "DeviceBuffer", n: int, g: float, srp, precision=0,
+                out: "DeviceBuffer" = None) -> "DeviceBuffer":
+        out = out or DeviceBuffer(max(1, n) * 12)
+        check(self.lib.mrp_mlp_forward(self.h, int(n), feat.ptr, gamma.ptr, float(g), Vec3.of(srp), int(precision),
+                                       out.ptr))
+        return out
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.mrp_network_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def srp_of(scatter_rate):
+    """(scatter_rate / 1.001)^4 per channel, as `RadiancePredict` forms it."""
+    sr = np.asarray(scatter_rate, dtype=np.float32) / np.float32(1.001)
+    return tuple(float(np.float32(float(x) ** 4.0)) for x in sr)
+
+
 class VolumeRender:
     """Handle-based twin of the reference `VolumeRender` (core/volume.hpp:45-146)."""
 
@@ -212,6 +270,18 @@ class VolumeRender:
         check(self.lib.mrp_render_offline_dev(self.h, ctypes.byref(p), target.ptr,
                                               sum64.ptr if sum64 is not None else None,
                                               ctypes.byref(stats) if stats is not None else None))
+
+    def set_network(self, net: "Network"):
+        self.network = net
+        check(self.lib.mrp_volume_set_network(self.h, net.h if net is not None else None))
+
+    def mrpnn_features(self, pos_act: DeviceBuffer, fx: DeviceBuffer, fy: DeviceBuffer, fz: DeviceBuffer, n: int,
+                       light_dir, alpha, g, seed=0, feat: DeviceBuffer = None, gamma: DeviceBuffer = None):
+        feat = feat or DeviceBuffer(max(1, n) * 576 * 4)
+        gamma = gamma or DeviceBuffer(max(1, n) * 4)
+        check(self.lib.mrp_mrpnn_features(self.h, int(n), pos_act.ptr, fx.ptr, fy.ptr, fz.ptr, Vec3.of(light_dir),
+                                          float(alpha), float(g), ctypes.c_uint64(seed), feat.ptr, gamma.ptr))
+        return feat, gamma
 
     def set_sample_buffer_budget(self, nbytes: int):
         check(self.lib.mrp_set_sample_buffer_budget(self.h, ctypes.c_size_t(int(nbytes))))

@@ -45,7 +45,9 @@ class RenderParams(ctypes.Structure):
 class TraceStats(ctypes.Structure):
     _fields_ = [
         ("lookups", ctypes.c_uint64), ("paths", ctypes.c_uint64), ("scatters", ctypes.c_uint64),
+        ("max_path_steps", ctypes.c_uint64),
         ("kernel_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+        ("first_vertex_ms", ctypes.c_double),
         ("launches", ctypes.c_int), ("grid_blocks", ctypes.c_int),
     ]
 
@@ -83,6 +85,11 @@ SIGNATURES = {
     "mrp_render_offline": (I, [P, ctypes.POINTER(RenderParams), P, ctypes.POINTER(TraceStats)]),
     "mrp_render_offline_dev": (I, [P, ctypes.POINTER(RenderParams), P, P, ctypes.POINTER(TraceStats)]),
     "mrp_set_sample_buffer_budget": (I, [P, SZ]),
+    "mrp_network_create": (I, [P, SZ, ctypes.POINTER(P)]),
+    "mrp_network_destroy": (I, [P]),
+    "mrp_volume_set_network": (I, [P, P]),
+    "mrp_mrpnn_features": (I, [P, I, P, P, P, P, Vec3, F, F, ctypes.c_uint64, P, P]),
+    "mrp_mlp_forward": (I, [P, I, P, P, F, Vec3, I, P]),
 }
 
 _lib = None

@@ -43,6 +43,8 @@ struct Volume {
     bool lut_valid = false;
     float lut_g = -100.0f;
     uint64_t lut_seed = 0;
+    const float* nn_weights = nullptr;       // FP32 MRPNN weights in HBM (owned by an mrp_network)
+    const void* nn_packed = nullptr;         // BF16 MFMA-packed weights (owned by an mrp_network)
     // scratch
     uint32_t* queues = nullptr;              // kNumQueues counters
     unsigned long long* counters = nullptr;  // [lookups, paths, scatters, spare]
@@ -58,8 +60,13 @@ inline int tr_mip_res(int m) { return 128 >> m; }
 int ensure_samples(Volume* v, size_t bytes);
 
 // Kernel entry points implemented in the .hip files (host launchers).
+int check_params(const Volume* v, const mrp_render_params* p);
+int launch_accumulate(Volume* v, const void* samples, size_t P, int spp_begin, int n, float* target, double* sum64);
+int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st);
 int launch_update(Volume* v);
 int launch_update_tr(Volume* v, vec3 light_dir, float alpha);
+int ensure_tr(Volume* v, mrp_vec3 light_dir, float alpha);       // cached Update_TR
+int ensure_hg_lut(Volume* v, float g, uint64_t seed);           // cached UpdateHGLut
 int launch_update_hg_lut(Volume* v, float g, uint64_t seed);
 
 }  // namespace mrp

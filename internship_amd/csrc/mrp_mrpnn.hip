@@ -1,0 +1,541 @@
+// mrp_mrpnn.hip - MRPNN inference path on gfx950 (FP32 reference-parity path).
+//
+// Reference: `NNPredict<MRPNN>` (core/render.cuh:469-523) and
+// `RadiancePredict` (core/radiancePredict.cu:307-486), driven by the offline
+// camera kernel (core/volume.cu:147-179).  The reference does everything for
+// one pixel in one thread (about 7 KB of per-thread arrays).  Here the work is
+// split into stages with their own launch shapes:
+//
+//   1. first_vertex_kernel : one thread per (sample, pixel) - first free flight
+//      (g = 0), random view/light frames, single-scatter Tr * HG;
+//   2. gather_kernel       : one thread per (point, tap) - 192 taps x
+//      {density mip, transmittance mip, HG table} -> 576 features + gamma;
+//   3. mlp (FP32 here, BF16 MFMA in mrp_mlp_bf16.hip) - the network;
+//   4. combine_kernel      : (predict + single) * light * albedo, or sky.
+//
+// The per-(sample, pixel) results then go through the same in-order running
+// mean as the path tracer (launch_accumulate).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "mrp_hglut.h"
+#include "mrp_internal.h"
+#include "mrp_mlp.h"
+#include "mrp_mrpnn.h"
+#include "mrp_weights_gen.h"
+
+namespace mrp {
+
+__constant__ float c_tap_offset[kTaps];
+__constant__ int c_tap_mip[kTaps];
+__constant__ float c_tap_v[kTaps];
+__constant__ float c_stencil[kShellTaps * 3];
+
+struct NNArgs {
+    const float* grid;
+    int res;
+    float smax, alpha, g, tr_scale;
+    vec3 ori, forward, right, up, light, lx, light_color, sky, scatter_rate;
+    int width, height;
+    uint32_t seed_lo, seed_hi;
+    int spp_begin;       // absolute sample index of item 0's sample
+    uint32_t item0;      // first (sample-major) item of this chunk within the batch
+    uint32_t n_items;    // items in this chunk
+    unsigned long long* counters;
+};
+
+// ---------------------------------------------------------------- stage 1
+// Straight-line delta tracking for the first-vertex stage: a few dozen lookups
+// per path, so a plain one-thread-per-path loop is adequate here.
+__device__ bool fv_free_flight(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir, float dis, vec3* npos, uint32_t* lk) {
+    float t = 0.0f;
+    int loop_num = 0;
+    while (loop_num++ < 10000) {
+        float rk = rng_next(rng);
+        t -= logf_(1.0f - rk) / a.smax;
+        if (t > dis) return false;
+        rk = rng_next(rng);
+        const float d = trilinear_border(a.grid, a.res, pos + (dir * t));
+        *lk += 1u;
+        if ((d * a.alpha) / a.smax > rk) break;
+        if (d < 0.0f) t -= d;
+    }
+    (void)rng_next(rng);  // HG direction draws of `DeterminateNextVertex` (direction unused)
+    (void)rng_next(rng);
+    *npos = (dir * t) + pos;
+    return true;
+}
+
+__device__ float fv_transmittance(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir, float dis, uint32_t* lk) {
+    float tr = 1.0f, t = 0.0f;
+    int loop_num = 0;
+    while (loop_num++ < 10000 && tr > 0.5f) {
+        const float rk = rng_next(rng);
+        t -= logf_(1.0f - rk) / a.smax * a.tr_scale;
+        if (t > dis) break;
+        const float d = trilinear_border(a.grid, a.res, pos + (dir * t));
+        *lk += 1u;
+        tr *= 1.0f - fmax_(0.0f, (d * a.alpha) / a.smax);
+        if (d < 0.0f) t -= d;
+    }
+    while (loop_num++ < 10000 && tr > 0.001f) {
+        const float rk = rng_next(rng);
+        t -= logf_(1.0f - rk) / a.smax * a.tr_scale;
+        if (t > dis) break;
+        const float d = trilinear_border(a.grid, a.res, pos + (dir * t));
+        *lk += 1u;
+        if (rng_next(rng) < fmax_(0.0f, (d * a.alpha) / a.smax)) {
+            tr = 0.0f;
+            break;
+        }
+        if (d < 0.0f) t -= d;
+    }
+    return tr;
+}
+
+// pos_act = (pos, active), fx = (X, single), fy = (Y, 0), fz = (Z, dist)
+__global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __restrict__ pos_act,
+                                                           float4* __restrict__ fx, float4* __restrict__ fy,
+                                                           float4* __restrict__ fz) {
+    __shared__ unsigned red[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lk = 0u;
+    if (i < a.n_items) {
+        const uint32_t P = (uint32_t)(a.width * a.height);
+        const uint32_t item = a.item0 + i;
+        const uint32_t s = item / P, pix = item - s * P;
+        Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        const float xu = rng_next(&rng);
+        const float xv = rng_next(&rng);
+        const int row = (int)(pix / (uint32_t)a.width), col = (int)(pix - (uint32_t)row * (uint32_t)a.width);
+        vec3 dir = camera_dir(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv);
+        dir = normalize(dir);  // NNPredict
+        const vec3 gdir = normalize(dir);  // GetSamplePoint
+        vec3 pos = v3(0, 0, 0);
+        int active = 0;
+        const float off = ray_box_offset(a.ori, gdir);
+        if (off >= 0.0f) {
+            const vec3 sp = a.ori + gdir * off;
+            vec3 np;
+            if (fv_free_flight(&rng, a, sp, gdir, ray_box_distance(sp, gdir), &np, &lk)) {
+                pos = np;
+                active = 1;
+            }
+        }
+        vec3 X, Y, Z, LX, LY, LZ;
+        frame_from_normal(&rng, dir, &X, &Y, &Z);
+        frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
+        const float phase = hg_phase(dot(X, LX), a.g);
+        const float single = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk) * phase;
+        const float dist = active ? distance(a.ori, pos) : -1.0f;
+        pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
+        fx[i] = make_float4(X.x, X.y, X.z, single);
+        fy[i] = make_float4(Y.x, Y.y, Y.z, 0.0f);
+        fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
+    }
+    red[threadIdx.x] = lk;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&a.counters[0], (unsigned long long)red[0]);
+}
+
+// ---------------------------------------------------------------- stage 2
+struct GatherArgs {
+    const float* mips[kNumMips];     // padded density mips
+    const float* trs[kNumTrMips];    // unpadded transmittance mips
+    const float* lut;                // HG table
+    vec3 light, lx;
+    float alpha, g;
+    uint32_t n_items;
+};
+
+// One block (192 threads) per point, one thread per tap.  feat[item][576] =
+// [Xd | Xt | Xh] (FP32), gamma[item] = acos(X . LX).
+__global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
+                                                       const float4* __restrict__ fx, const float4* __restrict__ fy,
+                                                       const float4* __restrict__ fz, float* __restrict__ feat,
+                                                       float* __restrict__ gamma) {
+    const uint32_t item = blockIdx.x;
+    const int i = threadIdx.x;
+    if (item >= a.n_items) return;
+    const float4 pa = pos_act[item], x4 = fx[item], y4 = fy[item], z4 = fz[item];
+    const vec3 p = v3(pa.x, pa.y, pa.z);
+    const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
+    float* out = feat + (size_t)item * kFeatures;
+    if (i == 0) gamma[item] = acosf_(dot(X, a.lx));
+    if (pa.w == 0.0f) {
+        out[i] = 0.0f;
+        out[kTaps + i] = 0.0f;
+        out[2 * kTaps + i] = 0.0f;
+        return;
+    }
+    const int mip = c_tap_mip[i];
+    const float off = c_tap_offset[i];
+    vec3 pi;
+    if (i < kShellTaps) {
+        pi = p + (X * c_stencil[3 * i] + Y * c_stencil[3 * i + 1] + Z * c_stencil[3 * i + 2]) * off;
+    } else {
+        pi = p + a.light * off;
+    }
+    const float* m = a.mips[mip];
+    out[i] = logf_(1.0f + a.alpha * trilinear_border(m, 256 >> mip, pi) / 64.0f);
+    float xh;
+    if (i == 0) {
+        xh = logf_(hg_phase(dot(X, a.light), a.g) + 1.0f);
+    } else {
+        const vec3 ms = normalize(pi - p);
+        const float u = dot(X, ms) * 0.5f + 0.5f;
+        const float u2 = dot(a.lx, ms) * 0.5f + 0.5f;
+        const float v = c_tap_v[i];
+        xh = logf_(bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u, v) *
+                       bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u2, v) +
+                   1.0f);
+    }
+    out[2 * kTaps + i] = xh;
+    const int tm = mip - 1 > 0 ? mip - 1 : 0;
+    const int tr = 128 >> tm;
+    float val;
+    if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
+        const float o = ray_box_offset(pi, a.lx);
+        val = o >= 0.0f ? trilinear_clamp(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
+    } else {
+        val = trilinear_clamp(a.trs[tm], tr, pi + 0.5f);
+    }
+    out[kTaps + i] = val;
+}
+
+// ---------------------------------------------------------------- stage 3 (FP32)
+__global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ w, const float* __restrict__ feat,
+                                                      const float* __restrict__ gamma, float g, vec3 srp,
+                                                      uint32_t n_items, float* __restrict__ pred) {
+    const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
+    if (item >= n_items) return;
+    float o[3];
+    mlp_forward_fp32(w, feat + (size_t)item * kFeatures, gamma[item], g, srp, o);
+    pred[3 * item] = o[0];
+    pred[3 * item + 1] = o[1];
+    pred[3 * item + 2] = o[2];
+}
+
+// ---------------------------------------------------------------- stage 4
+__global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act, const float4* __restrict__ fx,
+                                                      const float4* __restrict__ fz, const float* __restrict__ pred,
+                                                      vec3 light_color, vec3 scatter_rate, vec3 sky, uint32_t n_items,
+                                                      float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_items) return;
+    const float single = fx[i].w;
+    if (pos_act[i].w != 0.0f) {
+        const vec3 p = v3(pred[3 * i] + single, pred[3 * i + 1] + single, pred[3 * i + 2] + single);
+        const vec3 r = p * light_color * scatter_rate;
+        out[i] = make_float4(r.x, r.y, r.z, fz[i].w);
+    } else {
+        out[i] = make_float4(sky.x, sky.y, sky.z, -1.0f);
+    }
+}
+
+__global__ void clear_counters_kernel(unsigned long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
+
+static bool g_tables_ready[64] = {};
+
+static int upload_tap_tables(int device) {
+    if (device >= 0 && device < 64 && g_tables_ready[device]) return MRP_OK;
+    TapTable t;
+    descriptor_schedule(&t);
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_offset), t.offset, sizeof(t.offset)));
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_mip), t.mip, sizeof(t.mip)));
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_v), t.lut_v, sizeof(t.lut_v)));
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_stencil), &t.stencil[0][0], sizeof(t.stencil)));
+    if (device >= 0 && device < 64) g_tables_ready[device] = true;
+    return MRP_OK;
+}
+
+// Per-item scratch of the MRPNN stages (bytes): 4 float4 first-vertex
+// records, 576 FP32 features, gamma, 3 predictions.
+constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 4 + 12;
+
+struct NNScratch {
+    void* base = nullptr;
+    size_t bytes = 0;
+};
+static NNScratch g_nn_scratch[64];
+
+static int ensure_nn_scratch(int device, size_t bytes, void** out) {
+    NNScratch& s = g_nn_scratch[device & 63];
+    if (s.bytes < bytes) {
+        if (s.base) {
+            MRP_HIP(hipDeviceSynchronize());
+            MRP_HIP(hipFree(s.base));
+        }
+        s.base = nullptr;
+        s.bytes = 0;
+        if (hipMalloc(&s.base, bytes) != hipSuccess) return fail(MRP_E_NOMEM, "MRPNN scratch allocation failed");
+        s.bytes = bytes;
+    }
+    *out = s.base;
+    return MRP_OK;
+}
+
+int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
+    if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
+    if (p->mlp_precision != 0 && p->mlp_precision != 1)
+        return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32) or 1 (BF16)");
+    if (p->mlp_precision == 1 && !v->nn_packed) return fail(MRP_E_STATE, "network has no BF16 packing");
+    int rc = upload_tap_tables(v->device);
+    if (rc) return rc;
+    rc = ensure_tr(v, p->light_dir, p->alpha);
+    if (rc) return rc;
+    rc = ensure_hg_lut(v, p->g, p->seed);
+    if (rc) return rc;
+
+    const int W = p->width, H = p->height;
+    const size_t P = (size_t)W * H;
+    int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16), (size_t)p->spp_count));
+    rc = ensure_samples(v, (size_t)batch * P * 16);
+    if (rc) return rc;
+    const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 20, (size_t)batch * P));
+    void* scratch = nullptr;
+    rc = ensure_nn_scratch(v->device, chunk_items * kNNItemBytes + 4096, &scratch);
+    if (rc) return rc;
+    char* sp = (char*)scratch;
+    float4* pos_act = (float4*)sp;
+    float4* fx = pos_act + chunk_items;
+    float4* fy = fx + chunk_items;
+    float4* fz = fy + chunk_items;
+    float* feat = (float*)(fz + chunk_items);
+    float* gam = feat + chunk_items * kFeatures;
+    float* pred = gam + chunk_items;
+
+    NNArgs a;
+    a.grid = v->grid;
+    a.res = v->res;
+    a.alpha = p->alpha;
+    a.smax = v->max_density * p->alpha;
+    a.g = p->g;
+    a.tr_scale = p->tr_scale;
+    a.ori = V(p->ori);
+    a.forward = normalize(-a.ori);
+    a.right = V(p->right);
+    a.up = V(p->up);
+    a.light = normalize(V(p->light_dir));
+    a.lx = normalize(a.light);  // first axis of the light frame (GetMatrixFromNormal normalises again)
+    a.light_color = V(p->light_color);
+    a.sky = V(p->sky);
+    a.scatter_rate = V(p->scatter_rate);
+    a.width = W;
+    a.height = H;
+    a.seed_lo = (uint32_t)p->seed;
+    a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.counters = v->counters;
+
+    GatherArgs ga;
+    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mips[m];
+    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_mips[m];
+    ga.lut = v->hg_lut;
+    ga.light = a.light;
+    ga.lx = a.lx;
+    ga.alpha = p->alpha;
+    ga.g = p->g;
+
+    const vec3 sr = v3(a.scatter_rate.x / 1.001f, a.scatter_rate.y / 1.001f, a.scatter_rate.z / 1.001f);
+    const vec3 srp = v3((float)std::pow((double)sr.x, 4.0), (float)std::pow((double)sr.y, 4.0),
+                        (float)std::pow((double)sr.z, 4.0));
+
+    hipEvent_t e0, e1;
+    MRP_HIP(hipEventCreate(&e0));
+    MRP_HIP(hipEventCreate(&e1));
+    MRP_HIP(hipEventRecord(e0, v->stream));
+    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, v->stream, v->counters, 4);
+    double mlp_ms = 0.0, gather_ms = 0.0, fv_ms = 0.0;
+    hipEvent_t t0, t1, t2, t3;
+    MRP_HIP(hipEventCreate(&t0));
+    MRP_HIP(hipEventCreate(&t1));
+    MRP_HIP(hipEventCreate(&t2));
+    MRP_HIP(hipEventCreate(&t3));
+    for (int s0 = 0; s0 < p->spp_count; s0 += batch) {
+        const int n = std::min(batch, p->spp_count - s0);
+        a.spp_begin = p->spp_begin + s0;
+        const size_t total = (size_t)n * P;
+        for (size_t i0 = 0; i0 < total; i0 += chunk_items) {
+            const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
+            a.item0 = (uint32_t)i0;
+            a.n_items = cnt;
+            ga.n_items = cnt;
+            MRP_HIP(hipEventRecord(t0, v->stream));
+            first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz);
+            MRP_HIP(hipGetLastError());
+            MRP_HIP(hipEventRecord(t1, v->stream));
+            gather_kernel<<<cnt, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, feat, gam);
+            MRP_HIP(hipGetLastError());
+            MRP_HIP(hipEventRecord(t2, v->stream));
+            if (p->mlp_precision == 0) {
+                mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt, pred);
+            } else {
+                rc = launch_mlp_bf16(v->nn_packed, feat, gam, p->g, srp, cnt, pred, v->stream);
+                if (rc) return rc;
+            }
+            MRP_HIP(hipGetLastError());
+            MRP_HIP(hipEventRecord(t3, v->stream));
+            combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, fx, fz, pred, a.light_color,
+                                                                      a.scatter_rate, a.sky, cnt,
+                                                                      (float4*)v->samples + i0);
+            MRP_HIP(hipGetLastError());
+            MRP_HIP(hipEventSynchronize(t3));
+            float ms;
+            MRP_HIP(hipEventElapsedTime(&ms, t0, t1));
+            fv_ms += ms;
+            MRP_HIP(hipEventElapsedTime(&ms, t1, t2));
+            gather_ms += ms;
+            MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
+            mlp_ms += ms;
+        }
+        rc = launch_accumulate(v, v->samples, P, a.spp_begin, n, target, sum64);
+        if (rc) return rc;
+    }
+    MRP_HIP(hipEventRecord(e1, v->stream));
+    MRP_HIP(hipEventSynchronize(e1));
+    float total_ms = 0.0f;
+    MRP_HIP(hipEventElapsedTime(&total_ms, e0, e1));
+    unsigned long long cnt4[4] = {0, 0, 0, 0};
+    MRP_HIP(hipMemcpy(cnt4, v->counters, sizeof(cnt4), hipMemcpyDeviceToHost));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    hipEventDestroy(t2);
+    hipEventDestroy(t3);
+    if (st) {
+        st->lookups = cnt4[0];
+        st->paths = (uint64_t)p->spp_count * P;
+        st->scatters = 0;
+        st->max_path_steps = 0;
+        st->kernel_ms = mlp_ms;       // the network
+        st->reduce_ms = gather_ms;    // the descriptor gather
+        st->total_ms = total_ms;
+        st->launches = 0;
+        st->grid_blocks = 0;
+        st->first_vertex_ms = fv_ms;
+    }
+    return MRP_OK;
+}
+
+}  // namespace mrp
+
+using namespace mrp;
+
+struct mrp_network {
+    float* w = nullptr;    // FP32 weights (HBM)
+    void* packed = nullptr;  // BF16 MFMA operand packing (HBM)
+    int device = 0;
+};
+
+extern "C" {
+
+int mrp_network_create(const float* weights, size_t count, mrp_network** out) {
+    if (!weights || !out) return fail(MRP_E_INVALID, "NULL argument");
+    *out = nullptr;
+    if (count != (size_t)MRP_W_TOTAL) return fail(MRP_E_INVALID, "weight count must be 49673 (NNWeight.cuh layout)");
+    mrp_network* n = new (std::nothrow) mrp_network();
+    if (!n) return fail(MRP_E_NOMEM, "host allocation failed");
+    hipGetDevice(&n->device);
+    if (hipMalloc(&n->w, count * sizeof(float)) != hipSuccess) {
+        delete n;
+        return fail(MRP_E_NOMEM, "weight allocation failed");
+    }
+    if (hipMemcpy(n->w, weights, count * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(n->w);
+        delete n;
+        return fail(MRP_E_HIP, "weight upload failed");
+    }
+    const int rc = pack_mlp_bf16(weights, &n->packed);
+    if (rc) {
+        hipFree(n->w);
+        delete n;
+        return rc;
+    }
+    *out = n;
+    return MRP_OK;
+}
+
+int mrp_network_destroy(mrp_network* n) {
+    if (!n) return MRP_OK;
+    hipFree(n->w);
+    hipFree(n->packed);
+    delete n;
+    return MRP_OK;
+}
+
+int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
+                       const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
+                       uint64_t seed, float* feat_dev, float* gamma_dev) {
+    if (!vol || !pos_act_dev || !frame_x_dev || !frame_y_dev || !frame_z_dev || !feat_dev || !gamma_dev)
+        return fail(MRP_E_INVALID, "NULL argument");
+    if (n < 0) return fail(MRP_E_INVALID, "n must be >= 0");
+    if (n == 0) return MRP_OK;
+    Volume* v = &vol->v;
+    MRP_HIP(hipSetDevice(v->device));
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
+    int rc = upload_tap_tables(v->device);
+    if (rc) return rc;
+    rc = ensure_tr(v, light_dir, alpha);
+    if (rc) return rc;
+    rc = ensure_hg_lut(v, g, seed);
+    if (rc) return rc;
+    GatherArgs ga;
+    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mips[m];
+    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_mips[m];
+    ga.lut = v->hg_lut;
+    ga.light = normalize(V(light_dir));
+    ga.lx = normalize(ga.light);
+    ga.alpha = alpha;
+    ga.g = g;
+    ga.n_items = (uint32_t)n;
+    gather_kernel<<<n, kTaps, 0, v->stream>>>(ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev,
+                                              (const float4*)frame_y_dev, (const float4*)frame_z_dev, feat_dev,
+                                              gamma_dev);
+    MRP_HIP(hipGetLastError());
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    return MRP_OK;
+}
+
+int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
+                    mrp_vec3 srp, int precision, float* out_dev) {
+    if (!net || !feat_dev || !gamma_dev || !out_dev) return fail(MRP_E_INVALID, "NULL argument");
+    if (n < 0) return fail(MRP_E_INVALID, "n must be >= 0");
+    if (n == 0) return MRP_OK;
+    MRP_HIP(hipSetDevice(net->device));
+    const vec3 s = V(srp);
+    if (precision == 0) {
+        mlp_fp32_kernel<<<(n + 63) / 64, 64>>>(net->w, feat_dev, gamma_dev, g, s, (uint32_t)n, out_dev);
+        MRP_HIP(hipGetLastError());
+    } else if (precision == 1) {
+        if (!net->packed) return fail(MRP_E_STATE, "network has no BF16 packing");
+        const int rc = launch_mlp_bf16(net->packed, feat_dev, gamma_dev, g, s, (uint32_t)n, out_dev, nullptr);
+        if (rc) return rc;
+    } else {
+        return fail(MRP_E_INVALID, "precision must be 0 (FP32) or 1 (BF16)");
+    }
+    MRP_HIP(hipDeviceSynchronize());
+    return MRP_OK;
+}
+
+int mrp_volume_set_network(mrp_volume* vol, const mrp_network* n) {
+    if (!vol) return fail(MRP_E_INVALID, "NULL volume");
+    vol->v.nn_weights = n ? n->w : nullptr;
+    vol->v.nn_packed = n ? n->packed : nullptr;
+    return MRP_OK;
+}
+
+}  // extern "C"

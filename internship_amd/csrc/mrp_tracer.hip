@@ -95,28 +95,43 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
     float t = 0.0f, dis = 0.0f, tfirst = -1.0f, tr = 1.0f, light_phase = 0.0f;
     int scatter_num = 0, loop_num = 0;
     uint32_t n_lookups = 0, n_paths = 0, n_scatters = 0;
+    uint32_t path_iters = 0, max_iters = 0;  // loop iterations of the current / longest path
 
     const int P = a.width * a.height;
+    // Wave-private slice [wq_cur, wq_end) of queue wq, claimed kChunk items at a
+    // time so that the device-scope atomic (a round trip of microseconds across
+    // XCDs) is paid once per kChunk paths rather than once per iteration.
+    constexpr uint32_t kChunk = 128u;
+    uint32_t wq_cur = 0u, wq_end = 0u;
+    int wq = q;
 
     for (;;) {
         // ------------------------------------------------ refill idle lanes
         const bool want = (state == ST_IDLE);
         const unsigned long long mask = __ballot(want);
-        if (mask != 0ull && qtries < kNumQueues) {
-            const int cnt = __popcll(mask);
-            const int leader = __ffsll((long long)mask) - 1;
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+        if (mask != 0ull && wq_cur == wq_end && qtries < kNumQueues) {
             uint32_t base = 0u;
-            if ((int)lane == leader) base = atomicAdd(&a.queues[q], (uint32_t)cnt);
-            base = __shfl(base, leader);
-            const int qf = q;
-            if (base + (uint32_t)cnt > a.items_per_queue) {  // this queue is exhausted: move on
+            if (lane == 0u) base = atomicAdd(&a.queues[q], kChunk);
+            base = __shfl(base, 0);
+            if (base >= a.items_per_queue) {  // this queue is exhausted: move on
                 q = (q + 1) & 7;
                 qtries += 1;
+            } else {
+                wq = q;
+                wq_cur = base;
+                wq_end = min(base + kChunk, a.items_per_queue);
             }
-            if (want) {
+        }
+        if (mask != 0ull && wq_cur < wq_end) {
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            const uint32_t avail = wq_end - wq_cur;
+            const uint32_t take = min(avail, (uint32_t)__popcll(mask));
+            const uint32_t base = wq_cur;
+            const int qf = wq;
+            wq_cur += take;
+            if (want && (uint32_t)rank < take) {
                 const uint32_t k = base + (uint32_t)rank;
-                if (k < a.items_per_queue) {
+                {
                     const uint32_t per_sample = (uint32_t)a.tiles_per_band * 64u;
                     const uint32_t s = k / per_sample;
                     const uint32_t r = k - s * per_sample;
@@ -130,6 +145,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                         pix = row * a.width + col;
                         sidx = (int)s;
                         state = ST_INIT;
+                        path_iters = 0;
                         rng.n = 0u;
                         rng.hi = 0u;
                     }
@@ -137,7 +153,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
             }
         }
         if (__ballot(state != ST_IDLE) == 0ull) {
-            if (qtries >= kNumQueues) break;
+            if (qtries >= kNumQueues && wq_cur == wq_end) break;
             continue;
         }
 
@@ -157,6 +173,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
             rng.hi += 4u;
         }
 
+        path_iters++;
         int ev = EV_NONE;
         bool lookup = false;
         vec3 lp = v3(0, 0, 0);
@@ -290,6 +307,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
         if (ev == EV_DONE) {
             a.out[(size_t)sidx * (size_t)P + (size_t)pix] = make_float4(res.x, res.y, res.z, tfirst);
             n_paths++;
+            max_iters = max(max_iters, path_iters);
             state = ST_IDLE;
         }
     }
@@ -297,6 +315,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
     if (n_lookups) atomicAdd(&a.counters[0], (unsigned long long)n_lookups);
     if (n_paths) atomicAdd(&a.counters[1], (unsigned long long)n_paths);
     if (n_scatters) atomicAdd(&a.counters[2], (unsigned long long)n_scatters);
+    if (max_iters) atomicMax(&a.counters[3], (unsigned long long)max_iters);
 }
 
 // Fold per-sample results into the running mean in sample order
@@ -339,7 +358,14 @@ __global__ void clear_u64_kernel(unsigned long long* p, int n) {
 
 static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
 
-static int check_params(const Volume* v, const mrp_render_params* p) {
+int launch_accumulate(Volume* v, const void* samples, size_t P, int spp_begin, int n, float* target, double* sum64) {
+    accumulate_kernel<<<(unsigned)((P + 255) / 256), 256, 0, v->stream>>>((const float4*)samples, (int)P, spp_begin, n,
+                                                                       target, sum64);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
+}
+
+int check_params(const Volume* v, const mrp_render_params* p) {
     if (!p) return fail(MRP_E_INVALID, "params is NULL");
     if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() must run before rendering");
     if (p->width <= 0 || p->height <= 0 || (long long)p->width * p->height > (1ll << 28))
@@ -348,7 +374,8 @@ static int check_params(const Volume* v, const mrp_render_params* p) {
         return fail(MRP_E_INVALID, "sample range out of range");
     if (!(p->alpha > 0.0f)) return fail(MRP_E_INVALID, "alpha must be > 0");
     if (p->multi_scatter < 0) return fail(MRP_E_INVALID, "multi_scatter must be >= 0");
-    if (p->render_type != MRP_RENDER_PT) return fail(MRP_E_INVALID, "render_type: only MRP_RENDER_PT here");
+    if (p->render_type != MRP_RENDER_PT && p->render_type != MRP_RENDER_MRPNN)
+        return fail(MRP_E_INVALID, "render_type must be MRP_RENDER_PT or MRP_RENDER_MRPNN");
     return MRP_OK;
 }
 
@@ -356,6 +383,7 @@ static int check_params(const Volume* v, const mrp_render_params* p) {
 static int trace_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
     int rc = check_params(v, p);
     if (rc) return rc;
+    if (p->render_type == MRP_RENDER_MRPNN) return mrpnn_dev(v, p, target, sum64, st);
     const int W = p->width, H = p->height;
     const size_t P = (size_t)W * H;
     const size_t per_sample_bytes = P * sizeof(float4);
@@ -445,6 +473,7 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
         st->lookups = cnt[0];
         st->paths = cnt[1];
         st->scatters = cnt[2];
+        st->max_path_steps = cnt[3];
         st->kernel_ms = kernel_ms;
         st->reduce_ms = reduce_ms;
         st->total_ms = total;

@@ -252,6 +252,33 @@ int launch_update_hg_lut(Volume* v, float g, uint64_t seed) {
     return MRP_OK;
 }
 
+int ensure_tr(Volume* v, mrp_vec3 light_dir, float alpha) {
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
+    if (!(alpha >= 0.0f)) return fail(MRP_E_INVALID, "alpha must be >= 0");
+    const vec3 l = v3(light_dir.x, light_dir.y, light_dir.z);
+    if (!(dot(l, l) > 0.0f)) return fail(MRP_E_INVALID, "light_dir must be non-zero");
+    if (v->tr_valid && v->tr_light.x == light_dir.x && v->tr_light.y == light_dir.y && v->tr_light.z == light_dir.z &&
+        v->tr_alpha == alpha)
+        return MRP_OK;
+    const int rc = launch_update_tr(v, normalize(l), alpha);
+    if (rc) return rc;
+    v->tr_valid = true;
+    v->tr_light = light_dir;
+    v->tr_alpha = alpha;
+    return MRP_OK;
+}
+
+int ensure_hg_lut(Volume* v, float g, uint64_t seed) {
+    if (!(g > -1.0f && g < 1.0f)) return fail(MRP_E_INVALID, "g must be in (-1, 1)");
+    if (v->lut_valid && v->lut_g == g && v->lut_seed == seed) return MRP_OK;
+    const int rc = launch_update_hg_lut(v, g, seed);
+    if (rc) return rc;
+    v->lut_valid = true;
+    v->lut_g = g;
+    v->lut_seed = seed;
+    return MRP_OK;
+}
+
 }  // namespace mrp
 
 using namespace mrp;
@@ -339,21 +366,8 @@ int mrp_volume_get_mip(const mrp_volume* h, int mip, float* host_out) {
 
 int mrp_volume_update_tr(mrp_volume* h, mrp_vec3 light_dir, float alpha) {
     if (!h) return fail(MRP_E_INVALID, "NULL volume");
-    Volume* v = &h->v;
-    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
-    if (!(alpha >= 0.0f)) return fail(MRP_E_INVALID, "alpha must be >= 0");
-    const vec3 l = v3(light_dir.x, light_dir.y, light_dir.z);
-    if (!(dot(l, l) > 0.0f)) return fail(MRP_E_INVALID, "light_dir must be non-zero");
-    if (v->tr_valid && v->tr_light.x == light_dir.x && v->tr_light.y == light_dir.y && v->tr_light.z == light_dir.z &&
-        v->tr_alpha == alpha)
-        return MRP_OK;
-    MRP_HIP(hipSetDevice(v->device));
-    const int rc = launch_update_tr(v, normalize(l), alpha);
-    if (rc) return rc;
-    v->tr_valid = true;
-    v->tr_light = light_dir;
-    v->tr_alpha = alpha;
-    return MRP_OK;
+    MRP_HIP(hipSetDevice(h->v.device));
+    return ensure_tr(&h->v, light_dir, alpha);
 }
 
 int mrp_volume_get_tr_mip(const mrp_volume* h, int mip, float* host_out) {
@@ -369,16 +383,8 @@ int mrp_volume_get_tr_mip(const mrp_volume* h, int mip, float* host_out) {
 
 int mrp_volume_update_hg_lut(mrp_volume* h, float g, uint64_t seed) {
     if (!h) return fail(MRP_E_INVALID, "NULL volume");
-    Volume* v = &h->v;
-    if (!(g > -1.0f && g < 1.0f)) return fail(MRP_E_INVALID, "g must be in (-1, 1)");
-    if (v->lut_valid && v->lut_g == g && v->lut_seed == seed) return MRP_OK;
-    MRP_HIP(hipSetDevice(v->device));
-    const int rc = launch_update_hg_lut(v, g, seed);
-    if (rc) return rc;
-    v->lut_valid = true;
-    v->lut_g = g;
-    v->lut_seed = seed;
-    return MRP_OK;
+    MRP_HIP(hipSetDevice(h->v.device));
+    return ensure_hg_lut(&h->v, g, seed);
 }
 
 int mrp_volume_get_hg_lut(const mrp_volume* h, float* host_out) {
