@@ -13,6 +13,7 @@ Prints one JSON line (stdout); progress goes to stderr.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
 """
 import argparse
+import ctypes
 import json
 import math
 import os
@@ -80,6 +81,8 @@ def main(argv=None):
     ap.add_argument("--spp-per-step", type=int, default=1,
                     help="1-spp frames traced per launch (a step renders that many frames)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="first run one device-to-device copy of this many MiB (reference kernel for HBM counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -87,8 +90,16 @@ def main(argv=None):
         return 2
 
     import internship_amd as ia
-    ia.load()
+    lib = ia.load()
     W, H, res = a.width, a.height, a.res
+    if a.copy_calibration_mib > 0:
+        nb = a.copy_calibration_mib << 20
+        src, dst = ia.DeviceBuffer(nb), ia.DeviceBuffer(nb)
+        src.zero()
+        ia.check(lib.mrp_memcpy_d2d(dst.ptr, src.ptr, ctypes.c_size_t(nb)))
+        log("calibration copy of %d MiB done" % a.copy_calibration_mib)
+        src.free()
+        dst.free()
     log("generating %d^3 cloud on the GPU" % res)
     grid_dev = ia.DeviceBuffer(res ** 3 * 4)
     ia.generate_cloud(res, seed=1, device_out=grid_dev)

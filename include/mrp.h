@@ -105,6 +105,7 @@ int mrp_free(void* dev);
 int mrp_memcpy_h2d(void* dev_dst, const void* host_src, size_t bytes);
 int mrp_memcpy_d2h(void* host_dst, const void* dev_src, size_t bytes);
 int mrp_memset_zero(void* dev, size_t bytes);
+int mrp_memcpy_d2d(void* dev_dst, const void* dev_src, size_t bytes);
 
 /* ------------------------------------------------------------------ volume */
 typedef struct mrp_volume mrp_volume;

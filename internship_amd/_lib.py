@@ -70,6 +70,7 @@ SIGNATURES = {
     "mrp_memcpy_h2d": (I, [P, P, SZ]),
     "mrp_memcpy_d2h": (I, [P, P, SZ]),
     "mrp_memset_zero": (I, [P, SZ]),
+    "mrp_memcpy_d2d": (I, [P, P, SZ]),
     "mrp_volume_create": (I, [I, ctypes.POINTER(P)]),
     "mrp_volume_destroy": (I, [P]),
     "mrp_volume_resolution": (I, [P, ctypes.POINTER(I)]),

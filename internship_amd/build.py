@@ -33,7 +33,7 @@ HIPCC_FLAGS = [
     "-Wno-unused-command-line-argument",
     "-I" + os.path.join(ROOT, "include"),
 ]
-GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+GXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-ffp-contract=off", "-mfma", "-Wall", "-Wno-unused-function",
              "-I" + os.path.join(ROOT, "include")]
 
 

@@ -81,6 +81,12 @@ int mrp_memcpy_d2h(void* dst, const void* src, size_t bytes) {
     MRP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
     return MRP_OK;
 }
+int mrp_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return MRP_OK;
+    if (!dst || !src) return fail(MRP_E_INVALID, "NULL pointer");
+    MRP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return MRP_OK;
+}
 int mrp_memset_zero(void* dev, size_t bytes) {
     if (bytes == 0) return MRP_OK;
     if (!dev) return fail(MRP_E_INVALID, "NULL pointer");
@@ -91,7 +97,7 @@ int mrp_memset_zero(void* dev, size_t bytes) {
 int mrp_volume_create(int resolution, mrp_volume** out) {
     if (!out) return fail(MRP_E_INVALID, "out is NULL");
     *out = nullptr;
-    if (resolution < 2 || resolution > 2048) return fail(MRP_E_INVALID, "resolution must be in [2, 2048]");
+    if (resolution < 2 || resolution > 1024) return fail(MRP_E_INVALID, "resolution must be in [2, 1024]");
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(MRP_E_HIP, "no HIP device");
     mrp_volume* h = new (std::nothrow) mrp_volume();
@@ -112,6 +118,8 @@ int mrp_volume_create(int resolution, mrp_volume** out) {
         return bail(MRP_E_HIP, "hipStreamCreate");
     if (hipMalloc(&v->grid, padded_count(resolution) * sizeof(float)) != hipSuccess)
         return bail(MRP_E_NOMEM, "grid allocation failed");
+    if (hipMalloc(&v->cells, cell_count(resolution) * 2 * sizeof(float4)) != hipSuccess)
+        return bail(MRP_E_NOMEM, "cell grid allocation failed");
     for (int m = 0; m < kNumMips; ++m)
         if (hipMalloc(&v->mips[m], padded_count(mip_res(m)) * sizeof(float)) != hipSuccess)
             return bail(MRP_E_NOMEM, "mip allocation failed");
@@ -133,6 +141,7 @@ int mrp_volume_destroy(mrp_volume* h) {
     Volume* v = &h->v;
     if (v->stream) hipStreamSynchronize(v->stream);
     hipFree(v->grid);
+    hipFree(v->cells);
     for (int m = 0; m < kNumMips; ++m) hipFree(v->mips[m]);
     for (int m = 0; m < kNumTrMips; ++m) hipFree(v->tr_mips[m]);
     hipFree(v->hg_lut);

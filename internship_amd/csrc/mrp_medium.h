@@ -18,6 +18,25 @@ struct Grid {
     const float* data;
     int res;
 };
+
+// Majorant constants of the trackers.  The reference divides by the
+// majorant SMax = maxDensity * alpha at every step (`t -= log(1-xi)/SMax`,
+// `S/SMax`, core/render.cuh:101,109,145,158); here the three reciprocals are
+// formed once and every step multiplies (a rounding-level difference; the
+// reference's own -use_fast_math build divides approximately).
+struct Majorant {
+    float inv_smax;     // 1/SMax: free-flight distance scale
+    float inv_smax_tr;  // tr_scale/SMax: shadow-ray distance scale
+    float ratio;        // alpha/SMax: density -> collision probability
+};
+MRP_HD Majorant make_majorant(float max_density, float alpha, float tr_scale) {
+    const float smax = max_density * alpha;
+    Majorant m;
+    m.inv_smax = 1.0f / smax;
+    m.inv_smax_tr = tr_scale / smax;
+    m.ratio = alpha / smax;
+    return m;
+}
 MRP_HD size_t padded_index(int res, int x, int y, int z) {
     const size_t p = (size_t)res + 2;
     return ((size_t)(x + 1) * p + (size_t)(y + 1)) * p + (size_t)(z + 1);
@@ -49,6 +68,39 @@ MRP_HD float trilinear_border_uv(const float* data, int res, float ux, float uy,
     return lerp(lerp(lerp(c000, c001, wz), lerp(c010, c011, wz), wy),
                 lerp(lerp(c100, c101, wz), lerp(c110, c111, wz), wy), wx);
 }
+
+// Corner-expanded cell layout of the tracer's density grid: cell (x,y,z) for
+// x,y,z in [-1, res-1] holds the eight corners of its trilinear neighbourhood
+// as two float4 (c000 c001 c010 c011 | c100 c101 c110 c111), cells z fastest.
+// A lookup is then one 32-byte aligned read from a single cache line instead
+// of four 8-byte reads from four lines (z-pair, y+1, x+1, x+1/y+1) of a
+// linear grid; the price is 8x the footprint (4.3 GB for 512^3).
+MRP_HD size_t cell_index(int res, int x, int y, int z) {
+    const size_t c = (size_t)res + 1;
+    return ((size_t)(x + 1) * c + (size_t)(y + 1)) * c + (size_t)(z + 1);
+}
+MRP_HD size_t cell_count(int res) {
+    const size_t c = (size_t)res + 1;
+    return c * c * c;
+}
+
+// Density at box position p from the cell layout: the same arithmetic as
+// trilinear_border (same corners, same weights, same blend order).
+#if defined(__HIPCC__)
+__device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cells, int res, vec3 p) {
+    const float fr = (float)res;
+    const float tx = (p.x + 0.5f) * fr - 0.5f;
+    const float ty = (p.y + 0.5f) * fr - 0.5f;
+    const float tz = (p.z + 0.5f) * fr - 0.5f;
+    if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) return 0.0f;
+    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
+    const float4* c = cells + 2 * cell_index(res, (int)fx, (int)fy, (int)fz);
+    const float4 lo = c[0], hi = c[1];
+    return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
+                wx);
+}
+#endif
 
 // Density at box position p in [-0.5,0.5]^3: uv = p + 0.5 as in `Density`
 // (core/render.cuh:67-71) and `MipDensityDynamic` (core/renderResources.cu:42-67).

@@ -36,9 +36,10 @@ __constant__ float c_tap_v[kTaps];
 __constant__ float c_stencil[kShellTaps * 3];
 
 struct NNArgs {
-    const float* grid;
+    const float4* cells;
     int res;
     float smax, alpha, g, tr_scale;
+    Majorant mj;
     vec3 ori, forward, right, up, light, lx, light_color, sky, scatter_rate;
     int width, height;
     uint32_t seed_lo, seed_hi;
@@ -56,12 +57,12 @@ __device__ bool fv_free_flight(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir, fl
     int loop_num = 0;
     while (loop_num++ < 10000) {
         float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) / a.smax;
+        t -= logf_(1.0f - rk) * a.mj.inv_smax;
         if (t > dis) return false;
         rk = rng_next(rng);
-        const float d = trilinear_border(a.grid, a.res, pos + (dir * t));
+        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
         *lk += 1u;
-        if ((d * a.alpha) / a.smax > rk) break;
+        if (d * a.mj.ratio > rk) break;
         if (d < 0.0f) t -= d;
     }
     (void)rng_next(rng);  // HG direction draws of `DeterminateNextVertex` (direction unused)
@@ -75,20 +76,20 @@ __device__ float fv_transmittance(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir,
     int loop_num = 0;
     while (loop_num++ < 10000 && tr > 0.5f) {
         const float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) / a.smax * a.tr_scale;
+        t -= logf_(1.0f - rk) * a.mj.inv_smax_tr;
         if (t > dis) break;
-        const float d = trilinear_border(a.grid, a.res, pos + (dir * t));
+        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
         *lk += 1u;
-        tr *= 1.0f - fmax_(0.0f, (d * a.alpha) / a.smax);
+        tr *= 1.0f - fmax_(0.0f, d * a.mj.ratio);
         if (d < 0.0f) t -= d;
     }
     while (loop_num++ < 10000 && tr > 0.001f) {
         const float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) / a.smax * a.tr_scale;
+        t -= logf_(1.0f - rk) * a.mj.inv_smax_tr;
         if (t > dis) break;
-        const float d = trilinear_border(a.grid, a.res, pos + (dir * t));
+        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
         *lk += 1u;
-        if (rng_next(rng) < fmax_(0.0f, (d * a.alpha) / a.smax)) {
+        if (rng_next(rng) < fmax_(0.0f, d * a.mj.ratio)) {
             tr = 0.0f;
             break;
         }
@@ -319,10 +320,11 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     float* pred = gam + chunk_items;
 
     NNArgs a;
-    a.grid = v->grid;
+    a.cells = v->cells;
     a.res = v->res;
     a.alpha = p->alpha;
     a.smax = v->max_density * p->alpha;
+    a.mj = make_majorant(v->max_density, p->alpha, p->tr_scale);
     a.g = p->g;
     a.tr_scale = p->tr_scale;
     a.ori = V(p->ori);

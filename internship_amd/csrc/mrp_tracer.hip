@@ -34,14 +34,17 @@
 
 namespace mrp {
 
-enum : int { ST_IDLE = 0, ST_INIT = 1, ST_FF = 2, ST_TR1 = 3, ST_TR2 = 4 };
-enum : int { EV_NONE = 0, EV_ACCEPT = 1, EV_ESCAPE = 2, EV_TRDONE = 3, EV_DONE = 4 };
+// Path states: idle lane, pending path start, free flight, ratio tracking and
+// delta tracking toward the light, pending real collision, pending end of the
+// shadow ray, finished.
+enum : int { ST_IDLE = 0, ST_INIT = 1, ST_FF = 2, ST_TR1 = 3, ST_TR2 = 4, ST_ACC = 5, ST_TRD = 6, ST_DONE = 7 };
 
 struct TraceArgs {
-    const float* grid;
+    const float4* cells;
     int res;
     float smax, alpha, g, tr_scale;
     vec3 ori, forward, right, up, light_dir, light_color, sky, scatter_rate;
+    Majorant mj;
     int width, height, multi_scatter;
     uint32_t seed_lo, seed_hi;
     int spp_begin, spp_count;
@@ -77,8 +80,19 @@ __device__ __forceinline__ float ring_take(Ring& r) {
 }
 
 __device__ __forceinline__ float fetch_density(const TraceArgs& a, vec3 p) {
-    return trilinear_border(a.grid, a.res, p);
+    return trilinear_cells(a.cells, a.res, p);
 }
+
+// Pending-event batching.  An event (path start, real collision, end of a
+// shadow ray) runs a long divergent block (camera ray and box entry; HG
+// direction sampling; box exit along the new direction).  With one event per
+// ~25 steps per lane, some lane of 64 needs each block in almost every
+// iteration, so executing it eagerly costs the whole wave several hundred
+// instructions per lookup.  Instead a lane that hits an event parks in a
+// pending state and the block runs when at least kEventBatch lanes wait for
+// it, or when fewer than kMinStepping lanes are still stepping.
+constexpr int kEventBatch = 16;
+constexpr int kMinStepping = 40;
 
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
     const uint32_t lane = __lane_id();
@@ -106,32 +120,35 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
     int wq = q;
 
     for (;;) {
+        const bool stepping = (state == ST_FF) | (state == ST_TR1) | (state == ST_TR2);
+        const int n_stepping = __popcll(__ballot(stepping));
+        const bool flush = n_stepping < kMinStepping;
+
         // ------------------------------------------------ refill idle lanes
         const bool want = (state == ST_IDLE);
         const unsigned long long mask = __ballot(want);
-        if (mask != 0ull && wq_cur == wq_end && qtries < kNumQueues) {
-            uint32_t base = 0u;
-            if (lane == 0u) base = atomicAdd(&a.queues[q], kChunk);
-            base = __shfl(base, 0);
-            if (base >= a.items_per_queue) {  // this queue is exhausted: move on
-                q = (q + 1) & 7;
-                qtries += 1;
-            } else {
-                wq = q;
-                wq_cur = base;
-                wq_end = min(base + kChunk, a.items_per_queue);
+        if (mask != 0ull && (flush || __popcll(mask) >= kEventBatch)) {
+            if (wq_cur == wq_end && qtries < kNumQueues) {
+                uint32_t base = 0u;
+                if (lane == 0u) base = atomicAdd(&a.queues[q], kChunk);
+                base = __shfl(base, 0);
+                if (base >= a.items_per_queue) {  // this queue is exhausted: move on
+                    q = (q + 1) & 7;
+                    qtries += 1;
+                } else {
+                    wq = q;
+                    wq_cur = base;
+                    wq_end = min(base + kChunk, a.items_per_queue);
+                }
             }
-        }
-        if (mask != 0ull && wq_cur < wq_end) {
-            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
-            const uint32_t avail = wq_end - wq_cur;
-            const uint32_t take = min(avail, (uint32_t)__popcll(mask));
-            const uint32_t base = wq_cur;
-            const int qf = wq;
-            wq_cur += take;
-            if (want && (uint32_t)rank < take) {
-                const uint32_t k = base + (uint32_t)rank;
-                {
+            if (wq_cur < wq_end) {
+                const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+                const uint32_t take = min(wq_end - wq_cur, (uint32_t)__popcll(mask));
+                const uint32_t base = wq_cur;
+                const int qf = wq;
+                wq_cur += take;
+                if (want && (uint32_t)rank < take) {
+                    const uint32_t k = base + (uint32_t)rank;
                     const uint32_t per_sample = (uint32_t)a.tiles_per_band * 64u;
                     const uint32_t s = k / per_sample;
                     const uint32_t r = k - s * per_sample;
@@ -172,46 +189,99 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
             }
             rng.hi += 4u;
         }
+        if (state != ST_IDLE) path_iters++;
 
-        path_iters++;
-        int ev = EV_NONE;
-        bool lookup = false;
-        vec3 lp = v3(0, 0, 0);
-        float rk2 = 0.0f;
-
-        if (state == ST_INIT) {
-            // camera ray (core/volume.cu:158-163) then CalculateRadiance entry (core/render.cuh:205-226)
-            const float xu = ring_take(rng);
-            const float xv = ring_take(rng);
-            const int row = pix / a.width, col = pix - row * a.width;
-            vec3 dir = camera_dir(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv);
-            dir = normalize(dir);
-            const float off = ray_box_offset(a.ori, dir);
-            tfirst = -1.0f;
-            if (off < 0.0f) {
-                res = a.sky;
-                ev = EV_DONE;
-            } else {
-                pos = a.ori + dir * off;
-                rdir = dir;
-                csr = v3(1, 1, 1);
-                res = v3(0, 0, 0);
-                scatter_num = 0;
-                if (a.multi_scatter <= 0) {
-                    ev = EV_DONE;
+        // ------------------------------------------------ path starts
+        {
+            const unsigned long long m = __ballot(state == ST_INIT);
+            if (m != 0ull && (flush || __popcll(m) >= kEventBatch) && state == ST_INIT) {
+                // camera ray (core/volume.cu:158-163), CalculateRadiance entry (core/render.cuh:205-226)
+                const float xu = ring_take(rng);
+                const float xv = ring_take(rng);
+                const int row = pix / a.width, col = pix - row * a.width;
+                vec3 dir = camera_dir(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv);
+                dir = normalize(dir);
+                const float off = ray_box_offset(a.ori, dir);
+                tfirst = -1.0f;
+                if (off < 0.0f) {
+                    res = a.sky;
+                    state = ST_DONE;
                 } else {
+                    pos = a.ori + dir * off;
+                    rdir = dir;
+                    csr = v3(1, 1, 1);
+                    res = v3(0, 0, 0);
+                    scatter_num = 0;
+                    if (a.multi_scatter <= 0) {
+                        state = ST_DONE;
+                    } else {
+                        dis = ray_box_distance(pos, rdir);
+                        t = 0.0f;
+                        loop_num = 0;
+                        state = ST_FF;
+                    }
+                }
+            }
+        }
+        // ------------------------------------------------ real collisions (DeterminateNextVertex tail)
+        {
+            const unsigned long long m = __ballot(state == ST_ACC);
+            if (m != 0ull && (flush || __popcll(m) >= kEventBatch) && state == ST_ACC) {
+                const float e0 = ring_take(rng);
+                const float e1 = ring_take(rng);
+                ndir = hg_sample_dir(e0, e1, rdir, a.g);
+                const vec3 npos = (rdir * t) + pos;
+                n_scatters++;
+                if (scatter_num == 0) tfirst = distance(a.ori, npos);
+                pos = npos;
+                csr = csr * a.scatter_rate;
+                dis = ray_box_distance(pos, a.light_dir);
+                light_phase = hg_phase(dot(rdir, a.light_dir), a.g);
+                tr = 1.0f;
+                t = 0.0f;
+                loop_num = 0;
+                state = ST_TR1;
+            }
+        }
+        // ------------------------------------------------ shadow rays done (next-event estimate)
+        {
+            const unsigned long long m = __ballot(state == ST_TRD);
+            if (m != 0ull && (flush || __popcll(m) >= kEventBatch) && state == ST_TRD) {
+                res = res + a.light_color * csr * (tr * light_phase);
+                rdir = ndir;
+                scatter_num++;
+                if (scatter_num < a.multi_scatter) {
                     dis = ray_box_distance(pos, rdir);
                     t = 0.0f;
                     loop_num = 0;
                     state = ST_FF;
+                } else {
+                    state = ST_DONE;
                 }
             }
-        } else if (state != ST_IDLE) {
+        }
+        // ------------------------------------------------ finished paths
+        if (state == ST_DONE) {
+            a.out[(size_t)sidx * (size_t)P + (size_t)pix] = make_float4(res.x, res.y, res.z, tfirst);
+            n_paths++;
+            max_iters = max(max_iters, path_iters);
+            state = ST_IDLE;
+        }
+
+        // ------------------------------------------------ one tracking step
+        bool lookup = false;
+        vec3 lp = v3(0, 0, 0);
+        float rk2 = 0.0f;
+        if ((state == ST_FF) | (state == ST_TR1) | (state == ST_TR2)) {
             // loop-condition checks of the reference while loops
+            bool go = true;
             if (state == ST_FF) {
                 const bool c = loop_num < 10000;
                 loop_num++;
-                if (!c) ev = EV_ACCEPT;
+                if (!c) {
+                    state = ST_ACC;  // loop cap: scatter at the current distance
+                    go = false;
+                }
             } else {
                 if (state == ST_TR1) {
                     const bool c = loop_num < 10000;
@@ -221,16 +291,19 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                 if (state == ST_TR2) {
                     const bool c = loop_num < 10000;
                     loop_num++;
-                    if (!(c && tr > 0.001f)) ev = EV_TRDONE;
+                    if (!(c && tr > 0.001f)) {
+                        state = ST_TRD;
+                        go = false;
+                    }
                 }
             }
-            if (ev == EV_NONE) {
+            if (go) {
                 const float rk = ring_take(rng);
-                const float scale = (state == ST_FF) ? 1.0f : a.tr_scale;
-                t -= logf_(1.0f - rk) / a.smax * scale;
+                t -= logf_(1.0f - rk) * (state == ST_FF ? a.mj.inv_smax : a.mj.inv_smax_tr);
                 if (t > dis) {
                     if (state == ST_FF) {
-                        ev = EV_ESCAPE;
+                        res = res + a.sky * csr;  // escaped: environment
+                        state = ST_DONE;
                     } else {
                         if (state == ST_TR1) {
                             // the delta-tracking loop still runs once: its check, one
@@ -239,7 +312,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                             loop_num++;
                             if (c && tr > 0.001f) (void)ring_take(rng);
                         }
-                        ev = EV_TRDONE;
+                        state = ST_TRD;
                     }
                 } else {
                     lookup = true;
@@ -250,13 +323,12 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
         }
 
         // ------------------------------------------------ the one lookup site
-        float d = 0.0f;
         if (lookup) {
-            d = fetch_density(a, lp);
+            const float d = fetch_density(a, lp);
             n_lookups++;
-            const float ratio = (d * a.alpha) / a.smax;
+            const float ratio = d * a.mj.ratio;
             if (state == ST_FF) {
-                if (ratio > rk2) ev = EV_ACCEPT;
+                if (ratio > rk2) state = ST_ACC;
                 else if (d < 0.0f) t -= d;
             } else if (state == ST_TR1) {
                 tr *= 1.0f - fmax_(0.0f, ratio);
@@ -265,46 +337,13 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                 const float rk3 = ring_take(rng);
                 if (rk3 < fmax_(0.0f, ratio)) {
                     tr = 0.0f;
-                    ev = EV_TRDONE;
+                    state = ST_TRD;
                 } else if (d < 0.0f) {
                     t -= d;
                 }
             }
         }
-
-        // ------------------------------------------------ events
-        if (ev == EV_ACCEPT) {
-            const float e0 = ring_take(rng);
-            const float e1 = ring_take(rng);
-            ndir = hg_sample_dir(e0, e1, rdir, a.g);
-            const vec3 npos = (rdir * t) + pos;
-            n_scatters++;
-            if (scatter_num == 0) tfirst = distance(a.ori, npos);
-            pos = npos;
-            csr = csr * a.scatter_rate;
-            dis = ray_box_distance(pos, a.light_dir);
-            light_phase = hg_phase(dot(rdir, a.light_dir), a.g);
-            tr = 1.0f;
-            t = 0.0f;
-            loop_num = 0;
-            state = ST_TR1;
-        } else if (ev == EV_ESCAPE) {
-            res = res + a.sky * csr;
-            ev = EV_DONE;
-        } else if (ev == EV_TRDONE) {
-            res = res + a.light_color * csr * (tr * light_phase);
-            rdir = ndir;
-            scatter_num++;
-            if (scatter_num < a.multi_scatter) {
-                dis = ray_box_distance(pos, rdir);
-                t = 0.0f;
-                loop_num = 0;
-                state = ST_FF;
-            } else {
-                ev = EV_DONE;
-            }
-        }
-        if (ev == EV_DONE) {
+        if (state == ST_DONE) {
             a.out[(size_t)sidx * (size_t)P + (size_t)pix] = make_float4(res.x, res.y, res.z, tfirst);
             n_paths++;
             max_iters = max(max_iters, path_iters);
@@ -398,10 +437,11 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
     if (rc) return rc;
 
     TraceArgs a;
-    a.grid = v->grid;
+    a.cells = v->cells;
     a.res = v->res;
     a.alpha = p->alpha;
     a.smax = v->max_density * p->alpha;
+    a.mj = make_majorant(v->max_density, p->alpha, p->tr_scale);
     a.g = p->g;
     a.tr_scale = p->tr_scale;
     a.ori = V(p->ori);

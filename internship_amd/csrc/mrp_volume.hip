@@ -51,6 +51,23 @@ __global__ void unpad_kernel(const float* __restrict__ src, int res, float* __re
     }
 }
 
+// cells[(x,y,z)] = the 8 corners (x..x+1, y..y+1, z..z+1) of the padded grid,
+// for x,y,z in [-1, res-1].
+__global__ void build_cells_kernel(const float* __restrict__ grid, int res, float4* __restrict__ cells) {
+    const int c = res + 1;
+    const size_t n = (size_t)c * c * c;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t P = (size_t)res + 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % c) - 1;
+        const int y = (int)((i / c) % c) - 1;
+        const int x = (int)(i / ((size_t)c * c)) - 1;
+        const float* g = grid + padded_index(res, x, y, z);
+        cells[2 * i] = make_float4(g[0], g[1], g[P], g[P + 1]);
+        cells[2 * i + 1] = make_float4(g[P * P], g[P * P + 1], g[P * P + P], g[P * P + P + 1]);
+    }
+}
+
 // max over the interior of a padded grid; result as float bits in *out
 // (initialised to bits(1e-5f) > 0, so unsigned ordering equals float ordering).
 __global__ __launch_bounds__(256) void max_kernel(const float* __restrict__ src, int res, unsigned* out) {
@@ -182,6 +199,7 @@ int launch_update(Volume* v) {
     MRP_HIP(hipStreamSynchronize(v->stream));
     hipFree(dmax);
     v->max_density = u2f(bits);
+    build_cells_kernel<<<blocks_for(cell_count(res)), 256, 0, v->stream>>>(v->grid, res, v->cells);
 
     // source chain: halve (by trilinear resampling) while above 2x256
     int r = 256;

@@ -73,6 +73,7 @@ struct Medium {
     float max_density;
     float alpha, g, tr_scale;
     vec3 sky, scatter_rate, light_color;
+    Majorant mj;  // reciprocals of SMax = max_density * alpha
 };
 
 struct Counters {
@@ -82,26 +83,23 @@ struct Counters {
 // `Tr` (core/render.cuh:93-136): ratio tracking while tr > 0.5, then delta
 // tracking while tr > 0.001, sharing one 10000-iteration cap.
 float transmittance(Rng* rng, const Medium& m, vec3 pos, vec3 dir, float dis, Counters* c) {
-    const float smax = m.max_density * m.alpha;
     float tr = 1.0f;
     float t = 0.0f;
     int loop_num = 0;
     while (loop_num++ < 10000 && tr > 0.5f) {
         const float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) / smax * m.tr_scale;
+        t -= logf_(1.0f - rk) * m.mj.inv_smax_tr;
         if (t > dis) break;
         const float d = density(m.grid, pos + (dir * t), &c->lookups);
-        const float S = d * m.alpha;
-        tr *= 1.0f - fmax_(0.0f, S / smax);
+        tr *= 1.0f - fmax_(0.0f, d * m.mj.ratio);
         if (d < 0.0f) t -= d;
     }
     while (loop_num++ < 10000 && tr > 0.001f) {
         const float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) / smax * m.tr_scale;
+        t -= logf_(1.0f - rk) * m.mj.inv_smax_tr;
         if (t > dis) break;
         const float d = density(m.grid, pos + (dir * t), &c->lookups);
-        const float S = d * m.alpha;
-        if (rng_next(rng) < fmax_(0.0f, S / smax)) {
+        if (rng_next(rng) < fmax_(0.0f, d * m.mj.ratio)) {
             tr = 0.0f;
             break;
         }
@@ -115,12 +113,11 @@ float transmittance(Rng* rng, const Medium& m, vec3 pos, vec3 dir, float dis, Co
 // draws (first -> cosine, second -> azimuth).
 bool next_vertex(Rng* rng, const Medium& m, float g, vec3 pos, vec3 dir, float dis, vec3* next_pos, vec3* next_dir,
                  Counters* c) {
-    const float smax = m.max_density * m.alpha;
     float t = 0.0f;
     int loop_num = 0;
     while (loop_num++ < 10000) {
         float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) / smax;
+        t -= logf_(1.0f - rk) * m.mj.inv_smax;
         if (t > dis) {
             *next_pos = v3(0, 0, 0);
             *next_dir = v3(0, 0, 0);
@@ -128,8 +125,7 @@ bool next_vertex(Rng* rng, const Medium& m, float g, vec3 pos, vec3 dir, float d
         }
         rk = rng_next(rng);
         const float d = density(m.grid, pos + (dir * t), &c->lookups);
-        const float S = d * m.alpha;
-        if (S / smax > rk) break;
+        if (d * m.mj.ratio > rk) break;
         if (d < 0.0f) t -= d;
     }
     const float e0 = rng_next(rng);
@@ -779,6 +775,7 @@ int oracle_render_mrpnn(const float* grid, int res, float max_density, const flo
     m.sky = V(p->sky);
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
+    m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
     const vec3 ori = V(p->ori), up = V(p->up), right = V(p->right), light = V(p->light_dir);
     const vec3 forward = normalize(-ori);
     const vec3 Ln = normalize(light);
@@ -855,6 +852,7 @@ int oracle_render(const float* grid, int res, float max_density, const mrp_rende
     m.sky = V(p->sky);
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
+    m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
     const vec3 ori = V(p->ori), up = V(p->up), right = V(p->right), light = V(p->light_dir);
     const vec3 forward = normalize(-ori);
     const int W = p->width, H = p->height;
