@@ -9,7 +9,7 @@
 //   * One persistent launch covers every (pixel, sample) path of a batch.
 //     Workgroup b reads work queue b % 8, i.e. the queue of the XCD the
 //     workgroup is dispatched to, and each queue owns one horizontal band of
-//     the image (8x8-pixel tiles, sample-major), so rays that share an XCD's
+//     the image (8x8-pixel tiles, all samples of a tile together), so rays that share an XCD's
 //     L2 also share voxels.  A wave that drains its queue steals from the next.
 //   * Lanes fetch items in wave-batched atomics (one atomicAdd per wave) and
 //     regenerate a new path as soon as theirs ends, so lanes stay busy despite
@@ -149,10 +149,12 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                 wq_cur += take;
                 if (want && (uint32_t)rank < take) {
                     const uint32_t k = base + (uint32_t)rank;
-                    const uint32_t per_sample = (uint32_t)a.tiles_per_band * 64u;
-                    const uint32_t s = k / per_sample;
-                    const uint32_t r = k - s * per_sample;
-                    const uint32_t tile = r >> 6, w = r & 63u;
+                    // tile-major: all samples of one 8x8 tile are consecutive items, so
+                    // the paths in flight on an XCD start from a compact patch of image
+                    const uint32_t per_tile = (uint32_t)a.spp_count * 64u;
+                    const uint32_t tile = k / per_tile;
+                    const uint32_t r = k - tile * per_tile;
+                    const uint32_t s = r >> 6, w = r & 63u;
                     const int ty = (int)(tile / (uint32_t)a.tiles_x), tx = (int)(tile % (uint32_t)a.tiles_x);
                     const int col = tx * 8 + (int)(w & 7u);
                     const int row0 = qf * a.band_rows;
