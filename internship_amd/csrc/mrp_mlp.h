@@ -191,7 +191,7 @@ MRP_HD void mlp_forward_fp32(const float* w, const float* feat, float gamma, flo
 
 // BF16 MFMA path (mrp_mlp_bf16.hip)
 int pack_mlp_bf16(const float* weights_host, void** packed_dev);
-int launch_mlp_bf16(const void* packed, const float* feat, const float* gamma, float g, vec3 srp, uint32_t n_items,
-                    float* pred, hipStream_t stream);
+int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, const float* gamma, float g, vec3 srp,
+                    uint32_t n_items, float* pred, hipStream_t stream);
 
 }  // namespace mrp

@@ -1,19 +1,551 @@
-// mrp_mlp_bf16.hip - BF16 MFMA MRPNN network (placeholder until the MFMA kernel lands).
+// mrp_mlp_bf16.hip - the MRPNN network on gfx950 matrix cores (BF16 operands,
+// FP32 accumulation).
+//
+// The reference evaluates the network (`RadiancePredict` lines 370-486,
+// core/radiancePredict.cu) one pixel per thread with scalar loops.  Here one
+// wave evaluates 32 descriptors at once: every layer is a v_mfma_f32_32x32x16_bf16
+// product D[32 out x 32 items] = W[32 x 16] x X[16 x 32 items], tiled over the
+// layer's rows (32) and inputs (16).
+//
+// Register layout ("D-order").  Lane l works on item n = l % 32 and lane group
+// kg = l / 32.  An MFMA result leaves row r = 8*(v/4) + 4*kg + (v%4) of item n in
+// register v (v = 0..15).  The next layer consumes those registers directly as
+// its B operand (K block b = registers 8b..8b+7) because its weight columns are
+// permuted on the host to the same order: A element e of K block kb = 2j+b for
+// lane group kg multiplies input feature 32j + 16b + 8*(e/4) + 4*kg + (e%4).
+// Descriptor features and pooled statistics are loaded from HBM straight into
+// that order (two float4 per 16-feature block), so no layer needs a shuffle.
+// Layers narrower than 32x16 are zero-padded (rows >= M and features >= K have
+// zero weights; padded inputs are kept finite).
+//
+// Squeeze-excitation gates (8x8 -> ReLU -> 3x8 -> sigmoid) also run as MFMA
+// products; the three gates are exchanged between the two lane groups of an
+// item with one cross-lane move.  The per-segment means and maxima the gates
+// start from come precomputed from the gather kernel.
 #include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "mrp_internal.h"
 #include "mrp_mlp.h"
+#include "mrp_mrpnn.h"
+#include "mrp_weights_gen.h"
 
 namespace mrp {
 
-int pack_mlp_bf16(const float* weights_host, void** packed_dev) {
-    (void)weights_host;
-    *packed_dev = nullptr;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------- layer table
+enum : int {
+    LY_SE0 = 0,         // 12 SE first layers (8x8, no bias)
+    LY_SE1 = 12,        // 12 SE second layers (3x8, no bias)
+    LY_STREAM = 24,     // 12 blocks x 3 streams (SxS + bias)
+    LY_L81 = 60,        // 3 (32x32 + bias)
+    LY_LD41 = 63,       // 3 (8x8 + bias)
+    LY_FIN1 = 66,       // 16x75, no bias
+    LY_FIN2 = 67,       // 6x16, no bias
+    LY_LC0 = 68,        // 128x128
+    LY_LC1 = 69,        // 64x128
+    LY_LC2 = 70,        // 32x64
+    LY_LX = 71,         // 16x32
+    LY_LXR = 72,        // 12 residual 16x16
+    LY_COUNT = 84
+};
+
+struct LayerDesc {
+    int w, b;      // offsets into the FP32 weight blob (b < 0: no bias)
+    int M, K;      // rows, inputs
+    int tile;      // first A tile (64 lanes x 8 bf16) in the packed buffer
+    int btile;     // first bias tile (2 lane groups x 16 floats) or -1
+};
+
+__constant__ LayerDesc c_layers[LY_COUNT];
+
+static void layer_table(LayerDesc* L) {
+    const int se0[12] = {MRP_W_LSE01W, MRP_W_LSE11W, MRP_W_LSE21W, MRP_W_LSE31W, MRP_W_LSE41W, MRP_W_LSE51W,
+                         MRP_W_LSE61W, MRP_W_LSE71W, MRP_W_LDSE01W, MRP_W_LDSE11W, MRP_W_LDSE21W, MRP_W_LDSE31W};
+    const int se1[12] = {MRP_W_LSE02W, MRP_W_LSE12W, MRP_W_LSE22W, MRP_W_LSE32W, MRP_W_LSE42W, MRP_W_LSE52W,
+                         MRP_W_LSE62W, MRP_W_LSE72W, MRP_W_LDSE02W, MRP_W_LDSE12W, MRP_W_LDSE22W, MRP_W_LDSE32W};
+    const int sw[12][3] = {{MRP_W_L01W, MRP_W_L_Tr01W, MRP_W_L_Hg01W},   {MRP_W_L11W, MRP_W_L_Tr11W, MRP_W_L_Hg11W},
+                           {MRP_W_L21W, MRP_W_L_Tr21W, MRP_W_L_Hg21W},   {MRP_W_L31W, MRP_W_L_Tr31W, MRP_W_L_Hg31W},
+                           {MRP_W_L41W, MRP_W_L_Tr41W, MRP_W_L_Hg41W},   {MRP_W_L51W, MRP_W_L_Tr51W, MRP_W_L_Hg51W},
+                           {MRP_W_L61W, MRP_W_L_Tr61W, MRP_W_L_Hg61W},   {MRP_W_L71W, MRP_W_L_Tr71W, MRP_W_L_Hg71W},
+                           {MRP_W_LD01W, MRP_W_LD_Tr01W, MRP_W_LD_Hg01W}, {MRP_W_LD11W, MRP_W_LD_Tr11W, MRP_W_LD_Hg11W},
+                           {MRP_W_LD21W, MRP_W_LD_Tr21W, MRP_W_LD_Hg21W}, {MRP_W_LD31W, MRP_W_LD_Tr31W, MRP_W_LD_Hg31W}};
+    const int sb[12][3] = {{MRP_W_L01B, MRP_W_L_Tr01B, MRP_W_L_Hg01B},   {MRP_W_L11B, MRP_W_L_Tr11B, MRP_W_L_Hg11B},
+                           {MRP_W_L21B, MRP_W_L_Tr21B, MRP_W_L_Hg21B},   {MRP_W_L31B, MRP_W_L_Tr31B, MRP_W_L_Hg31B},
+                           {MRP_W_L41B, MRP_W_L_Tr41B, MRP_W_L_Hg41B},   {MRP_W_L51B, MRP_W_L_Tr51B, MRP_W_L_Hg51B},
+                           {MRP_W_L61B, MRP_W_L_Tr61B, MRP_W_L_Hg61B},   {MRP_W_L71B, MRP_W_L_Tr71B, MRP_W_L_Hg71B},
+                           {MRP_W_LD01B, MRP_W_LD_Tr01B, MRP_W_LD_Hg01B}, {MRP_W_LD11B, MRP_W_LD_Tr11B, MRP_W_LD_Hg11B},
+                           {MRP_W_LD21B, MRP_W_LD_Tr21B, MRP_W_LD_Hg21B}, {MRP_W_LD31B, MRP_W_LD_Tr31B, MRP_W_LD_Hg31B}};
+    const int ssize[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
+    for (int i = 0; i < 12; ++i) {
+        L[LY_SE0 + i] = {se0[i], -1, 8, 8, 0, -1};
+        L[LY_SE1 + i] = {se1[i], -1, 3, 8, 0, -1};
+        for (int k = 0; k < 3; ++k) L[LY_STREAM + 3 * i + k] = {sw[i][k], sb[i][k], ssize[i], ssize[i], 0, -1};
+    }
+    const int l8w[3] = {MRP_W_L81W, MRP_W_L_Tr81W, MRP_W_L_Hg81W}, l8b[3] = {MRP_W_L81B, MRP_W_L_Tr81B, MRP_W_L_Hg81B};
+    const int d4w[3] = {MRP_W_LD41W, MRP_W_LD_Tr41W, MRP_W_LD_Hg41W};
+    const int d4b[3] = {MRP_W_LD41B, MRP_W_LD_Tr41B, MRP_W_LD_Hg41B};
+    for (int k = 0; k < 3; ++k) {
+        L[LY_L81 + k] = {l8w[k], l8b[k], 32, 32, 0, -1};
+        L[LY_LD41 + k] = {d4w[k], d4b[k], 8, 8, 0, -1};
+    }
+    L[LY_FIN1] = {MRP_W_LSEFin1W, -1, 16, 75, 0, -1};
+    L[LY_FIN2] = {MRP_W_LSEFin2W, -1, 6, 16, 0, -1};
+    L[LY_LC0] = {MRP_W_LC0W, MRP_W_LC0B, 128, 128, 0, -1};
+    L[LY_LC1] = {MRP_W_LC1W, MRP_W_LC1B, 64, 128, 0, -1};
+    L[LY_LC2] = {MRP_W_LC2W, MRP_W_LC2B, 32, 64, 0, -1};
+    L[LY_LX] = {MRP_W_LXW, MRP_W_LXB, 16, 32, 0, -1};
+    const int stride = MRP_W_LX1W - MRP_W_LX0W;
+    for (int r = 0; r < 12; ++r) L[LY_LXR + r] = {MRP_W_LX0W + r * stride, MRP_W_LX0W + r * stride + 256, 16, 16, 0, -1};
+    int tile = 0, btile = 0;
+    for (int i = 0; i < LY_COUNT; ++i) {
+        const int mt = (L[i].M + 31) / 32, kb = (L[i].K + 15) / 16;
+        L[i].tile = tile;
+        tile += mt * kb;
+        if (L[i].b >= 0) {
+            L[i].btile = btile;
+            btile += mt;
+        }
+    }
+}
+
+static int total_tiles(const LayerDesc* L, int* btiles) {
+    int t = 0, b = 0;
+    for (int i = 0; i < LY_COUNT; ++i) {
+        t += ((L[i].M + 31) / 32) * ((L[i].K + 15) / 16);
+        if (L[i].b >= 0) b += (L[i].M + 31) / 32;
+    }
+    *btiles = b;
+    return t;
+}
+
+// input feature multiplied by A element e of K block kb for lane group kg
+static inline int korder(int kb, int kg, int e) {
+    const int j = kb / 2, b = kb % 2;
+    return 32 * j + 16 * b + 8 * (e / 4) + 4 * kg + (e % 4);
+}
+// output row held in register v by lane group kg (MFMA result layout)
+MRP_HD int vrow(int v, int kg) { return 8 * (v / 4) + 4 * kg + (v % 4); }
+
+static uint16_t to_bf16_rne(float x) {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+
+// Packed buffer: [tiles][64 lanes][8 bf16], then [bias tiles][2][16] FP32, then
+// 64 FP32 of small VALU layers: LGGS W (24) and b (8), LX12 W (16) and b (1).
+constexpr int kExtraFloats = 64;
+int pack_mlp_bf16(const float* w, void** packed_dev) {
+    LayerDesc L[LY_COUNT];
+    layer_table(L);
+    int nb = 0;
+    const int nt = total_tiles(L, &nb);
+    std::vector<uint16_t> a((size_t)nt * 64 * 8, 0);
+    std::vector<float> bias((size_t)nb * 32, 0.0f);
+    for (int i = 0; i < LY_COUNT; ++i) {
+        const LayerDesc& d = L[i];
+        const int mtn = (d.M + 31) / 32, kbn = (d.K + 15) / 16;
+        for (int mt = 0; mt < mtn; ++mt)
+            for (int kb = 0; kb < kbn; ++kb) {
+                uint16_t* t = &a[((size_t)(d.tile + mt * kbn + kb)) * 512];
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int m = mt * 32 + l % 32, f = korder(kb, l / 32, e);
+                        const float val = (m < d.M && f < d.K) ? w[d.w + m * d.K + f] : 0.0f;
+                        t[l * 8 + e] = to_bf16_rne(val);
+                    }
+            }
+        if (d.b >= 0)
+            for (int mt = 0; mt < mtn; ++mt)
+                for (int kg = 0; kg < 2; ++kg)
+                    for (int v = 0; v < 16; ++v) {
+                        const int r = mt * 32 + vrow(v, kg);
+                        bias[((size_t)(d.btile + mt) * 2 + kg) * 16 + v] = r < d.M ? w[d.b + r] : 0.0f;
+                    }
+    }
+    float extra[kExtraFloats] = {};
+    for (int i = 0; i < 24; ++i) extra[i] = w[MRP_W_LGGSW + i];
+    for (int i = 0; i < 8; ++i) extra[24 + i] = w[MRP_W_LGGSB + i];
+    for (int i = 0; i < 16; ++i) extra[32 + i] = w[MRP_W_LX12W + i];
+    extra[48] = w[MRP_W_LX12B];
+    const size_t abytes = a.size() * 2, bbytes = bias.size() * 4;
+    void* dev = nullptr;
+    if (hipMalloc(&dev, abytes + bbytes + sizeof(extra)) != hipSuccess)
+        return fail(MRP_E_NOMEM, "packed weight allocation failed");
+    if (hipMemcpy(dev, a.data(), abytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((char*)dev + abytes, bias.data(), bbytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy((char*)dev + abytes + bbytes, extra, sizeof(extra), hipMemcpyHostToDevice) != hipSuccess) {
+        hipFree(dev);
+        return fail(MRP_E_HIP, "packed weight upload failed");
+    }
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_layers), L, sizeof(L)));
+    *packed_dev = dev;
     return MRP_OK;
 }
 
-int launch_mlp_bf16(const void*, const float*, const float*, float, vec3, uint32_t, float*, hipStream_t) {
-    return fail(MRP_E_STATE, "BF16 MFMA network not available");
+// ---------------------------------------------------------------- device helpers
+struct Packed {
+    const bf16x8* a;   // A tiles
+    const float* bias; // bias tiles
+};
+
+__device__ __forceinline__ bf16x8 to_b(const f32x16& x, int half) {
+    bf16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (__bf16)x[8 * half + e];
+    return r;
+}
+
+__device__ __forceinline__ f32x16 zero16() {
+    f32x16 z;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) z[v] = 0.0f;
+    return z;
+}
+
+// D (MT tiles) = W x B (KB blocks) [+ bias], optional ReLU.
+template <int MT, int KB>
+__device__ __forceinline__ void dense(const Packed& pk, int layer, const bf16x8 (&B)[KB], f32x16 (&D)[MT], bool relu,
+                                      int lane) {
+    const LayerDesc d = c_layers[layer];
+    const int kg = lane >> 5;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+        f32x16 acc;
+        if (d.btile >= 0) {
+            const float4* bp = (const float4*)(pk.bias + ((size_t)(d.btile + mt) * 2 + kg) * 16);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 bb = bp[q];
+                acc[4 * q] = bb.x;
+                acc[4 * q + 1] = bb.y;
+                acc[4 * q + 2] = bb.z;
+                acc[4 * q + 3] = bb.w;
+            }
+        } else {
+            acc = zero16();
+        }
+        const bf16x8* at = pk.a + (size_t)(d.tile + mt * KB) * 64 + lane;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(at[kb * 64], B[kb], acc, 0, 0, 0);
+        if (relu) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) acc[v] = acc[v] >= 0.0f ? acc[v] : 0.0f;
+        }
+        D[mt] = acc;
+    }
+}
+
+// 16 features [base, base+16) of a row in D-order into registers v = 8*half..8*half+7
+__device__ __forceinline__ void load16(const float* row, int base, int kg, f32x16& x, int half) {
+    const float4 lo = *(const float4*)(row + base + 4 * kg);
+    const float4 hi = *(const float4*)(row + base + 8 + 4 * kg);
+    x[8 * half + 0] = lo.x;
+    x[8 * half + 1] = lo.y;
+    x[8 * half + 2] = lo.z;
+    x[8 * half + 3] = lo.w;
+    x[8 * half + 4] = hi.x;
+    x[8 * half + 5] = hi.y;
+    x[8 * half + 6] = hi.z;
+    x[8 * half + 7] = hi.w;
+}
+
+__device__ __forceinline__ float sigmoid_(float x) { return 1.0f / (1.0f + expf_(-x)); }
+
+// Squeeze-excitation gates of block `blk`: returns the 3 sigmoid gates of this lane's item.
+__device__ __forceinline__ void se_gates(const Packed& pk, int blk, const float* poolrow, float g, float gamma,
+                                         int lane, float* gate) {
+    const int kg = lane >> 5;
+    // t = [mean A, mean B, mean C, max A, max B, max C, g, gamma]; lane group kg holds t[4kg..4kg+3]
+    f32x16 t = zero16();
+    if (kg == 0) {
+        t[0] = poolrow[3 * blk];
+        t[1] = poolrow[3 * blk + 1];
+        t[2] = poolrow[3 * blk + 2];
+        t[3] = poolrow[36 + 3 * blk];
+    } else {
+        t[0] = poolrow[36 + 3 * blk + 1];
+        t[1] = poolrow[36 + 3 * blk + 2];
+        t[2] = g;
+        t[3] = gamma;
+    }
+    bf16x8 b1[1] = {to_b(t, 0)};
+    f32x16 h[1];
+    dense<1, 1>(pk, LY_SE0 + blk, b1, h, true, lane);
+    bf16x8 b2[1] = {to_b(h[0], 0)};
+    f32x16 o[1];
+    dense<1, 1>(pk, LY_SE1 + blk, b2, o, false, lane);
+    // rows 0..2 live in lane group 0 (registers 0..2)
+    const int src = lane & 31;
+    gate[0] = sigmoid_(__shfl(o[0][0], src));
+    gate[1] = sigmoid_(__shfl(o[0][1], src));
+    gate[2] = sigmoid_(__shfl(o[0][2], src));
+}
+
+// ---------------------------------------------------------------- kernel
+__global__ __launch_bounds__(256) void mlp_bf16_kernel(const bf16x8* __restrict__ pa, const float* __restrict__ pbias,
+                                                       const float* __restrict__ pextra, const float* __restrict__ feat, const float* __restrict__ pool,
+                                                       const float* __restrict__ gamma_in, float g, vec3 srp,
+                                                       uint32_t n_items, float* __restrict__ pred) {
+    const int lane = (int)__lane_id();
+    const int kg = lane >> 5;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t item0 = wave * 32u;
+    if (item0 >= n_items) return;  // uniform over the wave
+    const uint32_t item = min(item0 + (uint32_t)(lane & 31), n_items - 1u);  // tail lanes recompute the last item
+    const Packed pk{pa, pbias};
+    const float* frow = feat + (size_t)item * kFeatures;
+    const float* prow = pool + (size_t)item * 72;
+    const float gamma = gamma_in[item];
+
+    // ------------------------------------------------ the two descriptor streams
+    // prev[k]: previous block output of stream k in D-order (X_ValA2/B2/C2 window)
+    f32x16 prev[3] = {zero16(), zero16(), zero16()};
+    f32x16 comb[3];    // L81 outputs (32 each)
+    f32x16 lout[3];    // LD41 outputs (8 each, registers 0..3)
+    const int from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
+    const int size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
+#pragma unroll
+    for (int blk = 0; blk < 12; ++blk) {
+        const int S = size[blk];
+        const bool residual = (blk != 0 && blk != 8);
+        const bool rep = (blk == 1 || blk == 4);  // REP3 after this block, ADD3 otherwise
+        float gate[3];
+        se_gates(pk, blk, prow, g, gamma, lane, gate);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float* srow = frow + k * kTaps + from[blk];
+            f32x16 xv = zero16();
+            if (S == 8) {
+                const float4 q = *(const float4*)(srow + 4 * kg);
+                xv[0] = q.x;
+                xv[1] = q.y;
+                xv[2] = q.z;
+                xv[3] = q.w;
+            } else {
+                load16(srow, 0, kg, xv, 0);
+                if (S == 32) load16(srow, 16, kg, xv, 1);
+            }
+            f32x16 x;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) x[v] = xv[v] * gate[k] + (residual ? prev[k][v] : 0.0f);
+            f32x16 y[1];
+            if (S == 32) {
+                bf16x8 b[2] = {to_b(x, 0), to_b(x, 1)};
+                dense<1, 2>(pk, LY_STREAM + 3 * blk + k, b, y, true, lane);
+            } else {
+                bf16x8 b[1] = {to_b(x, 0)};
+                dense<1, 1>(pk, LY_STREAM + 3 * blk + k, b, y, true, lane);
+            }
+            if (rep) {
+                // window = [block output | this block's input features]
+                f32x16 w = y[0];
+                if (S == 8) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) w[4 + v] = xv[v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 8; ++v) w[8 + v] = xv[v];
+                }
+                prev[k] = w;
+            } else {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) prev[k][v] = y[0][v] + xv[v];
+            }
+        }
+        if (blk == 7) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                bf16x8 b[2] = {to_b(prev[k], 0), to_b(prev[k], 1)};
+                f32x16 o[1];
+                dense<1, 2>(pk, LY_L81 + k, b, o, true, lane);
+                comb[k] = o[0];
+                prev[k] = zero16();
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        bf16x8 b[1] = {to_b(prev[k], 0)};
+        f32x16 o[1];
+        dense<1, 1>(pk, LY_LD41 + k, b, o, true, lane);
+        lout[k] = o[0];
+    }
+
+    // ------------------------------------------------ head, once per distinct channel
+    const float sr[3] = {srp.x, srp.y, srp.z};
+    float P[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        if (c > 0 && sr[c] == sr[0]) { P[c] = P[0]; continue; }
+        if (c == 2 && sr[2] == sr[1]) { P[2] = P[1]; continue; }
+        // SE of the head: LSEFin1 (16x75) on the pool + [g, gamma, srp], LSEFin2 (6x16)
+        bf16x8 bp[5];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            f32x16 t;
+            load16(prow, 32 * j, kg, t, 0);
+            load16(prow, 32 * j + 16, kg, t, 1);
+            bp[2 * j] = to_b(t, 0);
+            bp[2 * j + 1] = to_b(t, 1);
+        }
+        {
+            f32x16 t = zero16();
+            // features 64..79: 64+8*(e/4)+4kg+e%4; 72 = g, 73 = gamma, 74 = srp_c
+            if (kg == 0) {
+                const float4 q = *(const float4*)(prow + 64);
+                t[0] = q.x;
+                t[1] = q.y;
+                t[2] = q.z;
+                t[3] = q.w;
+                t[4] = g;
+                t[5] = gamma;
+                t[6] = sr[c];
+            } else {
+                const float4 q = *(const float4*)(prow + 68);
+                t[0] = q.x;
+                t[1] = q.y;
+                t[2] = q.z;
+                t[3] = q.w;
+            }
+            bp[4] = to_b(t, 0);
+        }
+        f32x16 s1[1];
+        dense<1, 5>(pk, LY_FIN1, bp, s1, true, lane);
+        bf16x8 bs[1] = {to_b(s1[0], 0)};
+        f32x16 s2[1];
+        dense<1, 1>(pk, LY_FIN2, bs, s2, false, lane);
+        float s[6];
+        const int src0 = lane & 31, src1 = (lane & 31) + 32;
+        s[0] = sigmoid_(__shfl(s2[0][0], src0));
+        s[1] = sigmoid_(__shfl(s2[0][1], src0));
+        s[2] = sigmoid_(__shfl(s2[0][2], src0));
+        s[3] = sigmoid_(__shfl(s2[0][3], src0));
+        s[4] = sigmoid_(__shfl(s2[0][0], src1));
+        s[5] = sigmoid_(__shfl(s2[0][1], src1));
+        // LGGS (8x3) rows 4kg..4kg+3 on [g, gamma, srp_c]
+        f32x16 x3;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 4 * kg + v;
+            float acc = 0.0f;
+            acc = fmaf(g, pextra[r * 3 + 0], acc);
+            acc = fmaf(gamma, pextra[r * 3 + 1], acc);
+            acc = fmaf(sr[c], pextra[r * 3 + 2], acc);
+            acc += pextra[24 + r];
+            x3[12 + v] = acc >= 0.0f ? acc : 0.0f;
+            x3[v] = lout[0][v] * s[3];
+            x3[4 + v] = lout[1][v] * s[4];
+            x3[8 + v] = lout[2][v] * s[5];
+        }
+        bf16x8 b0[8];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            f32x16 t;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) t[v] = comb[j][v] * s[j];
+            b0[2 * j] = to_b(t, 0);
+            b0[2 * j + 1] = to_b(t, 1);
+        }
+        b0[6] = to_b(x3, 0);
+        b0[7] = to_b(x3, 1);
+        f32x16 h0[4];
+        dense<4, 8>(pk, LY_LC0, b0, h0, true, lane);
+        bf16x8 b1[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            b1[2 * j] = to_b(h0[j], 0);
+            b1[2 * j + 1] = to_b(h0[j], 1);
+        }
+        f32x16 h1[2];
+        dense<2, 8>(pk, LY_LC1, b1, h1, true, lane);
+        bf16x8 b2[4] = {to_b(h1[0], 0), to_b(h1[0], 1), to_b(h1[1], 0), to_b(h1[1], 1)};
+        f32x16 h2[1];
+        dense<1, 4>(pk, LY_LC2, b2, h2, true, lane);
+        bf16x8 b3[2] = {to_b(h2[0], 0), to_b(h2[0], 1)};
+        f32x16 a[1];
+        dense<1, 2>(pk, LY_LX, b3, a, true, lane);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            bf16x8 ba[1] = {to_b(a[0], 0)};
+            f32x16 hb[1];
+            dense<1, 1>(pk, LY_LXR + 2 * r, ba, hb, true, lane);
+            bf16x8 bb[1] = {to_b(hb[0], 0)};
+            f32x16 hc[1];
+            dense<1, 1>(pk, LY_LXR + 2 * r + 1, bb, hc, true, lane);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) a[0][v] = a[0][v] + hc[0][v];
+        }
+        // LX12 (1x16): rows 0..15 are registers 0..7 split over the two lane groups
+        float part = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) part = fmaf(a[0][v], pextra[32 + vrow(v, kg)], part);
+        part += __shfl_xor(part, 32);
+        const float pc = part + pextra[48];
+        P[c] = pc >= 0.0f ? pc : 0.0f;
+    }
+    if (kg == 0 && item0 + (uint32_t)lane < n_items) {
+        pred[3 * item] = fmax_(expf_(P[0]) - 1.0f, 0.0f) * sr[0];
+        pred[3 * item + 1] = fmax_(expf_(P[1]) - 1.0f, 0.0f) * sr[1];
+        pred[3 * item + 2] = fmax_(expf_(P[2]) - 1.0f, 0.0f) * sr[2];
+    }
+}
+
+// Per-segment means / maxima from FP32 descriptors (used when the caller has
+// no gather-side pool, e.g. mrp_mlp_forward on arbitrary descriptors).
+__global__ void pool_kernel(const float* __restrict__ feat, uint32_t n_items, float* __restrict__ pool) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_items * 72) return;
+    const size_t item = t / 72;
+    const int i = (int)(t % 72);
+    const int from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
+    const int size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
+    const int q = i % 36, blk = q / 3, k = q % 3;
+    const float* seg = feat + item * kFeatures + k * kTaps + from[blk];
+    float r;
+    if (i < 36) {
+        float sum = 0.0f;
+        for (int j = 0; j < size[blk]; ++j) sum += seg[j];
+        r = sum / (float)size[blk];
+    } else {
+        r = -10000.0f;
+        for (int j = 0; j < size[blk]; ++j) r = fmax_(r, seg[j]);
+    }
+    pool[t] = r;
+}
+
+int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, const float* gamma, float g, vec3 srp,
+                    uint32_t n_items, float* pred, hipStream_t stream) {
+    if (n_items == 0) return MRP_OK;
+    LayerDesc L[LY_COUNT];
+    layer_table(L);
+    int nb = 0;
+    const int nt = total_tiles(L, &nb);
+    const bf16x8* pa = (const bf16x8*)packed;
+    const float* pb = (const float*)((const char*)packed + (size_t)nt * 64 * 16);
+    const float* pe = pb + (size_t)nb * 32;
+    float* tmp = nullptr;
+    if (!pool) {
+        MRP_HIP(hipMalloc(&tmp, (size_t)n_items * 72 * sizeof(float)));
+        pool_kernel<<<(unsigned)(((size_t)n_items * 72 + 255) / 256), 256, 0, stream>>>(feat, n_items, tmp);
+        pool = tmp;
+    }
+    const uint32_t waves = (n_items + 31) / 32;
+    mlp_bf16_kernel<<<(waves + 3) / 4, 256, 0, stream>>>(pa, pb, pe, feat, pool, gamma, g, srp, n_items, pred);
+    hipError_t e = hipGetLastError();
+    if (tmp) {
+        hipStreamSynchronize(stream);
+        hipFree(tmp);
+    }
+    if (e != hipSuccess) return hip_fail(e, "mlp_bf16_kernel");
+    return MRP_OK;
 }
 
 }  // namespace mrp

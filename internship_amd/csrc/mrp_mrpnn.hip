@@ -157,59 +157,86 @@ struct GatherArgs {
     uint32_t n_items;
 };
 
+// Segments of the 12 squeeze-excitation blocks (`SE`/`SERES` call sites,
+// core/radiancePredict.cu:396-434): tap range [from, from+size) per block, in
+// pool order (block b writes means at 3b+k and maxes at 36+3b+k, k = stream).
+__constant__ int c_seg_from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
+__constant__ int c_seg_size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
+
 // One block (192 threads) per point, one thread per tap.  feat[item][576] =
-// [Xd | Xt | Xh] (FP32), gamma[item] = acos(X . LX).
+// [Xd | Xt | Xh] (FP32), gamma[item] = acos(X . LX), and pool[item][72] =
+// the per-segment means (0..35) and maxima (36..71) that the network's
+// squeeze-excitation blocks start from (`AVGMAX`, core/radiancePredict.cu:21-33:
+// ascending sum then /size; max from -10000).
 __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, float* __restrict__ feat,
-                                                       float* __restrict__ gamma) {
+                                                       float* __restrict__ gamma, float* __restrict__ pool) {
+    __shared__ float sf[kFeatures];
     const uint32_t item = blockIdx.x;
     const int i = threadIdx.x;
-    if (item >= a.n_items) return;
+    if (item >= a.n_items) return;  // uniform over the block
     const float4 pa = pos_act[item], x4 = fx[item], y4 = fy[item], z4 = fz[item];
     const vec3 p = v3(pa.x, pa.y, pa.z);
     const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
     float* out = feat + (size_t)item * kFeatures;
     if (i == 0) gamma[item] = acosf_(dot(X, a.lx));
-    if (pa.w == 0.0f) {
-        out[i] = 0.0f;
-        out[kTaps + i] = 0.0f;
-        out[2 * kTaps + i] = 0.0f;
-        return;
+    float xd = 0.0f, xt = 0.0f, xh = 0.0f;
+    if (pa.w != 0.0f) {
+        const int mip = c_tap_mip[i];
+        const float off = c_tap_offset[i];
+        vec3 pi;
+        if (i < kShellTaps) {
+            pi = p + (X * c_stencil[3 * i] + Y * c_stencil[3 * i + 1] + Z * c_stencil[3 * i + 2]) * off;
+        } else {
+            pi = p + a.light * off;
+        }
+        const float* m = a.mips[mip];
+        xd = logf_(1.0f + a.alpha * trilinear_border(m, 256 >> mip, pi) / 64.0f);
+        if (i == 0) {
+            xh = logf_(hg_phase(dot(X, a.light), a.g) + 1.0f);
+        } else {
+            const vec3 ms = normalize(pi - p);
+            const float u = dot(X, ms) * 0.5f + 0.5f;
+            const float u2 = dot(a.lx, ms) * 0.5f + 0.5f;
+            const float v = c_tap_v[i];
+            xh = logf_(bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u, v) *
+                           bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u2, v) +
+                       1.0f);
+        }
+        const int tm = mip - 1 > 0 ? mip - 1 : 0;
+        const int tr = 128 >> tm;
+        if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
+            const float o = ray_box_offset(pi, a.lx);
+            xt = o >= 0.0f ? trilinear_clamp(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
+        } else {
+            xt = trilinear_clamp(a.trs[tm], tr, pi + 0.5f);
+        }
     }
-    const int mip = c_tap_mip[i];
-    const float off = c_tap_offset[i];
-    vec3 pi;
-    if (i < kShellTaps) {
-        pi = p + (X * c_stencil[3 * i] + Y * c_stencil[3 * i + 1] + Z * c_stencil[3 * i + 2]) * off;
-    } else {
-        pi = p + a.light * off;
-    }
-    const float* m = a.mips[mip];
-    out[i] = logf_(1.0f + a.alpha * trilinear_border(m, 256 >> mip, pi) / 64.0f);
-    float xh;
-    if (i == 0) {
-        xh = logf_(hg_phase(dot(X, a.light), a.g) + 1.0f);
-    } else {
-        const vec3 ms = normalize(pi - p);
-        const float u = dot(X, ms) * 0.5f + 0.5f;
-        const float u2 = dot(a.lx, ms) * 0.5f + 0.5f;
-        const float v = c_tap_v[i];
-        xh = logf_(bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u, v) *
-                       bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u2, v) +
-                   1.0f);
-    }
+    out[i] = xd;
+    out[kTaps + i] = xt;
     out[2 * kTaps + i] = xh;
-    const int tm = mip - 1 > 0 ? mip - 1 : 0;
-    const int tr = 128 >> tm;
-    float val;
-    if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
-        const float o = ray_box_offset(pi, a.lx);
-        val = o >= 0.0f ? trilinear_clamp(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
-    } else {
-        val = trilinear_clamp(a.trs[tm], tr, pi + 0.5f);
+    if (pool) {
+        sf[i] = xd;
+        sf[kTaps + i] = xt;
+        sf[2 * kTaps + i] = xh;
+        __syncthreads();
+        if (i < 72) {
+            const int q = i % 36, blk = q / 3, k = q % 3;
+            const float* seg = sf + k * kTaps + c_seg_from[blk];
+            const int n = c_seg_size[blk];
+            float r;
+            if (i < 36) {
+                float sum = 0.0f;
+                for (int j = 0; j < n; ++j) sum += seg[j];
+                r = sum / (float)n;
+            } else {
+                r = -10000.0f;
+                for (int j = 0; j < n; ++j) r = fmax_(r, seg[j]);
+            }
+            pool[(size_t)item * 72 + i] = r;
+        }
     }
-    out[kTaps + i] = val;
 }
 
 // ---------------------------------------------------------------- stage 3 (FP32)
@@ -265,7 +292,7 @@ static int upload_tap_tables(int device) {
 
 // Per-item scratch of the MRPNN stages (bytes): 4 float4 first-vertex
 // records, 576 FP32 features, gamma, 3 predictions.
-constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 4 + 12;
+constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12;
 
 struct NNScratch {
     void* base = nullptr;
@@ -316,7 +343,8 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     float4* fy = fx + chunk_items;
     float4* fz = fy + chunk_items;
     float* feat = (float*)(fz + chunk_items);
-    float* gam = feat + chunk_items * kFeatures;
+    float* pool = feat + chunk_items * kFeatures;
+    float* gam = pool + chunk_items * 72;
     float* pred = gam + chunk_items;
 
     NNArgs a;
@@ -379,13 +407,13 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
             first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz);
             MRP_HIP(hipGetLastError());
             MRP_HIP(hipEventRecord(t1, v->stream));
-            gather_kernel<<<cnt, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, feat, gam);
+            gather_kernel<<<cnt, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, feat, gam, pool);
             MRP_HIP(hipGetLastError());
             MRP_HIP(hipEventRecord(t2, v->stream));
             if (p->mlp_precision == 0) {
                 mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt, pred);
             } else {
-                rc = launch_mlp_bf16(v->nn_packed, feat, gam, p->g, srp, cnt, pred, v->stream);
+                rc = launch_mlp_bf16(v->nn_packed, feat, pool, gam, p->g, srp, cnt, pred, v->stream);
                 if (rc) return rc;
             }
             MRP_HIP(hipGetLastError());
@@ -506,7 +534,7 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     ga.n_items = (uint32_t)n;
     gather_kernel<<<n, kTaps, 0, v->stream>>>(ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev,
                                               (const float4*)frame_y_dev, (const float4*)frame_z_dev, feat_dev,
-                                              gamma_dev);
+                                              gamma_dev, nullptr);
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
     return MRP_OK;
@@ -524,7 +552,7 @@ int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const 
         MRP_HIP(hipGetLastError());
     } else if (precision == 1) {
         if (!net->packed) return fail(MRP_E_STATE, "network has no BF16 packing");
-        const int rc = launch_mlp_bf16(net->packed, feat_dev, gamma_dev, g, s, (uint32_t)n, out_dev, nullptr);
+        const int rc = launch_mlp_bf16(net->packed, feat_dev, nullptr, gamma_dev, g, s, (uint32_t)n, out_dev, nullptr);
         if (rc) return rc;
     } else {
         return fail(MRP_E_INVALID, "precision must be 0 (FP32) or 1 (BF16)");

@@ -165,3 +165,38 @@ def test_gpu_render_mrpnn_fp32_matches_oracle(cloud64, oracle_tables):
     rel = diff / np.maximum(np.abs(cpu), 1e-12)
     _say("exact %.4f, max rel %.3g" % (np.mean(gpu == cpu), rel.max()))
     assert np.all((rel <= 1e-5) | (diff <= 1e-7))
+
+
+def _synthetic_descriptors(n, seed):
+    rng = np.random.default_rng(seed)
+    feat = np.concatenate([rng.uniform(0, 3, (n, 192)), rng.uniform(0, 1, (n, 192)), rng.uniform(0, 2.5, (n, 192))],
+                          axis=1).astype(np.float32)
+    gamma = rng.uniform(0, np.pi, n).astype(np.float32)
+    return feat, gamma
+
+
+def test_gpu_mlp_bf16_tracks_fp32_oracle():
+    """BF16 operands / FP32 accumulation on MFMA versus the FP32 oracle.
+    Tolerance: BF16 keeps 8 significand bits (relative rounding 2^-9 per
+    operand); through 20 layers the network output stays within a few percent
+    (relative L2 over the batch <= 3e-2, per-item relative error <= 1.5e-1)."""
+    _say("case: BF16 MFMA network on 1000 synthetic descriptors (ragged tail)")
+    n = 1000  # not a multiple of the 32-item wave tile
+    feat, gamma = _synthetic_descriptors(n, 12)
+    w = load_reference_weights()
+    for sr in ((0.99, 0.99, 0.99), (0.99, 0.95, 0.9)):
+        srp = srp_of(sr)
+        net = Network(w)
+        fb = DeviceBuffer(feat.nbytes)
+        fb.upload(feat)
+        gb = DeviceBuffer(gamma.nbytes)
+        gb.upload(gamma)
+        out = net.forward(fb, gb, n, 0.857, srp, precision=1).download(np.float32, (n, 3))
+        net.close()
+        ref = oracle.mrpnn_forward(w, feat, gamma, 0.857, srp).astype(np.float64)
+        rel_l2 = np.linalg.norm(out - ref) / np.linalg.norm(ref)
+        per = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+        _say("srp %s: rel L2 %.3g, max per-item rel %.3g, median %.3g" % (sr, rel_l2, per.max(), np.median(per)))
+        assert np.all(np.isfinite(out))
+        assert rel_l2 <= 3e-2
+        assert per.max() <= 1.5e-1
