@@ -1,21 +1,29 @@
 """Throughput benchmark of the MRPNN rendering hot path on one MI355X.
 
-Workload (N=1): the plain delta-tracking renderer on the 512^3 Perlin-fBm cloud
-(seed 1, 5 octaves, base frequency 4, lacunarity 2, gain 0.5, threshold 0.4,
-sphere falloff), extinction scale 256, albedo 0.999, HG g 0.857, sun
-normalize(0,1,0.3) + sky 0.1, 1920x1080 pinhole (fov 40 deg, distance 2.5),
-512 bounces.  One step = one 1080p frame of 1 sample per pixel folded into the
-resident running mean (`VolumeRender::Render`, core/volume.cu:410-467, one
-frame of its loop).  Inputs are resident in HBM before the timed region.
+Scene (N=1): the 512^3 Perlin-fBm cloud (seed 1, 5 octaves, base frequency 4,
+lacunarity 2, gain 0.5, threshold 0.4, sphere falloff), extinction scale 256,
+albedo 0.999, HG g 0.857, sun normalize(0,1,0.3) + constant sky 0.1,
+1920x1080 pinhole (vertical fov 40 deg, distance 2.5), 512 bounces.
+Inputs are generated on the GPU and resident in HBM before any timed region.
 
-Prints one JSON line (stdout); progress goes to stderr.
+Default mode (headline): frames/s of the BF16 MLP renderer at 1080p, one
+sample per pixel per frame (`VolumeRender::Render` with RenderType MRPNN,
+core/volume.cu:410-467), timed over K frames after W warm-up frames.  Equal-
+RMSE protocol beside it: RMSE over linear RGB against a 4096-spp delta-tracking
+oracle with FP64 per-pixel sums; the smallest delta-tracking spp N with
+RMSE(DT_N) <= RMSE(MLP) (powers of two, then bisection); speed-up =
+fps(MLP) / fps(DT_N), where fps(DT_N) is measured on one N-spp launch.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|tracer]
+                  [--no-cpu-baseline] [--oracle-spp 4096]
+
+Mode tracer: frames/s of the plain tracer (1 spp per frame, --spp-per-step
+frames per launch) with its HBM roofline.  Prints one JSON line on stdout;
+progress goes to stderr.
 """
 import argparse
 import ctypes
 import json
-import math
 import os
 import sys
 import time
@@ -28,70 +36,269 @@ if ROOT not in sys.path:
 
 METRIC = "1080p frames/s, BF16 MLP vs plain delta tracking at equal RMSE; HBM/MFMA %"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_LOOKUP = 32  # eight FP32 corners of one trilinear density lookup
+BF16_PEAK_TFLOPS = 2500.0
+BYTES_PER_LOOKUP = 32          # eight FP32 corners of one trilinear density lookup
+MLP_FLOP_PER_PASS = 97072      # 48,536 MACs per descriptor per colour pass
+GATHER_BYTES_PER_POINT = 18400  # 192 x 32 (density) + 192 x 32 (transmittance) + 382 x 16 (HG table)
+
+ORACLE_SEED = 1000
+DT_SEED = 2000
+MLP_SEED = 0
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def config2_params(ia, width, height, spp_begin, spp_count, multi=512, seed=0):
+def scene_params(ia, width, height, spp_begin, spp_count, multi=512, seed=0, rt=0, precision=1):
     ori = 2.5 * np.array([0.75, 0.75, 0.75]) / np.linalg.norm([0.75, 0.75, 0.75])
     right, up = ia.camera_basis(ori, 40.0, width / height)
     light = np.array([0.0, 1.0, 0.3]) / np.linalg.norm([0.0, 1.0, 0.3])
     V = ia.Vec3.of
     return ia.RenderParams(width, height, V(ori), V(up), V(right), V(light), V((1, 1, 1)), V((0.1, 0.1, 0.1)),
                            V((0.999, 0.999, 0.999)), 0.857, 256.0, 1.0, multi, spp_begin, spp_count, seed,
-                           ia.PT, 0)
+                           rt, precision)
 
 
-def cpu_baseline(ia, vol, grid_dev, res, width, height, multi, target_s=15.0):
-    """Time the CPU oracle (the reference algorithm restated in C++/OpenMP) on
-    a bounded band of rows of the same frame; scale to frames/s."""
+def rmse(img, ref):
+    d = img.astype(np.float64) - ref
+    return float(np.sqrt(np.mean(d * d)))
+
+
+def luma_rmse100(img, ref):
+    """RMSE x 100 of tone-mapped 8-bit luma, the reference's image metric
+    (tools/Compare.py:31-42), with ACES (core/camera.cu:74-160)."""
+    from internship_amd.tonemap import aces_np
+
+    def luma8(x):
+        t = np.clip(aces_np(np.asarray(x, np.float32)), 0, 1)
+        v = np.floor(t * 255.0)
+        return (0.299 * v[..., 0] + 0.587 * v[..., 1] + 0.114 * v[..., 2]) / 255.0
+
+    d = luma8(img) - luma8(ref)
+    return float(100.0 * np.sqrt(np.mean(d * d)))
+
+
+class Scene:
+    def __init__(self, ia, res, width, height, bounces):
+        self.ia, self.res, self.W, self.H, self.bounces = ia, res, width, height, bounces
+        log("generating %d^3 cloud on the GPU" % res)
+        self.grid = ia.DeviceBuffer(res ** 3 * 4)
+        ia.generate_cloud(res, seed=1, device_out=self.grid)
+        self.vol = ia.VolumeRender(res)
+        self.vol.set_datas(self.grid)
+        self.vol.update()
+        log("max density %.6f" % self.vol.max_density)
+        self.target = ia.DeviceBuffer(width * height * 3 * 4)
+        self.st = ia.TraceStats()
+
+    def render(self, spp_begin, spp_count, seed, rt=0, precision=1, sum64=None, fresh=True):
+        if fresh:
+            self.target.zero()
+        p = scene_params(self.ia, self.W, self.H, spp_begin, spp_count, self.bounces, seed, rt, precision)
+        t0 = time.perf_counter()
+        self.vol.render_dev(p, self.target, sum64, self.st)
+        return time.perf_counter() - t0
+
+    def image(self):
+        return self.target.download(np.float32, (self.H, self.W, 3))
+
+
+def oracle_image(ia, sc, spp):
+    sum64 = ia.DeviceBuffer(sc.W * sc.H * 3 * 8)
+    sum64.zero()
+    batch = 64
+    t0 = time.perf_counter()
+    for s in range(0, spp, batch):
+        n = min(batch, spp - s)
+        sc.render(s, n, ORACLE_SEED, sum64=sum64, fresh=(s == 0))
+        if (s // batch) % 16 == 0:
+            log("oracle: %d / %d spp (%.1f s)" % (s + n, spp, time.perf_counter() - t0))
+    img = sum64.download(np.float64, (sc.H, sc.W, 3)) / spp
+    log("oracle %d spp done in %.1f s, mean %.5f" % (spp, time.perf_counter() - t0, img.mean()))
+    return img
+
+
+def dt_rmse(sc, n, ref):
+    sc.render(0, n, DT_SEED)
+    return rmse(sc.image(), ref)
+
+
+def dt_time(sc, n, reps=2):
+    best = (1e30, 0.0, 0)
+    for _ in range(reps):
+        t = sc.render(0, n, DT_SEED)
+        if t < best[0]:
+            best = (t, sc.st.kernel_ms, sc.st.lookups)
+    return best
+
+
+def equal_rmse_search(sc, target_rmse, ref, max_spp):
+    """Smallest N with RMSE(DT_N) <= target: powers of two, then bisection."""
+    hist = {}
+    n = 1
+    while True:
+        hist[n] = dt_rmse(sc, n, ref)
+        log("DT %d spp: RMSE %.5g (target %.5g)" % (n, hist[n], target_rmse))
+        if hist[n] <= target_rmse or n >= max_spp:
+            break
+        n *= 2
+    if hist[n] > target_rmse:
+        return n, hist, False
+    lo, hi = n // 2, n
+    while hi - lo > 1 and lo >= 1:
+        mid = (lo + hi) // 2
+        hist[mid] = dt_rmse(sc, mid, ref)
+        log("DT %d spp: RMSE %.5g" % (mid, hist[mid]))
+        if hist[mid] <= target_rmse:
+            hi = mid
+        else:
+            lo = mid
+    return hi, hist, True
+
+
+def cpu_baseline(sc, target_s=15.0):
+    """Time the CPU oracle (the reference tracer restated in C++/OpenMP) on a
+    bounded band of rows of one 1-spp frame of the same scene; scale to frames/s."""
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
-    grid = grid_dev.download(np.float32, (res, res, res))
-    md = vol.max_density
-    p = config2_params(ia, width, height, 0, 1, multi)
-    r0 = height // 2 - 2
+    grid = sc.grid.download(np.float32, (sc.res, sc.res, sc.res))
+    md = sc.vol.max_density
+    p = scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED, 0, 0)
+    r0 = sc.H // 2 - 2
     t0 = time.perf_counter()
     oracle.render(grid, md, p, threads=threads, rows=(r0, r0 + 4))
     probe = time.perf_counter() - t0
-    rows = int(max(4, min(height, 4 * target_s / max(probe, 1e-3))))
-    r0 = max(0, height // 2 - rows // 2)
+    rows = int(max(4, min(sc.H, 4 * target_s / max(probe, 1e-3))))
+    r0 = max(0, sc.H // 2 - rows // 2)
     t0 = time.perf_counter()
     _, lookups, _ = oracle.render(grid, md, p, threads=threads, rows=(r0, r0 + rows))
     dt = time.perf_counter() - t0
-    frames_per_s = (rows / height) / dt
-    return {"value": frames_per_s, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "oracle tracer (C++/OpenMP, %d threads), rows %d..%d of one 1 spp 1920x1080 frame "
-                      "(%.1f s, %.3g lookups/s), scaled to a full frame" % (threads, r0, r0 + rows, dt,
-                                                                            lookups / dt)}
+    return {"value": (rows / sc.H) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "plain delta-tracking tracer, CPU oracle (C++/OpenMP, %d threads), rows %d..%d of one "
+                      "1-spp %dx%d frame (%.1f s, %.3g lookups/s), scaled to a full frame"
+                      % (threads, r0, r0 + rows, sc.W, sc.H, dt, lookups / dt)}
+
+
+def run_tracer(a, ia, sc):
+    def step(i):
+        p = scene_params(ia, sc.W, sc.H, i * a.spp_per_step, a.spp_per_step, sc.bounces, 0, 0, 0)
+        sc.vol.render_dev(p, sc.target, None, sc.st)
+        return sc.st.kernel_ms, sc.st.lookups
+
+    sc.target.zero()
+    for i in range(a.warmup):
+        step(i)
+    kernel_ms, lookups = 0.0, 0
+    t0 = time.perf_counter()
+    for i in range(a.warmup, a.warmup + a.steps):
+        km, lk = step(i)
+        kernel_ms += km
+        lookups += lk
+    elapsed = time.perf_counter() - t0
+    avg_ms = kernel_ms / a.steps
+    achieved = (lookups / a.steps) * BYTES_PER_LOOKUP / (avg_ms * 1e-3) / 1e9
+    log("tracer: %.2f ms/step, kernel %.2f ms, %.1f GB/s algorithmic" % (1000 * elapsed / a.steps, avg_ms, achieved))
+    return {
+        "value": a.steps * a.spp_per_step / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "f32",
+        "config": {"workload": "plain delta-tracking tracer, 512^3 Perlin-fBm cloud, %dx%d, 1 spp/frame, "
+                               "%d frames/launch, %d bounces" % (sc.W, sc.H, a.spp_per_step, sc.bounces),
+                   "width": sc.W, "height": sc.H, "grid": sc.res, "spp_per_step": a.spp_per_step},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "trace_kernel",
+                     "kernel_ms": avg_ms, "lookups_per_launch": lookups / a.steps,
+                     "bytes_per_lookup": BYTES_PER_LOOKUP},
+        "extra": {"glookups_per_s": lookups / (kernel_ms * 1e-3) / 1e9, "max_path_steps": sc.st.max_path_steps},
+    }
+
+
+def run_headline(a, ia, sc):
+    net = ia.Network()
+    sc.vol.set_network(net)
+    P = sc.W * sc.H
+    ref = oracle_image(ia, sc, a.oracle_spp)  # untimed
+    # MLP renderer: timed 1-spp frames folded into the running mean in HBM
+    sc.target.zero()
+    for i in range(a.warmup):
+        sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
+    fv = ga = ml = 0.0
+    t0 = time.perf_counter()
+    for i in range(a.warmup, a.warmup + a.steps):
+        sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
+        fv += sc.st.first_vertex_ms
+        ga += sc.st.reduce_ms
+        ml += sc.st.kernel_ms
+    elapsed = time.perf_counter() - t0
+    fps_mlp = a.steps / elapsed
+    fv, ga, ml = fv / a.steps, ga / a.steps, ml / a.steps
+    log("MLP renderer: %.2f ms/frame (first vertex %.2f, gather %.2f, network %.2f ms)"
+        % (1000 * elapsed / a.steps, fv, ga, ml))
+    # quality of one 1-spp frame with the BF16 and the FP32 network
+    sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
+    img_mlp = sc.image()
+    rmse_mlp = rmse(img_mlp, ref)
+    sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=0)
+    fp32_ms = sc.st.kernel_ms
+    rmse_mlp32 = rmse(sc.image(), ref)
+    log("MLP 1-spp RMSE: BF16 %.5g, FP32 %.5g (FP32 network %.2f ms)" % (rmse_mlp, rmse_mlp32, fp32_ms))
+    # the plain tracer at equal RMSE
+    n_eq, hist, reached = equal_rmse_search(sc, rmse_mlp, ref, a.dt_max_spp)
+    t_dt, dt_kernel_ms, dt_lookups = dt_time(sc, n_eq)
+    fps_dt = 1.0 / t_dt
+    sc.render(0, 1, DT_SEED)
+    img_dt1 = sc.image()
+    log("DT %d spp: %.2f ms/frame -> speed-up %.2fx" % (n_eq, 1000 * t_dt, fps_mlp / fps_dt))
+    mlp_achieved = MLP_FLOP_PER_PASS * P / (ml * 1e-3) / 1e12
+    gather_achieved = GATHER_BYTES_PER_POINT * P / (ga * 1e-3) / 1e9
+    dt_achieved = dt_lookups * BYTES_PER_LOOKUP / (dt_kernel_ms * 1e-3) / 1e9
+    net.close()
+    return {
+        "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
+        "config": {"workload": "MRPNN renderer (BF16 MFMA network), 512^3 Perlin-fBm cloud seed 1, %dx%d, "
+                               "1 spp/frame; vs plain delta tracking at equal RMSE against a %d-spp FP64 oracle"
+                               % (sc.W, sc.H, a.oracle_spp),
+                   "width": sc.W, "height": sc.H, "grid": sc.res, "alpha": 256, "albedo": 0.999, "g": 0.857,
+                   "bounces": sc.bounces, "oracle_spp": a.oracle_spp},
+        "roofline": {"bound": "mfma", "achieved": mlp_achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": mlp_achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "mlp_bf16_kernel",
+                     "kernel_ms": ml, "flop_per_launch": MLP_FLOP_PER_PASS * P},
+        "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "dt_spp": n_eq,
+                       "dt_reached_target": reached, "rmse_dt": hist.get(n_eq), "fps_mlp": fps_mlp,
+                       "fps_dt_at_equal_rmse": fps_dt, "speedup": fps_mlp / fps_dt,
+                       "dt_rmse_by_spp": {str(k): v for k, v in sorted(hist.items())},
+                       "luma_rmse100_mlp": luma_rmse100(img_mlp, ref),
+                       "luma_rmse100_dt1": luma_rmse100(img_dt1, ref)},
+        "kernels": {"first_vertex_ms": fv, "gather_ms": ga, "mlp_bf16_ms": ml, "mlp_fp32_ms": fp32_ms,
+                    "gather_gbs_algorithmic": gather_achieved, "gather_hbm_frac": gather_achieved / HBM_PEAK_GBS,
+                    "tracer_kernel_ms_at_equal_rmse": dt_kernel_ms, "tracer_gbs_algorithmic": dt_achieved,
+                    "tracer_hbm_frac": dt_achieved / HBM_PEAK_GBS},
+    }
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", choices=["headline", "tracer"], default="headline")
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=512)
-    ap.add_argument("--spp-per-step", type=int, default=1,
-                    help="1-spp frames traced per launch (a step renders that many frames)")
+    ap.add_argument("--oracle-spp", type=int, default=4096)
+    ap.add_argument("--dt-max-spp", type=int, default=4096)
+    ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (reference kernel for HBM counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the image-tile / spp sharding is described in DESIGN.md)", file=sys.stderr)
+              "(image-tile / sample sharding is described in DESIGN.md)", file=sys.stderr)
         return 2
 
     import internship_amd as ia
     lib = ia.load()
-    W, H, res = a.width, a.height, a.res
     if a.copy_calibration_mib > 0:
         nb = a.copy_calibration_mib << 20
         src, dst = ia.DeviceBuffer(nb), ia.DeviceBuffer(nb)
@@ -100,78 +307,18 @@ def main(argv=None):
         log("calibration copy of %d MiB done" % a.copy_calibration_mib)
         src.free()
         dst.free()
-    log("generating %d^3 cloud on the GPU" % res)
-    grid_dev = ia.DeviceBuffer(res ** 3 * 4)
-    ia.generate_cloud(res, seed=1, device_out=grid_dev)
-    vol = ia.VolumeRender(res)
-    vol.set_datas(grid_dev)
-    vol.update()
-    log("max density %.6f" % vol.max_density)
-    target = ia.DeviceBuffer(W * H * 3 * 4)
-    target.zero()
-    st = ia.TraceStats()
-
-    def step(i):
-        p = config2_params(ia, W, H, i * a.spp_per_step, a.spp_per_step, a.bounces)
-        vol.render_dev(p, target, None, st)
-        return st.kernel_ms, st.lookups, st.paths, st.scatters
-
-    for i in range(a.warmup):
-        km, lk, _, _ = step(i)
-        log("warmup %d: trace kernel %.2f ms, %.3g lookups" % (i, km, lk))
-    kernel_ms = 0.0
-    lookups = paths = scatters = 0
-    t0 = time.perf_counter()
-    for i in range(a.warmup, a.warmup + a.steps):
-        km, lk, pa, sc = step(i)
-        kernel_ms += km
-        lookups += lk
-        paths += pa
-        scatters += sc
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms_per_step = 1000.0 * elapsed / a.steps
-    fps = a.steps * a.spp_per_step / elapsed
-    avg_kernel_ms = kernel_ms / a.steps
-    achieved = (lookups / a.steps) * BYTES_PER_LOOKUP / (avg_kernel_ms * 1e-3) / 1e9
-    log("%.2f ms/frame, trace kernel %.2f ms, %.3g lookups/frame, %.1f GB/s algorithmic"
-        % (ms_per_step, avg_kernel_ms, lookups / a.steps, achieved))
-    img = target.download(np.float32, (H, W, 3))
-    if not np.all(np.isfinite(img)):
-        log("non-finite pixels in the image")
-        return 1
-
+    sc = Scene(ia, a.res, a.width, a.height, a.bounces)
+    res = run_headline(a, ia, sc) if a.mode == "headline" else run_tracer(a, ia, sc)
     base = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")
-        base = cpu_baseline(ia, vol, grid_dev, res, W, H, a.bounces)
+        base = cpu_baseline(sc)
         log("cpu baseline %.4g frames/s (%s)" % (base["value"], base["sample"]))
-
-    out = {
-        "metric": METRIC,
-        "value": fps,
-        "unit": "frames/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": "plain delta-tracking tracer, %d^3 Perlin-fBm cloud (seed 1), %dx%d, 1 spp/frame, %d frames/launch, "
-                               "%d bounces, alpha 256, albedo 0.999, g 0.857" % (res, W, H, a.spp_per_step, a.bounces),
-                   "width": W, "height": H, "grid": res, "spp_per_step": a.spp_per_step, "bounces": a.bounces},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "trace_kernel", "kernel_ms": avg_kernel_ms,
-                     "lookups_per_launch": lookups / a.steps, "bytes_per_lookup": BYTES_PER_LOOKUP},
-        "cpu_baseline": base,
-        "extra": {"paths_per_frame": paths / a.steps, "scatters_per_frame": scatters / a.steps,
-                  "glookups_per_s": lookups / (kernel_ms * 1e-3) / 1e9, "max_path_steps": st.max_path_steps,
-                  "image_mean": float(img.mean()), "grid_blocks": st.grid_blocks},
-    }
+    out = {"metric": METRIC, "value": res.pop("value"), "unit": "frames/s", "n_gpus": 1, "steps": a.steps,
+           "warmup": a.warmup, "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": res.pop("dtype"), "data": "synthetic"}
+    out.update(res)
+    out["cpu_baseline"] = base
     print(json.dumps(out), flush=True)
     return 0
 

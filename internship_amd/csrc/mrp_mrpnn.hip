@@ -333,7 +333,8 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16), (size_t)p->spp_count));
     rc = ensure_samples(v, (size_t)batch * P * 16);
     if (rc) return rc;
-    const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 20, (size_t)batch * P));
+    // one chunk covers a whole 1080p frame (2.07M points, about 6 GB of scratch)
+    const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, (size_t)batch * P));
     void* scratch = nullptr;
     rc = ensure_nn_scratch(v->device, chunk_items * kNNItemBytes + 4096, &scratch);
     if (rc) return rc;
