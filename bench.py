@@ -10,13 +10,14 @@ Default mode (headline): frames/s of the BF16 MLP renderer at 1080p, one
 sample per pixel per frame (`VolumeRender::Render` with RenderType MRPNN,
 core/volume.cu:410-467), timed over K frames after W warm-up frames.  Equal-
 RMSE protocol beside it: RMSE over linear RGB against a 4096-spp delta-tracking
-oracle with FP64 per-pixel sums; the smallest delta-tracking spp N with
+ground truth with FP64 per-pixel sums; the smallest delta-tracking spp N with
 RMSE(DT_N) <= RMSE(MLP) (powers of two, then bisection); speed-up =
 fps(MLP) / fps(DT_N), where fps(DT_N) is measured on one N-spp launch.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|tracer]
-                  [--no-cpu-baseline] [--oracle-spp 4096]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer]
+                  [--no-cpu-baseline] [--truth-spp 4096]
 
+Mode mlp: only the timed MLP-renderer frames of the headline (for profiling).
 Mode tracer: frames/s of the plain tracer (1 spp per frame, --spp-per-step
 frames per launch) with its HBM roofline.  Prints one JSON line on stdout;
 progress goes to stderr.
@@ -40,8 +41,10 @@ BF16_PEAK_TFLOPS = 2500.0
 BYTES_PER_LOOKUP = 32          # eight FP32 corners of one trilinear density lookup
 MLP_FLOP_PER_PASS = 97072      # 48,536 MACs per descriptor per colour pass
 GATHER_BYTES_PER_POINT = 18400  # 192 x 32 (density) + 192 x 32 (transmittance) + 382 x 16 (HG table)
+MLP_WORKLOAD = "cloud512_1080p_1spp"
+TRACER_WORKLOAD = "cloud512_1080p_%dspp_per_launch"
 
-ORACLE_SEED = 1000
+TRUTH_SEED = 1000
 DT_SEED = 2000
 MLP_SEED = 0
 
@@ -104,18 +107,19 @@ class Scene:
         return self.target.download(np.float32, (self.H, self.W, 3))
 
 
-def oracle_image(ia, sc, spp):
+def ground_truth_image(ia, sc, spp):
+    """Ground-truth image: `spp` frames of the GPU tracer summed in FP64 per pixel."""
     sum64 = ia.DeviceBuffer(sc.W * sc.H * 3 * 8)
     sum64.zero()
     batch = 64
     t0 = time.perf_counter()
     for s in range(0, spp, batch):
         n = min(batch, spp - s)
-        sc.render(s, n, ORACLE_SEED, sum64=sum64, fresh=(s == 0))
+        sc.render(s, n, TRUTH_SEED, sum64=sum64, fresh=(s == 0))
         if (s // batch) % 16 == 0:
-            log("oracle: %d / %d spp (%.1f s)" % (s + n, spp, time.perf_counter() - t0))
+            log("ground truth: %d / %d spp (%.1f s)" % (s + n, spp, time.perf_counter() - t0))
     img = sum64.download(np.float64, (sc.H, sc.W, 3)) / spp
-    log("oracle %d spp done in %.1f s, mean %.5f" % (spp, time.perf_counter() - t0, img.mean()))
+    log("ground truth %d spp done in %.1f s, mean %.5f" % (spp, time.perf_counter() - t0, img.mean()))
     return img
 
 
@@ -159,25 +163,30 @@ def equal_rmse_search(sc, target_rmse, ref, max_spp):
 
 def cpu_baseline(sc, target_s=15.0):
     """Time the CPU oracle (the reference tracer restated in C++/OpenMP) on a
-    bounded band of rows of one 1-spp frame of the same scene; scale to frames/s."""
+    bounded sample of the same scene: a band of rows of one 1-spp frame, or
+    whole 1-spp frames when one frame takes less than the target; frames/s."""
     import oracle
     threads = max(1, min(16, os.cpu_count() or 1))
     grid = sc.grid.download(np.float32, (sc.res, sc.res, sc.res))
     md = sc.vol.max_density
-    p = scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED, 0, 0)
     r0 = sc.H // 2 - 2
     t0 = time.perf_counter()
-    oracle.render(grid, md, p, threads=threads, rows=(r0, r0 + 4))
+    oracle.render(grid, md, scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED), threads=threads,
+                  rows=(r0, r0 + 4))
     probe = time.perf_counter() - t0
     rows = int(max(4, min(sc.H, 4 * target_s / max(probe, 1e-3))))
+    frames = 1
+    if rows == sc.H:
+        frames = int(max(1, min(64, target_s / max(probe * sc.H / 4, 1e-3))))
     r0 = max(0, sc.H // 2 - rows // 2)
+    p = scene_params(sc.ia, sc.W, sc.H, 0, frames, sc.bounces, DT_SEED)
     t0 = time.perf_counter()
     _, lookups, _ = oracle.render(grid, md, p, threads=threads, rows=(r0, r0 + rows))
     dt = time.perf_counter() - t0
-    return {"value": (rows / sc.H) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "plain delta-tracking tracer, CPU oracle (C++/OpenMP, %d threads), rows %d..%d of one "
-                      "1-spp %dx%d frame (%.1f s, %.3g lookups/s), scaled to a full frame"
-                      % (threads, r0, r0 + rows, sc.W, sc.H, dt, lookups / dt)}
+    return {"value": frames * (rows / sc.H) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "plain delta-tracking tracer, CPU oracle (C++/OpenMP, %d threads), rows %d..%d of %d "
+                      "1-spp %dx%d frame(s) (%.1f s, %.3g lookups/s), scaled to full frames"
+                      % (threads, r0, r0 + rows, frames, sc.W, sc.H, dt, lookups / dt)}
 
 
 def run_tracer(a, ia, sc):
@@ -205,19 +214,30 @@ def run_tracer(a, ia, sc):
                                "%d frames/launch, %d bounces" % (sc.W, sc.H, a.spp_per_step, sc.bounces),
                    "width": sc.W, "height": sc.H, "grid": sc.res, "spp_per_step": a.spp_per_step},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "trace_kernel",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": hbm_traffic("trace_kernel", TRACER_WORKLOAD % a.spp_per_step),
+                     "kernel": "trace_kernel",
                      "kernel_ms": avg_ms, "lookups_per_launch": lookups / a.steps,
                      "bytes_per_lookup": BYTES_PER_LOOKUP},
         "extra": {"glookups_per_s": lookups / (kernel_ms * 1e-3) / 1e9, "max_path_steps": sc.st.max_path_steps},
     }
 
 
-def run_headline(a, ia, sc):
-    net = ia.Network()
-    sc.vol.set_network(net)
-    P = sc.W * sc.H
-    ref = oracle_image(ia, sc, a.oracle_spp)  # untimed
-    # MLP renderer: timed 1-spp frames folded into the running mean in HBM
+def hbm_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes summarised in profiles/hbm_traffic.json
+    (each counter scaled by its reading for a copy kernel of known size)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic.json")) as f:
+            table = json.load(f)
+    except (OSError, ValueError):
+        return None
+    rec = table.get(kernel, {}).get(workload)
+    return None if rec is None else rec.get("bytes_per_launch")
+
+
+def time_mlp(a, ia, sc):
+    """K timed 1-spp MRPNN frames folded into the running mean in HBM."""
     sc.target.zero()
     for i in range(a.warmup):
         sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
@@ -229,10 +249,48 @@ def run_headline(a, ia, sc):
         ga += sc.st.reduce_ms
         ml += sc.st.kernel_ms
     elapsed = time.perf_counter() - t0
-    fps_mlp = a.steps / elapsed
     fv, ga, ml = fv / a.steps, ga / a.steps, ml / a.steps
     log("MLP renderer: %.2f ms/frame (first vertex %.2f, gather %.2f, network %.2f ms)"
         % (1000 * elapsed / a.steps, fv, ga, ml))
+    return elapsed, fv, ga, ml
+
+
+def mlp_roofline(sc, ml):
+    P = sc.W * sc.H
+    achieved = MLP_FLOP_PER_PASS * P / (ml * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / BF16_PEAK_TFLOPS, "traffic": hbm_traffic("mlp_bf16_kernel", MLP_WORKLOAD),
+            "kernel": "mlp_bf16_kernel", "kernel_ms": ml, "flop_per_launch": MLP_FLOP_PER_PASS * P}
+
+
+def headline_config(sc, a):
+    return {"workload": "MRPNN renderer (BF16 MFMA network), 512^3 Perlin-fBm cloud seed 1, %dx%d, 1 spp/frame; "
+                        "vs plain delta tracking at equal RMSE against a %d-spp FP64 ground-truth image"
+                        % (sc.W, sc.H, a.truth_spp),
+            "width": sc.W, "height": sc.H, "grid": sc.res, "alpha": 256, "albedo": 0.999, "g": 0.857,
+            "bounces": sc.bounces, "truth_spp": a.truth_spp}
+
+
+def run_mlp(a, ia, sc):
+    """Only the timed MLP-renderer frames (no ground truth, no equal-RMSE search)."""
+    net = ia.Network()
+    sc.vol.set_network(net)
+    elapsed, fv, ga, ml = time_mlp(a, ia, sc)
+    net.close()
+    cfg = headline_config(sc, a)
+    cfg["workload"] = cfg["workload"].split(";")[0]
+    return {"value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
+            "config": cfg, "roofline": mlp_roofline(sc, ml),
+            "kernels": {"first_vertex_ms": fv, "gather_ms": ga, "mlp_bf16_ms": ml}}
+
+
+def run_headline(a, ia, sc):
+    net = ia.Network()
+    sc.vol.set_network(net)
+    P = sc.W * sc.H
+    ref = ground_truth_image(ia, sc, a.truth_spp)  # untimed
+    elapsed, fv, ga, ml = time_mlp(a, ia, sc)
+    fps_mlp = a.steps / elapsed
     # quality of one 1-spp frame with the BF16 and the FP32 network
     sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
     img_mlp = sc.image()
@@ -248,20 +306,13 @@ def run_headline(a, ia, sc):
     sc.render(0, 1, DT_SEED)
     img_dt1 = sc.image()
     log("DT %d spp: %.2f ms/frame -> speed-up %.2fx" % (n_eq, 1000 * t_dt, fps_mlp / fps_dt))
-    mlp_achieved = MLP_FLOP_PER_PASS * P / (ml * 1e-3) / 1e12
     gather_achieved = GATHER_BYTES_PER_POINT * P / (ga * 1e-3) / 1e9
     dt_achieved = dt_lookups * BYTES_PER_LOOKUP / (dt_kernel_ms * 1e-3) / 1e9
     net.close()
     return {
         "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
-        "config": {"workload": "MRPNN renderer (BF16 MFMA network), 512^3 Perlin-fBm cloud seed 1, %dx%d, "
-                               "1 spp/frame; vs plain delta tracking at equal RMSE against a %d-spp FP64 oracle"
-                               % (sc.W, sc.H, a.oracle_spp),
-                   "width": sc.W, "height": sc.H, "grid": sc.res, "alpha": 256, "albedo": 0.999, "g": 0.857,
-                   "bounces": sc.bounces, "oracle_spp": a.oracle_spp},
-        "roofline": {"bound": "mfma", "achieved": mlp_achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": mlp_achieved / BF16_PEAK_TFLOPS, "traffic": None, "kernel": "mlp_bf16_kernel",
-                     "kernel_ms": ml, "flop_per_launch": MLP_FLOP_PER_PASS * P},
+        "config": headline_config(sc, a),
+        "roofline": mlp_roofline(sc, ml),
         "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "dt_spp": n_eq,
                        "dt_reached_target": reached, "rmse_dt": hist.get(n_eq), "fps_mlp": fps_mlp,
                        "fps_dt_at_equal_rmse": fps_dt, "speedup": fps_mlp / fps_dt,
@@ -280,12 +331,12 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["headline", "tracer"], default="headline")
+    ap.add_argument("--mode", choices=["headline", "mlp", "tracer"], default="headline")
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--bounces", type=int, default=512)
-    ap.add_argument("--oracle-spp", type=int, default=4096)
+    ap.add_argument("--truth-spp", type=int, default=4096)
     ap.add_argument("--dt-max-spp", type=int, default=4096)
     ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -308,7 +359,7 @@ def main(argv=None):
         src.free()
         dst.free()
     sc = Scene(ia, a.res, a.width, a.height, a.bounces)
-    res = run_headline(a, ia, sc) if a.mode == "headline" else run_tracer(a, ia, sc)
+    res = {"headline": run_headline, "mlp": run_mlp, "tracer": run_tracer}[a.mode](a, ia, sc)
     base = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")
