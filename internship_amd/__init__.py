@@ -35,7 +35,8 @@ __all__ = ["VolumeRender", "Camera", "DeviceBuffer", "PT", "MRPNN", "MrpError", 
 
 
 def _ptr(a: np.ndarray):
-    return ctypes.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to `a`, so temporaries made for the call stay alive through it
+    return a.ctypes.data_as(ctypes.c_void_p)
 
 
 class DeviceBuffer:

@@ -15,7 +15,8 @@ _lib = None
 
 
 def _ptr(a):
-    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+    # data_as keeps a reference to `a`, so temporaries made for the call stay alive through it
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 
 
 def load():
