@@ -14,10 +14,11 @@ ground truth with FP64 per-pixel sums; the smallest delta-tracking spp N with
 RMSE(DT_N) <= RMSE(MLP) (powers of two, then bisection); speed-up =
 fps(MLP) / fps(DT_N), where fps(DT_N) is measured on one N-spp launch.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer|frame]
                   [--no-cpu-baseline] [--truth-spp 4096]
 
 Mode mlp: only the timed MLP-renderer frames of the headline (for profiling).
+Mode frame: progressive path-traced frames with histograms, denoiser and tone map.
 Mode tracer: frames/s of the plain tracer (1 spp per frame, --spp-per-step
 frames per launch) with its HBM roofline.  Prints one JSON line on stdout;
 progress goes to stderr.
@@ -41,6 +42,7 @@ BF16_PEAK_TFLOPS = 2500.0
 BYTES_PER_LOOKUP = 32          # eight FP32 corners of one trilinear density lookup
 MLP_FLOP_PER_PASS = 97072      # 48,536 MACs per descriptor per colour pass
 GATHER_BYTES_PER_POINT = 18400  # 192 x 32 (density) + 192 x 32 (transmittance) + 382 x 16 (HG table)
+FRAME_POST_BYTES_PER_PIXEL = 12 + 132 + 132 + 12 + 4  # update: target + histogram r/w; denoise: read both; pack
 MLP_WORKLOAD = "cloud512_1080p_1spp"
 TRACER_WORKLOAD = "cloud512_1080p_%dspp_per_launch"
 
@@ -284,6 +286,59 @@ def run_mlp(a, ia, sc):
             "kernels": {"first_vertex_ms": fv, "gather_ms": ga, "mlp_bf16_ms": ml}}
 
 
+def run_frame(a, ia, sc):
+    """Progressive frames (`VolumeRender::Render(target, histo, target2, ...)`):
+    the path tracer with the checkerboard on (half the pixels per call), the
+    per-pixel histograms, the 11x11 histogram-distance denoiser, ACES and the
+    BGRA8 pack, K timed calls."""
+    W, H = sc.W, sc.H
+    target = ia.DeviceBuffer(W * H * 12)
+    histo = ia.DeviceBuffer(W * H * 132)
+    bgra = ia.DeviceBuffer(W * H * 4)
+    for b in (target, histo, bgra):
+        b.zero()
+    sc.vol.set_checkerboard(True)
+    sc.vol.set_scatter_rate(0.999)
+    sc.vol.set_env(0.1)
+    ori = 2.5 * np.array([0.75, 0.75, 0.75]) / np.linalg.norm([0.75, 0.75, 0.75])
+    right, up = ia.camera_basis(ori, 40.0, W / H)
+    light = np.array([0.0, 1.0, 0.3]) / np.linalg.norm([0.0, 1.0, 0.3])
+    st = ia.TraceStats()
+
+    def frame(i):
+        sc.vol.render_frame(target, histo, bgra, (W, H), ori, up, right, light, alpha=256.0,
+                            multi_scatter=sc.bounces, g=0.857, randseed=i, rt=ia.PT, tone=2, denoise=True, stats=st)
+        return st.kernel_ms, st.reduce_ms, st.lookups
+
+    for i in range(a.warmup):
+        frame(i)
+    tr_ms = post_ms = 0.0
+    lookups = 0
+    t0 = time.perf_counter()
+    for i in range(a.warmup, a.warmup + a.steps):
+        k, r, lk = frame(i)
+        tr_ms += k
+        post_ms += r
+        lookups += lk
+    elapsed = time.perf_counter() - t0
+    tr_ms, post_ms = tr_ms / a.steps, post_ms / a.steps
+    achieved = (lookups / a.steps) * BYTES_PER_LOOKUP / (tr_ms * 1e-3) / 1e9
+    post_gbs = W * H * FRAME_POST_BYTES_PER_PIXEL / (post_ms * 1e-3) / 1e9
+    log("progressive frames: %.2f ms/frame (trace %.2f, histogram + denoise + tone + pack %.3f ms)"
+        % (1000 * elapsed / a.steps, tr_ms, post_ms))
+    return {
+        "value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "f32",
+        "config": {"workload": "progressive path-traced frames, 512^3 Perlin-fBm cloud, %dx%d, checkerboard "
+                               "(half the pixels per frame), histogram denoiser, ACES, BGRA8" % (W, H),
+                   "width": W, "height": H, "grid": sc.res, "bounces": sc.bounces},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "trace_kernel",
+                     "kernel_ms": tr_ms, "lookups_per_launch": lookups / a.steps,
+                     "bytes_per_lookup": BYTES_PER_LOOKUP},
+        "kernels": {"trace_ms": tr_ms, "post_ms": post_ms, "post_gbs_compulsory": post_gbs},
+    }
+
+
 def run_headline(a, ia, sc):
     net = ia.Network()
     sc.vol.set_network(net)
@@ -331,7 +386,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["headline", "mlp", "tracer"], default="headline")
+    ap.add_argument("--mode", choices=["headline", "mlp", "tracer", "frame"], default="headline")
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -359,7 +414,7 @@ def main(argv=None):
         src.free()
         dst.free()
     sc = Scene(ia, a.res, a.width, a.height, a.bounces)
-    res = {"headline": run_headline, "mlp": run_mlp, "tracer": run_tracer}[a.mode](a, ia, sc)
+    res = {"headline": run_headline, "mlp": run_mlp, "tracer": run_tracer, "frame": run_frame}[a.mode](a, ia, sc)
     base = None
     if not a.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")

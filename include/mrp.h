@@ -166,6 +166,37 @@ int mrp_render_offline(mrp_volume* vol, const mrp_render_params* p, float* rgb_h
 int mrp_render_offline_dev(mrp_volume* vol, const mrp_render_params* p, float* target_dev, double* sum64_dev,
                            mrp_trace_stats* stats);
 
+/* One progressive frame: `VolumeRender::Render(float3* target, Histogram*
+ * histo, unsigned* target2, int2 size, ..., int randseed, RenderType rt,
+ * int toneType, bool denoise)` (core/volume.cu:469-547), all buffers
+ * caller-owned in HBM (the reference's GUI passes device pointers and a mapped
+ * pixel buffer, GUI.hpp:138-168):
+ *   target_dev : width*height*3 FP32; PT: running mean; MRPNN: the frame
+ *                blended with the reprojected previous frame
+ *   histo_dev  : width*height histograms; PT: per-pixel statistics (cleared
+ *                per pixel at its first frame); MRPNN: the previous frame
+ *   bgra_dev   : width*height 0xAARRGGBB after exposure and tone curve
+ * r.spp_begin is the frame number (`randseed`: 0 restarts the running means),
+ * r.spp_count must be 1.  The volume keeps the state the reference keeps in
+ * globals: the checkerboard parity (toggles every call), a call counter that
+ * selects the random stream (`rand_cpu`), the previous camera (reprojection)
+ * and whether the previous frame was MRPNN (histograms are cleared when
+ * switching to MRPNN).  Rows follow the reference's progressive camera:
+ * v = (row + xi)/height, row 0 at the bottom. */
+typedef struct mrp_frame_params {
+    mrp_render_params r;
+    int tone;          /* 0 none, 1 gamma 2.2, 2 ACES */
+    int denoise;       /* PT: histogram-distance filter of noisy pixels */
+    int checkerboard;  /* `SetCheckboard`: render one checkerboard half per call */
+    float exposure;    /* `SetExposure` */
+} mrp_frame_params;
+int mrp_render_frame(mrp_volume* vol, const mrp_frame_params* p, float* target_dev, mrp_histogram* histo_dev,
+                     uint32_t* bgra_dev, mrp_trace_stats* stats);
+/* Forget the progressive state (parity 0, call counter 0, no previous camera). */
+int mrp_volume_reset_frames(mrp_volume* vol);
+/* The progressive state: checkerboard parity of the next call and calls made. */
+int mrp_volume_frame_state(const mrp_volume* vol, int* flip, uint32_t* calls);
+
 /* Cap (bytes, default 4 GiB) of the per-volume buffer that holds the
  * per-(sample, pixel) results of one batch; larger sample counts are split
  * into batches. */

@@ -25,7 +25,7 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import CloudParams, MrpError, RenderParams, TraceStats, Vec3, check, load
+from ._lib import CloudParams, FrameParams, Histogram, MrpError, RenderParams, TraceStats, Vec3, check, load
 
 PT = 0
 MRPNN = 2
@@ -179,6 +179,7 @@ class VolumeRender:
         self.tr_scale = 1.0
         self.sky = (0.0, 0.0, 0.0)
         self.exposure = 1.0
+        self.checkerboard = True  # the reference's default (core/volume.hpp:50)
         self.seed = 0
 
     # ---- data
@@ -246,6 +247,9 @@ class VolumeRender:
     def set_exposure(self, e):
         self.exposure = float(e)
 
+    def set_checkerboard(self, on: bool):
+        self.checkerboard = bool(on)
+
     # ---- rendering
     def params(self, size, ori, up, right, light_dir, rt=PT, g=0.857, alpha=1.0, light_color=(1, 1, 1),
                multi_scatter=512, sample_num=1024, spp_begin=0, seed=None, mlp_precision=0) -> RenderParams:
@@ -271,6 +275,27 @@ class VolumeRender:
         check(self.lib.mrp_render_offline_dev(self.h, ctypes.byref(p), target.ptr,
                                               sum64.ptr if sum64 is not None else None,
                                               ctypes.byref(stats) if stats is not None else None))
+
+    def render_frame(self, target: DeviceBuffer, histo: DeviceBuffer, target2: DeviceBuffer, size, ori, up, right,
+                     light_dir, light_color=(1, 1, 1), alpha=1.0, multi_scatter=1, g=0.0, randseed=0, rt=PT,
+                     tone=2, denoise=False, mlp_precision=1, stats: TraceStats = None):
+        """One progressive frame (`VolumeRender::Render(float3* target, Histogram*
+        histo, unsigned* target2, ...)`, core/volume.cu:469-547) into
+        caller-owned HBM buffers: target (w*h*3 FP32), histo (w*h Histogram),
+        target2 (w*h BGRA8 words)."""
+        p = self.params(size, ori, up, right, light_dir, rt, g, alpha, light_color, multi_scatter, 1, randseed,
+                        None, mlp_precision)
+        fp = FrameParams(p, int(tone), 1 if denoise else 0, 1 if self.checkerboard else 0, float(self.exposure))
+        check(self.lib.mrp_render_frame(self.h, ctypes.byref(fp), target.ptr, histo.ptr, target2.ptr,
+                                        ctypes.byref(stats) if stats is not None else None))
+
+    def reset_frames(self):
+        check(self.lib.mrp_volume_reset_frames(self.h))
+
+    def frame_state(self):
+        flip, calls = ctypes.c_int(), ctypes.c_uint32()
+        check(self.lib.mrp_volume_frame_state(self.h, ctypes.byref(flip), ctypes.byref(calls)))
+        return flip.value, calls.value
 
     def set_network(self, net: "Network"):
         self.network = net

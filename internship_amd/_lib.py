@@ -42,6 +42,13 @@ class RenderParams(ctypes.Structure):
     ]
 
 
+class FrameParams(ctypes.Structure):
+    _fields_ = [
+        ("r", RenderParams), ("tone", ctypes.c_int), ("denoise", ctypes.c_int), ("checkerboard", ctypes.c_int),
+        ("exposure", ctypes.c_float),
+    ]
+
+
 class TraceStats(ctypes.Structure):
     _fields_ = [
         ("lookups", ctypes.c_uint64), ("paths", ctypes.c_uint64), ("scatters", ctypes.c_uint64),
@@ -86,6 +93,9 @@ SIGNATURES = {
     "mrp_render_offline": (I, [P, ctypes.POINTER(RenderParams), P, ctypes.POINTER(TraceStats)]),
     "mrp_render_offline_dev": (I, [P, ctypes.POINTER(RenderParams), P, P, ctypes.POINTER(TraceStats)]),
     "mrp_set_sample_buffer_budget": (I, [P, SZ]),
+    "mrp_render_frame": (I, [P, ctypes.POINTER(FrameParams), P, P, P, ctypes.POINTER(TraceStats)]),
+    "mrp_volume_reset_frames": (I, [P]),
+    "mrp_volume_frame_state": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)]),
     "mrp_network_create": (I, [P, SZ, ctypes.POINTER(P)]),
     "mrp_network_destroy": (I, [P]),
     "mrp_volume_set_network": (I, [P, P]),

@@ -52,6 +52,20 @@ struct Volume {
     void* samples = nullptr;                 // per-(sample, pixel) float4 results
     size_t samples_bytes = 0;
     size_t sample_budget = (size_t)4 << 30;
+    // progressive-frame state kept between mrp_render_frame calls, like the
+    // reference's globals `flip`, `rand_cpu`, `lori/lup/lright` and
+    // `last_predict` (core/volume.cu:50-56,404-407,469-547)
+    int frame_flip = 0;
+    uint32_t frame_calls = 0;
+    bool frame_have_last = false;
+    bool frame_last_predict = true;
+    mrp_vec3 frame_last_ori{0, 0, 0}, frame_last_up{0, 0, 0}, frame_last_right{0, 0, 0};
+};
+
+// Which pixels a launch renders and which camera row convention it uses.
+struct PixelMap {
+    int checker = -1;  // -1: every pixel; 0/1: the checkerboard half of checker_col
+    int rows_up = 0;   // 1: progressive camera (row 0 at the bottom)
 };
 
 inline int mip_res(int m) { return 256 >> m; }
@@ -64,6 +78,13 @@ int ensure_samples(Volume* v, size_t bytes);
 int check_params(const Volume* v, const mrp_render_params* p);
 int launch_accumulate(Volume* v, const void* samples, size_t P, int spp_begin, int n, float* target, double* sum64);
 int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st);
+// One batch of per-(sample, pixel) float4 results (rgb, first-hit distance or
+// -1) in v->samples[(s - s_begin) * P + pixel] for the pixels `map` selects.
+int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n);
+int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
+                  double* gather_ms, double* mlp_ms);
+int clear_counters(Volume* v);
+int read_counters(Volume* v, unsigned long long out[4]);
 int launch_update(Volume* v);
 int launch_update_tr(Volume* v, vec3 light_dir, float alpha);
 int ensure_tr(Volume* v, mrp_vec3 light_dir, float alpha);       // cached Update_TR

@@ -239,6 +239,29 @@ MRP_HD vec3 camera_dir(vec3 forward, vec3 right, vec3 up, int row, int col, int 
     return normalize(d);
 }
 
+// The same ray with the row convention selectable: the progressive camera of
+// `RenderCamera(target, histo, ...)` (core/volume.cu:69-70) takes
+// v = (row + xi)/h (row 0 at the bottom, the GL convention of its pixel
+// buffer) where the offline camera takes v = 1 - (row + xi)/h.
+MRP_HD vec3 camera_dir_rows(vec3 forward, vec3 right, vec3 up, int row, int col, int w, int h, float xi_u, float xi_v,
+                            int rows_up) {
+    const float u = 1.0f - ((float)col + xi_u) / (float)w;
+    const float vv = ((float)row + xi_v) / (float)h;
+    const float v = rows_up ? vv : 1.0f - vv;
+    const vec3 d = forward + (right * (u * 2.0f - 1.0f)) + (up * (v * 2.0f - 1.0f));
+    return normalize(d);
+}
+
+// Checkerboard rendering renders, in each call, the pixels whose row + column
+// + flip is even (`RenderCamera`, core/volume.cu:61-62): logical column k of
+// row r is image column 2k + ((r + flip) & 1).  checker < 0: every pixel.
+MRP_HD int checker_col(int row, int col_l, int checker) {
+    return checker < 0 ? col_l : 2 * col_l + ((row + checker) & 1);
+}
+MRP_HD bool checker_rendered(int row, int col, int checker) {
+    return checker < 0 || ((row + col + checker) & 1) == 0;
+}
+
 // Random orthonormal frame whose first axis is v1: `GetMatrixFromNormal`
 // (core/render.cuh:355-369).  The rejection loop is bounded (64 tries, then a
 // fixed helper axis) so that degenerate input cannot hang a wavefront.

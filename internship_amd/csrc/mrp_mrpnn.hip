@@ -42,6 +42,7 @@ struct NNArgs {
     Majorant mj;
     vec3 ori, forward, right, up, light, lx, light_color, sky, scatter_rate;
     int width, height;
+    int checker, rows_up, wl;  // pixel selection (checker_col), camera row convention, logical width
     uint32_t seed_lo, seed_hi;
     int spp_begin;       // absolute sample index of item 0's sample
     uint32_t item0;      // first (sample-major) item of this chunk within the batch
@@ -98,7 +99,10 @@ __device__ float fv_transmittance(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir,
     return tr;
 }
 
-// pos_act = (pos, active), fx = (X, single), fy = (Y, 0), fz = (Z, dist)
+// pos_act = (pos, active), fx = (X, single), fy = (Y, output slot as bits),
+// fz = (Z, dist).  Items run over the logical pixels of `a`'s pixel map; the
+// output slot (s - spp_begin) * P + pixel is 0xffffffff for a logical pixel
+// past the image edge (odd widths under the checkerboard).
 __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __restrict__ pos_act,
                                                            float4* __restrict__ fx, float4* __restrict__ fy,
                                                            float4* __restrict__ fz) {
@@ -107,36 +111,46 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
     uint32_t lk = 0u;
     if (i < a.n_items) {
         const uint32_t P = (uint32_t)(a.width * a.height);
+        const uint32_t Pl = (uint32_t)(a.wl * a.height);
         const uint32_t item = a.item0 + i;
-        const uint32_t s = item / P, pix = item - s * P;
-        Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
-        const float xu = rng_next(&rng);
-        const float xv = rng_next(&rng);
-        const int row = (int)(pix / (uint32_t)a.width), col = (int)(pix - (uint32_t)row * (uint32_t)a.width);
-        vec3 dir = camera_dir(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv);
-        dir = normalize(dir);  // NNPredict
-        const vec3 gdir = normalize(dir);  // GetSamplePoint
-        vec3 pos = v3(0, 0, 0);
-        int active = 0;
-        const float off = ray_box_offset(a.ori, gdir);
-        if (off >= 0.0f) {
-            const vec3 sp = a.ori + gdir * off;
-            vec3 np;
-            if (fv_free_flight(&rng, a, sp, gdir, ray_box_distance(sp, gdir), &np, &lk)) {
-                pos = np;
-                active = 1;
+        const uint32_t s = item / Pl, l = item - s * Pl;
+        const int row = (int)(l / (uint32_t)a.wl), col_l = (int)(l - (uint32_t)row * (uint32_t)a.wl);
+        const int col = checker_col(row, col_l, a.checker);
+        if (col >= a.width) {
+            pos_act[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            fx[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+            fy[i] = make_float4(0.0f, 1.0f, 0.0f, __uint_as_float(0xffffffffu));
+            fz[i] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+        } else {
+            const uint32_t pix = (uint32_t)row * (uint32_t)a.width + (uint32_t)col;
+            Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+            const float xu = rng_next(&rng);
+            const float xv = rng_next(&rng);
+            vec3 dir = camera_dir_rows(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv, a.rows_up);
+            dir = normalize(dir);  // NNPredict
+            const vec3 gdir = normalize(dir);  // GetSamplePoint
+            vec3 pos = v3(0, 0, 0);
+            int active = 0;
+            const float off = ray_box_offset(a.ori, gdir);
+            if (off >= 0.0f) {
+                const vec3 sp = a.ori + gdir * off;
+                vec3 np;
+                if (fv_free_flight(&rng, a, sp, gdir, ray_box_distance(sp, gdir), &np, &lk)) {
+                    pos = np;
+                    active = 1;
+                }
             }
+            vec3 X, Y, Z, LX, LY, LZ;
+            frame_from_normal(&rng, dir, &X, &Y, &Z);
+            frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
+            const float phase = hg_phase(dot(X, LX), a.g);
+            const float single = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk) * phase;
+            const float dist = active ? distance(a.ori, pos) : -1.0f;
+            pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
+            fx[i] = make_float4(X.x, X.y, X.z, single);
+            fy[i] = make_float4(Y.x, Y.y, Y.z, __uint_as_float(s * P + pix));
+            fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
         }
-        vec3 X, Y, Z, LX, LY, LZ;
-        frame_from_normal(&rng, dir, &X, &Y, &Z);
-        frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
-        const float phase = hg_phase(dot(X, LX), a.g);
-        const float single = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk) * phase;
-        const float dist = active ? distance(a.ori, pos) : -1.0f;
-        pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
-        fx[i] = make_float4(X.x, X.y, X.z, single);
-        fy[i] = make_float4(Y.x, Y.y, Y.z, 0.0f);
-        fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -254,18 +268,21 @@ __global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ 
 
 // ---------------------------------------------------------------- stage 4
 __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act, const float4* __restrict__ fx,
-                                                      const float4* __restrict__ fz, const float* __restrict__ pred,
-                                                      vec3 light_color, vec3 scatter_rate, vec3 sky, uint32_t n_items,
+                                                      const float4* __restrict__ fy, const float4* __restrict__ fz,
+                                                      const float* __restrict__ pred, vec3 light_color,
+                                                      vec3 scatter_rate, vec3 sky, uint32_t n_items,
                                                       float4* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
+    const uint32_t slot = __float_as_uint(fy[i].w);
+    if (slot == 0xffffffffu) return;
     const float single = fx[i].w;
     if (pos_act[i].w != 0.0f) {
         const vec3 p = v3(pred[3 * i] + single, pred[3 * i + 1] + single, pred[3 * i + 2] + single);
         const vec3 r = p * light_color * scatter_rate;
-        out[i] = make_float4(r.x, r.y, r.z, fz[i].w);
+        out[slot] = make_float4(r.x, r.y, r.z, fz[i].w);
     } else {
-        out[i] = make_float4(sky.x, sky.y, sky.z, -1.0f);
+        out[slot] = make_float4(sky.x, sky.y, sky.z, -1.0f);
     }
 }
 
@@ -316,7 +333,15 @@ static int ensure_nn_scratch(int device, size_t bytes, void** out) {
     return MRP_OK;
 }
 
-int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
+static vec3 srp_of(vec3 scatter_rate) {
+    const vec3 sr = v3(scatter_rate.x / 1.001f, scatter_rate.y / 1.001f, scatter_rate.z / 1.001f);
+    return v3((float)std::pow((double)sr.x, 4.0), (float)std::pow((double)sr.y, 4.0), (float)std::pow((double)sr.z, 4.0));
+}
+
+// First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
+// of the pixels `map` selects, in chunks of at most 4M vertices.
+int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
+                  double* gather_ms, double* mlp_ms) {
     if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
     if (p->mlp_precision != 0 && p->mlp_precision != 1)
         return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32) or 1 (BF16)");
@@ -329,12 +354,10 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     if (rc) return rc;
 
     const int W = p->width, H = p->height;
-    const size_t P = (size_t)W * H;
-    int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16), (size_t)p->spp_count));
-    rc = ensure_samples(v, (size_t)batch * P * 16);
-    if (rc) return rc;
-    // one chunk covers a whole 1080p frame (2.07M points, about 6 GB of scratch)
-    const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, (size_t)batch * P));
+    const int wl = map.checker < 0 ? W : (W + 1) / 2;
+    const size_t Pl = (size_t)wl * H;
+    const size_t total = (size_t)n * Pl;
+    const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, total));
     void* scratch = nullptr;
     rc = ensure_nn_scratch(v->device, chunk_items * kNNItemBytes + 4096, &scratch);
     if (rc) return rc;
@@ -367,8 +390,12 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     a.scatter_rate = V(p->scatter_rate);
     a.width = W;
     a.height = H;
+    a.checker = map.checker;
+    a.rows_up = map.rows_up;
+    a.wl = wl;
     a.seed_lo = (uint32_t)p->seed;
     a.seed_hi = (uint32_t)(p->seed >> 32);
+    a.spp_begin = s_begin;
     a.counters = v->counters;
 
     GatherArgs ga;
@@ -379,74 +406,81 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     ga.lx = a.lx;
     ga.alpha = p->alpha;
     ga.g = p->g;
+    const vec3 srp = srp_of(a.scatter_rate);
 
-    const vec3 sr = v3(a.scatter_rate.x / 1.001f, a.scatter_rate.y / 1.001f, a.scatter_rate.z / 1.001f);
-    const vec3 srp = v3((float)std::pow((double)sr.x, 4.0), (float)std::pow((double)sr.y, 4.0),
-                        (float)std::pow((double)sr.z, 4.0));
-
-    hipEvent_t e0, e1;
-    MRP_HIP(hipEventCreate(&e0));
-    MRP_HIP(hipEventCreate(&e1));
-    MRP_HIP(hipEventRecord(e0, v->stream));
-    hipLaunchKernelGGL(clear_counters_kernel, dim3(1), dim3(64), 0, v->stream, v->counters, 4);
-    double mlp_ms = 0.0, gather_ms = 0.0, fv_ms = 0.0;
     hipEvent_t t0, t1, t2, t3;
     MRP_HIP(hipEventCreate(&t0));
     MRP_HIP(hipEventCreate(&t1));
     MRP_HIP(hipEventCreate(&t2));
     MRP_HIP(hipEventCreate(&t3));
-    for (int s0 = 0; s0 < p->spp_count; s0 += batch) {
-        const int n = std::min(batch, p->spp_count - s0);
-        a.spp_begin = p->spp_begin + s0;
-        const size_t total = (size_t)n * P;
-        for (size_t i0 = 0; i0 < total; i0 += chunk_items) {
-            const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
-            a.item0 = (uint32_t)i0;
-            a.n_items = cnt;
-            ga.n_items = cnt;
-            MRP_HIP(hipEventRecord(t0, v->stream));
-            first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz);
-            MRP_HIP(hipGetLastError());
-            MRP_HIP(hipEventRecord(t1, v->stream));
-            gather_kernel<<<cnt, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, feat, gam, pool);
-            MRP_HIP(hipGetLastError());
-            MRP_HIP(hipEventRecord(t2, v->stream));
-            if (p->mlp_precision == 0) {
-                mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt, pred);
-            } else {
-                rc = launch_mlp_bf16(v->nn_packed, feat, pool, gam, p->g, srp, cnt, pred, v->stream);
-                if (rc) return rc;
-            }
-            MRP_HIP(hipGetLastError());
-            MRP_HIP(hipEventRecord(t3, v->stream));
-            combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, fx, fz, pred, a.light_color,
-                                                                      a.scatter_rate, a.sky, cnt,
-                                                                      (float4*)v->samples + i0);
-            MRP_HIP(hipGetLastError());
-            MRP_HIP(hipEventSynchronize(t3));
-            float ms;
-            MRP_HIP(hipEventElapsedTime(&ms, t0, t1));
-            fv_ms += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, t1, t2));
-            gather_ms += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
-            mlp_ms += ms;
+    for (size_t i0 = 0; i0 < total && rc == MRP_OK; i0 += chunk_items) {
+        const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
+        a.item0 = (uint32_t)i0;
+        a.n_items = cnt;
+        ga.n_items = cnt;
+        MRP_HIP(hipEventRecord(t0, v->stream));
+        first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz);
+        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventRecord(t1, v->stream));
+        gather_kernel<<<cnt, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, feat, gam, pool);
+        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventRecord(t2, v->stream));
+        if (p->mlp_precision == 0) {
+            mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt, pred);
+        } else {
+            rc = launch_mlp_bf16(v->nn_packed, feat, pool, gam, p->g, srp, cnt, pred, v->stream);
+            if (rc) break;
         }
-        rc = launch_accumulate(v, v->samples, P, a.spp_begin, n, target, sum64);
-        if (rc) return rc;
+        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventRecord(t3, v->stream));
+        combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, fx, fy, fz, pred, a.light_color,
+                                                                  a.scatter_rate, a.sky, cnt, (float4*)v->samples);
+        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventSynchronize(t3));
+        float ms;
+        MRP_HIP(hipEventElapsedTime(&ms, t0, t1));
+        if (fv_ms) *fv_ms += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, t1, t2));
+        if (gather_ms) *gather_ms += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
+        if (mlp_ms) *mlp_ms += ms;
     }
-    MRP_HIP(hipEventRecord(e1, v->stream));
-    MRP_HIP(hipEventSynchronize(e1));
-    float total_ms = 0.0f;
-    MRP_HIP(hipEventElapsedTime(&total_ms, e0, e1));
-    unsigned long long cnt4[4] = {0, 0, 0, 0};
-    MRP_HIP(hipMemcpy(cnt4, v->counters, sizeof(cnt4), hipMemcpyDeviceToHost));
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     hipEventDestroy(t0);
     hipEventDestroy(t1);
     hipEventDestroy(t2);
     hipEventDestroy(t3);
+    return rc;
+}
+
+int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
+    const int W = p->width, H = p->height;
+    const size_t P = (size_t)W * H;
+    const int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16), (size_t)p->spp_count));
+    int rc = ensure_samples(v, (size_t)batch * P * 16);
+    if (rc) return rc;
+    hipEvent_t e0, e1;
+    MRP_HIP(hipEventCreate(&e0));
+    MRP_HIP(hipEventCreate(&e1));
+    MRP_HIP(hipEventRecord(e0, v->stream));
+    rc = clear_counters(v);
+    double mlp_ms = 0.0, gather_ms = 0.0, fv_ms = 0.0;
+    const PixelMap map;
+    for (int s0 = 0; s0 < p->spp_count && rc == MRP_OK; s0 += batch) {
+        const int n = std::min(batch, p->spp_count - s0);
+        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &fv_ms, &gather_ms, &mlp_ms);
+        if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
+    }
+    float total_ms = 0.0f;
+    unsigned long long cnt4[4] = {0, 0, 0, 0};
+    if (rc == MRP_OK) {
+        MRP_HIP(hipEventRecord(e1, v->stream));
+        MRP_HIP(hipEventSynchronize(e1));
+        MRP_HIP(hipEventElapsedTime(&total_ms, e0, e1));
+        rc = read_counters(v, cnt4);
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (rc) return rc;
     if (st) {
         st->lookups = cnt4[0];
         st->paths = (uint64_t)p->spp_count * P;

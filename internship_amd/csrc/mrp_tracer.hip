@@ -46,6 +46,7 @@ struct TraceArgs {
     vec3 ori, forward, right, up, light_dir, light_color, sky, scatter_rate;
     Majorant mj;
     int width, height, multi_scatter;
+    int checker, rows_up, wl;  // pixel selection (checker_col) and camera row convention; wl = logical width
     uint32_t seed_lo, seed_hi;
     int spp_begin, spp_count;
     int band_rows, tiles_x, tiles_per_band;
@@ -156,11 +157,12 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                     const uint32_t r = k - tile * per_tile;
                     const uint32_t s = r >> 6, w = r & 63u;
                     const int ty = (int)(tile / (uint32_t)a.tiles_x), tx = (int)(tile % (uint32_t)a.tiles_x);
-                    const int col = tx * 8 + (int)(w & 7u);
+                    const int col_l = tx * 8 + (int)(w & 7u);
                     const int row0 = qf * a.band_rows;
                     const int row = row0 + ty * 8 + (int)(w >> 3);
                     const int row_end = min(a.height, row0 + a.band_rows);
-                    if (col < a.width && row < row_end) {
+                    const int col = checker_col(row, col_l, a.checker);
+                    if (col_l < a.wl && col < a.width && row < row_end) {
                         pix = row * a.width + col;
                         sidx = (int)s;
                         state = ST_INIT;
@@ -201,7 +203,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                 const float xu = ring_take(rng);
                 const float xv = ring_take(rng);
                 const int row = pix / a.width, col = pix - row * a.width;
-                vec3 dir = camera_dir(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv);
+                vec3 dir = camera_dir_rows(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv, a.rows_up);
                 dir = normalize(dir);
                 const float off = ray_box_offset(a.ori, dir);
                 tfirst = -1.0f;
@@ -420,6 +422,77 @@ int check_params(const Volume* v, const mrp_render_params* p) {
     return MRP_OK;
 }
 
+static void fill_trace_args(Volume* v, const mrp_render_params* p, PixelMap map, TraceArgs* a) {
+    const int W = p->width, H = p->height;
+    const int wl = map.checker < 0 ? W : (W + 1) / 2;
+    const int band_rows = (H + kNumQueues - 1) / kNumQueues;
+    a->cells = v->cells;
+    a->res = v->res;
+    a->alpha = p->alpha;
+    a->smax = v->max_density * p->alpha;
+    a->mj = make_majorant(v->max_density, p->alpha, p->tr_scale);
+    a->g = p->g;
+    a->tr_scale = p->tr_scale;
+    a->ori = V(p->ori);
+    a->forward = normalize(-a->ori);
+    a->right = V(p->right);
+    a->up = V(p->up);
+    a->light_dir = normalize(V(p->light_dir));
+    a->light_color = V(p->light_color);
+    a->sky = V(p->sky);
+    a->scatter_rate = V(p->scatter_rate);
+    a->width = W;
+    a->height = H;
+    a->multi_scatter = p->multi_scatter;
+    a->checker = map.checker;
+    a->rows_up = map.rows_up;
+    a->wl = wl;
+    a->seed_lo = (uint32_t)p->seed;
+    a->seed_hi = (uint32_t)(p->seed >> 32);
+    a->band_rows = band_rows;
+    a->tiles_x = (wl + 7) / 8;
+    a->tiles_per_band = a->tiles_x * ((band_rows + 7) / 8);
+    a->queues = v->queues;
+    a->counters = v->counters;
+    a->out = (float4*)v->samples;
+}
+
+static uint64_t trace_items_per_sample(const TraceArgs& a) { return (uint64_t)a.tiles_per_band * 64u; }
+
+static int trace_grid(Volume* v) {
+    int occ = 0;
+    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel, 256, 0));
+    occ = std::max(1, std::min(occ, 8));
+    return std::max(kNumQueues, v->num_cus * occ);
+}
+
+// One persistent tracer launch over samples [s_begin, s_begin + n) (absolute
+// RNG sample indices) of the pixels that `map` selects; path results go to
+// v->samples[(s - s_begin) * P + pixel].  The caller sized v->samples.
+int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n) {
+    TraceArgs a;
+    fill_trace_args(v, p, map, &a);
+    a.spp_begin = s_begin;
+    a.spp_count = n;
+    a.items_per_queue = (uint32_t)(trace_items_per_sample(a) * (uint64_t)n);
+    clear_u32_kernel<<<1, 64, 0, v->stream>>>(v->queues, kNumQueues);
+    trace_kernel<<<trace_grid(v), 256, 0, v->stream>>>(a);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
+}
+
+int clear_counters(Volume* v) {
+    clear_u64_kernel<<<1, 64, 0, v->stream>>>(v->counters, 4);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
+}
+
+int read_counters(Volume* v, unsigned long long out[4]) {
+    MRP_HIP(hipMemcpyAsync(out, v->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, v->stream));
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    return MRP_OK;
+}
+
 // Trace samples [spp_begin, spp_begin+spp_count) into target/sum64 (device).
 static int trace_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
     int rc = check_params(v, p);
@@ -428,48 +501,16 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
     const int W = p->width, H = p->height;
     const size_t P = (size_t)W * H;
     const size_t per_sample_bytes = P * sizeof(float4);
+    const PixelMap map;
+    TraceArgs probe;
+    fill_trace_args(v, p, map, &probe);
+    const uint64_t per_sample_items = trace_items_per_sample(probe);
     int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / per_sample_bytes, (size_t)p->spp_count));
-    const int band_rows = (H + kNumQueues - 1) / kNumQueues;
-    const int tiles_x = (W + 7) / 8;
-    const int tiles_per_band = tiles_x * ((band_rows + 7) / 8);
-    const uint64_t per_sample_items = (uint64_t)tiles_per_band * 64u;
     batch = (int)std::min<uint64_t>((uint64_t)batch, (uint64_t)(((1ull << 32) - (1ull << 26)) / per_sample_items));
     if (batch < 1) return fail(MRP_E_INVALID, "image too large for one work queue");
     rc = ensure_samples(v, (size_t)batch * per_sample_bytes);
     if (rc) return rc;
-
-    TraceArgs a;
-    a.cells = v->cells;
-    a.res = v->res;
-    a.alpha = p->alpha;
-    a.smax = v->max_density * p->alpha;
-    a.mj = make_majorant(v->max_density, p->alpha, p->tr_scale);
-    a.g = p->g;
-    a.tr_scale = p->tr_scale;
-    a.ori = V(p->ori);
-    a.forward = normalize(-a.ori);
-    a.right = V(p->right);
-    a.up = V(p->up);
-    a.light_dir = normalize(V(p->light_dir));
-    a.light_color = V(p->light_color);
-    a.sky = V(p->sky);
-    a.scatter_rate = V(p->scatter_rate);
-    a.width = W;
-    a.height = H;
-    a.multi_scatter = p->multi_scatter;
-    a.seed_lo = (uint32_t)p->seed;
-    a.seed_hi = (uint32_t)(p->seed >> 32);
-    a.band_rows = band_rows;
-    a.tiles_x = tiles_x;
-    a.tiles_per_band = tiles_per_band;
-    a.queues = v->queues;
-    a.counters = v->counters;
-    a.out = (float4*)v->samples;
-
-    int occ = 0;
-    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel, 256, 0));
-    occ = std::max(1, std::min(occ, 8));
-    const int blocks = std::max(kNumQueues, v->num_cus * occ);
+    const int blocks = trace_grid(v);
 
     hipEvent_t e0, e1, e2, e3;
     MRP_HIP(hipEventCreate(&e0));
@@ -479,19 +520,17 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
     double kernel_ms = 0.0, reduce_ms = 0.0;
     int launches = 0;
     MRP_HIP(hipEventRecord(e0, v->stream));
-    clear_u64_kernel<<<1, 64, 0, v->stream>>>(v->counters, 4);
+    rc = clear_counters(v);
+    if (rc) return rc;
     for (int s0 = 0; s0 < p->spp_count; s0 += batch) {
         const int n = std::min(batch, p->spp_count - s0);
-        a.spp_begin = p->spp_begin + s0;
-        a.spp_count = n;
-        a.items_per_queue = (uint32_t)(per_sample_items * (uint64_t)n);
-        clear_u32_kernel<<<1, 64, 0, v->stream>>>(v->queues, kNumQueues);
+        const int sb = p->spp_begin + s0;
         MRP_HIP(hipEventRecord(e1, v->stream));
-        trace_kernel<<<blocks, 256, 0, v->stream>>>(a);
-        MRP_HIP(hipGetLastError());
+        rc = trace_samples(v, p, map, sb, n);
+        if (rc) return rc;
         MRP_HIP(hipEventRecord(e2, v->stream));
-        accumulate_kernel<<<(unsigned)((P + 255) / 256), 256, 0, v->stream>>>((const float4*)v->samples, (int)P,
-                                                                           a.spp_begin, n, target, sum64);
+        accumulate_kernel<<<(unsigned)((P + 255) / 256), 256, 0, v->stream>>>((const float4*)v->samples, (int)P, sb, n,
+                                                                           target, sum64);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(e3, v->stream));
         MRP_HIP(hipEventSynchronize(e3));
@@ -505,12 +544,12 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
     float total = 0.0f;
     MRP_HIP(hipEventElapsedTime(&total, e0, e3));
     unsigned long long cnt[4] = {0, 0, 0, 0};
-    MRP_HIP(hipMemcpyAsync(cnt, v->counters, sizeof(cnt), hipMemcpyDeviceToHost, v->stream));
-    MRP_HIP(hipStreamSynchronize(v->stream));
+    rc = read_counters(v, cnt);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     hipEventDestroy(e2);
     hipEventDestroy(e3);
+    if (rc) return rc;
     if (st) {
         st->lookups = cnt[0];
         st->paths = cnt[1];

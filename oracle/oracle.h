@@ -13,6 +13,14 @@
  * oracle_render            core/render.cuh:93-258, volume.cu:147-179 closed forms: empty volume = sky,
  *                                                                    albedo 0 = Beer-Lambert, single
  *                                                                    scattering quadrature (statistical)
+ * oracle_render_frame      core/volume.cu:57-124,181-287,469-547      constant image passes the filter;
+ *                                                                    low variance passes through; brute
+ *                                                                    force on small images (numpy);
+ *                                                                    running mean = oracle_render
+ * oracle_tone_pack         core/render.cuh:640-732, volume.cu:248-263 Gamma / ACES closed forms (numpy)
+ * oracle_histo_distance    core/volume.cu:187-207                    chi2(h,h) = 0, scale invariance, hand values
+ * oracle_reproject         core/volume.cu:125-141                    static camera maps to itself; pinhole
+ *                                                                    projection into a moved camera (numpy)
  * Later sections of oracle.cpp list their own pins.
  */
 #ifndef MRP_ORACLE_H_
@@ -49,6 +57,17 @@ int oracle_render_mrpnn(const float* grid, int res, float max_density, const flo
 int oracle_render(const float* grid, int res, float max_density, const mrp_render_params* p, float* target,
                   double* sum64, uint64_t* lookups_out, uint64_t* scatters_out, int threads, int row_begin,
                   int row_end);
+
+int oracle_render_frame(const float* grid, int res, float max_density, const float* mips, const float* trs,
+                        const float* lut, const float* weights, const mrp_frame_params* fp, uint32_t call_index,
+                        int flip, int have_last, mrp_vec3 last_ori, mrp_vec3 last_up, mrp_vec3 last_right,
+                        int last_predict, float* target, mrp_histogram* histo, uint32_t* bgra,
+                        uint64_t* lookups_out, int threads);
+int oracle_denoise_pack(const float* target, mrp_histogram* histo, int W, int H, int frame, int checker,
+                        int checkerboard, int denoise, int predict, int tone, float exposure, uint32_t* bgra);
+int oracle_tone_pack(const float* rgb, int n, float exposure, int tone, uint32_t* out);
+int oracle_reproject(const float* pts, int n, mrp_vec3 lori, mrp_vec3 lup, mrp_vec3 lright, int W, int H, int* out);
+int oracle_histo_distance(const mrp_histogram* a, const mrp_histogram* b, float* dist, float* weight);
 
 #ifdef __cplusplus
 }

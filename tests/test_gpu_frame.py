@@ -1,0 +1,132 @@
+"""GPU parity of the progressive frame (`mrp_render_frame`) with the CPU
+oracle: histogram accumulation, checkerboard, denoise, tone map and pack for
+path-traced frames; reprojection for MRPNN frames; several frames in a row on
+the same seeded inputs, called through the C ABI."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import internship_amd as ia
+import oracle
+from internship_amd import DeviceBuffer, FrameParams, Network, TraceStats, VolumeRender, load_reference_weights
+from tests.conftest import sphere_grid
+
+pytestmark = pytest.mark.gpu
+
+LIGHT = np.array([0.0, 1.0, 0.3]) / np.linalg.norm([0.0, 1.0, 0.3])
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _params(w, h, frame, rt, ori_dir=(0.75, 0.75, 0.75), multi=16, checkerboard=True, denoise=True, tone=2,
+            precision=0):
+    ori = 2.5 * np.asarray(ori_dir, float) / np.linalg.norm(ori_dir)
+    right, up = ia.camera_basis(ori, 40.0, w / h)
+    V = ia.Vec3.of
+    r = ia.RenderParams(w, h, V(ori), V(up), V(right), V(LIGHT), V((1, 1, 1)), V((0.1, 0.1, 0.1)),
+                        V((0.99, 0.99, 0.99)), 0.857, 64.0, 1.0, multi, frame, 1, 7, rt, precision)
+    return FrameParams(r, tone, 1 if denoise else 0, 1 if checkerboard else 0, 1.0)
+
+
+class _Bufs:
+    def __init__(self, w, h):
+        self.w, self.h = w, h
+        self.target = DeviceBuffer(w * h * 12)
+        self.histo = DeviceBuffer(w * h * 132)
+        self.bgra = DeviceBuffer(w * h * 4)
+        for b in (self.target, self.histo, self.bgra):
+            b.zero()
+
+    def read(self):
+        return (self.target.download(np.float32, (self.h, self.w, 3)),
+                self.histo.download(np.float32, (self.h, self.w, 33)),
+                self.bgra.download(np.uint32, (self.h, self.w)))
+
+
+def _gpu_frame(vol, fp, bufs):
+    st = TraceStats()
+    ia.check(vol.lib.mrp_render_frame(vol.h, ctypes.byref(fp), bufs.target.ptr, bufs.histo.ptr, bufs.bgra.ptr,
+                                      ctypes.byref(st)))
+    return st
+
+
+def _run_pt(grid, w, h, calls, checkerboard):
+    vol = VolumeRender(grid.shape[0])
+    vol.set_datas(grid)
+    vol.update()
+    md = vol.max_density
+    bufs = _Bufs(w, h)
+    tgt = np.zeros((h, w, 3), np.float32)
+    his = np.zeros((h, w, 33), np.float32)
+    last = None
+    try:
+        for call in range(calls):
+            fp = _params(w, h, call, ia.PT, checkerboard=checkerboard)
+            flip, ncalls = vol.frame_state()
+            assert ncalls == call and flip == call % 2
+            _say("case: PT frame %d, %dx%d, checkerboard %d" % (call, w, h, checkerboard))
+            st = _gpu_frame(vol, fp, bufs)
+            bgra_o, lk = oracle.render_frame(grid, md, fp, call, flip, tgt, his, last=last)
+            last = (fp.r.ori.tolist(), fp.r.up.tolist(), fp.r.right.tolist())
+            gt, gh, gb = bufs.read()
+            assert st.lookups == lk
+            assert np.array_equal(gt, tgt), np.abs(gt - tgt).max()
+            assert np.array_equal(gh, his)
+            assert np.array_equal(gb, bgra_o), (gb != bgra_o).sum()
+    finally:
+        vol.close()
+    return his
+
+
+def test_gpu_pt_frames_checkerboard_denoise_match_oracle():
+    his = _run_pt(sphere_grid(32), 20, 14, 5, True)
+    assert np.all(his[..., 30] >= 2)
+
+
+def test_gpu_pt_frames_full_odd_size_match_oracle():
+    _run_pt(sphere_grid(32), 21, 13, 3, False)
+
+
+def test_gpu_mrpnn_frames_reprojection_match_oracle():
+    grid = oracle.generate_cloud(32, seed=3)
+    w, h = 17, 12  # odd width: the checkerboard's last logical column is partly outside
+    vol = VolumeRender(32)
+    vol.set_datas(grid)
+    vol.update()
+    md = vol.max_density
+    wts = load_reference_weights()
+    net = Network(wts)
+    vol.set_network(net)
+    mips, mc = oracle.mips(grid)
+    bufs = _Bufs(w, h)
+    tgt = np.zeros((h, w, 3), np.float32)
+    his = np.zeros((h, w, 33), np.float32)
+    last = None
+    try:
+        for call, ori_dir in enumerate([(0.75, 0.75, 0.75), (0.75, 0.75, 0.75), (0.8, 0.7, 0.72)]):
+            fp = _params(w, h, call, ia.MRPNN, ori_dir=ori_dir)
+            flip, _ = vol.frame_state()
+            _say("case: MRPNN frame %d, %dx%d, camera %s" % (call, w, h, ori_dir))
+            _gpu_frame(vol, fp, bufs)
+            _, tc = oracle.tr_mips(mc, oracle.normalize3(LIGHT), 64.0)
+            lut = oracle.hg_lut(0.857, seed=7)
+            bgra_o, _ = oracle.render_frame(grid, md, fp, call, flip, tgt, his, last=last, last_predict=True,
+                                            mips_concat=mc, tr_concat=tc, lut=lut, weights=wts)
+            last = (fp.r.ori.tolist(), fp.r.up.tolist(), fp.r.right.tolist())
+            gt, gh, gb = bufs.read()
+            diff = np.abs(gt.astype(np.float64) - tgt)
+            rel = diff / np.maximum(np.abs(tgt), 1e-12)
+            _say("exact %.4f, max rel %.3g" % (np.mean(gt == tgt), rel.max()))
+            assert np.all((rel <= 1e-5) | (diff <= 1e-7))
+            assert np.all((np.abs(gh - his) <= 1e-5 * np.abs(his)) | (np.abs(gh - his) <= 1e-7))
+            ch = lambda x, s: (x >> s) & 255
+            for s in (0, 8, 16):
+                assert np.abs(ch(gb.astype(np.int64), s) - ch(bgra_o.astype(np.int64), s)).max() <= 1
+    finally:
+        vol.close()
+        net.close()
