@@ -197,6 +197,11 @@ int mrp_volume_reset_frames(mrp_volume* vol);
 /* The progressive state: checkerboard parity of the next call and calls made. */
 int mrp_volume_frame_state(const mrp_volume* vol, int* flip, uint32_t* calls);
 
+/* Tone curve on host colours (n RGB triples): `Camera::None/Gamma/ACES`
+ * (core/camera.cu:66-160, core/render.cuh:640-732); tone 0 none, 1 gamma,
+ * 2 ACES.  The same arithmetic as the frame path's device tone map. */
+int mrp_tone_map_host(const float* rgb_host, size_t n, int tone, float* out_host);
+
 /* Cap (bytes, default 4 GiB) of the per-volume buffer that holds the
  * per-(sample, pixel) results of one batch; larger sample counts are split
  * into batches. */

@@ -1,0 +1,363 @@
+// mrp_volume_render.hpp - C++ façade with the reference's class interface over
+// the C ABI of libmrp (mrp.h).  Header-only; no HIP or torch types.
+//
+// It keeps the signatures of `VolumeRender` (core/volume.hpp:45-146) and
+// `Camera` (core/camera.hpp:10-41), so that code written against the
+// reference (main.cu:9-47, the README example) compiles against this header
+// with `using namespace internship;`.  The differences:
+//   * `float3`, `int2` and `Histogram` are this header's PODs (same layouts as
+//     CUDA's float3/int2 and the reference's Histogram);
+//   * errors print the library's message and continue, as the reference's
+//     CheckError does (core/volume.cu:15); LastError() returns it;
+//   * the environment is a constant sky colour (SetEnvExp sets its level;
+//     SetHDRI is not available);
+//   * MRPNN rendering needs the trained weights: LoadNetwork(path) reads the
+//     49,673-float blob (internship_amd/data/mrpnn_weights.bin), or
+//     SetNetwork(weights, count) takes them from memory.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <fstream>
+#include <functional>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "mrp.h"
+
+namespace internship {
+
+struct float3 {
+    float x, y, z;
+};
+struct int2 {
+    int x, y;
+};
+inline float3 make_float3(float x, float y, float z) { return float3{x, y, z}; }
+inline float3 operator*(float3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
+inline float3 operator-(float3 a) { return {-a.x, -a.y, -a.z}; }
+
+typedef mrp_histogram Histogram;
+typedef std::function<float(int x, int y, int z, float u, float v, float w)> FillFunc;  // omp.hpp:10
+
+namespace detail {
+inline mrp_vec3 v(float3 a) { return mrp_vec3{a.x, a.y, a.z}; }
+inline float3 normalize(float3 a) {
+    const float l = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return {a.x / l, a.y / l, a.z / l};
+}
+inline float3 cross(float3 a, float3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline bool ok(int rc, const char* what) {
+    if (rc != MRP_OK) std::printf("%s: %s\n", what, mrp_last_error());
+    return rc == MRP_OK;
+}
+}  // namespace detail
+
+class VolumeRender {
+   public:
+    enum RenderType { PT, RPNN, MRPNN };
+
+    int resolution = 0;
+    float max_density = 0.00001f;
+
+    explicit VolumeRender(int res) : resolution(res), datas_((size_t)res * res * res, 0.0f) {
+        detail::ok(mrp_volume_create(res, &h_), "VolumeRender");
+    }
+
+    // `VolumeRender(string path)` (core/volume.cu:835-911): `path.bin` (int32
+    // resolution + FP32 grid) if present, else `.txt` (resolution, then the
+    // values) or `.vox` (256^3 bytes as text, x 64/255); writes `path.bin`.
+    explicit VolumeRender(const std::string& path) {
+        if (!load(path)) {
+            std::printf("File not found!\n");
+            return;
+        }
+        detail::ok(mrp_volume_create(resolution, &h_), "VolumeRender");
+        Update();
+        if (!have_bin_) {
+            if (FILE* f = std::fopen((path + ".bin").c_str(), "wb")) {
+                std::fwrite(&resolution, sizeof(int), 1, f);
+                std::fwrite(datas_.data(), sizeof(float), datas_.size(), f);
+                std::fclose(f);
+            }
+        }
+    }
+
+    VolumeRender(const VolumeRender&) = delete;
+    VolumeRender& operator=(const VolumeRender&) = delete;
+
+    ~VolumeRender() {
+        if (net_) mrp_network_destroy(net_);
+        if (h_) mrp_volume_destroy(h_);
+    }
+
+    void SetData(int x, int y, int z, float value) { datas_[((size_t)x * resolution + y) * resolution + z] = value; }
+
+    // `ParallelFill` (core/omp.cpp:4-18): f(x, y, z, u, v, w) at voxel centres.
+    void SetDatas(FillFunc f) {
+        const int r = resolution;
+#pragma omp parallel for
+        for (int i = 0; i < r; i++)
+            for (int j = 0; j < r; j++)
+                for (int k = 0; k < r; k++)
+                    datas_[((size_t)i * r + j) * r + k] =
+                        f(i, j, k, (float)((i + 0.5) / r), (float)((j + 0.5) / r), (float)((k + 0.5) / r));
+    }
+
+    void Update() {
+        if (!h_) return;
+        if (!detail::ok(mrp_volume_set_data(h_, datas_.data(), 0), "Update")) return;
+        if (!detail::ok(mrp_volume_update(h_), "Update")) return;
+        detail::ok(mrp_volume_max_density(h_, &max_density), "Update");
+    }
+    void Update_TR(float3 lightDir, float alpha = 64.0f, bool /*CPU*/ = false) {
+        detail::ok(mrp_volume_update_tr(h_, detail::v(lightDir), alpha), "Update_TR");
+    }
+    void UpdateHGLut(float g) { detail::ok(mrp_volume_update_hg_lut(h_, g, seed_), "UpdateHGLut"); }
+
+    // setters of the reference's device globals (core/volume.cu:1460-1493)
+    void SetCheckboard(bool on) { checkboard_ = on; }
+    void SetEnvExp(float e) {  // scales the constant sky (the reference scales its HDRI)
+        env_exp_ = e;
+        sky_ = base_sky_ * e;
+    }
+    void SetTrScale(float s) { tr_scale_ = s; }
+    void SetScatterRate(float r) { scatter_ = {r, r, r}; }
+    void SetScatterRate(float3 r) { scatter_ = r; }
+    void SetExposure(float e) { exposure_ = e; }
+    void SetHDRI(const std::string&) { std::printf("SetHDRI: HDRI environments are not available\n"); }
+    // this build's extras
+    void SetSky(float3 c) {
+        base_sky_ = c;
+        sky_ = c * env_exp_;
+    }
+    void SetSeed(uint64_t s) { seed_ = s; }
+    void SetMlpPrecision(int p) { precision_ = p; }  // 0 FP32, 1 BF16 MFMA
+    bool SetNetwork(const float* w, size_t count) {
+        if (net_) mrp_network_destroy(net_);
+        net_ = nullptr;
+        if (!detail::ok(mrp_network_create(w, count, &net_), "SetNetwork")) return false;
+        return detail::ok(mrp_volume_set_network(h_, net_), "SetNetwork");
+    }
+    bool LoadNetwork(const std::string& path) {
+        FILE* f = std::fopen(path.c_str(), "rb");
+        if (!f) {
+            std::printf("LoadNetwork: cannot open %s\n", path.c_str());
+            return false;
+        }
+        std::fseek(f, 0, SEEK_END);
+        const long bytes = std::ftell(f);
+        std::fseek(f, 0, SEEK_SET);
+        std::vector<float> w(bytes > 0 ? (size_t)bytes / sizeof(float) : 0);
+        const size_t got = std::fread(w.data(), sizeof(float), w.size(), f);
+        std::fclose(f);
+        return SetNetwork(w.data(), got);
+    }
+    const char* LastError() const { return mrp_last_error(); }
+    mrp_volume* Handle() { return h_; }
+
+    // Offline render (core/volume.cu:410-467): the mean of sampleNum samples.
+    std::vector<float3> Render(int2 size, float3 ori, float3 up, float3 right, float3 lightDir, RenderType rt = PT,
+                               float g = 0.857f, float alpha = 1, float3 lightColor = {1, 1, 1},
+                               int multiScatter = 512, int sampleNum = 1024) {
+        std::vector<float3> out((size_t)size.x * size.y, float3{0, 0, 0});
+        mrp_render_params p = params(size, ori, up, right, lightDir, rt, g, alpha, lightColor, multiScatter);
+        p.spp_begin = 0;
+        p.spp_count = sampleNum;
+        detail::ok(mrp_render_offline(h_, &p, reinterpret_cast<float*>(out.data()), nullptr), "Render");
+        return out;
+    }
+
+    // Progressive frame (core/volume.cu:469-547); target, histo_buffer and
+    // target2 are device pointers owned by the caller.
+    void Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, float3 ori, float3 up,
+                float3 right, float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1,
+                float g = 0, int randseed = 0, RenderType rt = PT, int toneType = 2, bool denoise = false) {
+        mrp_frame_params fp;
+        fp.r = params(size, ori, up, right, lightDir, rt, g, alpha, lightColor, multiScatter);
+        fp.r.spp_begin = randseed;
+        fp.r.spp_count = 1;
+        fp.tone = toneType;
+        fp.denoise = denoise ? 1 : 0;
+        fp.checkerboard = checkboard_ ? 1 : 0;
+        fp.exposure = exposure_;
+        detail::ok(mrp_render_frame(h_, &fp, reinterpret_cast<float*>(target), histo_buffer, target2, nullptr),
+                   "Render");
+    }
+
+   private:
+    mrp_render_params params(int2 size, float3 ori, float3 up, float3 right, float3 lightDir, RenderType rt, float g,
+                             float alpha, float3 lightColor, int multiScatter) const {
+        mrp_render_params p;
+        p.width = size.x;
+        p.height = size.y;
+        p.ori = detail::v(ori);
+        p.up = detail::v(up);
+        p.right = detail::v(right);
+        p.light_dir = detail::v(lightDir);
+        p.light_color = detail::v(lightColor);
+        p.sky = detail::v(sky_);
+        p.scatter_rate = detail::v(scatter_);
+        p.g = g;
+        p.alpha = alpha;
+        p.tr_scale = tr_scale_;
+        p.multi_scatter = multiScatter;
+        p.spp_begin = 0;
+        p.spp_count = 1;
+        p.seed = seed_;
+        p.render_type = rt == PT ? MRP_RENDER_PT : MRP_RENDER_MRPNN;
+        p.mlp_precision = precision_;
+        return p;
+    }
+
+    bool load(const std::string& path) {
+        if (FILE* f = std::fopen((path + ".bin").c_str(), "rb")) {
+            bool good = std::fread(&resolution, sizeof(int), 1, f) == 1 && resolution > 0 && resolution <= 1024;
+            if (good) {
+                datas_.assign((size_t)resolution * resolution * resolution, 0.0f);
+                good = std::fread(datas_.data(), sizeof(float), datas_.size(), f) == datas_.size();
+            }
+            std::fclose(f);
+            have_bin_ = good;
+            return good;
+        }
+        const std::string fmt = path.size() >= 3 ? path.substr(path.size() - 3) : "";
+        if (fmt == "txt") {
+            std::ifstream f(path);
+            if (!(f >> resolution) || resolution <= 0 || resolution > 1024) return false;
+            datas_.assign((size_t)resolution * resolution * resolution, 0.0f);
+            for (float& d : datas_)
+                if (!(f >> d)) break;
+            return true;
+        }
+        if (fmt == "vox") {
+            resolution = 256;
+            datas_.assign((size_t)256 * 256 * 256, 0.0f);
+            std::ifstream f(path);
+            std::string line;
+            size_t i = 0;
+            const float inv = 64.0f / 255.0f;
+            while (std::getline(f, line) && i < datas_.size()) {
+                if (line.empty() || line[0] == 'w' || line[0] == 'h' || line[0] == 'd') continue;
+                std::stringstream s(line);
+                int d;
+                for (int j = 0; j < 64 && (s >> d) && i < datas_.size(); ++j) datas_[i++] = (float)d * inv;
+            }
+            return true;
+        }
+        return false;
+    }
+
+    mrp_volume* h_ = nullptr;
+    mrp_network* net_ = nullptr;
+    std::vector<float> datas_;
+    bool have_bin_ = false;
+    bool checkboard_ = true;
+    float3 base_sky_{0.1f, 0.1f, 0.1f}, sky_{0.1f, 0.1f, 0.1f};
+    float env_exp_ = 1.0f;
+    float3 scatter_{1.001f, 1.001f, 1.001f};  // core/render.cuh:16
+    float tr_scale_ = 1.0f;
+    float exposure_ = 1.0f;
+    uint64_t seed_ = 0;
+    int precision_ = 1;
+};
+
+typedef std::function<float3(float3 radiance)> ToneMapFunction;
+
+// `SaveBMP` (core/camera.cu:10-64): 32-bit BGRA, top-down; writes `<filename>.bmp`.
+inline int SaveBMP(unsigned char* image, int w, int h, const char* filename) {
+    unsigned char header[54] = {0x42, 0x4d, 0, 0, 0, 0, 0, 0, 0, 0, 0x36, 0, 0, 0, 0x28, 0, 0, 0,
+                                0,    0,    0, 0, 0, 0, 0, 0, 0x01, 0, 0x20, 0, 0, 0, 0, 0, 0, 0,
+                                0x10, 0,    0, 0, 0, 0, 0, 0, 0,    0, 0,    0, 0, 0, 0, 0, 0, 0};
+    for (size_t b = 0; b < (size_t)w * h * 4; b += 4) std::swap(image[b], image[b + 2]);
+    const long size = (long)w * h * 4 + 54, height = -h;
+    for (int k = 0; k < 4; ++k) {
+        header[2 + k] = (unsigned char)((size >> (8 * k)) & 0xff);
+        header[18 + k] = (unsigned char)(((long)w >> (8 * k)) & 0xff);
+        header[22 + k] = (unsigned char)((height >> (8 * k)) & 0xff);
+    }
+    FILE* fp = std::fopen((std::string(filename) + ".bmp").c_str(), "wb");
+    if (!fp) return -1;
+    std::fwrite(header, 1, 54, fp);
+    std::fwrite(image, 1, (size_t)w * h * 4, fp);
+    std::fclose(fp);
+    return 0;
+}
+
+class Camera {
+   public:
+    int resolution;
+    std::string name;
+
+    static float3 tone(float3 c, int curve) {
+        float3 o;
+        mrp_tone_map_host(&c.x, 1, curve, &o.x);
+        return o;
+    }
+    static float3 None(float3 c) { return c; }
+    static float3 Gamma(float3 c) { return tone(c, 1); }
+    static float3 ACES(float3 c) { return tone(c, 2); }
+
+    // core/camera.cu:163-189: forward = -normalize(ori), right =
+    // normalize(cross(Y, forward)), up = cross(forward, right) (unit: 90 deg).
+    Camera(VolumeRender& volume, std::string name = "test camera", float3 position = {0.75f, 0.75f, 0.75f},
+           int resolution = 512)
+        : resolution(resolution), name(std::move(name)), volume_(&volume) {
+        SetPosition(position);
+    }
+    void SetPosition(float3 position) {
+        ori_ = position;
+        forward_ = detail::normalize(-ori_);
+        right_ = detail::normalize(detail::cross({0, 1, 0}, forward_));
+        up_ = detail::cross(forward_, right_);
+        if (position.x == 0 && position.z == 0) {
+            right_ = {1, 0, 0};
+            up_ = position.y > 0 ? float3{0, 0, 1} : float3{0, 0, -1};
+        }
+    }
+    float3 GetPosition() const { return ori_; }
+    void SetVolume(VolumeRender& v) { volume_ = &v; }
+
+    void Render(float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1, float g = 0,
+                int sampleNum = 1, ToneMapFunction tone = ACES, VolumeRender::RenderType rt = VolumeRender::PT,
+                float exp = 1.0f) {
+        const std::string affix = rt == VolumeRender::PT ? "_PT" : (rt == VolumeRender::RPNN ? "_RPNN" : "MRPNN");
+        RenderToFile("./" + name + affix, lightDir, lightColor, alpha, multiScatter, g, sampleNum, tone, rt, exp);
+    }
+
+    // core/camera.cu:226-252: render, tone(x * exp), saturate, x 255, SaveBMP(path).
+    void RenderToFile(std::string path, float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1,
+                      int multiScatter = 1, float g = 0, int sampleNum = 1, ToneMapFunction tone = ACES,
+                      VolumeRender::RenderType rt = VolumeRender::PT, float exp = 1.0f) {
+        const auto img = volume_->Render(int2{resolution, resolution}, ori_, up_, right_, lightDir, rt, g, alpha,
+                                         lightColor, multiScatter, sampleNum);
+        std::vector<unsigned char> data((size_t)resolution * resolution * 4);
+        auto sat = [](float x) { return x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x); };
+        for (size_t i = 0; i < img.size(); ++i) {
+            const float3 r = tone(img[i] * exp);
+            data[4 * i] = (unsigned char)(sat(r.x) * 255);
+            data[4 * i + 1] = (unsigned char)(sat(r.y) * 255);
+            data[4 * i + 2] = (unsigned char)(sat(r.z) * 255);
+            data[4 * i + 3] = 255;
+        }
+        SaveBMP(data.data(), resolution, resolution, path.c_str());
+    }
+
+    // core/camera.cu:255-257
+    void Render(float3* target, Histogram* histo_buffer, unsigned int* target2, int2 size, int frame, float3 lightDir,
+                float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1, float g = 0, int tone = 2,
+                VolumeRender::RenderType rt = VolumeRender::PT, bool denoise = false) {
+        volume_->Render(target, histo_buffer, target2, size, ori_, up_, right_, lightDir, lightColor, alpha,
+                        multiScatter, g, frame, rt, tone, denoise);
+    }
+
+   private:
+    float3 ori_{}, forward_{}, right_{}, up_{};
+    VolumeRender* volume_;
+};
+
+}  // namespace internship

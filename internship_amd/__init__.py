@@ -177,7 +177,7 @@ class VolumeRender:
         # state of the reference's device-global setters (core/render.cuh:14-16)
         self.scatter_rate = (1.001, 1.001, 1.001)
         self.tr_scale = 1.0
-        self.sky = (0.0, 0.0, 0.0)
+        self.sky = (0.1, 0.1, 0.1)  # constant sky (the reference needs an HDRI, core/render.cuh:57-59)
         self.exposure = 1.0
         self.checkerboard = True  # the reference's default (core/volume.hpp:50)
         self.seed = 0

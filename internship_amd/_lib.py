@@ -95,6 +95,7 @@ SIGNATURES = {
     "mrp_set_sample_buffer_budget": (I, [P, SZ]),
     "mrp_render_frame": (I, [P, ctypes.POINTER(FrameParams), P, P, P, ctypes.POINTER(TraceStats)]),
     "mrp_volume_reset_frames": (I, [P]),
+    "mrp_tone_map_host": (I, [P, SZ, ctypes.c_int, P]),
     "mrp_volume_frame_state": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)]),
     "mrp_network_create": (I, [P, SZ, ctypes.POINTER(P)]),
     "mrp_network_destroy": (I, [P]),

@@ -70,6 +70,23 @@ def build_lib(jobs: int = 4) -> str:
     return LIB
 
 
+FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_render.cpp")
+FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "facade_render")
+
+
+def build_facade_test() -> str:
+    """The C++ facade's test program, linked against libmrp.so (test infrastructure)."""
+    os.makedirs(os.path.dirname(FACADE_BIN), exist_ok=True)
+    gxx = shutil.which("g++") or "g++"
+    lib_dir = os.path.dirname(LIB)
+    rel = os.path.relpath(lib_dir, os.path.dirname(FACADE_BIN))
+    tmp = FACADE_BIN + ".tmp"
+    run([gxx, "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I" + os.path.join(ROOT, "include"), FACADE_SRC,
+         LIB, "-Wl,-rpath,$ORIGIN/" + rel, "-o", tmp])
+    os.replace(tmp, FACADE_BIN)
+    return FACADE_BIN
+
+
 def build_oracle() -> str:
     os.makedirs(os.path.dirname(ORACLE_LIB), exist_ok=True)
     gxx = shutil.which("g++") or "g++"
@@ -89,6 +106,8 @@ def main(argv=None):
         print("built", build_lib(a.j))
     if not a.lib_only:
         print("built", build_oracle())
+    if not a.oracle_only and not a.lib_only:
+        print("built", build_facade_test())
 
 
 if __name__ == "__main__":

@@ -5,6 +5,7 @@
 #include <new>
 #include <string>
 
+#include "mrp_frame.h"
 #include "mrp_internal.h"
 
 namespace mrp {
@@ -91,6 +92,20 @@ int mrp_memset_zero(void* dev, size_t bytes) {
     if (bytes == 0) return MRP_OK;
     if (!dev) return fail(MRP_E_INVALID, "NULL pointer");
     MRP_HIP(hipMemset(dev, 0, bytes));
+    return MRP_OK;
+}
+
+int mrp_tone_map_host(const float* rgb, size_t n, int tone, float* out) {
+    if ((!rgb || !out) && n) return fail(MRP_E_INVALID, "NULL pointer");
+    if (tone < 0 || tone > 2) return fail(MRP_E_INVALID, "tone must be 0 (none), 1 (gamma) or 2 (ACES)");
+    for (size_t i = 0; i < n; ++i) {
+        vec3 c = v3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+        if (tone == 1) c = tone_gamma(c);
+        else if (tone == 2) c = tone_aces(c);
+        out[3 * i] = c.x;
+        out[3 * i + 1] = c.y;
+        out[3 * i + 2] = c.z;
+    }
     return MRP_OK;
 }
 

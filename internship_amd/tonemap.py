@@ -75,7 +75,8 @@ def save_bmp(img: np.ndarray, path: str):
     bgra = to_bgra8(img)
     h, w, _ = bgra.shape
     size = 54 + w * h * 4
-    header = struct.pack("<2sIHHIIiiHHIIiiII", b"BM", size, 0, 0, 54, 40, w, -h, 1, 32, 0, 0, 0x10, 0, 0, 0)
+    # the 54-byte header of `SaveBMP` (core/camera.cu:10-64): 32-bit, top-down (negative height)
+    header = struct.pack("<2sIHHIIiiHHIIiiII", b"BM", size, 0, 0, 54, 40, w, -h, 1, 32, 0, 0x100000, 0, 0, 0, 0)
     with open(path, "wb") as f:
         f.write(header)
         f.write(bgra.tobytes())
