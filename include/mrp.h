@@ -207,6 +207,24 @@ int mrp_tone_map_host(const float* rgb_host, size_t n, int tone, float* out_host
  * into batches. */
 int mrp_set_sample_buffer_budget(mrp_volume* vol, size_t bytes);
 
+/* ------------------------------------------------------ training targets */
+/* One query of `GetSamples` (core/volume.cu:550-626): a scattering point, the
+ * direction the first HG lobe is drawn about, the unit light direction, and the
+ * medium (extinction scale, HG g, albedo). 48 bytes. */
+typedef struct mrp_sample_query {
+    mrp_vec3 pos;
+    mrp_vec3 dir;
+    mrp_vec3 light_dir;
+    float alpha, g, scatter;
+} mrp_sample_query;
+/* `GetSample` (core/render.cuh:276-315) for n queries in HBM: the mean over
+ * spp paths of the multiple-scattering radiance reaching the point (next-event
+ * light at every vertex, albedo 1.001*scatter per bounce, sky on escape), the
+ * high-order part the network is trained to predict.  out_dev: n x 3 FP32.
+ * Path k of query j draws from Philox stream (j, k, tag 1). */
+int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q_dev, mrp_vec3 light_color, mrp_vec3 sky,
+                    int multi_scatter, int spp, uint64_t seed, float* out_dev, mrp_trace_stats* stats);
+
 /* ---------------------------------------------------------------- network */
 /* Trained MRPNN weights: 49,673 FP32 values, the 146 arrays of
  * core/NNWeight.cuh concatenated in header order (row-major [out][in]).

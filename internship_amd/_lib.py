@@ -49,6 +49,11 @@ class FrameParams(ctypes.Structure):
     ]
 
 
+class SampleQuery(ctypes.Structure):
+    _fields_ = [("pos", Vec3), ("dir", Vec3), ("light_dir", Vec3), ("alpha", ctypes.c_float), ("g", ctypes.c_float),
+                ("scatter", ctypes.c_float)]
+
+
 class TraceStats(ctypes.Structure):
     _fields_ = [
         ("lookups", ctypes.c_uint64), ("paths", ctypes.c_uint64), ("scatters", ctypes.c_uint64),
@@ -96,6 +101,8 @@ SIGNATURES = {
     "mrp_render_frame": (I, [P, ctypes.POINTER(FrameParams), P, P, P, ctypes.POINTER(TraceStats)]),
     "mrp_volume_reset_frames": (I, [P]),
     "mrp_tone_map_host": (I, [P, SZ, ctypes.c_int, P]),
+    "mrp_get_samples": (I, [P, ctypes.c_int, P, Vec3, Vec3, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, P,
+                            ctypes.POINTER(TraceStats)]),
     "mrp_volume_frame_state": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)]),
     "mrp_network_create": (I, [P, SZ, ctypes.POINTER(P)]),
     "mrp_network_destroy": (I, [P]),

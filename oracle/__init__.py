@@ -259,3 +259,18 @@ def reproject(points, last_ori, last_up, last_right, w, h):
     load().oracle_reproject(_ptr(pts), ctypes.c_int(pts.shape[0]), Vec3.of(last_ori), Vec3.of(last_up),
                             Vec3.of(last_right), ctypes.c_int(w), ctypes.c_int(h), _ptr(out))
     return out
+
+
+def get_samples(grid, max_dens, queries, light_color, sky, multi, spp, seed=0, threads=0):
+    """`GetSample` targets on the CPU. queries: array of internship_amd._lib.SampleQuery.
+    Returns (out (n,3) FP32, lookups)."""
+    from internship_amd._lib import Vec3
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    n = len(queries)
+    out = np.zeros((n, 3), dtype=np.float32)
+    lk = ctypes.c_uint64()
+    rc = load().oracle_get_samples(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.c_float(max_dens), ctypes.c_int(n),
+                                   queries, Vec3.of(light_color), Vec3.of(sky), ctypes.c_int(multi), ctypes.c_int(spp),
+                                   ctypes.c_uint64(seed), _ptr(out), ctypes.byref(lk), ctypes.c_int(threads))
+    assert rc == 0
+    return out, lk.value

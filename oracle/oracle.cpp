@@ -1088,4 +1088,63 @@ int oracle_reproject(const float* pts, int n, mrp_vec3 lori, mrp_vec3 lup, mrp_v
     return 0;
 }
 
+// Training targets, `GetSample` (core/render.cuh:276-315), for n queries:
+// per path, an HG direction about q.dir, then up to multi bounces of delta
+// tracking with next-event light (albedo 1.001*scatter per bounce) and the sky
+// on escape; the paths of a query are added in sample order and divided by spp.
+// Path k of query j draws from Philox stream (j, k, tag 1).
+int oracle_get_samples(const float* grid, int res, float max_density, int n, const mrp_sample_query* q,
+                       mrp_vec3 light_color, mrp_vec3 sky, int multi, int spp, uint64_t seed, float* out,
+                       uint64_t* lookups_out, int threads) {
+    uint64_t total = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total)
+#endif
+    for (int j = 0; j < n; ++j) {
+        Medium m;
+        m.grid = OGrid{grid, res};
+        m.max_density = max_density;
+        m.alpha = q[j].alpha;
+        m.g = q[j].g;
+        m.tr_scale = 1.0f;
+        m.sky = V(sky);
+        m.scatter_rate = v3(q[j].scatter, q[j].scatter, q[j].scatter);
+        m.light_color = V(light_color);
+        m.mj = make_majorant(max_density, q[j].alpha, 1.0f);
+        const vec3 light = V(q[j].light_dir);
+        Counters c;
+        vec3 sum = v3(0, 0, 0);
+        for (int k = 0; k < spp; ++k) {
+            Rng rng = rng_make((uint32_t)j, (uint32_t)k, 1u, seed);
+            vec3 pos = V(q[j].pos);
+            const float e0 = rng_next(&rng);
+            const float e1 = rng_next(&rng);
+            vec3 ray = hg_sample_dir(e0, e1, V(q[j].dir), m.g);
+            vec3 csr = v3(1, 1, 1), res_k = v3(0, 0, 0);
+            for (int s = 0; s < multi; ++s) {
+                vec3 np, nd;
+                if (!next_vertex(&rng, m, m.g, pos, ray, ray_box_distance(pos, ray), &np, &nd, &c)) {
+                    res_k = res_k + m.sky * csr;
+                    break;
+                }
+                pos = np;
+                csr = (csr * 1.001f) * q[j].scatter;
+                const float phase = hg_phase(dot(ray, light), m.g);
+                const float tr = transmittance(&rng, m, pos, light, ray_box_distance(pos, light), &c);
+                res_k = res_k + m.light_color * csr * (tr * phase);
+                ray = nd;
+            }
+            sum = sum + res_k;
+        }
+        const float d = (float)spp;
+        out[3 * j] = sum.x / d;
+        out[3 * j + 1] = sum.y / d;
+        out[3 * j + 2] = sum.z / d;
+        total += c.lookups;
+    }
+    if (lookups_out) *lookups_out = total;
+    return 0;
+}
+
 }  // extern "C"

@@ -19,6 +19,9 @@
  *                                                                    running mean = oracle_render
  * oracle_tone_pack         core/render.cuh:640-732, volume.cu:248-263 Gamma / ACES closed forms (numpy)
  * oracle_histo_distance    core/volume.cu:187-207                    chi2(h,h) = 0, scale invariance, hand values
+ * oracle_get_samples       core/render.cuh:276-315                   transparent medium = sky; albedo 0 =
+ *                                                                    HG-averaged escape probability;
+ *                                                                    single scattering = FP64 quadrature
  * oracle_reproject         core/volume.cu:125-141                    static camera maps to itself; pinhole
  *                                                                    projection into a moved camera (numpy)
  * Later sections of oracle.cpp list their own pins.
@@ -66,6 +69,9 @@ int oracle_render_frame(const float* grid, int res, float max_density, const flo
 int oracle_denoise_pack(const float* target, mrp_histogram* histo, int W, int H, int frame, int checker,
                         int checkerboard, int denoise, int predict, int tone, float exposure, uint32_t* bgra);
 int oracle_tone_pack(const float* rgb, int n, float exposure, int tone, uint32_t* out);
+int oracle_get_samples(const float* grid, int res, float max_density, int n, const mrp_sample_query* q,
+                       mrp_vec3 light_color, mrp_vec3 sky, int multi, int spp, uint64_t seed, float* out,
+                       uint64_t* lookups_out, int threads);
 int oracle_reproject(const float* pts, int n, mrp_vec3 lori, mrp_vec3 lup, mrp_vec3 lright, int W, int H, int* out);
 int oracle_histo_distance(const mrp_histogram* a, const mrp_histogram* b, float* dist, float* weight);
 

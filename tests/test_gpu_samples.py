@@ -1,0 +1,65 @@
+"""GPU parity of the training targets (`mrp_get_samples`, `GetSample`) with the
+CPU oracle on the same Philox streams, including sample chunks that carry the
+running sums across launches."""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import internship_amd as ia
+import oracle
+from internship_amd import DeviceBuffer, TraceStats, VolumeRender
+from internship_amd._lib import SampleQuery, Vec3
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _random_queries(n, seed):
+    rng = np.random.default_rng(seed)
+    q = (SampleQuery * n)()
+    for i in range(n):
+        p = rng.uniform(-0.3, 0.3, 3)
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        L = rng.normal(size=3)
+        L[1] = abs(L[1]) + 0.5
+        L /= np.linalg.norm(L)
+        q[i] = SampleQuery(Vec3.of(p), Vec3.of(d), Vec3.of(L), float(rng.uniform(16, 128)),
+                           float(rng.uniform(0.7, 0.9)), float(rng.uniform(0.9, 0.999)))
+    return q
+
+
+# one chunk; with a 1 MiB budget, chunks of 2184 samples (the second carries the sums over)
+@pytest.mark.parametrize("spp,budget", [(16, None), (2200, 1 << 20)])
+def test_gpu_get_samples_match_oracle(spp, budget):
+    grid = oracle.generate_cloud(32, seed=5)
+    n, multi = 40, 24
+    _say("case: GetSample %d queries x %d spp, %d bounces, budget %s" % (n, spp, multi, budget))
+    vol = VolumeRender(32)
+    try:
+        vol.set_datas(grid)
+        vol.update()
+        if budget is not None:
+            vol.set_sample_buffer_budget(budget)
+        q = _random_queries(n, 11)
+        qb = DeviceBuffer(ctypes.sizeof(q))
+        ia.check(vol.lib.mrp_memcpy_h2d(qb.ptr, ctypes.cast(q, ctypes.c_void_p), ctypes.c_size_t(ctypes.sizeof(q))))
+        out = DeviceBuffer(n * 12)
+        st = TraceStats()
+        ia.check(vol.lib.mrp_get_samples(vol.h, n, qb.ptr, Vec3.of((1, 1, 1)), Vec3.of((0.1, 0.1, 0.1)), multi, spp,
+                                         ctypes.c_uint64(3), out.ptr, ctypes.byref(st)))
+        g = out.download(np.float32, (n, 3))
+        c, lk = oracle.get_samples(grid, vol.max_density, q, (1, 1, 1), (0.1, 0.1, 0.1), multi, spp, seed=3,
+                                  threads=16)
+        _say("exact %.3f, lookups %d vs %d" % (np.mean(g == c), st.lookups, lk))
+        assert st.lookups == lk
+        assert np.array_equal(g, c)
+        assert g.max() > 0.01
+    finally:
+        vol.close()
