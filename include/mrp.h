@@ -252,6 +252,32 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
 int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
                     mrp_vec3 srp, int precision, float* out_dev);
 
+/* --------------------------------------------------------------- training */
+/* Training of the MRPNN network on the GPU (forward, backward, Adam), derived
+ * from the inference network (core/radiancePredict.cu:370-486); the
+ * reference trains with Adam, lr 1e-3 (train/train.py:536-546).  A sample is
+ * the gather's 576 features (feat_dev, batch x 576 FP32) plus aux_dev
+ * (batch x float4: gamma = acos(X . LX), g, srp = (scatter/1.001)^4, target).
+ * The prediction is p = max(exp(P)-1, 0) * srp for one channel; the loss is
+ * the batch mean of (p - target)^2 / (target^2 + 0.01), reduced in FP64.
+ * Weight gradients are FP64; master weights and Adam moments are FP32. */
+typedef struct mrp_trainer mrp_trainer;
+int mrp_trainer_create(const float* weights_host, size_t count, int max_batch, mrp_trainer** out);
+int mrp_trainer_destroy(mrp_trainer* t);
+/* Floats of per-sample activation record (HBM: record_floats * max_batch * 4 bytes). */
+int mrp_trainer_record_floats(const mrp_trainer* t, int* out);
+/* Loss and gradient of the current weights on a batch (no update); grad_host
+ * (49,673 doubles) may be NULL. */
+int mrp_trainer_gradient(mrp_trainer* t, int batch, const float* feat_dev, const float* aux_dev, double* grad_host,
+                         double* loss_out);
+/* Gradient + one Adam step (beta1 0.9, beta2 0.999, eps 1e-8). loss_out: the
+ * batch loss before the step. */
+int mrp_trainer_step(mrp_trainer* t, int batch, const float* feat_dev, const float* aux_dev, float lr,
+                     double* loss_out);
+int mrp_trainer_get_weights(const mrp_trainer* t, float* host_out);
+/* Predictions p of the last gradient/step call (batch floats). */
+int mrp_trainer_predictions(const mrp_trainer* t, int batch, float* host_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -157,6 +157,52 @@ class Network:
             pass
 
 
+class Trainer:
+    """GPU training of the MRPNN weights (mrp_trainer_*): FP32 forward and
+    backward, FP64 weight gradients, Adam on FP32 master weights.  A batch is
+    feat (B x 576 FP32 in HBM, the gather's layout) and aux (B x 4 FP32:
+    gamma, g, srp, target)."""
+
+    def __init__(self, weights: np.ndarray = None, max_batch: int = 16384):
+        self.lib = load()
+        w = load_reference_weights() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
+        self.h = ctypes.c_void_p()
+        self.max_batch = int(max_batch)
+        check(self.lib.mrp_trainer_create(_ptr(w), ctypes.c_size_t(w.size), self.max_batch, ctypes.byref(self.h)))
+
+    def gradient(self, feat: "DeviceBuffer", aux: "DeviceBuffer", batch: int):
+        g = np.zeros(49673, dtype=np.float64)
+        loss = ctypes.c_double()
+        check(self.lib.mrp_trainer_gradient(self.h, int(batch), feat.ptr, aux.ptr, _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def step(self, feat: "DeviceBuffer", aux: "DeviceBuffer", batch: int, lr: float = 1e-3) -> float:
+        loss = ctypes.c_double()
+        check(self.lib.mrp_trainer_step(self.h, int(batch), feat.ptr, aux.ptr, float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def weights(self) -> np.ndarray:
+        out = np.zeros(49673, dtype=np.float32)
+        check(self.lib.mrp_trainer_get_weights(self.h, _ptr(out)))
+        return out
+
+    def predictions(self, batch: int) -> np.ndarray:
+        out = np.zeros(int(batch), dtype=np.float32)
+        check(self.lib.mrp_trainer_predictions(self.h, int(batch), _ptr(out)))
+        return out
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.mrp_trainer_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def srp_of(scatter_rate):
     """(scatter_rate / 1.001)^4 per channel, as `RadiancePredict` forms it."""
     sr = np.asarray(scatter_rate, dtype=np.float32) / np.float32(1.001)

@@ -101,6 +101,13 @@ SIGNATURES = {
     "mrp_render_frame": (I, [P, ctypes.POINTER(FrameParams), P, P, P, ctypes.POINTER(TraceStats)]),
     "mrp_volume_reset_frames": (I, [P]),
     "mrp_tone_map_host": (I, [P, SZ, ctypes.c_int, P]),
+    "mrp_trainer_create": (I, [P, SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "mrp_trainer_destroy": (I, [P]),
+    "mrp_trainer_record_floats": (I, [P, ctypes.POINTER(ctypes.c_int)]),
+    "mrp_trainer_gradient": (I, [P, ctypes.c_int, P, P, P, ctypes.POINTER(ctypes.c_double)]),
+    "mrp_trainer_step": (I, [P, ctypes.c_int, P, P, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]),
+    "mrp_trainer_get_weights": (I, [P, P]),
+    "mrp_trainer_predictions": (I, [P, ctypes.c_int, P]),
     "mrp_get_samples": (I, [P, ctypes.c_int, P, Vec3, Vec3, ctypes.c_int, ctypes.c_int, ctypes.c_uint64, P,
                             ctypes.POINTER(TraceStats)]),
     "mrp_volume_frame_state": (I, [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint32)]),

@@ -189,9 +189,11 @@ MRP_HD void mlp_forward_fp32(const float* w, const float* feat, float gamma, flo
     for (int c = 0; c < 3; ++c) out[c] = fmax_(expf_(P[c]) - 1.0f, 0.0f) * sr[c];
 }
 
+#if defined(__HIPCC__)
 // BF16 MFMA path (mrp_mlp_bf16.hip)
 int pack_mlp_bf16(const float* weights_host, void** packed_dev);
 int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, const float* gamma, float g, vec3 srp,
                     uint32_t n_items, float* pred, hipStream_t stream);
+#endif
 
 }  // namespace mrp

@@ -274,3 +274,19 @@ def get_samples(grid, max_dens, queries, light_color, sky, multi, spp, seed=0, t
                                    ctypes.c_uint64(seed), _ptr(out), ctypes.byref(lk), ctypes.c_int(threads))
     assert rc == 0
     return out, lk.value
+
+
+def mrpnn_grad(weights, feat, aux):
+    """FP64 training loss and gradient: feat (n,576), aux (n,4) = (gamma, g, srp, target).
+    Returns (grad (49673,) float64, mean loss, predictions (n,))."""
+    w = np.ascontiguousarray(weights, dtype=np.float32)
+    feat = np.ascontiguousarray(feat, dtype=np.float32)
+    aux = np.ascontiguousarray(aux, dtype=np.float32)
+    n = feat.shape[0]
+    grad = np.zeros(w.size, dtype=np.float64)
+    pred = np.zeros(n, dtype=np.float64)
+    loss = ctypes.c_double()
+    rc = load().oracle_mrpnn_grad(_ptr(w), ctypes.c_int(n), _ptr(feat), _ptr(aux), _ptr(grad), ctypes.byref(loss),
+                                  _ptr(pred))
+    assert rc == 0
+    return grad, loss.value, pred

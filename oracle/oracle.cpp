@@ -27,6 +27,7 @@
 #include "../internship_amd/csrc/mrp_medium.h"
 #include "../internship_amd/csrc/mrp_mrpnn.h"
 #include "../internship_amd/csrc/mrp_procedural.h"
+#include "../internship_amd/csrc/mrp_train.h"
 #include "../internship_amd/csrc/mrp_weights_gen.h"
 
 #ifdef _OPENMP
@@ -1144,6 +1145,44 @@ int oracle_get_samples(const float* grid, int res, float max_density, int n, con
         total += c.lookups;
     }
     if (lookups_out) *lookups_out = total;
+    return 0;
+}
+
+namespace {
+struct HostRec {
+    std::vector<double> v;
+    double get(int f) const { return v[(size_t)f]; }
+    void put(int f, double x) { v[(size_t)f] = x; }
+};
+}  // namespace
+
+// Loss and gradient of the training objective in FP64 (mrp_train.h with
+// T = double): n samples, feat n x 576, aux n x 4 (gamma, g, srp, target).
+// grad (49,673) = d mean-loss / d w, accumulated sample by sample in order.
+int oracle_mrpnn_grad(const float* w, int n, const float* feat, const float* aux, double* grad, double* loss_mean,
+                      double* pred) {
+    if (!w || !feat || !aux || !grad || n <= 0) return -1;
+    TrainLayer L[kTrainLayers];
+    const int R = build_train_layers(L);
+    for (int i = 0; i < MRP_W_TOTAL; ++i) grad[i] = 0.0;
+    double loss = 0.0;
+    HostRec r;
+    r.v.assign((size_t)R, 0.0);
+    for (int s = 0; s < n; ++s) {
+        double p = 0.0;
+        const float* a = aux + 4 * (size_t)s;
+        loss += train_sample<double>(w, L, feat + (size_t)s * kFeatures, a[0], a[1], a[2], a[3], 1.0 / (double)n, r, &p);
+        if (pred) pred[s] = p;
+        for (int li = 0; li < kTrainLayers; ++li) {
+            const TrainLayer& l = L[li];
+            for (int i = 0; i < l.out; ++i) {
+                const double dz = r.v[(size_t)(l.dz_off + i)];
+                for (int j = 0; j < l.in; ++j) grad[l.w_off + i * l.in + j] += dz * r.v[(size_t)(l.x_off + j)];
+                if (l.b_off >= 0) grad[l.b_off + i] += dz;
+            }
+        }
+    }
+    if (loss_mean) *loss_mean = loss / (double)n;
     return 0;
 }
 
