@@ -82,4 +82,4 @@ def test_gpu_adam_step_and_training_lowers_the_loss():
     finally:
         tr.close()
     _say("loss %.5g -> %.5g" % (losses[0], losses[-1]))
-    assert losses[-1] < 0.5 * losses[0]
+    assert losses[-1] < 0.6 * losses[0]
