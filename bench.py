@@ -14,11 +14,12 @@ ground truth with FP64 per-pixel sums; the smallest delta-tracking spp N with
 RMSE(DT_N) <= RMSE(MLP) (powers of two, then bisection); speed-up =
 fps(MLP) / fps(DT_N), where fps(DT_N) is measured on one N-spp launch.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer|frame]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer|frame|train]
                   [--no-cpu-baseline] [--truth-spp 4096]
 
 Mode mlp: only the timed MLP-renderer frames of the headline (for profiling).
 Mode frame: progressive path-traced frames with histograms, denoiser and tone map.
+Mode train: GPU training data (descriptors + GetSample targets) and Adam steps, with an A/B test.
 Mode tracer: frames/s of the plain tracer (1 spp per frame, --spp-per-step
 frames per launch) with its HBM roofline.  Prints one JSON line on stdout;
 progress goes to stderr.
@@ -39,6 +40,7 @@ if ROOT not in sys.path:
 METRIC = "1080p frames/s, BF16 MLP vs plain delta tracking at equal RMSE; HBM/MFMA %"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_LOOKUP = 32          # eight FP32 corners of one trilinear density lookup
 MLP_FLOP_PER_PASS = 97072      # 48,536 MACs per descriptor per colour pass
 GATHER_BYTES_PER_POINT = 18400  # 192 x 32 (density) + 192 x 32 (transmittance) + 382 x 16 (HG table)
@@ -339,6 +341,74 @@ def run_frame(a, ia, sc):
     }
 
 
+def run_train(a, ia, sc):
+    """The learned part: GPU data generation (descriptors + GetSample targets)
+    on config-4 style procedural clouds, then K timed Adam steps of batch
+    --batch on the GPU trainer, starting from the shipped weights; A/B on
+    held-out clouds: relative L2 of the shipped vs the trained weights."""
+    from internship_amd import datagen
+    n_train, n_hold = a.train_clouds, 2
+    specs = datagen.random_specs(n_train + n_hold, seed=100)
+    t0 = time.perf_counter()
+    feat, aux, gst = datagen.make_dataset(specs[:n_train], res=128, n_per_cloud=a.points_per_cloud, spp=a.target_spp,
+                                          multi=a.target_bounces, seed=1, log=log)
+    hfeat, haux, _ = datagen.make_dataset(specs[n_train:], res=128, n_per_cloud=a.points_per_cloud,
+                                          spp=a.target_spp, multi=a.target_bounces, seed=2, log=log)
+    gen_s = time.perf_counter() - t0
+    log("data: %d training + %d held-out samples in %.1f s (targets %.1f s, %.3g paths/s)"
+        % (len(feat), len(hfeat), gen_s, gst["target_s"], gst["paths"] / max(gst["target_s"], 1e-9)))
+    perm = np.random.default_rng(0).permutation(len(feat))
+    feat, aux = np.ascontiguousarray(feat[perm]), np.ascontiguousarray(aux[perm])
+    fb = ia.DeviceBuffer(feat.nbytes)
+    fb.upload(feat)
+    ab = ia.DeviceBuffer(aux.nbytes)
+    ab.upload(aux)
+    B = min(a.batch, len(feat))
+    w0 = ia.load_reference_weights()
+    tr = ia.Trainer(w0, max_batch=B)
+
+    def batch_views(i):
+        off = (i * B) % (len(feat) - B + 1)
+        return datagen.DeviceView(fb, off * 576 * 4), datagen.DeviceView(ab, off * 16)
+
+    losses = []
+    for i in range(a.warmup):
+        f, x = batch_views(i)
+        losses.append(tr.step(f, x, B, a.lr))
+    t0 = time.perf_counter()
+    for i in range(a.warmup, a.warmup + a.steps):
+        f, x = batch_views(i)
+        losses.append(tr.step(f, x, B, a.lr))
+    elapsed = time.perf_counter() - t0
+    w1 = tr.weights()
+    tr.close()
+    sps = a.steps * B / elapsed
+    flops = 3 * MLP_FLOP_PER_PASS  # forward + backward (dX and dW) per sample
+    achieved = sps * flops / 1e12
+    ref_pred = datagen.predict(w0, hfeat, haux)
+    new_pred = datagen.predict(w1, hfeat, haux)
+    ab_res = {"heldout_samples": len(hfeat),
+              "rel_l2_shipped_weights": datagen.relative_l2(ref_pred, haux[:, 3]),
+              "rel_l2_trained_weights": datagen.relative_l2(new_pred, haux[:, 3]),
+              "train_loss_first": losses[0], "train_loss_last": losses[-1]}
+    log("A/B held-out rel L2: shipped %.4g, trained %.4g; loss %.4g -> %.4g"
+        % (ab_res["rel_l2_shipped_weights"], ab_res["rel_l2_trained_weights"], losses[0], losses[-1]))
+    return {
+        "metric": "MRPNN training samples/s (forward + backward + Adam, GPU)", "unit": "samples/s",
+        "value": sps, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "f32",
+        "config": {"workload": "MRPNN training, batch %d, %d procedural 128^3 clouds x %d points, GetSample targets "
+                               "%d spp / %d bounces" % (B, n_train, a.points_per_cloud, a.target_spp,
+                                                         a.target_bounces),
+                   "batch": B, "lr": a.lr},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "train_fwd_bwd_kernel + wgrad",
+                     "flop_per_sample": flops},
+        "datagen": {"seconds": gen_s, "target_seconds": gst["target_s"], "paths": gst["paths"],
+                    "target_lookups": gst["target_lookups"]},
+        "ab_test": ab_res,
+    }
+
+
 def run_headline(a, ia, sc):
     net = ia.Network()
     sc.vol.set_network(net)
@@ -386,7 +456,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["headline", "mlp", "tracer", "frame"], default="headline")
+    ap.add_argument("--mode", choices=["headline", "mlp", "tracer", "frame", "train"], default="headline")
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -395,6 +465,12 @@ def main(argv=None):
     ap.add_argument("--dt-max-spp", type=int, default=4096)
     ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
+    ap.add_argument("--lr", type=float, default=1e-4, help="train mode: Adam learning rate")
+    ap.add_argument("--train-clouds", type=int, default=16)
+    ap.add_argument("--points-per-cloud", type=int, default=8192)
+    ap.add_argument("--target-spp", type=int, default=32)
+    ap.add_argument("--target-bounces", type=int, default=64)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (reference kernel for HBM counters)")
     a = ap.parse_args(argv)
@@ -413,14 +489,16 @@ def main(argv=None):
         log("calibration copy of %d MiB done" % a.copy_calibration_mib)
         src.free()
         dst.free()
-    sc = Scene(ia, a.res, a.width, a.height, a.bounces)
-    res = {"headline": run_headline, "mlp": run_mlp, "tracer": run_tracer, "frame": run_frame}[a.mode](a, ia, sc)
+    sc = Scene(ia, a.res, a.width, a.height, a.bounces) if a.mode != "train" else None
+    res = {"headline": run_headline, "mlp": run_mlp, "tracer": run_tracer, "frame": run_frame,
+           "train": run_train}[a.mode](a, ia, sc)
     base = None
-    if not a.no_cpu_baseline:
+    if not a.no_cpu_baseline and a.mode != "train":
         log("timing the CPU oracle on a bounded sample")
         base = cpu_baseline(sc)
         log("cpu baseline %.4g frames/s (%s)" % (base["value"], base["sample"]))
-    out = {"metric": METRIC, "value": res.pop("value"), "unit": "frames/s", "n_gpus": 1, "steps": a.steps,
+    out = {"metric": res.pop("metric", METRIC), "value": res.pop("value"), "unit": res.pop("unit", "frames/s"),
+           "n_gpus": 1, "steps": a.steps,
            "warmup": a.warmup, "ms_per_step": res.pop("ms_per_step"), "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": res.pop("dtype"), "data": "synthetic"}
     out.update(res)
