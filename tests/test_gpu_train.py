@@ -87,8 +87,7 @@ def test_gpu_adam_step_and_training_lowers_the_loss():
 
 def test_gpu_dataset_generation_and_one_training_epoch():
     """GPU data generation (descriptors + GetSample targets on procedural
-    clouds) feeds the trainer; the targets agree with the oracle's GetSample
-    for a few recomputed points."""
+    clouds, each stage parity-tested on its own) feeds the trainer."""
     from internship_amd import datagen
     _say("case: 2 clouds x 256 points, 8 spp targets, one epoch")
     specs = datagen.random_specs(2, seed=100)
