@@ -14,12 +14,13 @@ ground truth with FP64 per-pixel sums; the smallest delta-tracking spp N with
 RMSE(DT_N) <= RMSE(MLP) (powers of two, then bisection); speed-up =
 fps(MLP) / fps(DT_N), where fps(DT_N) is measured on one N-spp launch.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer|frame|train]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode headline|mlp|tracer|frame|train|serve]
                   [--no-cpu-baseline] [--truth-spp 4096]
 
 Mode mlp: only the timed MLP-renderer frames of the headline (for profiling).
 Mode frame: progressive path-traced frames with histograms, denoiser and tone map.
 Mode train: GPU training data (descriptors + GetSample targets) and Adam steps, with an A/B test.
+Mode serve: 16 scenes x 4 cameras at 720p served by concurrent host threads; p50/p99 latency.
 Mode tracer: frames/s of the plain tracer (1 spp per frame, --spp-per-step
 frames per launch) with its HBM roofline.  Prints one JSON line on stdout;
 progress goes to stderr.
@@ -409,6 +410,84 @@ def run_train(a, ia, sc):
     }
 
 
+def run_serve(a, ia, sc):
+    """Batched serving: 16 Perlin-fBm clouds of 128..512^3 (seeds 200-215) x 4
+    orbit cameras at 1280x720, one 1-spp BF16 MRPNN frame per request, served
+    by --workers host threads, each owning a set of scenes (own volume, HIP
+    stream and buffers), so requests of different scenes overlap on the GPU."""
+    import concurrent.futures as cf
+    W, H = 1280, 720
+    sizes = [128, 256, 384, 512]
+    net = ia.Network()
+    scenes = []
+    t0 = time.perf_counter()
+    for i in range(16):
+        res = sizes[i % 4]
+        grid = ia.DeviceBuffer(res ** 3 * 4)
+        ia.generate_cloud(res, seed=200 + i, device_out=grid)
+        vol = ia.VolumeRender(res)
+        vol.set_datas(grid)
+        vol.update()
+        grid.free()
+        vol.set_network(net)
+        tgt = ia.DeviceBuffer(W * H * 12)
+        tgt.zero()
+        scenes.append((vol, tgt))
+    log("16 scenes prepared in %.1f s" % (time.perf_counter() - t0))
+    light = np.array([0.0, 1.0, 0.3]) / np.linalg.norm([0.0, 1.0, 0.3])
+    cams = []
+    for k in range(4):
+        az = np.radians(45.0 + 90.0 * k)
+        d = np.array([np.cos(az) * np.cos(np.radians(25)), np.sin(np.radians(25)), np.sin(az) * np.cos(np.radians(25))])
+        ori = 2.5 * d
+        right, up = ia.camera_basis(ori, 40.0, W / H)
+        cams.append((ori, up, right))
+
+    def request(si, ci, frame):
+        vol, tgt = scenes[si]
+        ori, up, right = cams[ci]
+        V = ia.Vec3.of
+        p = ia.RenderParams(W, H, V(ori), V(up), V(right), V(light), V((1, 1, 1)), V((0.1, 0.1, 0.1)),
+                            V((0.999, 0.999, 0.999)), 0.857, 256.0, 1.0, 512, frame, 1, 0, ia.MRPNN, 1)
+        t = time.perf_counter()
+        vol.render_dev(p, tgt)
+        return time.perf_counter() - t
+
+    def worker(wi, rounds):
+        lat = []
+        mine = [si for si in range(16) if si % a.workers == wi]
+        for r in range(rounds):
+            for si in mine:
+                for ci in range(4):
+                    lat.append(request(si, ci, r))
+        return lat
+
+    with cf.ThreadPoolExecutor(a.workers) as ex:  # warm-up: per-light tables, scratch
+        list(ex.map(lambda wi: worker(wi, 1), range(a.workers)))
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(a.workers) as ex:
+        lats = [x for l in ex.map(lambda wi: worker(wi, a.steps), range(a.workers)) for x in l]
+    elapsed = time.perf_counter() - t0
+    net.close()
+    for vol, tgt in scenes:
+        vol.close()
+        tgt.free()
+    lat_ms = np.array(lats) * 1000.0
+    n_req = len(lats)
+    log("serving: %d requests in %.2f s: %.1f frames/s, latency p50 %.2f ms, p99 %.2f ms"
+        % (n_req, elapsed, n_req / elapsed, np.percentile(lat_ms, 50), np.percentile(lat_ms, 99)))
+    return {
+        "metric": "720p MRPNN frames/s over 16 scenes x 4 cameras (batched serving)", "unit": "frames/s",
+        "value": n_req / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
+        "config": {"workload": "16 Perlin-fBm clouds (128..512^3, seeds 200-215) x 4 orbit cameras, 1280x720, "
+                               "1-spp BF16 MRPNN frames, %d host workers" % a.workers,
+                   "scenes": 16, "cameras": 4, "width": W, "height": H, "workers": a.workers},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p90": float(np.percentile(lat_ms, 90)),
+                       "p99": float(np.percentile(lat_ms, 99)), "max": float(lat_ms.max())},
+        "requests": n_req,
+    }
+
+
 def run_headline(a, ia, sc):
     net = ia.Network()
     sc.vol.set_network(net)
@@ -456,7 +535,7 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--mode", choices=["headline", "mlp", "tracer", "frame", "train"], default="headline")
+    ap.add_argument("--mode", choices=["headline", "mlp", "tracer", "frame", "train", "serve"], default="headline")
     ap.add_argument("--res", type=int, default=512)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
@@ -466,6 +545,7 @@ def main(argv=None):
     ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
+    ap.add_argument("--workers", type=int, default=4, help="serve mode: host threads")
     ap.add_argument("--lr", type=float, default=1e-4, help="train mode: Adam learning rate")
     ap.add_argument("--train-clouds", type=int, default=16)
     ap.add_argument("--points-per-cloud", type=int, default=8192)
@@ -489,11 +569,11 @@ def main(argv=None):
         log("calibration copy of %d MiB done" % a.copy_calibration_mib)
         src.free()
         dst.free()
-    sc = Scene(ia, a.res, a.width, a.height, a.bounces) if a.mode != "train" else None
+    sc = Scene(ia, a.res, a.width, a.height, a.bounces) if a.mode not in ("train", "serve") else None
     res = {"headline": run_headline, "mlp": run_mlp, "tracer": run_tracer, "frame": run_frame,
-           "train": run_train}[a.mode](a, ia, sc)
+           "train": run_train, "serve": run_serve}[a.mode](a, ia, sc)
     base = None
-    if not a.no_cpu_baseline and a.mode != "train":
+    if not a.no_cpu_baseline and a.mode not in ("train", "serve"):
         log("timing the CPU oracle on a bounded sample")
         base = cpu_baseline(sc)
         log("cpu baseline %.4g frames/s (%s)" % (base["value"], base["sample"]))

@@ -163,6 +163,7 @@ int mrp_volume_destroy(mrp_volume* h) {
     hipFree(v->queues);
     hipFree(v->counters);
     hipFree(v->samples);
+    hipFree(v->nn_scratch);
     if (v->stream) hipStreamDestroy(v->stream);
     delete h;
     return MRP_OK;

@@ -52,6 +52,8 @@ struct Volume {
     void* samples = nullptr;                 // per-(sample, pixel) float4 results
     size_t samples_bytes = 0;
     size_t sample_budget = (size_t)4 << 30;
+    void* nn_scratch = nullptr;              // MRPNN stages: vertices, features, pool, predictions
+    size_t nn_scratch_bytes = 0;
     // progressive-frame state kept between mrp_render_frame calls, like the
     // reference's globals `flip`, `rand_cpu`, `lori/lup/lright` and
     // `last_predict` (core/volume.cu:50-56,404-407,469-547)

@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -294,8 +295,10 @@ __global__ void clear_counters_kernel(unsigned long long* p, int n) {
 static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
 
 static bool g_tables_ready[64] = {};
+static std::mutex g_tables_mutex;
 
 static int upload_tap_tables(int device) {
+    std::lock_guard<std::mutex> lock(g_tables_mutex);  // volumes may render from several threads
     if (device >= 0 && device < 64 && g_tables_ready[device]) return MRP_OK;
     TapTable t;
     descriptor_schedule(&t);
@@ -311,25 +314,20 @@ static int upload_tap_tables(int device) {
 // records, 576 FP32 features, gamma, 3 predictions.
 constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12;
 
-struct NNScratch {
-    void* base = nullptr;
-    size_t bytes = 0;
-};
-static NNScratch g_nn_scratch[64];
-
-static int ensure_nn_scratch(int device, size_t bytes, void** out) {
-    NNScratch& s = g_nn_scratch[device & 63];
-    if (s.bytes < bytes) {
-        if (s.base) {
-            MRP_HIP(hipDeviceSynchronize());
-            MRP_HIP(hipFree(s.base));
+// The MRPNN stages' scratch lives in the volume (nn_scratch), so volumes on
+// different streams can render concurrently.
+static int ensure_nn_scratch(Volume* v, size_t bytes, void** out) {
+    if (v->nn_scratch_bytes < bytes) {
+        if (v->nn_scratch) {
+            MRP_HIP(hipStreamSynchronize(v->stream));
+            MRP_HIP(hipFree(v->nn_scratch));
         }
-        s.base = nullptr;
-        s.bytes = 0;
-        if (hipMalloc(&s.base, bytes) != hipSuccess) return fail(MRP_E_NOMEM, "MRPNN scratch allocation failed");
-        s.bytes = bytes;
+        v->nn_scratch = nullptr;
+        v->nn_scratch_bytes = 0;
+        if (hipMalloc(&v->nn_scratch, bytes) != hipSuccess) return fail(MRP_E_NOMEM, "MRPNN scratch allocation failed");
+        v->nn_scratch_bytes = bytes;
     }
-    *out = s.base;
+    *out = v->nn_scratch;
     return MRP_OK;
 }
 
@@ -359,7 +357,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     const size_t total = (size_t)n * Pl;
     const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, total));
     void* scratch = nullptr;
-    rc = ensure_nn_scratch(v->device, chunk_items * kNNItemBytes + 4096, &scratch);
+    rc = ensure_nn_scratch(v, chunk_items * kNNItemBytes + 4096, &scratch);
     if (rc) return rc;
     char* sp = (char*)scratch;
     float4* pos_act = (float4*)sp;
