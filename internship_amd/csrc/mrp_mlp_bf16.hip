@@ -24,6 +24,7 @@
 // start from come precomputed from the gather kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -142,11 +143,14 @@ static uint16_t to_bf16_rne(float x) {
 // Packed buffer: [tiles][64 lanes][8 bf16], then [bias tiles][2][16] FP32, then
 // 64 FP32 of small VALU layers: LGGS W (24) and b (8), LX12 W (16) and b (1).
 constexpr int kExtraFloats = 64;
+// packed sizes (checked against layer_table) and the persistent kernel's shape
+constexpr int kTiles = 150, kBiasTiles = 62, kMlpBlock = 512, kMlpWaves = kMlpBlock / 64;
 int pack_mlp_bf16(const float* w, void** packed_dev) {
     LayerDesc L[LY_COUNT];
     layer_table(L);
     int nb = 0;
     const int nt = total_tiles(L, &nb);
+    if (nt != kTiles || nb != kBiasTiles) return fail(MRP_E_STATE, "packed weight tiles do not match the kernel's LDS layout");
     std::vector<uint16_t> a((size_t)nt * 64 * 8, 0);
     std::vector<float> bias((size_t)nb * 32, 0.0f);
     for (int i = 0; i < LY_COUNT; ++i) {
@@ -290,17 +294,37 @@ __device__ __forceinline__ void se_gates(const Packed& pk, int blk, const float*
 }
 
 // ---------------------------------------------------------------- kernel
-__global__ __launch_bounds__(256) void mlp_bf16_kernel(const bf16x8* __restrict__ pa, const float* __restrict__ pbias,
-                                                       const float* __restrict__ pextra, const float* __restrict__ feat, const float* __restrict__ pool,
-                                                       const float* __restrict__ gamma_in, float g, vec3 srp,
-                                                       uint32_t n_items, float* __restrict__ pred) {
+// Persistent: one workgroup of 8 waves per CU stages all weight tiles (150 KB
+// BF16), the bias tiles and the small VALU layers into LDS once, then each
+// wave evaluates tiles of 32 descriptors, reading every A operand from LDS.
+
+__global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __restrict__ packed,
+                                                             const float* __restrict__ feat,
+                                                             const float* __restrict__ pool,
+                                                             const float* __restrict__ gamma_in, float g, vec3 srp,
+                                                             uint32_t n_items, float* __restrict__ pred) {
+    __shared__ bf16x8 s_a[kTiles * 64];
+    __shared__ float s_bias[kBiasTiles * 32];
+    __shared__ float s_extra[kExtraFloats];
+    {
+        uint4* da = (uint4*)s_a;
+        for (int i = threadIdx.x; i < kTiles * 64; i += kMlpBlock) da[i] = packed[i];
+        const float4* sb = (const float4*)(packed + kTiles * 64);
+        float4* db = (float4*)s_bias;
+        for (int i = threadIdx.x; i < kBiasTiles * 8; i += kMlpBlock) db[i] = sb[i];
+        const float* se = (const float*)(packed + kTiles * 64) + kBiasTiles * 32;
+        for (int i = threadIdx.x; i < kExtraFloats; i += kMlpBlock) s_extra[i] = se[i];
+    }
+    __syncthreads();
     const int lane = (int)__lane_id();
     const int kg = lane >> 5;
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t item0 = wave * 32u;
-    if (item0 >= n_items) return;  // uniform over the wave
+    const Packed pk{s_a, s_bias};
+    const float* pextra = s_extra;
+    const uint32_t n_tiles = (n_items + 31u) / 32u;
+    for (uint32_t tile = blockIdx.x * kMlpWaves + (threadIdx.x >> 6); tile < n_tiles;
+         tile += gridDim.x * kMlpWaves) {
+    const uint32_t item0 = tile * 32u;
     const uint32_t item = min(item0 + (uint32_t)(lane & 31), n_items - 1u);  // tail lanes recompute the last item
-    const Packed pk{pa, pbias};
     const float* frow = feat + (size_t)item * kFeatures;
     const float* prow = pool + (size_t)item * 72;
     const float gamma = gamma_in[item];
@@ -309,7 +333,7 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(const bf16x8* __restrict_
     // prev[k]: previous block output of stream k in D-order (X_ValA2/B2/C2 window)
     f32x16 prev[3] = {zero16(), zero16(), zero16()};
     f32x16 comb[3];    // L81 outputs (32 each)
-    f32x16 lout[3];    // LD41 outputs (8 each, registers 0..3)
+    float lout[3][4];  // LD41 outputs (8 rows: registers 0..3 of each lane group)
     const int from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
     const int size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
 #pragma unroll
@@ -376,7 +400,8 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(const bf16x8* __restrict_
         bf16x8 b[1] = {to_b(prev[k], 0)};
         f32x16 o[1];
         dense<1, 1>(pk, LY_LD41 + k, b, o, true, lane);
-        lout[k] = o[0];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) lout[k][v] = o[0][v];
     }
 
     // ------------------------------------------------ head, once per distinct channel
@@ -496,6 +521,7 @@ __global__ __launch_bounds__(256) void mlp_bf16_kernel(const bf16x8* __restrict_
         pred[3 * item + 1] = fmax_(expf_(P[1]) - 1.0f, 0.0f) * sr[1];
         pred[3 * item + 2] = fmax_(expf_(P[2]) - 1.0f, 0.0f) * sr[2];
     }
+    }  // tile loop
 }
 
 // Per-segment means / maxima from FP32 descriptors (used when the caller has
@@ -521,24 +547,24 @@ __global__ void pool_kernel(const float* __restrict__ feat, uint32_t n_items, fl
     pool[t] = r;
 }
 
+static int g_num_cus[64] = {};
+
 int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, const float* gamma, float g, vec3 srp,
                     uint32_t n_items, float* pred, hipStream_t stream) {
     if (n_items == 0) return MRP_OK;
-    LayerDesc L[LY_COUNT];
-    layer_table(L);
-    int nb = 0;
-    const int nt = total_tiles(L, &nb);
-    const bf16x8* pa = (const bf16x8*)packed;
-    const float* pb = (const float*)((const char*)packed + (size_t)nt * 64 * 16);
-    const float* pe = pb + (size_t)nb * 32;
+    int dev = 0;
+    MRP_HIP(hipGetDevice(&dev));
+    int& cus = g_num_cus[dev & 63];
+    if (cus == 0) MRP_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     float* tmp = nullptr;
     if (!pool) {
         MRP_HIP(hipMalloc(&tmp, (size_t)n_items * 72 * sizeof(float)));
         pool_kernel<<<(unsigned)(((size_t)n_items * 72 + 255) / 256), 256, 0, stream>>>(feat, n_items, tmp);
         pool = tmp;
     }
-    const uint32_t waves = (n_items + 31) / 32;
-    mlp_bf16_kernel<<<(waves + 3) / 4, 256, 0, stream>>>(pa, pb, pe, feat, pool, gamma, g, srp, n_items, pred);
+    const uint32_t tiles = (n_items + 31) / 32;
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)cus, (tiles + kMlpWaves - 1) / kMlpWaves));
+    mlp_bf16_kernel<<<blocks, kMlpBlock, 0, stream>>>((const uint4*)packed, feat, pool, gamma, g, srp, n_items, pred);
     hipError_t e = hipGetLastError();
     if (tmp) {
         hipStreamSynchronize(stream);
