@@ -242,30 +242,53 @@ def hbm_traffic(kernel, workload):
 
 
 def time_mlp(a, ia, sc):
-    """K timed 1-spp MRPNN frames folded into the running mean in HBM."""
+    """K timed 1-spp MRPNN frames folded into the running mean in HBM.
+    Returns (elapsed s, dict of per-frame averages)."""
     sc.target.zero()
     for i in range(a.warmup):
         sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
-    fv = ga = ml = 0.0
+    acc = {"fv_ms": 0.0, "gather_ms": 0.0, "mlp_ms": 0.0, "descriptors": 0, "fv_lookups": 0}
     t0 = time.perf_counter()
     for i in range(a.warmup, a.warmup + a.steps):
         sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
-        fv += sc.st.first_vertex_ms
-        ga += sc.st.reduce_ms
-        ml += sc.st.kernel_ms
+        acc["fv_ms"] += sc.st.first_vertex_ms
+        acc["gather_ms"] += sc.st.reduce_ms
+        acc["mlp_ms"] += sc.st.kernel_ms
+        acc["descriptors"] += sc.st.scatters
+        acc["fv_lookups"] += sc.st.lookups
     elapsed = time.perf_counter() - t0
-    fv, ga, ml = fv / a.steps, ga / a.steps, ml / a.steps
-    log("MLP renderer: %.2f ms/frame (first vertex %.2f, gather %.2f, network %.2f ms)"
-        % (1000 * elapsed / a.steps, fv, ga, ml))
-    return elapsed, fv, ga, ml
+    k = {key: v / a.steps for key, v in acc.items()}
+    log("MLP renderer: %.2f ms/frame (first vertex %.2f, compaction + gather %.2f, network %.2f ms; %d descriptors"
+        " = %.1f%% of pixels)" % (1000 * elapsed / a.steps, k["fv_ms"], k["gather_ms"], k["mlp_ms"], k["descriptors"],
+                                  100.0 * k["descriptors"] / (sc.W * sc.H)))
+    return elapsed, k
 
 
-def mlp_roofline(sc, ml):
-    P = sc.W * sc.H
-    achieved = MLP_FLOP_PER_PASS * P / (ml * 1e-3) / 1e12
-    return {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / BF16_PEAK_TFLOPS, "traffic": hbm_traffic("mlp_bf16_kernel", MLP_WORKLOAD),
-            "kernel": "mlp_bf16_kernel", "kernel_ms": ml, "flop_per_launch": MLP_FLOP_PER_PASS * P}
+def mlp_rooflines(k):
+    """Rooflines of the three stages of an MRPNN frame; `roofline` is the one
+    that takes longest.  Network: FLOPs of the descriptors evaluated (first
+    vertices that scatter; escaping pixels are sky and skip gather and network).
+    First vertex: 32 B per trilinear lookup.  Gather: 18,400 B per descriptor."""
+    nd = k["descriptors"]
+    mlp = MLP_FLOP_PER_PASS * nd / (k["mlp_ms"] * 1e-3) / 1e12
+    fv = k["fv_lookups"] * BYTES_PER_LOOKUP / (k["fv_ms"] * 1e-3) / 1e9
+    ga = GATHER_BYTES_PER_POINT * nd / (k["gather_ms"] * 1e-3) / 1e9
+    r = {
+        "mlp_bf16_kernel": {"bound": "mfma", "achieved": mlp, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                            "frac": mlp / BF16_PEAK_TFLOPS, "traffic": hbm_traffic("mlp_bf16_kernel", MLP_WORKLOAD),
+                            "kernel": "mlp_bf16_kernel", "kernel_ms": k["mlp_ms"],
+                            "flop_per_launch": MLP_FLOP_PER_PASS * nd, "descriptors_per_launch": nd},
+        "first_vertex_kernel": {"bound": "hbm", "achieved": fv, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": fv / HBM_PEAK_GBS,
+                                "traffic": hbm_traffic("first_vertex_kernel", MLP_WORKLOAD),
+                                "kernel": "first_vertex_kernel", "kernel_ms": k["fv_ms"],
+                                "lookups_per_launch": k["fv_lookups"], "bytes_per_lookup": BYTES_PER_LOOKUP},
+        "gather_kernel": {"bound": "hbm", "achieved": ga, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ga / HBM_PEAK_GBS,
+                          "traffic": hbm_traffic("gather_kernel", MLP_WORKLOAD), "kernel": "gather_kernel",
+                          "kernel_ms": k["gather_ms"], "bytes_per_descriptor": GATHER_BYTES_PER_POINT},
+    }
+    dominant = max(r, key=lambda n: r[n]["kernel_ms"])
+    return r[dominant], r
 
 
 def headline_config(sc, a):
@@ -280,13 +303,13 @@ def run_mlp(a, ia, sc):
     """Only the timed MLP-renderer frames (no ground truth, no equal-RMSE search)."""
     net = ia.Network()
     sc.vol.set_network(net)
-    elapsed, fv, ga, ml = time_mlp(a, ia, sc)
+    elapsed, k = time_mlp(a, ia, sc)
     net.close()
     cfg = headline_config(sc, a)
     cfg["workload"] = cfg["workload"].split(";")[0]
+    roof, roofs = mlp_rooflines(k)
     return {"value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
-            "config": cfg, "roofline": mlp_roofline(sc, ml),
-            "kernels": {"first_vertex_ms": fv, "gather_ms": ga, "mlp_bf16_ms": ml}}
+            "config": cfg, "roofline": roof, "rooflines": roofs, "kernels": k}
 
 
 def run_frame(a, ia, sc):
@@ -493,7 +516,7 @@ def run_headline(a, ia, sc):
     sc.vol.set_network(net)
     P = sc.W * sc.H
     ref = ground_truth_image(ia, sc, a.truth_spp)  # untimed
-    elapsed, fv, ga, ml = time_mlp(a, ia, sc)
+    elapsed, k = time_mlp(a, ia, sc)
     fps_mlp = a.steps / elapsed
     # quality of one 1-spp frame with the BF16 and the FP32 network
     sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
@@ -510,23 +533,20 @@ def run_headline(a, ia, sc):
     sc.render(0, 1, DT_SEED)
     img_dt1 = sc.image()
     log("DT %d spp: %.2f ms/frame -> speed-up %.2fx" % (n_eq, 1000 * t_dt, fps_mlp / fps_dt))
-    gather_achieved = GATHER_BYTES_PER_POINT * P / (ga * 1e-3) / 1e9
     dt_achieved = dt_lookups * BYTES_PER_LOOKUP / (dt_kernel_ms * 1e-3) / 1e9
     net.close()
     return {
         "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
         "config": headline_config(sc, a),
-        "roofline": mlp_roofline(sc, ml),
+        "roofline": mlp_rooflines(k)[0], "rooflines": mlp_rooflines(k)[1],
         "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "dt_spp": n_eq,
                        "dt_reached_target": reached, "rmse_dt": hist.get(n_eq), "fps_mlp": fps_mlp,
                        "fps_dt_at_equal_rmse": fps_dt, "speedup": fps_mlp / fps_dt,
                        "dt_rmse_by_spp": {str(k): v for k, v in sorted(hist.items())},
                        "luma_rmse100_mlp": luma_rmse100(img_mlp, ref),
                        "luma_rmse100_dt1": luma_rmse100(img_dt1, ref)},
-        "kernels": {"first_vertex_ms": fv, "gather_ms": ga, "mlp_bf16_ms": ml, "mlp_fp32_ms": fp32_ms,
-                    "gather_gbs_algorithmic": gather_achieved, "gather_hbm_frac": gather_achieved / HBM_PEAK_GBS,
-                    "tracer_kernel_ms_at_equal_rmse": dt_kernel_ms, "tracer_gbs_algorithmic": dt_achieved,
-                    "tracer_hbm_frac": dt_achieved / HBM_PEAK_GBS},
+        "kernels": dict(k, mlp_fp32_ms=fp32_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms,
+                        tracer_gbs_algorithmic=dt_achieved, tracer_hbm_frac=dt_achieved / HBM_PEAK_GBS),
     }
 
 

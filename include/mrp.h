@@ -84,12 +84,12 @@ typedef struct mrp_render_params {
 typedef struct mrp_trace_stats {
     uint64_t lookups;       /* trilinear density lookups (8 FP32 corners each) */
     uint64_t paths;         /* (pixel, sample) paths traced */
-    uint64_t scatters;      /* scattering events */
+    uint64_t scatters;      /* scattering events (MRPNN: first vertices that scattered = descriptors evaluated) */
     uint64_t max_path_steps; /* loop iterations of the longest path (latency tail) */
     double kernel_ms;       /* device time of the tracing kernel(s) (MRPNN: of the network), HIP events */
-    double reduce_ms;       /* device time of the accumulation kernel(s) (MRPNN: of the descriptor gather) */
+    double reduce_ms;       /* device time of the accumulation kernel(s) (MRPNN: compaction + descriptor gather) */
     double total_ms;        /* device time of the whole call */
-    double first_vertex_ms; /* MRPNN only: device time of the first-vertex stage */
+    double first_vertex_ms; /* MRPNN only: device time of the first-vertex kernel */
     int launches;           /* tracing kernel launches */
     int grid_blocks;        /* persistent workgroups per launch */
 } mrp_trace_stats;

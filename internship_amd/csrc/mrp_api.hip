@@ -145,6 +145,14 @@ int mrp_volume_create(int resolution, mrp_volume** out) {
     }
     if (hipMalloc(&v->hg_lut, (size_t)kLutSize * kLutSize * sizeof(float)) != hipSuccess)
         return bail(MRP_E_NOMEM, "HG LUT allocation failed");
+    for (int m = 0; m < kNumMips; ++m)
+        if (hipMalloc(&v->mip_cells[m], cell_count(mip_res(m)) * 2 * sizeof(float4)) != hipSuccess)
+            return bail(MRP_E_NOMEM, "mip cell allocation failed");
+    for (int m = 0; m < kNumTrMips; ++m)
+        if (hipMalloc(&v->tr_cells[m], clamp_cell_count(tr_mip_res(m)) * 2 * sizeof(float4)) != hipSuccess)
+            return bail(MRP_E_NOMEM, "TR cell allocation failed");
+    if (hipMalloc(&v->lut_cells, lut_cell_count(kLutSize, kLutSize) * sizeof(float4)) != hipSuccess)
+        return bail(MRP_E_NOMEM, "HG LUT cell allocation failed");
     if (hipMalloc(&v->queues, kNumQueues * sizeof(uint32_t)) != hipSuccess) return bail(MRP_E_NOMEM, "queues");
     if (hipMalloc(&v->counters, 4 * sizeof(unsigned long long)) != hipSuccess) return bail(MRP_E_NOMEM, "counters");
     *out = h;
@@ -160,6 +168,9 @@ int mrp_volume_destroy(mrp_volume* h) {
     for (int m = 0; m < kNumMips; ++m) hipFree(v->mips[m]);
     for (int m = 0; m < kNumTrMips; ++m) hipFree(v->tr_mips[m]);
     hipFree(v->hg_lut);
+    for (int m = 0; m < kNumMips; ++m) hipFree(v->mip_cells[m]);
+    for (int m = 0; m < kNumTrMips; ++m) hipFree(v->tr_cells[m]);
+    hipFree(v->lut_cells);
     hipFree(v->queues);
     hipFree(v->counters);
     hipFree(v->samples);

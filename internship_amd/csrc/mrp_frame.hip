@@ -218,13 +218,14 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
     MRP_HIP(hipEventRecord(e0, v->stream));
     rc = clear_counters(v);
     double fv_ms = 0.0, gather_ms = 0.0, mlp_ms = 0.0;
+    uint64_t n_active = 0;
     if (rc == MRP_OK) {
         if (predict && !v->frame_last_predict) {
             const size_t nf = P * kHistoFloats;
             clear_histo_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, v->stream>>>(histo, nf);
             MRP_HIP(hipGetLastError());
         }
-        rc = predict ? mrpnn_samples(v, p, map, (int)v->frame_calls, 1, &fv_ms, &gather_ms, &mlp_ms)
+        rc = predict ? mrpnn_samples(v, p, map, (int)v->frame_calls, 1, &fv_ms, &gather_ms, &mlp_ms, &n_active)
                      : trace_samples(v, p, map, (int)v->frame_calls, 1);
     }
     if (rc == MRP_OK) {
@@ -264,7 +265,7 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
     if (st) {
         st->lookups = cnt[0];
         st->paths = predict ? (uint64_t)(fp->checkerboard ? (P + 1) / 2 : P) : cnt[1];
-        st->scatters = cnt[2];
+        st->scatters = predict ? n_active : cnt[2];
         st->max_path_steps = cnt[3];
         st->kernel_ms = predict ? mlp_ms : (double)render_ms;
         st->reduce_ms = post_ms;  // histogram update + denoise + tone + pack

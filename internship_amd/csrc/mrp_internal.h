@@ -41,6 +41,10 @@ struct Volume {
     mrp_vec3 tr_light{0, 0, 0};
     float tr_alpha = -1.0f;
     float* hg_lut = nullptr;         // kLutSize^2, x (cos) fastest
+    // corner-expanded copies read by the descriptor gather (one 32/16-byte read per fetch)
+    float4* mip_cells[kNumMips] = {};   // border cells of the padded mips, (r+1)^3 x 32 B
+    float4* tr_cells[kNumTrMips] = {};  // clamp cells of the TR mips, (r+2)^3 x 32 B
+    float4* lut_cells = nullptr;        // bilinear clamp cells of the HG table, 514^2 x 16 B
     bool lut_valid = false;
     float lut_g = -100.0f;
     uint64_t lut_seed = 0;
@@ -84,7 +88,7 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
 // -1) in v->samples[(s - s_begin) * P + pixel] for the pixels `map` selects.
 int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n);
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
-                  double* gather_ms, double* mlp_ms);
+                  double* gather_ms, double* mlp_ms, uint64_t* n_active);
 int clear_counters(Volume* v);
 int read_counters(Volume* v, unsigned long long out[4]);
 int launch_update(Volume* v);

@@ -102,6 +102,49 @@ __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cell
 }
 #endif
 
+// Clamp-addressed cells: cell (x,y,z) for x,y,z in [-1, res] holds the eight
+// clamp-to-edge corners of its neighbourhood in the order of trilinear_clamp,
+// (res+2)^3 cells; used for the transmittance mips by the descriptor gather.
+MRP_HD size_t clamp_cell_index(int res, int x, int y, int z) {
+    const size_t c = (size_t)res + 2;
+    return ((size_t)(x + 1) * c + (size_t)(y + 1)) * c + (size_t)(z + 1);
+}
+MRP_HD size_t clamp_cell_count(int res) {
+    const size_t c = (size_t)res + 2;
+    return c * c * c;
+}
+// Bilinear clamp cells of a W x H table: cell (x,y) for x in [-1, W], y in
+// [-1, H] holds (d[y0][x0], d[y0][x1], d[y1][x0], d[y1][x1]), (W+2)(H+2) cells.
+MRP_HD size_t lut_cell_count(int w, int h) { return ((size_t)w + 2) * ((size_t)h + 2); }
+
+#if defined(__HIPCC__)
+// trilinear_clamp on clamp-addressed cells: the same coordinates, weights and
+// blend order, one 32-byte read.
+__device__ __forceinline__ float trilinear_clamp_cells(const float4* __restrict__ cells, int res, vec3 uv) {
+    const float fr = (float)res;
+    float tx = uv.x * fr - 0.5f, ty = uv.y * fr - 0.5f, tz = uv.z * fr - 0.5f;
+    tx = tx >= -1.0f ? fmin_(tx, fr) : -1.0f;
+    ty = ty >= -1.0f ? fmin_(ty, fr) : -1.0f;
+    tz = tz >= -1.0f ? fmin_(tz, fr) : -1.0f;
+    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
+    const float4* c = cells + 2 * clamp_cell_index(res, (int)fx, (int)fy, (int)fz);
+    const float4 lo = c[0], hi = c[1];
+    return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
+                wx);
+}
+// bilinear_clamp on LUT cells: one 16-byte read.
+__device__ __forceinline__ float bilinear_cells(const float4* __restrict__ cells, int w, int h, float u, float v) {
+    float tx = u * (float)w - 0.5f, ty = v * (float)h - 0.5f;
+    tx = tx >= -1.0f ? fmin_(tx, (float)w) : -1.0f;
+    ty = ty >= -1.0f ? fmin_(ty, (float)h) : -1.0f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float wx = tx - fx, wy = ty - fy;
+    const float4 q = cells[((size_t)((int)fy + 1)) * ((size_t)w + 2) + (size_t)((int)fx + 1)];
+    return lerp(lerp(q.x, q.y, wx), lerp(q.z, q.w, wx), wy);
+}
+#endif
+
 // Density at box position p in [-0.5,0.5]^3: uv = p + 0.5 as in `Density`
 // (core/render.cuh:67-71) and `MipDensityDynamic` (core/renderResources.cu:42-67).
 MRP_HD float trilinear_border(const float* data, int res, vec3 p) {

@@ -141,11 +141,17 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
                     active = 1;
                 }
             }
-            vec3 X, Y, Z, LX, LY, LZ;
-            frame_from_normal(&rng, dir, &X, &Y, &Z);
-            frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
-            const float phase = hg_phase(dot(X, LX), a.g);
-            const float single = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk) * phase;
+            // A vertex whose free flight escaped is sky: its frames and shadow ray
+            // (the rest of NNPredict's draws) cannot change any output, so they are skipped.
+            vec3 X = v3(1, 0, 0), Y = v3(0, 1, 0), Z = v3(0, 0, 1);
+            float single = 0.0f;
+            if (active) {
+                vec3 LX, LY, LZ;
+                frame_from_normal(&rng, dir, &X, &Y, &Z);
+                frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
+                const float phase = hg_phase(dot(X, LX), a.g);
+                single = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk) * phase;
+            }
             const float dist = active ? distance(a.ori, pos) : -1.0f;
             pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
             fx[i] = make_float4(X.x, X.y, X.z, single);
@@ -164,9 +170,9 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 
 // ---------------------------------------------------------------- stage 2
 struct GatherArgs {
-    const float* mips[kNumMips];     // padded density mips
-    const float* trs[kNumTrMips];    // unpadded transmittance mips
-    const float* lut;                // HG table
+    const float4* mips[kNumMips];    // border cells of the density mips
+    const float4* trs[kNumTrMips];   // clamp cells of the transmittance mips
+    const float4* lut;               // bilinear clamp cells of the HG table
     vec3 light, lx;
     float alpha, g;
     uint32_t n_items;
@@ -183,73 +189,83 @@ __constant__ int c_seg_size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
 // the per-segment means (0..35) and maxima (36..71) that the network's
 // squeeze-excitation blocks start from (`AVGMAX`, core/radiancePredict.cu:21-33:
 // ascending sum then /size; max from -10000).
+// One workgroup (192 threads, one tap each) loops over kGatherItems points, so
+// the dispatcher launches 1/kGatherItems as many workgroups; the pooled
+// statistics use a double-buffered LDS row, one barrier per point.
+constexpr int kGatherItems = 16;
+
 __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
-                                                       const float4* __restrict__ fz, float* __restrict__ feat,
-                                                       float* __restrict__ gamma, float* __restrict__ pool) {
-    __shared__ float sf[kFeatures];
-    const uint32_t item = blockIdx.x;
+                                                       const float4* __restrict__ fz, const uint32_t* __restrict__ list,
+                                                       float* __restrict__ feat, float* __restrict__ gamma,
+                                                       float* __restrict__ pool) {
+    __shared__ float sbuf[2][kFeatures];
     const int i = threadIdx.x;
-    if (item >= a.n_items) return;  // uniform over the block
-    const float4 pa = pos_act[item], x4 = fx[item], y4 = fy[item], z4 = fz[item];
-    const vec3 p = v3(pa.x, pa.y, pa.z);
-    const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
-    float* out = feat + (size_t)item * kFeatures;
-    if (i == 0) gamma[item] = acosf_(dot(X, a.lx));
-    float xd = 0.0f, xt = 0.0f, xh = 0.0f;
-    if (pa.w != 0.0f) {
-        const int mip = c_tap_mip[i];
-        const float off = c_tap_offset[i];
-        vec3 pi;
-        if (i < kShellTaps) {
-            pi = p + (X * c_stencil[3 * i] + Y * c_stencil[3 * i + 1] + Z * c_stencil[3 * i + 2]) * off;
-        } else {
-            pi = p + a.light * off;
-        }
-        const float* m = a.mips[mip];
-        xd = logf_(1.0f + a.alpha * trilinear_border(m, 256 >> mip, pi) / 64.0f);
-        if (i == 0) {
-            xh = logf_(hg_phase(dot(X, a.light), a.g) + 1.0f);
-        } else {
-            const vec3 ms = normalize(pi - p);
-            const float u = dot(X, ms) * 0.5f + 0.5f;
-            const float u2 = dot(a.lx, ms) * 0.5f + 0.5f;
-            const float v = c_tap_v[i];
-            xh = logf_(bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u, v) *
-                           bilinear_clamp(a.lut, kHgLutSize, kHgLutSize, u2, v) +
-                       1.0f);
-        }
-        const int tm = mip - 1 > 0 ? mip - 1 : 0;
-        const int tr = 128 >> tm;
-        if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
-            const float o = ray_box_offset(pi, a.lx);
-            xt = o >= 0.0f ? trilinear_clamp(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
-        } else {
-            xt = trilinear_clamp(a.trs[tm], tr, pi + 0.5f);
-        }
-    }
-    out[i] = xd;
-    out[kTaps + i] = xt;
-    out[2 * kTaps + i] = xh;
-    if (pool) {
-        sf[i] = xd;
-        sf[kTaps + i] = xt;
-        sf[2 * kTaps + i] = xh;
-        __syncthreads();
-        if (i < 72) {
-            const int q = i % 36, blk = q / 3, k = q % 3;
-            const float* seg = sf + k * kTaps + c_seg_from[blk];
-            const int n = c_seg_size[blk];
-            float r;
-            if (i < 36) {
-                float sum = 0.0f;
-                for (int j = 0; j < n; ++j) sum += seg[j];
-                r = sum / (float)n;
+    const int mip = c_tap_mip[i];
+    const float off = c_tap_offset[i];
+    const float lut_v = c_tap_v[i];
+    const int tm = mip - 1 > 0 ? mip - 1 : 0;
+    const int tr = 128 >> tm;
+    const vec3 st = i < kShellTaps ? v3(c_stencil[3 * i], c_stencil[3 * i + 1], c_stencil[3 * i + 2]) : v3(0, 0, 0);
+    for (int it = 0; it < kGatherItems; ++it) {
+        const uint32_t item = blockIdx.x * kGatherItems + (uint32_t)it;
+        if (item >= a.n_items) break;  // uniform over the block
+        const uint32_t src = list ? list[item] : item;  // compacted: the vertex this descriptor belongs to
+        const float4 pa = pos_act[src], x4 = fx[src], y4 = fy[src], z4 = fz[src];
+        const vec3 p = v3(pa.x, pa.y, pa.z);
+        const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
+        float* out = feat + (size_t)item * kFeatures;
+        if (i == 0) gamma[item] = acosf_(dot(X, a.lx));
+        float xd = 0.0f, xt = 0.0f, xh = 0.0f;
+        if (pa.w != 0.0f) {
+            vec3 pi;
+            if (i < kShellTaps) {
+                pi = p + (X * st.x + Y * st.y + Z * st.z) * off;
             } else {
-                r = -10000.0f;
-                for (int j = 0; j < n; ++j) r = fmax_(r, seg[j]);
+                pi = p + a.light * off;
             }
-            pool[(size_t)item * 72 + i] = r;
+            xd = logf_(1.0f + a.alpha * trilinear_cells(a.mips[mip], 256 >> mip, pi) / 64.0f);
+            if (i == 0) {
+                xh = logf_(hg_phase(dot(X, a.light), a.g) + 1.0f);
+            } else {
+                const vec3 ms = normalize(pi - p);
+                const float u = dot(X, ms) * 0.5f + 0.5f;
+                const float u2 = dot(a.lx, ms) * 0.5f + 0.5f;
+                xh = logf_(bilinear_cells(a.lut, kHgLutSize, kHgLutSize, u, lut_v) *
+                               bilinear_cells(a.lut, kHgLutSize, kHgLutSize, u2, lut_v) +
+                           1.0f);
+            }
+            if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
+                const float o = ray_box_offset(pi, a.lx);
+                xt = o >= 0.0f ? trilinear_clamp_cells(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
+            } else {
+                xt = trilinear_clamp_cells(a.trs[tm], tr, pi + 0.5f);
+            }
+        }
+        out[i] = xd;
+        out[kTaps + i] = xt;
+        out[2 * kTaps + i] = xh;
+        if (pool) {
+            float* sf = sbuf[it & 1];
+            sf[i] = xd;
+            sf[kTaps + i] = xt;
+            sf[2 * kTaps + i] = xh;
+            __syncthreads();
+            if (i < 72) {
+                const int q = i % 36, blk = q / 3, k = q % 3;
+                const float* seg = sf + k * kTaps + c_seg_from[blk];
+                const int n = c_seg_size[blk];
+                float r;
+                if (i < 36) {
+                    float sum = 0.0f;
+                    for (int j = 0; j < n; ++j) sum += seg[j];
+                    r = sum / (float)n;
+                } else {
+                    r = -10000.0f;
+                    for (int j = 0; j < n; ++j) r = fmax_(r, seg[j]);
+                }
+                pool[(size_t)item * 72 + i] = r;
+            }
         }
     }
 }
@@ -270,6 +286,7 @@ __global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ 
 // ---------------------------------------------------------------- stage 4
 __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act, const float4* __restrict__ fx,
                                                       const float4* __restrict__ fy, const float4* __restrict__ fz,
+                                                      const uint32_t* __restrict__ slot_of,
                                                       const float* __restrict__ pred, vec3 light_color,
                                                       vec3 scatter_rate, vec3 sky, uint32_t n_items,
                                                       float4* __restrict__ out) {
@@ -279,13 +296,39 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
     if (slot == 0xffffffffu) return;
     const float single = fx[i].w;
     if (pos_act[i].w != 0.0f) {
-        const vec3 p = v3(pred[3 * i] + single, pred[3 * i + 1] + single, pred[3 * i + 2] + single);
+        const uint32_t k = slot_of[i];  // the vertex's compacted descriptor
+        const vec3 p = v3(pred[3 * k] + single, pred[3 * k + 1] + single, pred[3 * k + 2] + single);
         const vec3 r = p * light_color * scatter_rate;
         out[slot] = make_float4(r.x, r.y, r.z, fz[i].w);
     } else {
         out[slot] = make_float4(sky.x, sky.y, sky.z, -1.0f);
     }
 }
+
+// Active first vertices -> a dense list (wave ballot, one atomic per wave).
+// The list order depends on scheduling; every descriptor's result does not.
+__global__ __launch_bounds__(256) void compact_kernel(const float4* __restrict__ pos_act, uint32_t n,
+                                                      uint32_t* __restrict__ list, uint32_t* __restrict__ slot_of,
+                                                      uint32_t* __restrict__ count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool act = i < n && pos_act[i].w != 0.0f;
+    const unsigned long long m = __ballot(act);
+    const uint32_t lane = __lane_id();
+    uint32_t base = 0u;
+    if (lane == 0u && m != 0ull) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    if (i < n) {
+        if (act) {
+            const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+            list[k] = i;
+            slot_of[i] = k;
+        } else {
+            slot_of[i] = 0xffffffffu;
+        }
+    }
+}
+
+__global__ void clear_u32_one_kernel(uint32_t* p) { *p = 0u; }
 
 __global__ void clear_counters_kernel(unsigned long long* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -312,7 +355,7 @@ static int upload_tap_tables(int device) {
 
 // Per-item scratch of the MRPNN stages (bytes): 4 float4 first-vertex
 // records, 576 FP32 features, gamma, 3 predictions.
-constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12;
+constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12 + 8;  // + compaction list and slot
 
 // The MRPNN stages' scratch lives in the volume (nn_scratch), so volumes on
 // different streams can render concurrently.
@@ -339,7 +382,7 @@ static vec3 srp_of(vec3 scatter_rate) {
 // First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
 // of the pixels `map` selects, in chunks of at most 4M vertices.
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
-                  double* gather_ms, double* mlp_ms) {
+                  double* gather_ms, double* mlp_ms, uint64_t* n_active) {
     if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
     if (p->mlp_precision != 0 && p->mlp_precision != 1)
         return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32) or 1 (BF16)");
@@ -368,6 +411,9 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     float* pool = feat + chunk_items * kFeatures;
     float* gam = pool + chunk_items * 72;
     float* pred = gam + chunk_items;
+    uint32_t* list = (uint32_t*)(pred + 3 * chunk_items);
+    uint32_t* slot_of = list + chunk_items;
+    uint32_t* n_active_dev = (uint32_t*)v->queues;  // the tracer's queue counters are free during MRPNN frames
 
     NNArgs a;
     a.cells = v->cells;
@@ -397,54 +443,67 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     a.counters = v->counters;
 
     GatherArgs ga;
-    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mips[m];
-    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_mips[m];
-    ga.lut = v->hg_lut;
+    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mip_cells[m];
+    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_cells[m];
+    ga.lut = v->lut_cells;
     ga.light = a.light;
     ga.lx = a.lx;
     ga.alpha = p->alpha;
     ga.g = p->g;
     const vec3 srp = srp_of(a.scatter_rate);
 
-    hipEvent_t t0, t1, t2, t3;
+    hipEvent_t t0, tc, t2, t3;
     MRP_HIP(hipEventCreate(&t0));
-    MRP_HIP(hipEventCreate(&t1));
+    MRP_HIP(hipEventCreate(&tc));
     MRP_HIP(hipEventCreate(&t2));
     MRP_HIP(hipEventCreate(&t3));
     for (size_t i0 = 0; i0 < total && rc == MRP_OK; i0 += chunk_items) {
         const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
         a.item0 = (uint32_t)i0;
         a.n_items = cnt;
-        ga.n_items = cnt;
         MRP_HIP(hipEventRecord(t0, v->stream));
         first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz);
         MRP_HIP(hipGetLastError());
-        MRP_HIP(hipEventRecord(t1, v->stream));
-        gather_kernel<<<cnt, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, feat, gam, pool);
+        MRP_HIP(hipEventRecord(tc, v->stream));
+        clear_u32_one_kernel<<<1, 1, 0, v->stream>>>(n_active_dev);
+        compact_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, cnt, list, slot_of, n_active_dev);
         MRP_HIP(hipGetLastError());
-        MRP_HIP(hipEventRecord(t2, v->stream));
-        if (p->mlp_precision == 0) {
-            mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt, pred);
-        } else {
-            rc = launch_mlp_bf16(v->nn_packed, feat, pool, gam, p->g, srp, cnt, pred, v->stream);
-            if (rc) break;
+        uint32_t n_act = 0;
+        MRP_HIP(hipMemcpyAsync(&n_act, n_active_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
+        MRP_HIP(hipStreamSynchronize(v->stream));
+        if (n_active) *n_active += n_act;
+        ga.n_items = n_act;
+        if (n_act > 0) {
+            gather_kernel<<<(n_act + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz,
+                                                                                              list, feat, gam, pool);
+            MRP_HIP(hipGetLastError());
         }
-        MRP_HIP(hipGetLastError());
+        MRP_HIP(hipEventRecord(t2, v->stream));
+        if (n_act > 0) {
+            if (p->mlp_precision == 0) {
+                mlp_fp32_kernel<<<(n_act + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, n_act,
+                                                                         pred);
+            } else {
+                rc = launch_mlp_bf16(v->nn_packed, feat, pool, gam, p->g, srp, n_act, pred, v->stream);
+                if (rc) break;
+            }
+            MRP_HIP(hipGetLastError());
+        }
         MRP_HIP(hipEventRecord(t3, v->stream));
-        combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, fx, fy, fz, pred, a.light_color,
+        combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, fx, fy, fz, slot_of, pred, a.light_color,
                                                                   a.scatter_rate, a.sky, cnt, (float4*)v->samples);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventSynchronize(t3));
         float ms;
-        MRP_HIP(hipEventElapsedTime(&ms, t0, t1));
+        MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
         if (fv_ms) *fv_ms += ms;
-        MRP_HIP(hipEventElapsedTime(&ms, t1, t2));
+        MRP_HIP(hipEventElapsedTime(&ms, tc, t2));  // compaction, its count read-back and the gather
         if (gather_ms) *gather_ms += ms;
         MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
         if (mlp_ms) *mlp_ms += ms;
     }
     hipEventDestroy(t0);
-    hipEventDestroy(t1);
+    hipEventDestroy(tc);
     hipEventDestroy(t2);
     hipEventDestroy(t3);
     return rc;
@@ -462,10 +521,11 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     MRP_HIP(hipEventRecord(e0, v->stream));
     rc = clear_counters(v);
     double mlp_ms = 0.0, gather_ms = 0.0, fv_ms = 0.0;
+    uint64_t n_active = 0;
     const PixelMap map;
     for (int s0 = 0; s0 < p->spp_count && rc == MRP_OK; s0 += batch) {
         const int n = std::min(batch, p->spp_count - s0);
-        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &fv_ms, &gather_ms, &mlp_ms);
+        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &fv_ms, &gather_ms, &mlp_ms, &n_active);
         if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
     }
     float total_ms = 0.0f;
@@ -482,7 +542,7 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     if (st) {
         st->lookups = cnt4[0];
         st->paths = (uint64_t)p->spp_count * P;
-        st->scatters = 0;
+        st->scatters = n_active;  // first vertices that scattered: the descriptors gathered and evaluated
         st->max_path_steps = 0;
         st->kernel_ms = mlp_ms;       // the network
         st->reduce_ms = gather_ms;    // the descriptor gather
@@ -557,17 +617,17 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     rc = ensure_hg_lut(v, g, seed);
     if (rc) return rc;
     GatherArgs ga;
-    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mips[m];
-    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_mips[m];
-    ga.lut = v->hg_lut;
+    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mip_cells[m];
+    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_cells[m];
+    ga.lut = v->lut_cells;
     ga.light = normalize(V(light_dir));
     ga.lx = normalize(ga.light);
     ga.alpha = alpha;
     ga.g = g;
     ga.n_items = (uint32_t)n;
-    gather_kernel<<<n, kTaps, 0, v->stream>>>(ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev,
-                                              (const float4*)frame_y_dev, (const float4*)frame_z_dev, feat_dev,
-                                              gamma_dev, nullptr);
+    gather_kernel<<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
+        ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev, (const float4*)frame_y_dev,
+        (const float4*)frame_z_dev, nullptr, feat_dev, gamma_dev, nullptr);
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
     return MRP_OK;

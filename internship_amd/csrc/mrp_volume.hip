@@ -68,6 +68,37 @@ __global__ void build_cells_kernel(const float* __restrict__ grid, int res, floa
     }
 }
 
+// Clamp cells of an unpadded res^3 grid (trilinear_clamp's corners) for cell
+// coordinates in [-1, res].
+__global__ void build_clamp_cells_kernel(const float* __restrict__ src, int res, float4* __restrict__ cells) {
+    const int c = res + 2;
+    const size_t n = (size_t)c * c * c;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t R = (size_t)res;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int z = (int)(i % c) - 1, y = (int)((i / c) % c) - 1, x = (int)(i / ((size_t)c * c)) - 1;
+        const int hi = res - 1;
+        const int x0 = x < 0 ? 0 : (x > hi ? hi : x), x1 = x + 1 > hi ? hi : x + 1;
+        const int y0 = y < 0 ? 0 : (y > hi ? hi : y), y1 = y + 1 > hi ? hi : y + 1;
+        const int z0 = z < 0 ? 0 : (z > hi ? hi : z), z1 = z + 1 > hi ? hi : z + 1;
+#define MRP_AT(a, b, d) src[((size_t)(a) * R + (size_t)(b)) * R + (size_t)(d)]
+        cells[2 * i] = make_float4(MRP_AT(x0, y0, z0), MRP_AT(x0, y0, z1), MRP_AT(x0, y1, z0), MRP_AT(x0, y1, z1));
+        cells[2 * i + 1] = make_float4(MRP_AT(x1, y0, z0), MRP_AT(x1, y0, z1), MRP_AT(x1, y1, z0), MRP_AT(x1, y1, z1));
+#undef MRP_AT
+    }
+}
+
+// Bilinear clamp cells of a w x h table (x fastest) for cell coordinates in [-1, w] x [-1, h].
+__global__ void build_lut_cells_kernel(const float* __restrict__ lut, int w, int h, float4* __restrict__ cells) {
+    const int cw = w + 2, ch = h + 2;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cw * ch) return;
+    const int x = i % cw - 1, y = i / cw - 1;
+    const int x0 = x < 0 ? 0 : (x > w - 1 ? w - 1 : x), x1 = x + 1 > w - 1 ? w - 1 : x + 1;
+    const int y0 = y < 0 ? 0 : (y > h - 1 ? h - 1 : y), y1 = y + 1 > h - 1 ? h - 1 : y + 1;
+    cells[i] = make_float4(lut[y0 * w + x0], lut[y0 * w + x1], lut[y1 * w + x0], lut[y1 * w + x1]);
+}
+
 // max over the interior of a padded grid; result as float bits in *out
 // (initialised to bits(1e-5f) > 0, so unsigned ordering equals float ordering).
 __global__ __launch_bounds__(256) void max_kernel(const float* __restrict__ src, int res, unsigned* out) {
@@ -224,6 +255,9 @@ int launch_update(Volume* v) {
         const int rm = mip_res(m);
         box_kernel<<<blocks_for((size_t)rm * rm * rm), 256, 0, v->stream>>>(v->mips[m - 1], v->mips[m], rm);
     }
+    for (int m = 0; m < kNumMips; ++m)
+        build_cells_kernel<<<blocks_for(cell_count(mip_res(m))), 256, 0, v->stream>>>(v->mips[m], mip_res(m),
+                                                                                      v->mip_cells[m]);
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
     for (float* t : temps) hipFree(t);
@@ -245,6 +279,8 @@ int launch_update_tr(Volume* v, vec3 light, float alpha) {
         const float am = alpha / tr_alpha_scale(m);
         fill_tr_kernel<<<blocks_for((size_t)r * r * r), 256, 0, v->stream>>>(v->mips[m + 1], r, v->tr_mips[m], light,
                                                                              am);
+        build_clamp_cells_kernel<<<blocks_for(clamp_cell_count(r)), 256, 0, v->stream>>>(v->tr_mips[m], r,
+                                                                                         v->tr_cells[m]);
     }
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
@@ -264,6 +300,8 @@ int launch_update_hg_lut(Volume* v, float g, uint64_t seed) {
     dim3 grid(kHgLutSize / 8, kHgLutSize / 8);
     fill_hg_kernel<<<grid, 64, 0, v->stream>>>(v->hg_lut, dt, g, hg_lut_blend(g), (uint32_t)seed,
                                                (uint32_t)(seed >> 32));
+    const int nc = (int)lut_cell_count(kHgLutSize, kHgLutSize);
+    build_lut_cells_kernel<<<(nc + 255) / 256, 256, 0, v->stream>>>(v->hg_lut, kHgLutSize, kHgLutSize, v->lut_cells);
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
     hipFree(dt);
