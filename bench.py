@@ -41,6 +41,7 @@ if ROOT not in sys.path:
 METRIC = "1080p frames/s, BF16 MLP vs plain delta tracking at equal RMSE; HBM/MFMA %"
 HBM_PEAK_GBS = 8000.0
 BF16_PEAK_TFLOPS = 2500.0
+FP8_PEAK_TFLOPS = 5000.0
 FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_LOOKUP = 32          # eight FP32 corners of one trilinear density lookup
 MLP_FLOP_PER_PASS = 97072      # 48,536 MACs per descriptor per colour pass
@@ -246,11 +247,11 @@ def time_mlp(a, ia, sc):
     Returns (elapsed s, dict of per-frame averages)."""
     sc.target.zero()
     for i in range(a.warmup):
-        sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
+        sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=a.mlp_precision, fresh=False)
     acc = {"fv_ms": 0.0, "gather_ms": 0.0, "mlp_ms": 0.0, "descriptors": 0, "fv_lookups": 0}
     t0 = time.perf_counter()
     for i in range(a.warmup, a.warmup + a.steps):
-        sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=1, fresh=False)
+        sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=a.mlp_precision, fresh=False)
         acc["fv_ms"] += sc.st.first_vertex_ms
         acc["gather_ms"] += sc.st.reduce_ms
         acc["mlp_ms"] += sc.st.kernel_ms
@@ -264,20 +265,22 @@ def time_mlp(a, ia, sc):
     return elapsed, k
 
 
-def mlp_rooflines(k):
+def mlp_rooflines(k, precision=1):
     """Rooflines of the three stages of an MRPNN frame; `roofline` is the one
     that takes longest.  Network: FLOPs of the descriptors evaluated (first
-    vertices that scatter; escaping pixels are sky and skip gather and network).
+    vertices that scatter; escaping pixels are sky and skip gather and network)
+    against the dense BF16 (precision 1) or FP8 (precision 2) MFMA peak.
     First vertex: 32 B per trilinear lookup.  Gather: 18,400 B per descriptor."""
     nd = k["descriptors"]
     mlp = MLP_FLOP_PER_PASS * nd / (k["mlp_ms"] * 1e-3) / 1e12
+    kname, peak = ("mlp_fp8_kernel", FP8_PEAK_TFLOPS) if precision == 2 else ("mlp_bf16_kernel", BF16_PEAK_TFLOPS)
     fv = k["fv_lookups"] * BYTES_PER_LOOKUP / (k["fv_ms"] * 1e-3) / 1e9
     ga = GATHER_BYTES_PER_POINT * nd / (k["gather_ms"] * 1e-3) / 1e9
     r = {
-        "mlp_bf16_kernel": {"bound": "mfma", "achieved": mlp, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                            "frac": mlp / BF16_PEAK_TFLOPS, "traffic": hbm_traffic("mlp_bf16_kernel", MLP_WORKLOAD),
-                            "kernel": "mlp_bf16_kernel", "kernel_ms": k["mlp_ms"],
-                            "flop_per_launch": MLP_FLOP_PER_PASS * nd, "descriptors_per_launch": nd},
+        kname: {"bound": "mfma", "achieved": mlp, "peak": peak, "unit": "TFLOP/s",
+                "frac": mlp / peak, "traffic": hbm_traffic(kname, MLP_WORKLOAD),
+                "kernel": kname, "kernel_ms": k["mlp_ms"],
+                "flop_per_launch": MLP_FLOP_PER_PASS * nd, "descriptors_per_launch": nd},
         "first_vertex_kernel": {"bound": "hbm", "achieved": fv, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": fv / HBM_PEAK_GBS,
                                 "traffic": hbm_traffic("first_vertex_kernel", MLP_WORKLOAD),
@@ -292,7 +295,8 @@ def mlp_rooflines(k):
 
 
 def headline_config(sc, a):
-    return {"workload": "MRPNN renderer (BF16 MFMA network), 512^3 Perlin-fBm cloud seed 1, %dx%d, 1 spp/frame; "
+    net = "FP8-E4M3" if a.mlp_precision == 2 else ("FP32" if a.mlp_precision == 0 else "BF16")
+    return {"workload": "MRPNN renderer (" + net + " MFMA network), 512^3 Perlin-fBm cloud seed 1, %dx%d, 1 spp/frame; "
                         "vs plain delta tracking at equal RMSE against a %d-spp FP64 ground-truth image"
                         % (sc.W, sc.H, a.truth_spp),
             "width": sc.W, "height": sc.H, "grid": sc.res, "alpha": 256, "albedo": 0.999, "g": 0.857,
@@ -307,8 +311,9 @@ def run_mlp(a, ia, sc):
     net.close()
     cfg = headline_config(sc, a)
     cfg["workload"] = cfg["workload"].split(";")[0]
-    roof, roofs = mlp_rooflines(k)
-    return {"value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
+    roof, roofs = mlp_rooflines(k, a.mlp_precision)
+    return {"value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps,
+            "dtype": {0: "f32", 1: "bf16", 2: "fp8-e4m3"}[a.mlp_precision],
             "config": cfg, "roofline": roof, "rooflines": roofs, "kernels": k}
 
 
@@ -518,14 +523,18 @@ def run_headline(a, ia, sc):
     ref = ground_truth_image(ia, sc, a.truth_spp)  # untimed
     elapsed, k = time_mlp(a, ia, sc)
     fps_mlp = a.steps / elapsed
-    # quality of one 1-spp frame with the BF16 and the FP32 network
+    # quality of one 1-spp frame with the BF16, FP32 and FP8 networks
     sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
     img_mlp = sc.image()
     rmse_mlp = rmse(img_mlp, ref)
     sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=0)
     fp32_ms = sc.st.kernel_ms
     rmse_mlp32 = rmse(sc.image(), ref)
-    log("MLP 1-spp RMSE: BF16 %.5g, FP32 %.5g (FP32 network %.2f ms)" % (rmse_mlp, rmse_mlp32, fp32_ms))
+    sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=2)
+    fp8_ms = sc.st.kernel_ms
+    rmse_mlp8 = rmse(sc.image(), ref)
+    log("MLP 1-spp RMSE: BF16 %.5g, FP32 %.5g, FP8 %.5g (networks: FP32 %.2f ms, FP8 %.2f ms)"
+        % (rmse_mlp, rmse_mlp32, rmse_mlp8, fp32_ms, fp8_ms))
     # the plain tracer at equal RMSE
     n_eq, hist, reached = equal_rmse_search(sc, rmse_mlp, ref, a.dt_max_spp)
     t_dt, dt_kernel_ms, dt_lookups = dt_time(sc, n_eq)
@@ -538,14 +547,15 @@ def run_headline(a, ia, sc):
     return {
         "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
         "config": headline_config(sc, a),
-        "roofline": mlp_rooflines(k)[0], "rooflines": mlp_rooflines(k)[1],
-        "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "dt_spp": n_eq,
+        "roofline": mlp_rooflines(k, a.mlp_precision)[0], "rooflines": mlp_rooflines(k, a.mlp_precision)[1],
+        "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "rmse_mlp_fp8": rmse_mlp8,
+                       "dt_spp": n_eq,
                        "dt_reached_target": reached, "rmse_dt": hist.get(n_eq), "fps_mlp": fps_mlp,
                        "fps_dt_at_equal_rmse": fps_dt, "speedup": fps_mlp / fps_dt,
                        "dt_rmse_by_spp": {str(k): v for k, v in sorted(hist.items())},
                        "luma_rmse100_mlp": luma_rmse100(img_mlp, ref),
                        "luma_rmse100_dt1": luma_rmse100(img_dt1, ref)},
-        "kernels": dict(k, mlp_fp32_ms=fp32_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms,
+        "kernels": dict(k, mlp_fp32_ms=fp32_ms, mlp_fp8_ms=fp8_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms,
                         tracer_gbs_algorithmic=dt_achieved, tracer_hbm_frac=dt_achieved / HBM_PEAK_GBS),
     }
 
@@ -564,6 +574,8 @@ def main(argv=None):
     ap.add_argument("--dt-max-spp", type=int, default=4096)
     ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mlp-precision", type=int, choices=[0, 1, 2], default=1,
+                    help="headline/mlp modes: network of the timed frames (0 FP32, 1 BF16, 2 FP8-E4M3)")
     ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
     ap.add_argument("--workers", type=int, default=4, help="serve mode: host threads")
     ap.add_argument("--lr", type=float, default=1e-4, help="train mode: Adam learning rate")

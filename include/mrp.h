@@ -248,7 +248,9 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
                        const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
                        uint64_t seed, float* feat_dev, float* gamma_dev);
 /* The network on n descriptors (HBM): out[n][3] = max(exp(P_c)-1, 0)*srp_c,
- * srp = (scatter_rate/1.001)^4 per channel.  precision 0 = FP32, 1 = BF16 MFMA. */
+ * srp = (scatter_rate/1.001)^4 per channel.  precision 0 = FP32, 1 = BF16 MFMA,
+ * 2 = FP8-E4M3 MFMA (power-of-two scales: per layer for the weights, per layer
+ * and descriptor for the activations; FP32 accumulation, FP32 bias). */
 int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
                     mrp_vec3 srp, int precision, float* out_dev);
 

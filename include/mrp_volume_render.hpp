@@ -136,7 +136,7 @@ class VolumeRender {
         sky_ = c * env_exp_;
     }
     void SetSeed(uint64_t s) { seed_ = s; }
-    void SetMlpPrecision(int p) { precision_ = p; }  // 0 FP32, 1 BF16 MFMA
+    void SetMlpPrecision(int p) { precision_ = p; }  // 0 FP32, 1 BF16 MFMA, 2 FP8-E4M3 MFMA
     bool SetNetwork(const float* w, size_t count) {
         if (net_) mrp_network_destroy(net_);
         net_ = nullptr;

@@ -22,7 +22,7 @@ LIB = os.path.join(ROOT, "internship_amd", "libmrp.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
-HIP_SOURCES = ["mrp_api.hip", "mrp_volume.hip", "mrp_tracer.hip", "mrp_mrpnn.hip", "mrp_mlp_bf16.hip",
+HIP_SOURCES = ["mrp_api.hip", "mrp_volume.hip", "mrp_tracer.hip", "mrp_mrpnn.hip", "mrp_mlp_mfma.hip",
                "mrp_frame.hip", "mrp_samples.hip", "mrp_train.hip"]
 ORACLE_SOURCES = ["oracle.cpp"]
 

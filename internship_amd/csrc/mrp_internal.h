@@ -50,6 +50,7 @@ struct Volume {
     uint64_t lut_seed = 0;
     const float* nn_weights = nullptr;       // FP32 MRPNN weights in HBM (owned by an mrp_network)
     const void* nn_packed = nullptr;         // BF16 MFMA-packed weights (owned by an mrp_network)
+    const void* nn_packed_fp8 = nullptr;     // FP8-E4M3 MFMA-packed weights (owned by an mrp_network)
     // scratch
     uint32_t* queues = nullptr;              // kNumQueues counters
     unsigned long long* counters = nullptr;  // [lookups, paths, scatters, spare]

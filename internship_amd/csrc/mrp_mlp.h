@@ -189,11 +189,25 @@ MRP_HD void mlp_forward_fp32(const float* w, const float* feat, float gamma, flo
     for (int c = 0; c < 3; ++c) out[c] = fmax_(expf_(P[c]) - 1.0f, 0.0f) * sr[c];
 }
 
+// Wave-tiled BF16 descriptors (the renderer's MFMA path: written by the
+// gather, read by the MFMA networks).  Descriptors are grouped in tiles of 32;
+// for stream k (0..2) and 16-feature chunk c (0..11) of a tile, lane
+// l = 32*kg + n owns one 16-byte chunk holding the BF16 features
+// 16c + 8h + 4kg + e (h, e = 0..1, 0..3; element 4h + e) of descriptor n: the
+// register order of the network's B operands, so one wave load of 1 KB
+// fetches a whole 16-feature block of 32 descriptors.  1152 bytes per descriptor.
+constexpr int kStreamChunks = kTaps / 16;
+constexpr size_t kTileBytes = (size_t)3 * kStreamChunks * 64 * 16;
+MRP_HD inline size_t tile_chunk(uint32_t item, int k, int c, int kg) {
+    return (((size_t)(item >> 5) * 3 + (size_t)k) * kStreamChunks + (size_t)c) * 64 + (size_t)(kg * 32) + (item & 31u);
+}
+
 #if defined(__HIPCC__)
-// BF16 MFMA path (mrp_mlp_bf16.hip)
-int pack_mlp_bf16(const float* weights_host, void** packed_dev);
-int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, const float* gamma, float g, vec3 srp,
-                    uint32_t n_items, float* pred, hipStream_t stream);
+// MFMA paths (mrp_mlp_mfma.hip); fmt 0: BF16 operands, 1: FP8-E4M3 operands.
+// Descriptors: FP32 rows (feat, `tiled` == nullptr) or wave-tiled BF16 (`tiled`).
+int pack_mlp(const float* weights_host, int fmt, void** packed_dev);
+int launch_mlp(const void* packed, int fmt, const float* feat, const void* tiled, const float* pool,
+               const float* gamma, float g, vec3 srp, uint32_t n_items, float* pred, hipStream_t stream);
 #endif
 
 }  // namespace mrp

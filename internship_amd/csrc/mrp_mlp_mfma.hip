@@ -1,5 +1,5 @@
-// mrp_mlp_bf16.hip - the MRPNN network on gfx950 matrix cores (BF16 operands,
-// FP32 accumulation).
+// mrp_mlp_mfma.hip - the MRPNN network on gfx950 matrix cores: BF16 or FP8-E4M3
+// operands, FP32 accumulation.
 //
 // The reference evaluates the network (`RadiancePredict` lines 370-486,
 // core/radiancePredict.cu) one pixel per thread with scalar loops.  Here one
@@ -22,6 +22,20 @@
 // products; the three gates are exchanged between the two lane groups of an
 // item with one cross-lane move.  The per-segment means and maxima the gates
 // start from come precomputed from the gather kernel.
+//
+// FP8 (v_mfma_f32_32x32x16_fp8_fp8, OCP E4M3) uses the same tiles and register
+// order with 8 one-byte elements per lane.  Scales are powers of two, so scaling
+// and unscaling are exact:
+//   - weights: one scale per layer, 2^(7 - floor(log2 max|W|)), which puts the
+//     layer's largest weight in [128, 256) (E4M3 reaches 448);
+//   - activations: one scale per layer and descriptor, chosen the same way
+//     from the descriptor's largest input magnitude to that layer (both lane
+//     groups of the item, one cross-lane max).  Each lane's accumulator
+//     column belongs to one descriptor, so unscaling is one multiply per
+//     output: D = (W's x X's) * 2^-(sw + sx) + bias.
+// Weights and activations are rounded by the same hardware conversion
+// (v_cvt_pk_fp8_f32, round to nearest even), so both operands share one
+// byte format by construction.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +52,12 @@ namespace mrp {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// Operand formats of the packed network.
+enum class Fmt : int { BF16 = 0, FP8 = 1 };
+template <Fmt F> struct OpOf;
+template <> struct OpOf<Fmt::BF16> { typedef bf16x8 T; };  // 8 BF16 per lane (16 B)
+template <> struct OpOf<Fmt::FP8> { typedef long T; };     // 8 E4M3 per lane (8 B)
 
 // ---------------------------------------------------------------- layer table
 enum : int {
@@ -140,30 +160,72 @@ static uint16_t to_bf16_rne(float x) {
     return (uint16_t)(r >> 16);
 }
 
-// Packed buffer: [tiles][64 lanes][8 bf16], then [bias tiles][2][16] FP32, then
-// 64 FP32 of small VALU layers: LGGS W (24) and b (8), LX12 W (16) and b (1).
-constexpr int kExtraFloats = 64;
+// Packed buffer: [tiles][64 lanes][8 elements] (BF16: 16 B, FP8: 8 B per lane),
+// then [bias tiles][2][16] FP32, then 64 FP32 of small VALU layers: LGGS W (24)
+// and b (8), LX12 W (16) and b (1), then 128 FP32 per-layer weight unscales
+// (FP8: 2^-sw of layer i at index i; BF16: 1).
+constexpr int kExtraFloats = 64, kScaleFloats = 128;
 // packed sizes (checked against layer_table) and the persistent kernel's shape
 constexpr int kTiles = 150, kBiasTiles = 62, kMlpBlock = 512, kMlpWaves = kMlpBlock / 64;
-int pack_mlp_bf16(const float* w, void** packed_dev) {
+static_assert(LY_COUNT <= kScaleFloats, "one weight scale per layer");
+
+template <Fmt F>
+constexpr size_t tile_bytes() {
+    return (size_t)kTiles * 64 * sizeof(typename OpOf<F>::T);
+}
+
+// 8 floats -> 8 E4M3 bytes (element e in byte e), hardware rounding.
+__device__ __forceinline__ long pack8_fp8(float a0, float a1, float a2, float a3, float a4, float a5, float a6,
+                                          float a7) {
+    int lo = __builtin_amdgcn_cvt_pk_fp8_f32(a0, a1, 0, false);
+    lo = __builtin_amdgcn_cvt_pk_fp8_f32(a2, a3, lo, true);
+    int hi = __builtin_amdgcn_cvt_pk_fp8_f32(a4, a5, 0, false);
+    hi = __builtin_amdgcn_cvt_pk_fp8_f32(a6, a7, hi, true);
+    return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned long)(unsigned)lo);
+}
+
+// Pre-scaled FP32 weight tiles -> E4M3 tiles, with the instruction that also
+// converts the activations.
+__global__ void fp8_tiles_kernel(const float* __restrict__ src, long* __restrict__ dst, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float* s = src + (size_t)i * 8;
+    dst[i] = pack8_fp8(s[0], s[1], s[2], s[3], s[4], s[5], s[6], s[7]);
+}
+
+int pack_mlp(const float* w, int fmt, void** packed_dev) {
+    if (fmt != (int)Fmt::BF16 && fmt != (int)Fmt::FP8) return fail(MRP_E_INVALID, "unknown packed network format");
+    const bool fp8 = fmt == (int)Fmt::FP8;
     LayerDesc L[LY_COUNT];
     layer_table(L);
     int nb = 0;
     const int nt = total_tiles(L, &nb);
     if (nt != kTiles || nb != kBiasTiles) return fail(MRP_E_STATE, "packed weight tiles do not match the kernel's LDS layout");
-    std::vector<uint16_t> a((size_t)nt * 64 * 8, 0);
+    std::vector<float> a((size_t)nt * 64 * 8, 0.0f);  // FP32 tiles (FP8: already scaled)
     std::vector<float> bias((size_t)nb * 32, 0.0f);
+    float unscale[kScaleFloats];
+    for (int i = 0; i < kScaleFloats; ++i) unscale[i] = 1.0f;
     for (int i = 0; i < LY_COUNT; ++i) {
         const LayerDesc& d = L[i];
+        float sw = 1.0f;
+        if (fp8) {
+            float m = 0.0f;
+            for (int r = 0; r < d.M * d.K; ++r) m = std::max(m, std::fabs(w[d.w + r]));
+            if (m > 0.0f) {
+                int e = 0;
+                (void)std::frexp(m, &e);  // m = f * 2^e, f in [0.5, 1): m * 2^(8-e) lies in [128, 256)
+                sw = std::ldexp(1.0f, 8 - e);
+                unscale[i] = std::ldexp(1.0f, e - 8);
+            }
+        }
         const int mtn = (d.M + 31) / 32, kbn = (d.K + 15) / 16;
         for (int mt = 0; mt < mtn; ++mt)
             for (int kb = 0; kb < kbn; ++kb) {
-                uint16_t* t = &a[((size_t)(d.tile + mt * kbn + kb)) * 512];
+                float* t = &a[((size_t)(d.tile + mt * kbn + kb)) * 512];
                 for (int l = 0; l < 64; ++l)
                     for (int e = 0; e < 8; ++e) {
                         const int m = mt * 32 + l % 32, f = korder(kb, l / 32, e);
-                        const float val = (m < d.M && f < d.K) ? w[d.w + m * d.K + f] : 0.0f;
-                        t[l * 8 + e] = to_bf16_rne(val);
+                        t[l * 8 + e] = (m < d.M && f < d.K) ? w[d.w + m * d.K + f] * sw : 0.0f;
                     }
             }
         if (d.b >= 0)
@@ -179,15 +241,34 @@ int pack_mlp_bf16(const float* w, void** packed_dev) {
     for (int i = 0; i < 8; ++i) extra[24 + i] = w[MRP_W_LGGSB + i];
     for (int i = 0; i < 16; ++i) extra[32 + i] = w[MRP_W_LX12W + i];
     extra[48] = w[MRP_W_LX12B];
-    const size_t abytes = a.size() * 2, bbytes = bias.size() * 4;
+    const size_t abytes = fp8 ? tile_bytes<Fmt::FP8>() : tile_bytes<Fmt::BF16>(), bbytes = bias.size() * 4;
     void* dev = nullptr;
-    if (hipMalloc(&dev, abytes + bbytes + sizeof(extra)) != hipSuccess)
+    if (hipMalloc(&dev, abytes + bbytes + sizeof(extra) + sizeof(unscale)) != hipSuccess)
         return fail(MRP_E_NOMEM, "packed weight allocation failed");
-    if (hipMemcpy(dev, a.data(), abytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy((char*)dev + abytes, bias.data(), bbytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy((char*)dev + abytes + bbytes, extra, sizeof(extra), hipMemcpyHostToDevice) != hipSuccess) {
+    hipError_t e = hipSuccess;
+    if (fp8) {
+        float* tmp = nullptr;
+        e = hipMalloc(&tmp, a.size() * sizeof(float));
+        if (e == hipSuccess) e = hipMemcpy(tmp, a.data(), a.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) {
+            const int n = nt * 64;
+            fp8_tiles_kernel<<<(n + 255) / 256, 256>>>(tmp, (long*)dev, n);
+            e = hipGetLastError();
+            if (e == hipSuccess) e = hipDeviceSynchronize();
+        }
+        if (tmp) hipFree(tmp);
+    } else {
+        std::vector<uint16_t> h(a.size());
+        for (size_t i = 0; i < a.size(); ++i) h[i] = to_bf16_rne(a[i]);
+        e = hipMemcpy(dev, h.data(), abytes, hipMemcpyHostToDevice);
+    }
+    char* p = (char*)dev + abytes;
+    if (e == hipSuccess) e = hipMemcpy(p, bias.data(), bbytes, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p + bbytes, extra, sizeof(extra), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p + bbytes + sizeof(extra), unscale, sizeof(unscale), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
         hipFree(dev);
-        return fail(MRP_E_HIP, "packed weight upload failed");
+        return hip_fail(e, "packed weight upload");
     }
     MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_layers), L, sizeof(L)));
     *packed_dev = dev;
@@ -195,9 +276,11 @@ int pack_mlp_bf16(const float* w, void** packed_dev) {
 }
 
 // ---------------------------------------------------------------- device helpers
+template <Fmt F>
 struct Packed {
-    const bf16x8* a;   // A tiles
-    const float* bias; // bias tiles
+    const typename OpOf<F>::T* a;  // A tiles
+    const float* bias;             // bias tiles
+    const float* unscale;          // per-layer weight unscales (FP8)
 };
 
 __device__ __forceinline__ bf16x8 to_b(const f32x16& x, int half) {
@@ -207,6 +290,19 @@ __device__ __forceinline__ bf16x8 to_b(const f32x16& x, int half) {
     return r;
 }
 
+__device__ __forceinline__ long to_fp8(const f32x16& x, int half, float s) {
+    const int o = 8 * half;
+    return pack8_fp8(x[o] * s, x[o + 1] * s, x[o + 2] * s, x[o + 3] * s, x[o + 4] * s, x[o + 5] * s, x[o + 6] * s,
+                     x[o + 7] * s);
+}
+
+__device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x16 mfma(long a, long b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, b, c, 0, 0, 0);
+}
+
 __device__ __forceinline__ f32x16 zero16() {
     f32x16 z;
 #pragma unroll
@@ -214,31 +310,71 @@ __device__ __forceinline__ f32x16 zero16() {
     return z;
 }
 
-// D (MT tiles) = W x B (KB blocks) [+ bias], optional ReLU.
-template <int MT, int KB>
-__device__ __forceinline__ void dense(const Packed& pk, int layer, const bf16x8 (&B)[KB], f32x16 (&D)[MT], bool relu,
+__device__ __forceinline__ void load_bias(const float* bias, int btile, int kg, f32x16& acc) {
+    const float4* bp = (const float4*)(bias + ((size_t)btile * 2 + kg) * 16);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 bb = bp[q];
+        acc[4 * q] = bb.x;
+        acc[4 * q + 1] = bb.y;
+        acc[4 * q + 2] = bb.z;
+        acc[4 * q + 3] = bb.w;
+    }
+}
+
+// D (MT tiles) = W x X [+ bias], optional ReLU.  K block kb of the input is
+// registers 8*(kb%2)..8*(kb%2)+7 of X[kb/2].
+template <Fmt F, int MT, int KB>
+__device__ __forceinline__ void dense(const Packed<F>& pk, int layer, const f32x16* X, f32x16 (&D)[MT], bool relu,
                                       int lane) {
     const LayerDesc d = c_layers[layer];
     const int kg = lane >> 5;
+    typename OpOf<F>::T B[KB];
+    float unscale = 1.0f;
+    if constexpr (F == Fmt::BF16) {
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) B[kb] = to_b(X[kb >> 1], kb & 1);
+    } else {
+        float amax = 0.0f;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) amax = fmax_(amax, fabsf(X[kb >> 1][8 * (kb & 1) + e]));
+        amax = fmax_(amax, __shfl_xor(amax, 32));  // the item's other lane group
+        // amax in [2^(eb-127), 2^(eb-126)): scale 2^(134-eb) puts it in [128, 256)
+        uint32_t eb = __float_as_uint(amax) >> 23;
+        eb = eb < 8u ? 8u : (eb > 254u ? 254u : eb);
+        const float sx = __uint_as_float((261u - eb) << 23);
+        unscale = __uint_as_float((eb - 7u) << 23) * pk.unscale[layer];
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) B[kb] = to_fp8(X[kb >> 1], kb & 1, sx);
+    }
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         f32x16 acc;
-        if (d.btile >= 0) {
-            const float4* bp = (const float4*)(pk.bias + ((size_t)(d.btile + mt) * 2 + kg) * 16);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float4 bb = bp[q];
-                acc[4 * q] = bb.x;
-                acc[4 * q + 1] = bb.y;
-                acc[4 * q + 2] = bb.z;
-                acc[4 * q + 3] = bb.w;
+        if constexpr (F == Fmt::BF16) {
+            if (d.btile >= 0) {
+                load_bias(pk.bias, d.btile + mt, kg, acc);
+            } else {
+                acc = zero16();
             }
         } else {
             acc = zero16();
         }
-        const bf16x8* at = pk.a + (size_t)(d.tile + mt * KB) * 64 + lane;
+        const typename OpOf<F>::T* at = pk.a + (size_t)(d.tile + mt * KB) * 64 + lane;
 #pragma unroll
-        for (int kb = 0; kb < KB; ++kb) acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(at[kb * 64], B[kb], acc, 0, 0, 0);
+        for (int kb = 0; kb < KB; ++kb) acc = mfma(at[kb * 64], B[kb], acc);
+        if constexpr (F == Fmt::FP8) {
+            if (d.btile >= 0) {
+                f32x16 b;
+                load_bias(pk.bias, d.btile + mt, kg, b);
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[v] = fmaf(acc[v], unscale, b[v]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < 16; ++v) acc[v] = acc[v] * unscale;
+            }
+        }
         if (relu) {
 #pragma unroll
             for (int v = 0; v < 16; ++v) acc[v] = acc[v] >= 0.0f ? acc[v] : 0.0f;
@@ -264,7 +400,8 @@ __device__ __forceinline__ void load16(const float* row, int base, int kg, f32x1
 __device__ __forceinline__ float sigmoid_(float x) { return 1.0f / (1.0f + expf_(-x)); }
 
 // Squeeze-excitation gates of block `blk`: returns the 3 sigmoid gates of this lane's item.
-__device__ __forceinline__ void se_gates(const Packed& pk, int blk, const float* poolrow, float g, float gamma,
+template <Fmt F>
+__device__ __forceinline__ void se_gates(const Packed<F>& pk, int blk, const float* poolrow, float g, float gamma,
                                          int lane, float* gate) {
     const int kg = lane >> 5;
     // t = [mean A, mean B, mean C, max A, max B, max C, g, gamma]; lane group kg holds t[4kg..4kg+3]
@@ -280,12 +417,10 @@ __device__ __forceinline__ void se_gates(const Packed& pk, int blk, const float*
         t[2] = g;
         t[3] = gamma;
     }
-    bf16x8 b1[1] = {to_b(t, 0)};
     f32x16 h[1];
-    dense<1, 1>(pk, LY_SE0 + blk, b1, h, true, lane);
-    bf16x8 b2[1] = {to_b(h[0], 0)};
+    dense<F, 1, 1>(pk, LY_SE0 + blk, &t, h, true, lane);
     f32x16 o[1];
-    dense<1, 1>(pk, LY_SE1 + blk, b2, o, false, lane);
+    dense<F, 1, 1>(pk, LY_SE1 + blk, h, o, false, lane);
     // rows 0..2 live in lane group 0 (registers 0..2)
     const int src = lane & 31;
     gate[0] = sigmoid_(__shfl(o[0][0], src));
@@ -294,40 +429,62 @@ __device__ __forceinline__ void se_gates(const Packed& pk, int blk, const float*
 }
 
 // ---------------------------------------------------------------- kernel
-// Persistent: one workgroup of 8 waves per CU stages all weight tiles (150 KB
-// BF16), the bias tiles and the small VALU layers into LDS once, then each
-// wave evaluates tiles of 32 descriptors, reading every A operand from LDS.
+// Persistent: one workgroup of 8 waves per CU stages all weight tiles (BF16
+// 150 KB, FP8 75 KB), the bias tiles and the small VALU layers into LDS once,
+// then each wave evaluates tiles of 32 descriptors, reading every A operand
+// from LDS.
 
-__global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __restrict__ packed,
-                                                             const float* __restrict__ feat,
-                                                             const float* __restrict__ pool,
-                                                             const float* __restrict__ gamma_in, float g, vec3 srp,
-                                                             uint32_t n_items, float* __restrict__ pred) {
-    __shared__ bf16x8 s_a[kTiles * 64];
-    __shared__ float s_bias[kBiasTiles * 32];
-    __shared__ float s_extra[kExtraFloats];
+struct MlpArgs {
+    const uint4* packed;
+    const float* feat;    // FP32 rows, or
+    const uint4* tiled;   // wave-tiled BF16 descriptors (tile_chunk)
+    const float* pool;
+    const float* gamma;
+    float g;
+    vec3 srp;
+    uint32_t n_items;
+    float* pred;
+};
+
+// 4 or 8 BF16 (two per 32-bit word, low half first) -> FP32 registers x[o..]
+__device__ __forceinline__ void unpack_bf16(uint32_t w, f32x16& x, int o) {
+    x[o] = __uint_as_float(w << 16);
+    x[o + 1] = __uint_as_float(w & 0xffff0000u);
+}
+
+template <Fmt F, bool kTiled>
+__device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpOf<F>::T* s_a, float* s_bias,
+                                               float* s_extra, float* s_unscale) {
     {
+        const uint4* packed = args.packed;
+        constexpr int kA16 = (int)(tile_bytes<F>() / 16);
         uint4* da = (uint4*)s_a;
-        for (int i = threadIdx.x; i < kTiles * 64; i += kMlpBlock) da[i] = packed[i];
-        const float4* sb = (const float4*)(packed + kTiles * 64);
+        for (int i = threadIdx.x; i < kA16; i += kMlpBlock) da[i] = packed[i];
+        const float4* sb = (const float4*)(packed + kA16);
         float4* db = (float4*)s_bias;
         for (int i = threadIdx.x; i < kBiasTiles * 8; i += kMlpBlock) db[i] = sb[i];
-        const float* se = (const float*)(packed + kTiles * 64) + kBiasTiles * 32;
+        const float* se = (const float*)(packed + kA16) + kBiasTiles * 32;
         for (int i = threadIdx.x; i < kExtraFloats; i += kMlpBlock) s_extra[i] = se[i];
+        if (s_unscale)
+            for (int i = threadIdx.x; i < kScaleFloats; i += kMlpBlock) s_unscale[i] = se[kExtraFloats + i];
     }
     __syncthreads();
+    const float* __restrict__ feat = args.feat;
+    const float* __restrict__ pool = args.pool;
+    const float g = args.g;
+    const uint32_t n_items = args.n_items;
     const int lane = (int)__lane_id();
     const int kg = lane >> 5;
-    const Packed pk{s_a, s_bias};
+    const Packed<F> pk{s_a, s_bias, s_unscale};
     const float* pextra = s_extra;
     const uint32_t n_tiles = (n_items + 31u) / 32u;
     for (uint32_t tile = blockIdx.x * kMlpWaves + (threadIdx.x >> 6); tile < n_tiles;
          tile += gridDim.x * kMlpWaves) {
     const uint32_t item0 = tile * 32u;
     const uint32_t item = min(item0 + (uint32_t)(lane & 31), n_items - 1u);  // tail lanes recompute the last item
-    const float* frow = feat + (size_t)item * kFeatures;
+    const float* frow = kTiled ? nullptr : feat + (size_t)item * kFeatures;
     const float* prow = pool + (size_t)item * 72;
-    const float gamma = gamma_in[item];
+    const float gamma = args.gamma[item];
 
     // ------------------------------------------------ the two descriptor streams
     // prev[k]: previous block output of stream k in D-order (X_ValA2/B2/C2 window)
@@ -342,31 +499,51 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
         const bool residual = (blk != 0 && blk != 8);
         const bool rep = (blk == 1 || blk == 4);  // REP3 after this block, ADD3 otherwise
         float gate[3];
-        se_gates(pk, blk, prow, g, gamma, lane, gate);
+        se_gates<F>(pk, blk, prow, g, gamma, lane, gate);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const float* srow = frow + k * kTaps + from[blk];
             f32x16 xv = zero16();
-            if (S == 8) {
-                const float4 q = *(const float4*)(srow + 4 * kg);
-                xv[0] = q.x;
-                xv[1] = q.y;
-                xv[2] = q.z;
-                xv[3] = q.w;
+            if constexpr (kTiled) {
+                const uint4* tc = args.tiled + tile_chunk(item, k, from[blk] >> 4, kg);
+                if (S == 8) {
+                    const uint2 q = ((const uint2*)tc)[(from[blk] >> 3) & 1];
+                    unpack_bf16(q.x, xv, 0);
+                    unpack_bf16(q.y, xv, 2);
+                } else {
+                    const uint4 q = tc[0];
+                    unpack_bf16(q.x, xv, 0);
+                    unpack_bf16(q.y, xv, 2);
+                    unpack_bf16(q.z, xv, 4);
+                    unpack_bf16(q.w, xv, 6);
+                    if (S == 32) {
+                        const uint4 r = tc[64];  // the next chunk of the stream
+                        unpack_bf16(r.x, xv, 8);
+                        unpack_bf16(r.y, xv, 10);
+                        unpack_bf16(r.z, xv, 12);
+                        unpack_bf16(r.w, xv, 14);
+                    }
+                }
             } else {
-                load16(srow, 0, kg, xv, 0);
-                if (S == 32) load16(srow, 16, kg, xv, 1);
+                const float* srow = frow + k * kTaps + from[blk];
+                if (S == 8) {
+                    const float4 q = *(const float4*)(srow + 4 * kg);
+                    xv[0] = q.x;
+                    xv[1] = q.y;
+                    xv[2] = q.z;
+                    xv[3] = q.w;
+                } else {
+                    load16(srow, 0, kg, xv, 0);
+                    if (S == 32) load16(srow, 16, kg, xv, 1);
+                }
             }
             f32x16 x;
 #pragma unroll
             for (int v = 0; v < 16; ++v) x[v] = xv[v] * gate[k] + (residual ? prev[k][v] : 0.0f);
             f32x16 y[1];
             if (S == 32) {
-                bf16x8 b[2] = {to_b(x, 0), to_b(x, 1)};
-                dense<1, 2>(pk, LY_STREAM + 3 * blk + k, b, y, true, lane);
+                dense<F, 1, 2>(pk, LY_STREAM + 3 * blk + k, &x, y, true, lane);
             } else {
-                bf16x8 b[1] = {to_b(x, 0)};
-                dense<1, 1>(pk, LY_STREAM + 3 * blk + k, b, y, true, lane);
+                dense<F, 1, 1>(pk, LY_STREAM + 3 * blk + k, &x, y, true, lane);
             }
             if (rep) {
                 // window = [block output | this block's input features]
@@ -387,9 +564,8 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
         if (blk == 7) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                bf16x8 b[2] = {to_b(prev[k], 0), to_b(prev[k], 1)};
                 f32x16 o[1];
-                dense<1, 2>(pk, LY_L81 + k, b, o, true, lane);
+                dense<F, 1, 2>(pk, LY_L81 + k, &prev[k], o, true, lane);
                 comb[k] = o[0];
                 prev[k] = zero16();
             }
@@ -397,14 +573,14 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        bf16x8 b[1] = {to_b(prev[k], 0)};
         f32x16 o[1];
-        dense<1, 1>(pk, LY_LD41 + k, b, o, true, lane);
+        dense<F, 1, 1>(pk, LY_LD41 + k, &prev[k], o, true, lane);
 #pragma unroll
         for (int v = 0; v < 4; ++v) lout[k][v] = o[0][v];
     }
 
     // ------------------------------------------------ head, once per distinct channel
+    const vec3 srp = args.srp;
     const float sr[3] = {srp.x, srp.y, srp.z};
     float P[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll 1
@@ -412,14 +588,11 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
         if (c > 0 && sr[c] == sr[0]) { P[c] = P[0]; continue; }
         if (c == 2 && sr[2] == sr[1]) { P[2] = P[1]; continue; }
         // SE of the head: LSEFin1 (16x75) on the pool + [g, gamma, srp], LSEFin2 (6x16)
-        bf16x8 bp[5];
+        f32x16 tp[3];
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-            f32x16 t;
-            load16(prow, 32 * j, kg, t, 0);
-            load16(prow, 32 * j + 16, kg, t, 1);
-            bp[2 * j] = to_b(t, 0);
-            bp[2 * j + 1] = to_b(t, 1);
+            load16(prow, 32 * j, kg, tp[j], 0);
+            load16(prow, 32 * j + 16, kg, tp[j], 1);
         }
         {
             f32x16 t = zero16();
@@ -440,13 +613,12 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
                 t[2] = q.z;
                 t[3] = q.w;
             }
-            bp[4] = to_b(t, 0);
+            tp[2] = t;
         }
         f32x16 s1[1];
-        dense<1, 5>(pk, LY_FIN1, bp, s1, true, lane);
-        bf16x8 bs[1] = {to_b(s1[0], 0)};
+        dense<F, 1, 5>(pk, LY_FIN1, tp, s1, true, lane);
         f32x16 s2[1];
-        dense<1, 1>(pk, LY_FIN2, bs, s2, false, lane);
+        dense<F, 1, 1>(pk, LY_FIN2, s1, s2, false, lane);
         float s[6];
         const int src0 = lane & 31, src1 = (lane & 31) + 32;
         s[0] = sigmoid_(__shfl(s2[0][0], src0));
@@ -455,8 +627,14 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
         s[3] = sigmoid_(__shfl(s2[0][3], src0));
         s[4] = sigmoid_(__shfl(s2[0][0], src1));
         s[5] = sigmoid_(__shfl(s2[0][1], src1));
+        // head input: [comb A * s0 | comb B * s1 | comb C * s2 | LD41 x s3..s5, LGGS]
+        f32x16 x0[4];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) x0[j][v] = comb[j][v] * s[j];
+        }
         // LGGS (8x3) rows 4kg..4kg+3 on [g, gamma, srp_c]
-        f32x16 x3;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = 4 * kg + v;
@@ -465,46 +643,25 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
             acc = fmaf(gamma, pextra[r * 3 + 1], acc);
             acc = fmaf(sr[c], pextra[r * 3 + 2], acc);
             acc += pextra[24 + r];
-            x3[12 + v] = acc >= 0.0f ? acc : 0.0f;
-            x3[v] = lout[0][v] * s[3];
-            x3[4 + v] = lout[1][v] * s[4];
-            x3[8 + v] = lout[2][v] * s[5];
+            x0[3][12 + v] = acc >= 0.0f ? acc : 0.0f;
+            x0[3][v] = lout[0][v] * s[3];
+            x0[3][4 + v] = lout[1][v] * s[4];
+            x0[3][8 + v] = lout[2][v] * s[5];
         }
-        bf16x8 b0[8];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) {
-            f32x16 t;
-#pragma unroll
-            for (int v = 0; v < 16; ++v) t[v] = comb[j][v] * s[j];
-            b0[2 * j] = to_b(t, 0);
-            b0[2 * j + 1] = to_b(t, 1);
-        }
-        b0[6] = to_b(x3, 0);
-        b0[7] = to_b(x3, 1);
         f32x16 h0[4];
-        dense<4, 8>(pk, LY_LC0, b0, h0, true, lane);
-        bf16x8 b1[8];
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            b1[2 * j] = to_b(h0[j], 0);
-            b1[2 * j + 1] = to_b(h0[j], 1);
-        }
+        dense<F, 4, 8>(pk, LY_LC0, x0, h0, true, lane);
         f32x16 h1[2];
-        dense<2, 8>(pk, LY_LC1, b1, h1, true, lane);
-        bf16x8 b2[4] = {to_b(h1[0], 0), to_b(h1[0], 1), to_b(h1[1], 0), to_b(h1[1], 1)};
+        dense<F, 2, 8>(pk, LY_LC1, h0, h1, true, lane);
         f32x16 h2[1];
-        dense<1, 4>(pk, LY_LC2, b2, h2, true, lane);
-        bf16x8 b3[2] = {to_b(h2[0], 0), to_b(h2[0], 1)};
+        dense<F, 1, 4>(pk, LY_LC2, h1, h2, true, lane);
         f32x16 a[1];
-        dense<1, 2>(pk, LY_LX, b3, a, true, lane);
+        dense<F, 1, 2>(pk, LY_LX, h2, a, true, lane);
 #pragma unroll
         for (int r = 0; r < 6; ++r) {
-            bf16x8 ba[1] = {to_b(a[0], 0)};
             f32x16 hb[1];
-            dense<1, 1>(pk, LY_LXR + 2 * r, ba, hb, true, lane);
-            bf16x8 bb[1] = {to_b(hb[0], 0)};
+            dense<F, 1, 1>(pk, LY_LXR + 2 * r, a, hb, true, lane);
             f32x16 hc[1];
-            dense<1, 1>(pk, LY_LXR + 2 * r + 1, bb, hc, true, lane);
+            dense<F, 1, 1>(pk, LY_LXR + 2 * r + 1, hb, hc, true, lane);
 #pragma unroll
             for (int v = 0; v < 8; ++v) a[0][v] = a[0][v] + hc[0][v];
         }
@@ -517,11 +674,44 @@ __global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(const uint4* __rest
         P[c] = pc >= 0.0f ? pc : 0.0f;
     }
     if (kg == 0 && item0 + (uint32_t)lane < n_items) {
+        float* pred = args.pred;
         pred[3 * item] = fmax_(expf_(P[0]) - 1.0f, 0.0f) * sr[0];
         pred[3 * item + 1] = fmax_(expf_(P[1]) - 1.0f, 0.0f) * sr[1];
         pred[3 * item + 2] = fmax_(expf_(P[2]) - 1.0f, 0.0f) * sr[2];
     }
     }  // tile loop
+}
+
+// Renderer: wave-tiled BF16 descriptors.
+__global__ __launch_bounds__(kMlpBlock) void mlp_bf16_kernel(MlpArgs args) {
+    __shared__ bf16x8 s_a[kTiles * 64];
+    __shared__ float s_bias[kBiasTiles * 32];
+    __shared__ float s_extra[kExtraFloats];
+    mlp_persistent<Fmt::BF16, true>(args, s_a, s_bias, s_extra, nullptr);
+}
+
+__global__ __launch_bounds__(kMlpBlock) void mlp_fp8_kernel(MlpArgs args) {
+    __shared__ long s_a[kTiles * 64];
+    __shared__ float s_bias[kBiasTiles * 32];
+    __shared__ float s_extra[kExtraFloats];
+    __shared__ float s_unscale[kScaleFloats];
+    mlp_persistent<Fmt::FP8, true>(args, s_a, s_bias, s_extra, s_unscale);
+}
+
+// Caller-given descriptors (mrp_mlp_forward): FP32 rows.
+__global__ __launch_bounds__(kMlpBlock) void mlp_bf16_rows_kernel(MlpArgs args) {
+    __shared__ bf16x8 s_a[kTiles * 64];
+    __shared__ float s_bias[kBiasTiles * 32];
+    __shared__ float s_extra[kExtraFloats];
+    mlp_persistent<Fmt::BF16, false>(args, s_a, s_bias, s_extra, nullptr);
+}
+
+__global__ __launch_bounds__(kMlpBlock) void mlp_fp8_rows_kernel(MlpArgs args) {
+    __shared__ long s_a[kTiles * 64];
+    __shared__ float s_bias[kBiasTiles * 32];
+    __shared__ float s_extra[kExtraFloats];
+    __shared__ float s_unscale[kScaleFloats];
+    mlp_persistent<Fmt::FP8, false>(args, s_a, s_bias, s_extra, s_unscale);
 }
 
 // Per-segment means / maxima from FP32 descriptors (used when the caller has
@@ -549,9 +739,12 @@ __global__ void pool_kernel(const float* __restrict__ feat, uint32_t n_items, fl
 
 static int g_num_cus[64] = {};
 
-int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, const float* gamma, float g, vec3 srp,
-                    uint32_t n_items, float* pred, hipStream_t stream) {
+int launch_mlp(const void* packed, int fmt, const float* feat, const void* tiled, const float* pool,
+               const float* gamma, float g, vec3 srp, uint32_t n_items, float* pred, hipStream_t stream) {
+    if (fmt != (int)Fmt::BF16 && fmt != (int)Fmt::FP8) return fail(MRP_E_INVALID, "unknown packed network format");
     if (n_items == 0) return MRP_OK;
+    if (tiled && !pool) return fail(MRP_E_INVALID, "tiled descriptors need the gather's pooled statistics");
+    if (!tiled && !feat) return fail(MRP_E_INVALID, "no descriptors");
     int dev = 0;
     MRP_HIP(hipGetDevice(&dev));
     int& cus = g_num_cus[dev & 63];
@@ -564,14 +757,24 @@ int launch_mlp_bf16(const void* packed, const float* feat, const float* pool, co
     }
     const uint32_t tiles = (n_items + 31) / 32;
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)cus, (tiles + kMlpWaves - 1) / kMlpWaves));
-    mlp_bf16_kernel<<<blocks, kMlpBlock, 0, stream>>>((const uint4*)packed, feat, pool, gamma, g, srp, n_items, pred);
+    const MlpArgs args{(const uint4*)packed, feat, (const uint4*)tiled, pool, gamma, g, srp, n_items, pred};
+    if (fmt == (int)Fmt::FP8) {
+        if (tiled)
+            mlp_fp8_kernel<<<blocks, kMlpBlock, 0, stream>>>(args);
+        else
+            mlp_fp8_rows_kernel<<<blocks, kMlpBlock, 0, stream>>>(args);
+    } else {
+        if (tiled)
+            mlp_bf16_kernel<<<blocks, kMlpBlock, 0, stream>>>(args);
+        else
+            mlp_bf16_rows_kernel<<<blocks, kMlpBlock, 0, stream>>>(args);
+    }
     hipError_t e = hipGetLastError();
     if (tmp) {
         hipStreamSynchronize(stream);
         hipFree(tmp);
     }
-    if (e != hipSuccess) return hip_fail(e, "mlp_bf16_kernel");
+    if (e != hipSuccess) return hip_fail(e, fmt == (int)Fmt::FP8 ? "mlp_fp8_kernel" : "mlp_bf16_kernel");
     return MRP_OK;
 }
-
 }  // namespace mrp

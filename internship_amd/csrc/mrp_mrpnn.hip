@@ -10,7 +10,7 @@
 //      (g = 0), random view/light frames, single-scatter Tr * HG;
 //   2. gather_kernel       : one thread per (point, tap) - 192 taps x
 //      {density mip, transmittance mip, HG table} -> 576 features + gamma;
-//   3. mlp (FP32 here, BF16 MFMA in mrp_mlp_bf16.hip) - the network;
+//   3. mlp (FP32 here, BF16 / FP8 MFMA in mrp_mlp_mfma.hip) - the network;
 //   4. combine_kernel      : (predict + single) * light * albedo, or sky.
 //
 // The per-(sample, pixel) results then go through the same in-order running
@@ -184,22 +184,32 @@ struct GatherArgs {
 __constant__ int c_seg_from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
 __constant__ int c_seg_size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
 
-// One block (192 threads) per point, one thread per tap.  feat[item][576] =
-// [Xd | Xt | Xh] (FP32), gamma[item] = acos(X . LX), and pool[item][72] =
-// the per-segment means (0..35) and maxima (36..71) that the network's
-// squeeze-excitation blocks start from (`AVGMAX`, core/radiancePredict.cu:21-33:
-// ascending sum then /size; max from -10000).
+// One thread per tap: the descriptor [Xd | Xt | Xh] (576 features),
+// gamma[item] = acos(X . LX), and pool[item][72] = the per-segment means
+// (0..35) and maxima (36..71) that the network's squeeze-excitation blocks
+// start from (`AVGMAX`, core/radiancePredict.cu:21-33: ascending sum then
+// /size; max from -10000).  Descriptors go out as FP32 rows (feat[item][576])
+// or, with kTiled, as wave-tiled BF16 chunks (tile_chunk), which threads
+// 96..167 store 16 bytes at a time from the LDS copy of the descriptor.
 // One workgroup (192 threads, one tap each) loops over kGatherItems points, so
-// the dispatcher launches 1/kGatherItems as many workgroups; the pooled
-// statistics use a double-buffered LDS row, one barrier per point.
+// the dispatcher launches 1/kGatherItems as many workgroups; the LDS row is
+// double-buffered, one barrier per point.
 constexpr int kGatherItems = 16;
+static_assert(kGatherItems % 8 == 0, "the tiled gather flushes groups of 8 descriptors");
 
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <bool kTiled>
 __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
-                                                       float* __restrict__ feat, float* __restrict__ gamma,
+                                                       void* __restrict__ feat_out, float* __restrict__ gamma,
                                                        float* __restrict__ pool) {
     __shared__ float sbuf[2][kFeatures];
+    // tiled: BF16 chunks of kStageItems consecutive descriptors, flushed as whole 128-byte lines
+    constexpr int kStageItems = 8, kChunks = 3 * 2 * kStreamChunks;
+    __shared__ bf16x8_t stage[kTiled ? kChunks * kStageItems : 1];
+    float* __restrict__ feat = (float*)feat_out;
     const int i = threadIdx.x;
     const int mip = c_tap_mip[i];
     const float off = c_tap_offset[i];
@@ -214,7 +224,6 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
         const float4 pa = pos_act[src], x4 = fx[src], y4 = fy[src], z4 = fz[src];
         const vec3 p = v3(pa.x, pa.y, pa.z);
         const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
-        float* out = feat + (size_t)item * kFeatures;
         if (i == 0) gamma[item] = acosf_(dot(X, a.lx));
         float xd = 0.0f, xt = 0.0f, xh = 0.0f;
         if (pa.w != 0.0f) {
@@ -242,16 +251,31 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
                 xt = trilinear_clamp_cells(a.trs[tm], tr, pi + 0.5f);
             }
         }
-        out[i] = xd;
-        out[kTaps + i] = xt;
-        out[2 * kTaps + i] = xh;
-        if (pool) {
+        if (!kTiled) {
+            float* out = feat + (size_t)item * kFeatures;
+            out[i] = xd;
+            out[kTaps + i] = xt;
+            out[2 * kTaps + i] = xh;
+        }
+        if (kTiled || pool) {
             float* sf = sbuf[it & 1];
             sf[i] = xd;
             sf[kTaps + i] = xt;
             sf[2 * kTaps + i] = xh;
             __syncthreads();
-            if (i < 72) {
+            if (kTiled && i >= 96 && i < 96 + kChunks) {
+                // chunk q = (stream k, 16-feature chunk c, lane group kg)
+                const int q = i - 96, k = q / (2 * kStreamChunks), c = (q >> 1) % kStreamChunks, kg = q & 1;
+                const float* src = sf + k * kTaps + 16 * c + 4 * kg;
+                bf16x8_t v;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    v[e] = (__bf16)src[e];
+                    v[4 + e] = (__bf16)src[8 + e];
+                }
+                stage[q * kStageItems + (it & (kStageItems - 1))] = v;
+            }
+            if (pool && i < 72) {
                 const int q = i % 36, blk = q / 3, k = q % 3;
                 const float* seg = sf + k * kTaps + c_seg_from[blk];
                 const int n = c_seg_size[blk];
@@ -265,6 +289,19 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
                     for (int j = 0; j < n; ++j) r = fmax_(r, seg[j]);
                 }
                 pool[(size_t)item * 72 + i] = r;
+            }
+            if (kTiled && ((it & (kStageItems - 1)) == kStageItems - 1 || item + 1 >= a.n_items)) {
+                // uniform over the block; items start at multiples of kStageItems (kGatherItems is one)
+                __syncthreads();
+                const int cnt = (it & (kStageItems - 1)) + 1;
+                const uint32_t base = item + 1u - (uint32_t)cnt;
+                for (int idx = i; idx < kChunks * kStageItems; idx += kTaps) {
+                    const int q = idx / kStageItems, j = idx % kStageItems;
+                    if (j < cnt) {
+                        const int k = q / (2 * kStreamChunks), c = (q >> 1) % kStreamChunks, kg = q & 1;
+                        ((bf16x8_t*)feat_out)[tile_chunk(base + (uint32_t)j, k, c, kg)] = stage[idx];
+                    }
+                }
             }
         }
     }
@@ -384,9 +421,10 @@ static vec3 srp_of(vec3 scatter_rate) {
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
                   double* gather_ms, double* mlp_ms, uint64_t* n_active) {
     if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
-    if (p->mlp_precision != 0 && p->mlp_precision != 1)
-        return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32) or 1 (BF16)");
-    if (p->mlp_precision == 1 && !v->nn_packed) return fail(MRP_E_STATE, "network has no BF16 packing");
+    if (p->mlp_precision < 0 || p->mlp_precision > 2)
+        return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
+    const void* packed = p->mlp_precision == 2 ? v->nn_packed_fp8 : v->nn_packed;
+    if (p->mlp_precision != 0 && !packed) return fail(MRP_E_STATE, "network has no MFMA packing");
     int rc = upload_tap_tables(v->device);
     if (rc) return rc;
     rc = ensure_tr(v, p->light_dir, p->alpha);
@@ -399,6 +437,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     const size_t Pl = (size_t)wl * H;
     const size_t total = (size_t)n * Pl;
     const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, total));
+    static_assert(2 * kTileBytes == (size_t)32 * kFeatures * sizeof(float), "tiled BF16 descriptors fit the FP32 rows");
     void* scratch = nullptr;
     rc = ensure_nn_scratch(v, chunk_items * kNNItemBytes + 4096, &scratch);
     if (rc) return rc;
@@ -474,8 +513,13 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         if (n_active) *n_active += n_act;
         ga.n_items = n_act;
         if (n_act > 0) {
-            gather_kernel<<<(n_act + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz,
-                                                                                              list, feat, gam, pool);
+            // the MFMA networks read wave-tiled BF16 descriptors (same region: ceil(n/32) tiles
+            // of kTileBytes fit in n FP32 rows for n >= 32); the FP32 network reads FP32 rows
+            const unsigned gblocks = (n_act + kGatherItems - 1) / kGatherItems;
+            if (p->mlp_precision == 0)
+                gather_kernel<false><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam, pool);
+            else
+                gather_kernel<true><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam, pool);
             MRP_HIP(hipGetLastError());
         }
         MRP_HIP(hipEventRecord(t2, v->stream));
@@ -484,7 +528,8 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
                 mlp_fp32_kernel<<<(n_act + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, n_act,
                                                                          pred);
             } else {
-                rc = launch_mlp_bf16(v->nn_packed, feat, pool, gam, p->g, srp, n_act, pred, v->stream);
+                rc = launch_mlp(packed, p->mlp_precision - 1, nullptr, feat, pool, gam, p->g, srp, n_act, pred,
+                                v->stream);
                 if (rc) break;
             }
             MRP_HIP(hipGetLastError());
@@ -560,7 +605,8 @@ using namespace mrp;
 
 struct mrp_network {
     float* w = nullptr;    // FP32 weights (HBM)
-    void* packed = nullptr;  // BF16 MFMA operand packing (HBM)
+    void* packed = nullptr;      // BF16 MFMA operand packing (HBM)
+    void* packed_fp8 = nullptr;  // FP8-E4M3 MFMA operand packing and per-layer scales (HBM)
     int device = 0;
 };
 
@@ -582,9 +628,11 @@ int mrp_network_create(const float* weights, size_t count, mrp_network** out) {
         delete n;
         return fail(MRP_E_HIP, "weight upload failed");
     }
-    const int rc = pack_mlp_bf16(weights, &n->packed);
+    int rc = pack_mlp(weights, 0, &n->packed);
+    if (rc == MRP_OK) rc = pack_mlp(weights, 1, &n->packed_fp8);
     if (rc) {
         hipFree(n->w);
+        hipFree(n->packed);
         delete n;
         return rc;
     }
@@ -596,6 +644,7 @@ int mrp_network_destroy(mrp_network* n) {
     if (!n) return MRP_OK;
     hipFree(n->w);
     hipFree(n->packed);
+    hipFree(n->packed_fp8);
     delete n;
     return MRP_OK;
 }
@@ -625,7 +674,7 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     ga.alpha = alpha;
     ga.g = g;
     ga.n_items = (uint32_t)n;
-    gather_kernel<<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
+    gather_kernel<false><<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
         ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev, (const float4*)frame_y_dev,
         (const float4*)frame_z_dev, nullptr, feat_dev, gamma_dev, nullptr);
     MRP_HIP(hipGetLastError());
@@ -643,12 +692,14 @@ int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const 
     if (precision == 0) {
         mlp_fp32_kernel<<<(n + 63) / 64, 64>>>(net->w, feat_dev, gamma_dev, g, s, (uint32_t)n, out_dev);
         MRP_HIP(hipGetLastError());
-    } else if (precision == 1) {
-        if (!net->packed) return fail(MRP_E_STATE, "network has no BF16 packing");
-        const int rc = launch_mlp_bf16(net->packed, feat_dev, nullptr, gamma_dev, g, s, (uint32_t)n, out_dev, nullptr);
+    } else if (precision == 1 || precision == 2) {
+        const void* packed = precision == 2 ? net->packed_fp8 : net->packed;
+        if (!packed) return fail(MRP_E_STATE, "network has no MFMA packing");
+        const int rc =
+            launch_mlp(packed, precision - 1, feat_dev, nullptr, nullptr, gamma_dev, g, s, (uint32_t)n, out_dev, nullptr);
         if (rc) return rc;
     } else {
-        return fail(MRP_E_INVALID, "precision must be 0 (FP32) or 1 (BF16)");
+        return fail(MRP_E_INVALID, "precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
     }
     MRP_HIP(hipDeviceSynchronize());
     return MRP_OK;
@@ -658,6 +709,7 @@ int mrp_volume_set_network(mrp_volume* vol, const mrp_network* n) {
     if (!vol) return fail(MRP_E_INVALID, "NULL volume");
     vol->v.nn_weights = n ? n->w : nullptr;
     vol->v.nn_packed = n ? n->packed : nullptr;
+    vol->v.nn_packed_fp8 = n ? n->packed_fp8 : nullptr;
     return MRP_OK;
 }
 

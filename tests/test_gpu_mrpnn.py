@@ -179,11 +179,16 @@ def test_gpu_mlp_bf16_tracks_fp32_oracle():
     """BF16 operands / FP32 accumulation on MFMA versus the FP32 oracle.
     Tolerance: BF16 keeps 8 significand bits (relative rounding 2^-9 per
     operand); through 20 layers the network output stays within a few percent
-    (relative L2 over the batch <= 3e-2, per-item relative error <= 1.5e-1)."""
+    (relative L2 over the batch <= 3e-2, per-item relative error <= 1.5e-1).
+    Against the network in FP64 with the same BF16 rounding applied on the CPU
+    (tests/lowp.py) the distance is far smaller: relative L2 <= 5e-3."""
+    from tests.lowp import bf16_layer
+    from tests.test_oracle_mrpnn_pins import _weights64, forward64
     _say("case: BF16 MFMA network on 1000 synthetic descriptors (ragged tail)")
     n = 1000  # not a multiple of the 32-item wave tile
     feat, gamma = _synthetic_descriptors(n, 12)
     w = load_reference_weights()
+    w64 = _weights64()
     for sr in ((0.99, 0.99, 0.99), (0.99, 0.95, 0.9)):
         srp = srp_of(sr)
         net = Network(w)
@@ -196,7 +201,96 @@ def test_gpu_mlp_bf16_tracks_fp32_oracle():
         ref = oracle.mrpnn_forward(w, feat, gamma, 0.857, srp).astype(np.float64)
         rel_l2 = np.linalg.norm(out - ref) / np.linalg.norm(ref)
         per = np.abs(out - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
-        _say("srp %s: rel L2 %.3g, max per-item rel %.3g, median %.3g" % (sr, rel_l2, per.max(), np.median(per)))
+        emu = forward64(w64, feat, gamma.astype(np.float64), np.float64(np.float32(0.857)),
+                        np.asarray(srp, np.float32).astype(np.float64), quant=bf16_layer)
+        rel_emu = np.linalg.norm(out - emu) / np.linalg.norm(emu)
+        _say("srp %s: rel L2 %.3g (vs BF16 emulation %.3g), max per-item rel %.3g, median %.3g"
+             % (sr, rel_l2, rel_emu, per.max(), np.median(per)))
         assert np.all(np.isfinite(out))
         assert rel_l2 <= 3e-2
         assert per.max() <= 1.5e-1
+        assert rel_emu <= 5e-3
+
+
+def test_gpu_mlp_fp8_matches_e4m3_emulation():
+    """FP8-E4M3 operands / FP32 accumulation on MFMA against two references:
+      - the network in FP64 with the same E4M3 rounding applied on the CPU
+        (tests/lowp.py: weights with one power-of-two scale per layer, inputs
+        with one per layer and descriptor).  Most descriptors agree to float
+        precision: median per-item relative difference <= 1e-4, and at most 10%
+        of the items differ by more than 1e-3.  Those are items where FP32
+        arithmetic (versus FP64) moved an activation across an E4M3 rounding
+        boundary, which changes it by one E4M3 step (6-12%) and propagates;
+        the CPU emulation itself moves this much (3% of items, relative L2
+        0.036) under 1e-6 relative noise on the layer outputs, hence relative
+        L2 <= 0.06.  A wrong scale exponent, byte order or layer moves every item;
+      - the FP32 oracle: relative L2 <= 0.25 and median per-item error <= 0.18.
+        E4M3 keeps 4 significand bits against BF16's 8, so the error is about 16x
+        BF16's (0.012): 0.18 on this input, which the emulation reproduces.
+    The output must also differ from BF16's, which shows the FP8 instructions ran."""
+    from tests.lowp import fp8_layer
+    from tests.test_oracle_mrpnn_pins import _weights64, forward64
+    _say("case: FP8 MFMA network on 1000 synthetic descriptors (ragged tail)")
+    n = 1000
+    feat, gamma = _synthetic_descriptors(n, 12)
+    w = load_reference_weights()
+    w64 = _weights64()
+    for sr in ((0.99, 0.99, 0.99), (0.99, 0.95, 0.9)):
+        srp = srp_of(sr)
+        net = Network(w)
+        fb = DeviceBuffer(feat.nbytes)
+        fb.upload(feat)
+        gb = DeviceBuffer(gamma.nbytes)
+        gb.upload(gamma)
+        out8 = net.forward(fb, gb, n, 0.857, srp, precision=2).download(np.float32, (n, 3))
+        out16 = net.forward(fb, gb, n, 0.857, srp, precision=1).download(np.float32, (n, 3))
+        net.close()
+        ref = oracle.mrpnn_forward(w, feat, gamma, 0.857, srp).astype(np.float64)
+        emu = forward64(w64, feat, gamma.astype(np.float64), np.float64(np.float32(0.857)),
+                        np.asarray(srp, np.float32).astype(np.float64), quant=fp8_layer)
+        rel_emu = np.linalg.norm(out8 - emu) / np.linalg.norm(emu)
+        per_emu = np.abs(out8 - emu) / np.maximum(np.abs(emu), 1e-3 * np.abs(emu).max())
+        rel_l2 = np.linalg.norm(out8 - ref) / np.linalg.norm(ref)
+        rel16 = np.linalg.norm(out16 - ref) / np.linalg.norm(ref)
+        per = np.abs(out8 - ref) / np.maximum(np.abs(ref), 1e-3 * np.abs(ref).max())
+        _say("srp %s: FP8 vs E4M3 emulation: rel L2 %.3g, median per-item %.3g, items > 1e-3: %.3f; "
+             "vs FP32 rel L2 %.3g (BF16 %.3g), median per-item %.3g"
+             % (sr, rel_emu, np.median(per_emu), np.mean(per_emu > 1e-3), rel_l2, rel16, np.median(per)))
+        assert np.all(np.isfinite(out8))
+        assert not np.array_equal(out8, out16)
+        assert np.median(per_emu) <= 1e-4
+        assert np.mean(per_emu > 1e-3) <= 0.1
+        assert rel_emu <= 0.06
+        assert rel_l2 <= 0.25
+        assert np.median(per) <= 0.18
+
+
+def test_gpu_render_mrpnn_mfma_precisions_track_fp32(cloud64, oracle_tables):
+    """The MRPNN renderer with the BF16 and FP8 networks against the oracle's
+    FP32 image (same first vertices, descriptors and random streams; only the
+    network differs): relative L2 over the image <= 2e-2 (BF16) and <= 0.1
+    (FP8).  The predicted radiance is a part of each scattering pixel, so the
+    image errors are smaller than the network's (BF16 0.012, FP8 0.18)."""
+    _, mc, _, tc, lut = oracle_tables
+    w = load_reference_weights()
+    p = config1_params(size=48, spp=2, multi=1, seed=0)
+    p.render_type = 2
+    cpu = None
+    for prec, tol in ((1, 2e-2), (2, 0.1)):
+        _say("case: MRPNN render 48x48, 2 spp, cloud 64^3, precision %d" % prec)
+        p.mlp_precision = prec
+        vol = VolumeRender(64)
+        vol.set_datas(cloud64)
+        vol.update()
+        net = Network(w)
+        vol.set_network(net)
+        gpu = np.zeros((48, 48, 3), np.float32)
+        _lib.check(vol.lib.mrp_render_offline(vol.h, ctypes.byref(p), ctypes.c_void_p(gpu.ctypes.data), None))
+        vol.close()
+        net.close()
+        if cpu is None:
+            cpu = oracle.render_mrpnn(cloud64, oracle.max_density(cloud64), mc, tc, lut, w, p)
+        rel = np.linalg.norm(gpu - cpu) / np.linalg.norm(cpu)
+        _say("precision %d: image rel L2 %.3g" % (prec, rel))
+        assert np.all(np.isfinite(gpu))
+        assert rel <= tol

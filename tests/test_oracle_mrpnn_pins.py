@@ -295,9 +295,18 @@ def _relu(x):
     return np.maximum(x, 0)
 
 
-def forward64(W, feat, gamma, g, srp):
-    """FP64 restatement of RadiancePredict's network (core/radiancePredict.cu:370-486) in matrix form."""
+def forward64(W, feat, gamma, g, srp, quant=None):
+    """FP64 restatement of RadiancePredict's network (core/radiancePredict.cu:370-486) in matrix form.
+    quant(name, W, x) -> (W', x'), if given, replaces a layer's weights and
+    input before its product (reduced-precision emulation)."""
     n = feat.shape[0]
+
+    def lin(wname, x, bname=None):
+        Wm = W[wname]
+        if quant is not None:
+            Wm, x = quant(wname, Wm, x)
+        return _lin(Wm, x, None if bname is None else W[bname])
+
     xin = [feat[:, 0:192].astype(np.float64), feat[:, 192:384].astype(np.float64), feat[:, 384:576].astype(np.float64)]
     y = [np.zeros((n, 160)) for _ in range(3)]
     pool = np.zeros((n, 75))
@@ -313,13 +322,13 @@ def forward64(W, feat, gamma, g, srp):
             pool[:, 36 + pid + k] = np.maximum(seg[k].max(axis=1), -10000.0)
         t = np.concatenate([pool[:, pid:pid + 3], pool[:, 36 + pid:39 + pid],
                             np.full((n, 1), g), gamma[:, None].astype(np.float64)], axis=1)
-        h = _relu(_lin(W[sew0], t))
-        s = _sig(_lin(W[sew1], h))
+        h = _relu(lin(sew0, t))
+        s = _sig(lin(sew1, h))
         for k in range(3):
             x = seg[k] * s[:, k:k + 1]
             if res_from is not None:
                 x = x + y[k][:, res_from:res_from + size]
-            y[k][:, dst:dst + size] = _relu(_lin(W[nm[k] % (tag + "W")], x, W[nm[k] % (tag + "B")]))
+            y[k][:, dst:dst + size] = _relu(lin(nm[k] % (tag + "W"), x, nm[k] % (tag + "B")))
 
     def add3(src, dst, size):
         for k in range(3):
@@ -338,27 +347,27 @@ def forward64(W, feat, gamma, g, srp):
     se(names, "LSE51W", "LSE52W", 32, 64, 64, 32, 15, "51"); add3(64, 64, 32)
     se(names, "LSE61W", "LSE62W", 64, 96, 96, 32, 18, "61"); add3(96, 96, 32)
     se(names, "LSE71W", "LSE72W", 96, 128, 128, 32, 21, "71"); add3(128, 128, 32)
-    comb = [_relu(_lin(W[names[k] % "81W"], y[k][:, 128:160], W[names[k] % "81B"])) for k in range(3)]
+    comb = [_relu(lin(names[k] % "81W", y[k][:, 128:160], names[k] % "81B")) for k in range(3)]
     se(dnames, "LDSE01W", "LDSE02W", None, 160, 0, 8, 24, "01"); add3(160, 0, 8)
     se(dnames, "LDSE11W", "LDSE12W", 0, 168, 8, 8, 27, "11"); add3(168, 8, 8)
     se(dnames, "LDSE21W", "LDSE22W", 8, 176, 16, 8, 30, "21"); add3(176, 16, 8)
     se(dnames, "LDSE31W", "LDSE32W", 16, 184, 24, 8, 33, "31"); add3(184, 24, 8)
-    comb += [_relu(_lin(W[dnames[k] % "41W"], y[k][:, 24:32], W[dnames[k] % "41B"])) for k in range(3)]
+    comb += [_relu(lin(dnames[k] % "41W", y[k][:, 24:32], dnames[k] % "41B")) for k in range(3)]
     comb = np.concatenate(comb, axis=1)  # (n, 120)
     out = np.zeros((n, 3))
     for c in range(3):
         pool[:, 74] = srp[c]
-        gg = _relu(_lin(W["LGGSW"], pool[:, 72:75], W["LGGSB"]))
-        a = _relu(_lin(W["LSEFin1W"], pool))
-        s = _sig(_lin(W["LSEFin2W"], a))
+        gg = _relu(lin("LGGSW", pool[:, 72:75], "LGGSB"))
+        a = _relu(lin("LSEFin1W", pool))
+        s = _sig(lin("LSEFin2W", a))
         scale = np.repeat(s, [32, 32, 32, 8, 8, 8], axis=1)
         x = np.concatenate([comb * scale, gg], axis=1)
         for nm in ("LC0", "LC1", "LC2", "LX"):
-            x = _relu(_lin(W[nm + "W"], x, W[nm + "B"]))
+            x = _relu(lin(nm + "W", x, nm + "B"))
         for r in range(6):
-            h = _relu(_lin(W["LX%dW" % (2 * r)], x, W["LX%dB" % (2 * r)]))
-            x = x + _relu(_lin(W["LX%dW" % (2 * r + 1)], h, W["LX%dB" % (2 * r + 1)]))
-        p = _relu(_lin(W["LX12W"], x, W["LX12B"]))[:, 0]
+            h = _relu(lin("LX%dW" % (2 * r), x, "LX%dB" % (2 * r)))
+            x = x + _relu(lin("LX%dW" % (2 * r + 1), h, "LX%dB" % (2 * r + 1)))
+        p = _relu(lin("LX12W", x, "LX12B"))[:, 0]
         out[:, c] = np.maximum(np.exp(p) - 1, 0) * srp[c]
     return out
 
@@ -385,3 +394,23 @@ def test_network_equal_albedos_give_equal_channels():
     gamma = rng.random(16).astype(np.float32)
     out = oracle.mrpnn_forward(ia.load_reference_weights(), feat, gamma, 0.857, np.float32([0.95, 0.95, 0.95]))
     assert np.array_equal(out[:, 0], out[:, 1]) and np.array_equal(out[:, 0], out[:, 2])
+
+
+def test_lowp_rounding_known_values():
+    """The numpy E4M3 / BF16 rounding used to emulate the MFMA networks, on
+    values fixed by the formats (OCP E4M3: 3 fraction bits, smallest subnormal
+    2^-9, largest 448; BF16: 7 fraction bits) and round-half-to-even."""
+    from tests.lowp import bf16, e4m3, pow2_scale
+    assert e4m3(1.0) == 1.0
+    assert e4m3(1.0625) == 1.0        # halfway 1 / 1.125 -> even
+    assert e4m3(1.1875) == 1.25       # halfway 1.125 / 1.25 -> even
+    assert e4m3(248.0) == 256.0       # quantum 16 in [128, 256): 15.5 -> 16
+    assert e4m3(-3.3) == -3.25
+    assert e4m3(448.0) == 448.0 and e4m3(1e4) == 448.0
+    assert e4m3(2.0 ** -9) == 2.0 ** -9
+    assert e4m3(5.5 * 2.0 ** -9) == 6 * 2.0 ** -9   # subnormal halfway -> even
+    assert e4m3(2.0 ** -11) == 0.0
+    assert bf16(1 + 2.0 ** -8) == 1.0
+    assert bf16(1 + 3 * 2.0 ** -8) == 1 + 2.0 ** -6
+    assert pow2_scale(0.3) == 512.0 and 0.3 * 512 >= 128 and 0.3 * 512 < 256
+    assert pow2_scale(128.0) == 1.0 and pow2_scale(0.0) == 1.0
