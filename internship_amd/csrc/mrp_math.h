@@ -267,6 +267,18 @@ MRP_HD Rng rng_make(uint32_t id, uint32_t sample, uint32_t tag, uint64_t seed) {
     r.buf.v[0] = r.buf.v[1] = r.buf.v[2] = r.buf.v[3] = 0u;
     return r;
 }
+// Resume a stream at draw n (the state rng_next leaves after n draws).
+MRP_HD void rng_seek(Rng* r, uint32_t n) {
+    r->n = n;
+    if (n & 3u) {
+        u32x4 c;
+        c.v[0] = r->id;
+        c.v[1] = r->sample;
+        c.v[2] = n >> 2;
+        c.v[3] = r->tag;
+        r->buf = philox4x32_10(c, r->k0, r->k1);
+    }
+}
 MRP_HD float rng_next(Rng* r) {
     const uint32_t w = r->n & 3u;
     if (w == 0u) {

@@ -7,7 +7,10 @@
 // split into stages with their own launch shapes:
 //
 //   1. first_vertex_kernel : one thread per (sample, pixel) - first free flight
-//      (g = 0), random view/light frames, single-scatter Tr * HG;
+//      (g = 0), random view/light frames, HG to the light; compact_kernel lists
+//      the vertices that scatter, and shadow_kernel traces their shadow rays
+//      (single scatter Tr * HG) one thread per listed vertex, so the long
+//      ratio-tracking loops run in full waves;
 //   2. gather_kernel       : one thread per (point, tap) - 192 taps x
 //      {density mip, transmittance mip, HG table} -> 576 features + gamma;
 //   3. mlp (FP32 here, BF16 / FP8 MFMA in mrp_mlp_mfma.hip) - the network;
@@ -100,13 +103,15 @@ __device__ float fv_transmittance(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir,
     return tr;
 }
 
-// pos_act = (pos, active), fx = (X, single), fy = (Y, output slot as bits),
-// fz = (Z, dist).  Items run over the logical pixels of `a`'s pixel map; the
-// output slot (s - spp_begin) * P + pixel is 0xffffffff for a logical pixel
-// past the image edge (odd widths under the checkerboard).
+// pos_act = (pos, active), fx = (X, HG phase to the light; shadow_kernel
+// multiplies in the transmittance), fy = (Y, output slot as bits), fz = (Z,
+// dist), rng_pos = the stream's draw count where the shadow ray starts.
+// Items run over the logical pixels of `a`'s pixel map; the output slot
+// (s - spp_begin) * P + pixel is 0xffffffff for a logical pixel past the image
+// edge (odd widths under the checkerboard).
 __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __restrict__ pos_act,
                                                            float4* __restrict__ fx, float4* __restrict__ fy,
-                                                           float4* __restrict__ fz) {
+                                                           float4* __restrict__ fz, uint32_t* __restrict__ rng_pos) {
     __shared__ unsigned red[256];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t lk = 0u;
@@ -143,21 +148,55 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
             }
             // A vertex whose free flight escaped is sky: its frames and shadow ray
             // (the rest of NNPredict's draws) cannot change any output, so they are skipped.
+            // The shadow ray of a scattering vertex runs in shadow_kernel over the
+            // compacted vertices, from the draw count stored here.
             vec3 X = v3(1, 0, 0), Y = v3(0, 1, 0), Z = v3(0, 0, 1);
-            float single = 0.0f;
+            float phase = 0.0f;
             if (active) {
                 vec3 LX, LY, LZ;
                 frame_from_normal(&rng, dir, &X, &Y, &Z);
                 frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
-                const float phase = hg_phase(dot(X, LX), a.g);
-                single = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk) * phase;
+                phase = hg_phase(dot(X, LX), a.g);
+                rng_pos[i] = rng.n;
             }
             const float dist = active ? distance(a.ori, pos) : -1.0f;
             pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
-            fx[i] = make_float4(X.x, X.y, X.z, single);
+            fx[i] = make_float4(X.x, X.y, X.z, phase);
             fy[i] = make_float4(Y.x, Y.y, Y.z, __uint_as_float(s * P + pix));
             fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
         }
+    }
+    red[threadIdx.x] = lk;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&a.counters[0], (unsigned long long)red[0]);
+}
+
+// Shadow rays of the compacted scattering vertices (the `Tr` of NNPredict's
+// single-scattering term, core/render.cuh:469-523): the stream resumes where
+// first_vertex_kernel stopped, so every draw matches the one-thread-per-pixel
+// order.  fx.w = Tr * phase.
+__global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list, uint32_t n_act,
+                                                     const float4* __restrict__ pos_act, const float4* __restrict__ fy,
+                                                     const uint32_t* __restrict__ rng_pos, float4* __restrict__ fx) {
+    __shared__ unsigned red[256];
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lk = 0u;
+    if (k < n_act) {
+        const uint32_t i = list[k];
+        const uint32_t P = (uint32_t)(a.width * a.height);
+        const uint32_t slot = __float_as_uint(fy[i].w);
+        const uint32_t s = slot / P, pix = slot - s * P;
+        Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        rng_seek(&rng, rng_pos[i]);
+        const float4 pa = pos_act[i];
+        const vec3 pos = v3(pa.x, pa.y, pa.z);
+        const vec3 LX = normalize(a.light);  // the first axis frame_from_normal returns for the light
+        const float tr = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk);
+        fx[i].w = tr * fx[i].w;
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -392,7 +431,7 @@ static int upload_tap_tables(int device) {
 
 // Per-item scratch of the MRPNN stages (bytes): 4 float4 first-vertex
 // records, 576 FP32 features, gamma, 3 predictions.
-constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12 + 8;  // + compaction list and slot
+constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12 + 8 + 4;  // + list, slot, stream position
 
 // The MRPNN stages' scratch lives in the volume (nn_scratch), so volumes on
 // different streams can render concurrently.
@@ -452,6 +491,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     float* pred = gam + chunk_items;
     uint32_t* list = (uint32_t*)(pred + 3 * chunk_items);
     uint32_t* slot_of = list + chunk_items;
+    uint32_t* rng_pos = slot_of + chunk_items;
     uint32_t* n_active_dev = (uint32_t*)v->queues;  // the tracer's queue counters are free during MRPNN frames
 
     NNArgs a;
@@ -491,9 +531,11 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     ga.g = p->g;
     const vec3 srp = srp_of(a.scatter_rate);
 
-    hipEvent_t t0, tc, t2, t3;
+    hipEvent_t t0, tc, ts0, ts1, t2, t3;
     MRP_HIP(hipEventCreate(&t0));
     MRP_HIP(hipEventCreate(&tc));
+    MRP_HIP(hipEventCreate(&ts0));
+    MRP_HIP(hipEventCreate(&ts1));
     MRP_HIP(hipEventCreate(&t2));
     MRP_HIP(hipEventCreate(&t3));
     for (size_t i0 = 0; i0 < total && rc == MRP_OK; i0 += chunk_items) {
@@ -501,7 +543,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         a.item0 = (uint32_t)i0;
         a.n_items = cnt;
         MRP_HIP(hipEventRecord(t0, v->stream));
-        first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz);
+        first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(tc, v->stream));
         clear_u32_one_kernel<<<1, 1, 0, v->stream>>>(n_active_dev);
@@ -511,6 +553,12 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         MRP_HIP(hipMemcpyAsync(&n_act, n_active_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
         MRP_HIP(hipStreamSynchronize(v->stream));
         if (n_active) *n_active += n_act;
+        MRP_HIP(hipEventRecord(ts0, v->stream));
+        if (n_act > 0) {
+            shadow_kernel<<<(n_act + 255) / 256, 256, 0, v->stream>>>(a, list, n_act, pos_act, fy, rng_pos, fx);
+            MRP_HIP(hipGetLastError());
+        }
+        MRP_HIP(hipEventRecord(ts1, v->stream));
         ga.n_items = n_act;
         if (n_act > 0) {
             // the MFMA networks read wave-tiled BF16 descriptors (same region: ceil(n/32) tiles
@@ -539,16 +587,20 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
                                                                   a.scatter_rate, a.sky, cnt, (float4*)v->samples);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventSynchronize(t3));
-        float ms;
-        MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
-        if (fv_ms) *fv_ms += ms;
-        MRP_HIP(hipEventElapsedTime(&ms, tc, t2));  // compaction, its count read-back and the gather
-        if (gather_ms) *gather_ms += ms;
+        float ms, ms2;
+        MRP_HIP(hipEventElapsedTime(&ms, t0, tc));  // first vertices and their shadow rays
+        MRP_HIP(hipEventElapsedTime(&ms2, ts0, ts1));
+        if (fv_ms) *fv_ms += ms + ms2;
+        MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));  // compaction, its count read-back and the gather
+        MRP_HIP(hipEventElapsedTime(&ms2, ts1, t2));
+        if (gather_ms) *gather_ms += ms + ms2;
         MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
         if (mlp_ms) *mlp_ms += ms;
     }
     hipEventDestroy(t0);
     hipEventDestroy(tc);
+    hipEventDestroy(ts0);
+    hipEventDestroy(ts1);
     hipEventDestroy(t2);
     hipEventDestroy(t3);
     return rc;
