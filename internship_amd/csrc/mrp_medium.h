@@ -100,6 +100,44 @@ __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cell
     return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
                 wx);
 }
+
+// trilinear_cells in two halves, so that several lookups can be in flight:
+// issue() starts the two loads, finish() blends (the same arithmetic; a point
+// outside the cells gives zero corners and weights, which blend to +0).
+struct CellsFetch {
+    struct Token {
+        float4 lo, hi;
+        float wx, wy, wz;
+    };
+    const float4* __restrict__ cells;
+    int res;
+    __device__ __forceinline__ Token issue(vec3 p) const {
+        const float fr = (float)res;
+        const float tx = (p.x + 0.5f) * fr - 0.5f;
+        const float ty = (p.y + 0.5f) * fr - 0.5f;
+        const float tz = (p.z + 0.5f) * fr - 0.5f;
+        Token k;
+        if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) {
+            k.lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            k.hi = k.lo;
+            k.wx = k.wy = k.wz = 0.0f;
+            return k;
+        }
+        const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+        k.wx = tx - fx;
+        k.wy = ty - fy;
+        k.wz = tz - fz;
+        const float4* c = cells + 2 * cell_index(res, (int)fx, (int)fy, (int)fz);
+        k.lo = c[0];
+        k.hi = c[1];
+        return k;
+    }
+    __device__ __forceinline__ float finish(const Token& k) const {
+        const float4 lo = k.lo, hi = k.hi;
+        return lerp(lerp(lerp(lo.x, lo.y, k.wz), lerp(lo.z, lo.w, k.wz), k.wy),
+                    lerp(lerp(hi.x, hi.y, k.wz), lerp(hi.z, hi.w, k.wz), k.wy), k.wx);
+    }
+};
 #endif
 
 // Clamp-addressed cells: cell (x,y,z) for x,y,z in [-1, res] holds the eight
@@ -303,6 +341,22 @@ MRP_HD int checker_col(int row, int col_l, int checker) {
 }
 MRP_HD bool checker_rendered(int row, int col, int checker) {
     return checker < 0 || ((row + col + checker) & 1) == 0;
+}
+
+// Work order of a w x h pixel grid in 8 x 8 blocks: index l in [0, w*h) ->
+// (row, col).  Bands of 8 rows (the last may be lower) are split into blocks
+// of 8 columns (the last may be narrower), each walked row by row, so 64
+// consecutive items form one block: a wave's rays leave the camera close
+// together.  A bijection for every w, h > 0.
+MRP_HD void block_order(uint32_t l, uint32_t w, uint32_t h, int* row, int* col) {
+    const uint32_t band = l / (8u * w);
+    const uint32_t hb = h - 8u * band < 8u ? h - 8u * band : 8u;
+    const uint32_t lp = l - band * 8u * w;
+    const uint32_t bx = lp / (8u * hb);
+    const uint32_t r = lp - bx * 8u * hb;
+    const uint32_t wb = w - 8u * bx < 8u ? w - 8u * bx : 8u;
+    *row = (int)(8u * band + r / wb);
+    *col = (int)(8u * bx + r % wb);
 }
 
 // Random orthonormal frame whose first axis is v1: `GetMatrixFromNormal`

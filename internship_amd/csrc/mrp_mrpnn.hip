@@ -30,6 +30,7 @@
 #include "mrp_internal.h"
 #include "mrp_mlp.h"
 #include "mrp_mrpnn.h"
+#include "mrp_track.h"
 #include "mrp_weights_gen.h"
 
 namespace mrp {
@@ -55,53 +56,11 @@ struct NNArgs {
 };
 
 // ---------------------------------------------------------------- stage 1
-// Straight-line delta tracking for the first-vertex stage: a few dozen lookups
-// per path, so a plain one-thread-per-path loop is adequate here.
-__device__ bool fv_free_flight(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir, float dis, vec3* npos, uint32_t* lk) {
-    float t = 0.0f;
-    int loop_num = 0;
-    while (loop_num++ < 10000) {
-        float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) * a.mj.inv_smax;
-        if (t > dis) return false;
-        rk = rng_next(rng);
-        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
-        *lk += 1u;
-        if (d * a.mj.ratio > rk) break;
-        if (d < 0.0f) t -= d;
-    }
-    (void)rng_next(rng);  // HG direction draws of `DeterminateNextVertex` (direction unused)
-    (void)rng_next(rng);
-    *npos = (dir * t) + pos;
-    return true;
-}
-
-__device__ float fv_transmittance(Rng* rng, const NNArgs& a, vec3 pos, vec3 dir, float dis, uint32_t* lk) {
-    float tr = 1.0f, t = 0.0f;
-    int loop_num = 0;
-    while (loop_num++ < 10000 && tr > 0.5f) {
-        const float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) * a.mj.inv_smax_tr;
-        if (t > dis) break;
-        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
-        *lk += 1u;
-        tr *= 1.0f - fmax_(0.0f, d * a.mj.ratio);
-        if (d < 0.0f) t -= d;
-    }
-    while (loop_num++ < 10000 && tr > 0.001f) {
-        const float rk = rng_next(rng);
-        t -= logf_(1.0f - rk) * a.mj.inv_smax_tr;
-        if (t > dis) break;
-        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
-        *lk += 1u;
-        if (rng_next(rng) < fmax_(0.0f, d * a.mj.ratio)) {
-            tr = 0.0f;
-            break;
-        }
-        if (d < 0.0f) t -= d;
-    }
-    return tr;
-}
+// The first free flight and the shadow ray use the batched trackers of
+// mrp_track.h: kFVBatch lookups in flight per thread, with the draws and
+// results of the one-lookup-at-a-time loops.
+constexpr int kFVBatch = 4;
+constexpr int kMaxTrackSteps = 10000;
 
 // pos_act = (pos, active), fx = (X, HG phase to the light; shadow_kernel
 // multiplies in the transmittance), fy = (Y, output slot as bits), fz = (Z,
@@ -120,7 +79,8 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
         const uint32_t Pl = (uint32_t)(a.wl * a.height);
         const uint32_t item = a.item0 + i;
         const uint32_t s = item / Pl, l = item - s * Pl;
-        const int row = (int)(l / (uint32_t)a.wl), col_l = (int)(l - (uint32_t)row * (uint32_t)a.wl);
+        int row, col_l;
+        block_order(l, (uint32_t)a.wl, (uint32_t)a.height, &row, &col_l);  // waves cover 8 x 8 pixel blocks
         const int col = checker_col(row, col_l, a.checker);
         if (col >= a.width) {
             pos_act[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -141,7 +101,9 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
             if (off >= 0.0f) {
                 const vec3 sp = a.ori + gdir * off;
                 vec3 np;
-                if (fv_free_flight(&rng, a, sp, gdir, ray_box_distance(sp, gdir), &np, &lk)) {
+                const CellsFetch F{a.cells, a.res};
+                if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
+                                                  ray_box_distance(sp, gdir), kMaxTrackSteps, &np, &lk)) {
                     pos = np;
                     active = 1;
                 }
@@ -195,7 +157,9 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
         const float4 pa = pos_act[i];
         const vec3 pos = v3(pa.x, pa.y, pa.z);
         const vec3 LX = normalize(a.light);  // the first axis frame_from_normal returns for the light
-        const float tr = fv_transmittance(&rng, a, pos, LX, ray_box_distance(pos, LX), &lk);
+        const CellsFetch F{a.cells, a.res};
+        const float tr = transmittance_batched<kFVBatch>(&rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX,
+                                                         ray_box_distance(pos, LX), kMaxTrackSteps, &lk);
         fx[i].w = tr * fx[i].w;
     }
     red[threadIdx.x] = lk;
