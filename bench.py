@@ -244,52 +244,58 @@ def hbm_traffic(kernel, workload):
 
 def time_mlp(a, ia, sc):
     """K timed 1-spp MRPNN frames folded into the running mean in HBM.
-    Returns (elapsed s, dict of per-frame averages)."""
+    Returns (elapsed s, dict of per-frame averages: HIP-event time of each
+    kernel, lookups of the first-vertex and shadow kernels, descriptors)."""
     sc.target.zero()
     for i in range(a.warmup):
         sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=a.mlp_precision, fresh=False)
-    acc = {"fv_ms": 0.0, "gather_ms": 0.0, "mlp_ms": 0.0, "descriptors": 0, "fv_lookups": 0}
+    keys = ("fv_ms", "shadow_ms", "compact_ms", "gather_ms", "mlp_ms", "descriptors", "fv_lookups", "shadow_lookups")
+    acc = dict.fromkeys(keys, 0.0)
     t0 = time.perf_counter()
     for i in range(a.warmup, a.warmup + a.steps):
         sc.render(i, 1, MLP_SEED, rt=ia.MRPNN, precision=a.mlp_precision, fresh=False)
-        acc["fv_ms"] += sc.st.first_vertex_ms
-        acc["gather_ms"] += sc.st.reduce_ms
-        acc["mlp_ms"] += sc.st.kernel_ms
-        acc["descriptors"] += sc.st.scatters
-        acc["fv_lookups"] += sc.st.lookups
+        st = sc.st
+        acc["fv_ms"] += st.first_vertex_ms
+        acc["shadow_ms"] += st.shadow_ms
+        acc["compact_ms"] += st.compact_ms
+        acc["gather_ms"] += st.gather_ms
+        acc["mlp_ms"] += st.kernel_ms
+        acc["descriptors"] += st.scatters
+        acc["fv_lookups"] += st.lookups - st.shadow_lookups
+        acc["shadow_lookups"] += st.shadow_lookups
     elapsed = time.perf_counter() - t0
     k = {key: v / a.steps for key, v in acc.items()}
-    log("MLP renderer: %.2f ms/frame (first vertex %.2f, compaction + gather %.2f, network %.2f ms; %d descriptors"
-        " = %.1f%% of pixels)" % (1000 * elapsed / a.steps, k["fv_ms"], k["gather_ms"], k["mlp_ms"], k["descriptors"],
-                                  100.0 * k["descriptors"] / (sc.W * sc.H)))
+    log("MLP renderer: %.2f ms/frame (first vertex %.2f + shadow rays %.2f, compaction %.2f, gather %.2f, network "
+        "%.2f ms; %d descriptors = %.1f%% of pixels)"
+        % (1000 * elapsed / a.steps, k["fv_ms"], k["shadow_ms"], k["compact_ms"], k["gather_ms"], k["mlp_ms"],
+           k["descriptors"], 100.0 * k["descriptors"] / (sc.W * sc.H)))
     return elapsed, k
 
 
 def mlp_rooflines(k, precision=1):
-    """Rooflines of the three stages of an MRPNN frame; `roofline` is the one
-    that takes longest.  Network: FLOPs of the descriptors evaluated (first
+    """Rooflines of the kernels of an MRPNN frame; `roofline` is the one that
+    takes longest.  Network: FLOPs of the descriptors evaluated (first
     vertices that scatter; escaping pixels are sky and skip gather and network)
     against the dense BF16 (precision 1) or FP8 (precision 2) MFMA peak.
-    First vertex: 32 B per trilinear lookup.  Gather: 18,400 B per descriptor."""
+    First vertex and shadow rays: 32 B per trilinear lookup.  Gather: 18,400 B
+    per descriptor."""
     nd = k["descriptors"]
     mlp = MLP_FLOP_PER_PASS * nd / (k["mlp_ms"] * 1e-3) / 1e12
     kname, peak = ("mlp_fp8_kernel", FP8_PEAK_TFLOPS) if precision == 2 else ("mlp_bf16_kernel", BF16_PEAK_TFLOPS)
-    fv = k["fv_lookups"] * BYTES_PER_LOOKUP / (k["fv_ms"] * 1e-3) / 1e9
+    r = {kname: {"bound": "mfma", "achieved": mlp, "peak": peak, "unit": "TFLOP/s", "frac": mlp / peak,
+                 "traffic": hbm_traffic(kname, MLP_WORKLOAD), "kernel": kname, "kernel_ms": k["mlp_ms"],
+                 "flop_per_launch": MLP_FLOP_PER_PASS * nd, "descriptors_per_launch": nd}}
+    for name, lk, ms in (("first_vertex_kernel", k["fv_lookups"], k["fv_ms"]),
+                         ("shadow_kernel", k["shadow_lookups"], k["shadow_ms"])):
+        gbs = lk * BYTES_PER_LOOKUP / (ms * 1e-3) / 1e9
+        r[name] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                   "traffic": hbm_traffic(name, MLP_WORKLOAD), "kernel": name, "kernel_ms": ms,
+                   "lookups_per_launch": lk, "bytes_per_lookup": BYTES_PER_LOOKUP}
     ga = GATHER_BYTES_PER_POINT * nd / (k["gather_ms"] * 1e-3) / 1e9
-    r = {
-        kname: {"bound": "mfma", "achieved": mlp, "peak": peak, "unit": "TFLOP/s",
-                "frac": mlp / peak, "traffic": hbm_traffic(kname, MLP_WORKLOAD),
-                "kernel": kname, "kernel_ms": k["mlp_ms"],
-                "flop_per_launch": MLP_FLOP_PER_PASS * nd, "descriptors_per_launch": nd},
-        "first_vertex_kernel": {"bound": "hbm", "achieved": fv, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                "frac": fv / HBM_PEAK_GBS,
-                                "traffic": hbm_traffic("first_vertex_kernel", MLP_WORKLOAD),
-                                "kernel": "first_vertex_kernel", "kernel_ms": k["fv_ms"],
-                                "lookups_per_launch": k["fv_lookups"], "bytes_per_lookup": BYTES_PER_LOOKUP},
-        "gather_kernel": {"bound": "hbm", "achieved": ga, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ga / HBM_PEAK_GBS,
-                          "traffic": hbm_traffic("gather_kernel", MLP_WORKLOAD), "kernel": "gather_kernel",
-                          "kernel_ms": k["gather_ms"], "bytes_per_descriptor": GATHER_BYTES_PER_POINT},
-    }
+    r["gather_kernel"] = {"bound": "hbm", "achieved": ga, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": ga / HBM_PEAK_GBS, "traffic": hbm_traffic("gather_kernel", MLP_WORKLOAD),
+                          "kernel": "gather_kernel", "kernel_ms": k["gather_ms"],
+                          "bytes_per_descriptor": GATHER_BYTES_PER_POINT}
     dominant = max(r, key=lambda n: r[n]["kernel_ms"])
     return r[dominant], r
 

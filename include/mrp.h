@@ -89,9 +89,13 @@ typedef struct mrp_trace_stats {
     double kernel_ms;       /* device time of the tracing kernel(s) (MRPNN: of the network), HIP events */
     double reduce_ms;       /* device time of the accumulation kernel(s) (MRPNN: compaction + descriptor gather) */
     double total_ms;        /* device time of the whole call */
-    double first_vertex_ms; /* MRPNN only: device time of the first-vertex kernel */
+    double first_vertex_ms; /* MRPNN only: device time of the first-vertex kernel (free flights and frames) */
     int launches;           /* tracing kernel launches */
     int grid_blocks;        /* persistent workgroups per launch */
+    double shadow_ms;       /* MRPNN only: shadow rays of the first vertices that scatter */
+    double compact_ms;      /* MRPNN only: compaction of those vertices and the count's read-back */
+    double gather_ms;       /* MRPNN only: the descriptor gather kernel */
+    uint64_t shadow_lookups; /* MRPNN only: the part of `lookups` made by the shadow rays */
 } mrp_trace_stats;
 
 /* ------------------------------------------------------------------ errors */

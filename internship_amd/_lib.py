@@ -61,6 +61,8 @@ class TraceStats(ctypes.Structure):
         ("kernel_ms", ctypes.c_double), ("reduce_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
         ("first_vertex_ms", ctypes.c_double),
         ("launches", ctypes.c_int), ("grid_blocks", ctypes.c_int),
+        ("shadow_ms", ctypes.c_double), ("compact_ms", ctypes.c_double), ("gather_ms", ctypes.c_double),
+        ("shadow_lookups", ctypes.c_uint64),
     ]
 
     def asdict(self):

@@ -217,7 +217,7 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
     MRP_HIP(hipEventCreate(&e2));
     MRP_HIP(hipEventRecord(e0, v->stream));
     rc = clear_counters(v);
-    double fv_ms = 0.0, gather_ms = 0.0, mlp_ms = 0.0;
+    NNTimes tm;
     uint64_t n_active = 0;
     if (rc == MRP_OK) {
         if (predict && !v->frame_last_predict) {
@@ -225,7 +225,7 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
             clear_histo_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, v->stream>>>(histo, nf);
             MRP_HIP(hipGetLastError());
         }
-        rc = predict ? mrpnn_samples(v, p, map, (int)v->frame_calls, 1, &fv_ms, &gather_ms, &mlp_ms, &n_active)
+        rc = predict ? mrpnn_samples(v, p, map, (int)v->frame_calls, 1, &tm, &n_active)
                      : trace_samples(v, p, map, (int)v->frame_calls, 1);
     }
     if (rc == MRP_OK) {
@@ -263,14 +263,18 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
     v->frame_last_up = p->up;
     v->frame_last_right = p->right;
     if (st) {
-        st->lookups = cnt[0];
+        st->lookups = predict ? cnt[0] + cnt[2] : cnt[0];  // MRPNN: shadow-ray lookups are counted apart
+        st->shadow_lookups = predict ? cnt[2] : 0;
         st->paths = predict ? (uint64_t)(fp->checkerboard ? (P + 1) / 2 : P) : cnt[1];
         st->scatters = predict ? n_active : cnt[2];
         st->max_path_steps = cnt[3];
-        st->kernel_ms = predict ? mlp_ms : (double)render_ms;
+        st->kernel_ms = predict ? tm.network : (double)render_ms;
         st->reduce_ms = post_ms;  // histogram update + denoise + tone + pack
         st->total_ms = (double)render_ms + (double)post_ms;
-        st->first_vertex_ms = fv_ms;
+        st->first_vertex_ms = tm.first_vertex;
+        st->shadow_ms = tm.shadow;
+        st->compact_ms = tm.compact;
+        st->gather_ms = tm.gather;
         st->launches = 1;
         st->grid_blocks = 0;
     }

@@ -88,8 +88,12 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
 // One batch of per-(sample, pixel) float4 results (rgb, first-hit distance or
 // -1) in v->samples[(s - s_begin) * P + pixel] for the pixels `map` selects.
 int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n);
-int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
-                  double* gather_ms, double* mlp_ms, uint64_t* n_active);
+// Device time (HIP events) of each MRPNN stage, summed over calls.
+struct NNTimes {
+    double first_vertex = 0.0, compact = 0.0, shadow = 0.0, gather = 0.0, network = 0.0;
+};
+int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
+                  uint64_t* n_active);
 int clear_counters(Volume* v);
 int read_counters(Volume* v, unsigned long long out[4]);
 int launch_update(Volume* v);

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 // Shadow rays of the compacted scattering vertices (the `Tr` of NNPredict's
 // single-scattering term, core/render.cuh:469-523): the stream resumes where
 // first_vertex_kernel stopped, so every draw matches the one-thread-per-pixel
-// order.  fx.w = Tr * phase.
+// order.  fx.w = Tr * phase.  Its lookups are counted in counters[2].
 __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list, uint32_t n_act,
                                                      const float4* __restrict__ pos_act, const float4* __restrict__ fy,
                                                      const uint32_t* __restrict__ rng_pos, float4* __restrict__ fx) {
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && red[0]) atomicAdd(&a.counters[0], (unsigned long long)red[0]);
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&a.counters[2], (unsigned long long)red[0]);
 }
 
 // ---------------------------------------------------------------- stage 2
@@ -421,8 +421,8 @@ static vec3 srp_of(vec3 scatter_rate) {
 
 // First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
 // of the pixels `map` selects, in chunks of at most 4M vertices.
-int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, double* fv_ms,
-                  double* gather_ms, double* mlp_ms, uint64_t* n_active) {
+int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
+                  uint64_t* n_active) {
     if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
     if (p->mlp_precision < 0 || p->mlp_precision > 2)
         return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
@@ -495,7 +495,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     ga.g = p->g;
     const vec3 srp = srp_of(a.scatter_rate);
 
-    hipEvent_t t0, tc, ts0, ts1, t2, t3;
+    hipEvent_t t0, tc, ts0, ts1, t2, t3;  // stage boundaries: first vertex | compaction | shadow | gather | network
     MRP_HIP(hipEventCreate(&t0));
     MRP_HIP(hipEventCreate(&tc));
     MRP_HIP(hipEventCreate(&ts0));
@@ -551,15 +551,19 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
                                                                   a.scatter_rate, a.sky, cnt, (float4*)v->samples);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventSynchronize(t3));
-        float ms, ms2;
-        MRP_HIP(hipEventElapsedTime(&ms, t0, tc));  // first vertices and their shadow rays
-        MRP_HIP(hipEventElapsedTime(&ms2, ts0, ts1));
-        if (fv_ms) *fv_ms += ms + ms2;
-        MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));  // compaction, its count read-back and the gather
-        MRP_HIP(hipEventElapsedTime(&ms2, ts1, t2));
-        if (gather_ms) *gather_ms += ms + ms2;
-        MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
-        if (mlp_ms) *mlp_ms += ms;
+        if (times) {
+            float ms;
+            MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
+            times->first_vertex += ms;
+            MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));  // compaction and its count read-back
+            times->compact += ms;
+            MRP_HIP(hipEventElapsedTime(&ms, ts0, ts1));
+            times->shadow += ms;
+            MRP_HIP(hipEventElapsedTime(&ms, ts1, t2));
+            times->gather += ms;
+            MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
+            times->network += ms;
+        }
     }
     hipEventDestroy(t0);
     hipEventDestroy(tc);
@@ -581,12 +585,12 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     MRP_HIP(hipEventCreate(&e1));
     MRP_HIP(hipEventRecord(e0, v->stream));
     rc = clear_counters(v);
-    double mlp_ms = 0.0, gather_ms = 0.0, fv_ms = 0.0;
+    NNTimes tm;
     uint64_t n_active = 0;
     const PixelMap map;
     for (int s0 = 0; s0 < p->spp_count && rc == MRP_OK; s0 += batch) {
         const int n = std::min(batch, p->spp_count - s0);
-        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &fv_ms, &gather_ms, &mlp_ms, &n_active);
+        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &tm, &n_active);
         if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
     }
     float total_ms = 0.0f;
@@ -601,16 +605,20 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     hipEventDestroy(e1);
     if (rc) return rc;
     if (st) {
-        st->lookups = cnt4[0];
+        st->lookups = cnt4[0] + cnt4[2];
+        st->shadow_lookups = cnt4[2];
         st->paths = (uint64_t)p->spp_count * P;
         st->scatters = n_active;  // first vertices that scattered: the descriptors gathered and evaluated
         st->max_path_steps = 0;
-        st->kernel_ms = mlp_ms;       // the network
-        st->reduce_ms = gather_ms;    // the descriptor gather
+        st->kernel_ms = tm.network;
+        st->reduce_ms = tm.compact + tm.gather;
         st->total_ms = total_ms;
         st->launches = 0;
         st->grid_blocks = 0;
-        st->first_vertex_ms = fv_ms;
+        st->first_vertex_ms = tm.first_vertex;
+        st->shadow_ms = tm.shadow;
+        st->compact_ms = tm.compact;
+        st->gather_ms = tm.gather;
     }
     return MRP_OK;
 }

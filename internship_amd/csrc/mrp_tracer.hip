@@ -560,6 +560,8 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
         st->total_ms = total;
         st->launches = launches;
         st->grid_blocks = blocks;
+        st->first_vertex_ms = st->shadow_ms = st->compact_ms = st->gather_ms = 0.0;
+        st->shadow_lookups = 0;
     }
     return MRP_OK;
 }
