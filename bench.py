@@ -195,6 +195,26 @@ def cpu_baseline(sc, target_s=15.0):
                       % (threads, r0, r0 + rows, frames, sc.W, sc.H, dt, lookups / dt)}
 
 
+CEILING_BUFFER = 4 << 30  # about the size of the 512^3 cell array (4.3 GB)
+
+
+def random_read_ceiling(ia):
+    """The GPU's rate for random 32-byte reads (the access shape of one
+    trilinear cell lookup) over a 4 GiB buffer: mrp_measure_random_reads."""
+    r = ctypes.c_double(0.0)
+    ia.check(ia.load().mrp_measure_random_reads(BYTES_PER_LOOKUP, ctypes.c_uint64(CEILING_BUFFER), 64, ctypes.byref(r)))
+    log("random 32-byte reads over 4 GiB: %.1f G/s (%.0f GB/s useful, %.0f GB/s of 128-byte lines)"
+        % (r.value / 1e9, r.value * BYTES_PER_LOOKUP / 1e9, r.value * 128 / 1e9))
+    return {"width_bytes": BYTES_PER_LOOKUP, "buffer_bytes": CEILING_BUFFER, "reads_per_s": r.value,
+            "useful_gbs": r.value * BYTES_PER_LOOKUP / 1e9}
+
+
+def with_ceiling(roof, ceiling, lookups_per_s):
+    roof = dict(roof)
+    roof["frac_of_random_read_ceiling"] = lookups_per_s / ceiling["reads_per_s"]
+    return roof
+
+
 def run_tracer(a, ia, sc):
     def step(i):
         p = scene_params(ia, sc.W, sc.H, i * a.spp_per_step, a.spp_per_step, sc.bounces, 0, 0, 0)
@@ -214,17 +234,20 @@ def run_tracer(a, ia, sc):
     avg_ms = kernel_ms / a.steps
     achieved = (lookups / a.steps) * BYTES_PER_LOOKUP / (avg_ms * 1e-3) / 1e9
     log("tracer: %.2f ms/step, kernel %.2f ms, %.1f GB/s algorithmic" % (1000 * elapsed / a.steps, avg_ms, achieved))
+    ceiling = random_read_ceiling(ia)
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": hbm_traffic("trace_kernel", TRACER_WORKLOAD % a.spp_per_step),
+            "kernel": "trace_kernel",
+            "kernel_ms": avg_ms, "lookups_per_launch": lookups / a.steps,
+            "bytes_per_lookup": BYTES_PER_LOOKUP}
     return {
         "value": a.steps * a.spp_per_step / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "f32",
         "config": {"workload": "plain delta-tracking tracer, 512^3 Perlin-fBm cloud, %dx%d, 1 spp/frame, "
                                "%d frames/launch, %d bounces" % (sc.W, sc.H, a.spp_per_step, sc.bounces),
                    "width": sc.W, "height": sc.H, "grid": sc.res, "spp_per_step": a.spp_per_step},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": hbm_traffic("trace_kernel", TRACER_WORKLOAD % a.spp_per_step),
-                     "kernel": "trace_kernel",
-                     "kernel_ms": avg_ms, "lookups_per_launch": lookups / a.steps,
-                     "bytes_per_lookup": BYTES_PER_LOOKUP},
+        "roofline": with_ceiling(roof, ceiling, lookups / (kernel_ms * 1e-3)),
+        "random_read_ceiling": ceiling,
         "extra": {"glookups_per_s": lookups / (kernel_ms * 1e-3) / 1e9, "max_path_steps": sc.st.max_path_steps},
     }
 
@@ -272,13 +295,14 @@ def time_mlp(a, ia, sc):
     return elapsed, k
 
 
-def mlp_rooflines(k, precision=1):
+def mlp_rooflines(k, precision=1, ceiling=None):
     """Rooflines of the kernels of an MRPNN frame; `roofline` is the one that
     takes longest.  Network: FLOPs of the descriptors evaluated (first
     vertices that scatter; escaping pixels are sky and skip gather and network)
     against the dense BF16 (precision 1) or FP8 (precision 2) MFMA peak.
-    First vertex and shadow rays: 32 B per trilinear lookup.  Gather: 18,400 B
-    per descriptor."""
+    First vertex and shadow rays: 32 B per trilinear lookup, also given as a
+    fraction of the measured random-read ceiling.  Gather: 18,400 B per
+    descriptor."""
     nd = k["descriptors"]
     mlp = MLP_FLOP_PER_PASS * nd / (k["mlp_ms"] * 1e-3) / 1e12
     kname, peak = ("mlp_fp8_kernel", FP8_PEAK_TFLOPS) if precision == 2 else ("mlp_bf16_kernel", BF16_PEAK_TFLOPS)
@@ -291,6 +315,8 @@ def mlp_rooflines(k, precision=1):
         r[name] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                    "traffic": hbm_traffic(name, MLP_WORKLOAD), "kernel": name, "kernel_ms": ms,
                    "lookups_per_launch": lk, "bytes_per_lookup": BYTES_PER_LOOKUP}
+        if ceiling:
+            r[name] = with_ceiling(r[name], ceiling, lk / (ms * 1e-3))
     ga = GATHER_BYTES_PER_POINT * nd / (k["gather_ms"] * 1e-3) / 1e9
     r["gather_kernel"] = {"bound": "hbm", "achieved": ga, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": ga / HBM_PEAK_GBS, "traffic": hbm_traffic("gather_kernel", MLP_WORKLOAD),
@@ -317,10 +343,11 @@ def run_mlp(a, ia, sc):
     net.close()
     cfg = headline_config(sc, a)
     cfg["workload"] = cfg["workload"].split(";")[0]
-    roof, roofs = mlp_rooflines(k, a.mlp_precision)
+    ceiling = random_read_ceiling(ia)
+    roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling)
     return {"value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps,
             "dtype": {0: "f32", 1: "bf16", 2: "fp8-e4m3"}[a.mlp_precision],
-            "config": cfg, "roofline": roof, "rooflines": roofs, "kernels": k}
+            "config": cfg, "roofline": roof, "rooflines": roofs, "random_read_ceiling": ceiling, "kernels": k}
 
 
 def run_frame(a, ia, sc):
@@ -550,10 +577,13 @@ def run_headline(a, ia, sc):
     log("DT %d spp: %.2f ms/frame -> speed-up %.2fx" % (n_eq, 1000 * t_dt, fps_mlp / fps_dt))
     dt_achieved = dt_lookups * BYTES_PER_LOOKUP / (dt_kernel_ms * 1e-3) / 1e9
     net.close()
+    ceiling = random_read_ceiling(ia)
+    roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling)
     return {
-        "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
+        "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps,
+        "dtype": {0: "f32", 1: "bf16", 2: "fp8-e4m3"}[a.mlp_precision],
         "config": headline_config(sc, a),
-        "roofline": mlp_rooflines(k, a.mlp_precision)[0], "rooflines": mlp_rooflines(k, a.mlp_precision)[1],
+        "roofline": roof, "rooflines": roofs, "random_read_ceiling": ceiling,
         "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "rmse_mlp_fp8": rmse_mlp8,
                        "dt_spp": n_eq,
                        "dt_reached_target": reached, "rmse_dt": hist.get(n_eq), "fps_mlp": fps_mlp,
@@ -562,7 +592,8 @@ def run_headline(a, ia, sc):
                        "luma_rmse100_mlp": luma_rmse100(img_mlp, ref),
                        "luma_rmse100_dt1": luma_rmse100(img_dt1, ref)},
         "kernels": dict(k, mlp_fp32_ms=fp32_ms, mlp_fp8_ms=fp8_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms,
-                        tracer_gbs_algorithmic=dt_achieved, tracer_hbm_frac=dt_achieved / HBM_PEAK_GBS),
+                        tracer_gbs_algorithmic=dt_achieved, tracer_hbm_frac=dt_achieved / HBM_PEAK_GBS,
+                        tracer_frac_of_random_read_ceiling=dt_lookups / (dt_kernel_ms * 1e-3) / ceiling["reads_per_s"]),
     }
 
 

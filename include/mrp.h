@@ -111,6 +111,14 @@ int mrp_memcpy_d2h(void* host_dst, const void* dev_src, size_t bytes);
 int mrp_memset_zero(void* dev, size_t bytes);
 int mrp_memcpy_d2d(void* dev_dst, const void* dev_src, size_t bytes);
 
+/* ------------------------------------------------------------ measurement
+ * No counterpart in the reference: the GPU's ceiling for random reads of
+ * `width` bytes (16, 32 or 128) at hashed positions in a `buffer_bytes`
+ * buffer (a power of two, 1 MiB to 16 GiB), `iters` reads per thread, 4.2M
+ * threads; best of two timed launches.  It is the access shape of a trilinear
+ * cell lookup, the bound the tracer's lookup rate is read against. */
+int mrp_measure_random_reads(int width, uint64_t buffer_bytes, int iters, double* reads_per_s);
+
 /* ------------------------------------------------------------------ volume */
 typedef struct mrp_volume mrp_volume;
 

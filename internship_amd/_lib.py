@@ -85,6 +85,7 @@ SIGNATURES = {
     "mrp_memcpy_d2h": (I, [P, P, SZ]),
     "mrp_memset_zero": (I, [P, SZ]),
     "mrp_memcpy_d2d": (I, [P, P, SZ]),
+    "mrp_measure_random_reads": (I, [I, ctypes.c_uint64, I, ctypes.POINTER(ctypes.c_double)]),
     "mrp_volume_create": (I, [I, ctypes.POINTER(P)]),
     "mrp_volume_destroy": (I, [P]),
     "mrp_volume_resolution": (I, [P, ctypes.POINTER(I)]),
