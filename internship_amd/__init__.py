@@ -306,11 +306,13 @@ class VolumeRender:
                             int(sample_num), int(self.seed if seed is None else seed), int(rt), int(mlp_precision))
 
     def render(self, size, ori, up, right, light_dir, rt=PT, g=0.857, alpha=1.0, light_color=(1, 1, 1),
-               multi_scatter=512, sample_num=1024, seed=None, stats: TraceStats = None) -> np.ndarray:
+               multi_scatter=512, sample_num=1024, seed=None, stats: TraceStats = None,
+               mlp_precision=0) -> np.ndarray:
         """Offline render (`VolumeRender::Render`, core/volume.cu:410-467):
-        returns (height, width, 3) FP32, the mean of `sample_num` samples."""
+        returns (height, width, 3) FP32, the mean of `sample_num` samples.
+        mlp_precision (MRPNN): 0 FP32, 1 BF16 MFMA, 2 FP8-E4M3 MFMA."""
         p = self.params(size, ori, up, right, light_dir, rt, g, alpha, light_color, multi_scatter, sample_num,
-                        0, seed)
+                        0, seed, mlp_precision)
         out = np.zeros((p.height, p.width, 3), dtype=np.float32)
         check(self.lib.mrp_render_offline(self.h, ctypes.byref(p), _ptr(out),
                                           ctypes.byref(stats) if stats is not None else None))
