@@ -87,6 +87,16 @@ MRP_HD size_t cell_count(int res) {
 // Density at box position p from the cell layout: the same arithmetic as
 // trilinear_border (same corners, same weights, same blend order).
 #if defined(__HIPCC__)
+// 32-bit cell indices on the device: resolutions are at most 1024, so twice
+// the largest index (2 * 1025^3) stays below 2^32.
+__device__ __forceinline__ uint32_t cell_index32(int res, int x, int y, int z) {
+    const uint32_t c = (uint32_t)res + 1u;
+    return ((uint32_t)(x + 1) * c + (uint32_t)(y + 1)) * c + (uint32_t)(z + 1);
+}
+__device__ __forceinline__ uint32_t clamp_cell_index32(int res, int x, int y, int z) {
+    const uint32_t c = (uint32_t)res + 2u;
+    return ((uint32_t)(x + 1) * c + (uint32_t)(y + 1)) * c + (uint32_t)(z + 1);
+}
 __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cells, int res, vec3 p) {
     const float fr = (float)res;
     const float tx = (p.x + 0.5f) * fr - 0.5f;
@@ -95,7 +105,7 @@ __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cell
     if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) return 0.0f;
     const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
     const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
-    const float4* c = cells + 2 * cell_index(res, (int)fx, (int)fy, (int)fz);
+    const float4* c = cells + 2u * cell_index32(res, (int)fx, (int)fy, (int)fz);
     const float4 lo = c[0], hi = c[1];
     return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
                 wx);
@@ -127,7 +137,7 @@ struct CellsFetch {
         k.wx = tx - fx;
         k.wy = ty - fy;
         k.wz = tz - fz;
-        const float4* c = cells + 2 * cell_index(res, (int)fx, (int)fy, (int)fz);
+        const float4* c = cells + 2u * cell_index32(res, (int)fx, (int)fy, (int)fz);
         k.lo = c[0];
         k.hi = c[1];
         return k;
@@ -166,7 +176,7 @@ __device__ __forceinline__ float trilinear_clamp_cells(const float4* __restrict_
     tz = tz >= -1.0f ? fmin_(tz, fr) : -1.0f;
     const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
     const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
-    const float4* c = cells + 2 * clamp_cell_index(res, (int)fx, (int)fy, (int)fz);
+    const float4* c = cells + 2u * clamp_cell_index32(res, (int)fx, (int)fy, (int)fz);
     const float4 lo = c[0], hi = c[1];
     return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
                 wx);
@@ -178,7 +188,7 @@ __device__ __forceinline__ float bilinear_cells(const float4* __restrict__ cells
     ty = ty >= -1.0f ? fmin_(ty, (float)h) : -1.0f;
     const float fx = floorf(tx), fy = floorf(ty);
     const float wx = tx - fx, wy = ty - fy;
-    const float4 q = cells[((size_t)((int)fy + 1)) * ((size_t)w + 2) + (size_t)((int)fx + 1)];
+    const float4 q = cells[(uint32_t)((int)fy + 1) * ((uint32_t)w + 2u) + (uint32_t)((int)fx + 1)];
     return lerp(lerp(q.x, q.y, wx), lerp(q.z, q.w, wx), wy);
 }
 #endif

@@ -138,12 +138,20 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 }
 
 // Shadow rays of the compacted scattering vertices (the `Tr` of NNPredict's
-// single-scattering term, core/render.cuh:469-523): the stream resumes where
-// first_vertex_kernel stopped, so every draw matches the one-thread-per-pixel
-// order.  fx.w = Tr * phase.  Its lookups are counted in counters[2].
+// single-scattering term, core/render.cuh:469-523), one thread per listed
+// vertex: the stream resumes where first_vertex_kernel stopped, so every draw
+// matches the one-thread-per-pixel order.  single[i] = Tr * phase (phase in
+// fx.w).  Its lookups are counted in counters[2].
+// The walk runs in rounds of kFVBatch lookups whose speculation is shared by
+// ratio and delta tracking, so lanes in different phases wait for memory
+// together.  Measured alternatives, all slower: persistent lanes that claim
+// the next vertex when done (0.78 against 0.27 ms at 1080p), 2-8 lanes per
+// ray sharing one walk (0.35-0.64 ms), and this kernel on a second stream
+// beside the gather (both slowed by about what the overlap saved).
 __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list, uint32_t n_act,
                                                      const float4* __restrict__ pos_act, const float4* __restrict__ fy,
-                                                     const uint32_t* __restrict__ rng_pos, float4* __restrict__ fx) {
+                                                     const uint32_t* __restrict__ rng_pos,
+                                                     const float4* __restrict__ fx, float* __restrict__ single) {
     __shared__ unsigned red[256];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t lk = 0u;
@@ -160,7 +168,7 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
         const CellsFetch F{a.cells, a.res};
         const float tr = transmittance_batched<kFVBatch>(&rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX,
                                                          ray_box_distance(pos, LX), kMaxTrackSteps, &lk);
-        fx[i].w = tr * fx[i].w;
+        single[i] = tr * fx[i].w;
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -324,7 +332,8 @@ __global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------- stage 4
-__global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act, const float4* __restrict__ fx,
+__global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act,
+                                                      const float* __restrict__ single_in,
                                                       const float4* __restrict__ fy, const float4* __restrict__ fz,
                                                       const uint32_t* __restrict__ slot_of,
                                                       const float* __restrict__ pred, vec3 light_color,
@@ -334,8 +343,8 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
     if (i >= n_items) return;
     const uint32_t slot = __float_as_uint(fy[i].w);
     if (slot == 0xffffffffu) return;
-    const float single = fx[i].w;
     if (pos_act[i].w != 0.0f) {
+        const float single = single_in[i];
         const uint32_t k = slot_of[i];  // the vertex's compacted descriptor
         const vec3 p = v3(pred[3 * k] + single, pred[3 * k + 1] + single, pred[3 * k + 2] + single);
         const vec3 r = p * light_color * scatter_rate;
@@ -377,6 +386,7 @@ __global__ void clear_counters_kernel(unsigned long long* p, int n) {
 
 static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
 
+
 static bool g_tables_ready[64] = {};
 static std::mutex g_tables_mutex;
 
@@ -395,7 +405,7 @@ static int upload_tap_tables(int device) {
 
 // Per-item scratch of the MRPNN stages (bytes): 4 float4 first-vertex
 // records, 576 FP32 features, gamma, 3 predictions.
-constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12 + 8 + 4;  // + list, slot, stream position
+constexpr size_t kNNItemBytes = 4 * 16 + kFeatures * 4 + 72 * 4 + 4 + 12 + 8 + 4 + 4;  // + list, slot, stream position, single
 
 // The MRPNN stages' scratch lives in the volume (nn_scratch), so volumes on
 // different streams can render concurrently.
@@ -456,6 +466,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     uint32_t* list = (uint32_t*)(pred + 3 * chunk_items);
     uint32_t* slot_of = list + chunk_items;
     uint32_t* rng_pos = slot_of + chunk_items;
+    float* single = (float*)(rng_pos + chunk_items);
     uint32_t* n_active_dev = (uint32_t*)v->queues;  // the tracer's queue counters are free during MRPNN frames
 
     NNArgs a;
@@ -495,13 +506,10 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     ga.g = p->g;
     const vec3 srp = srp_of(a.scatter_rate);
 
-    hipEvent_t t0, tc, ts0, ts1, t2, t3;  // stage boundaries: first vertex | compaction | shadow | gather | network
-    MRP_HIP(hipEventCreate(&t0));
-    MRP_HIP(hipEventCreate(&tc));
-    MRP_HIP(hipEventCreate(&ts0));
-    MRP_HIP(hipEventCreate(&ts1));
-    MRP_HIP(hipEventCreate(&t2));
-    MRP_HIP(hipEventCreate(&t3));
+    // Stage boundaries: first vertex | compaction | shadow rays | gather | network | combine.
+    hipEvent_t t0, tc, ts0, ts1, t2, t3, tj;
+    hipEvent_t* evs[7] = {&t0, &tc, &ts0, &ts1, &t2, &t3, &tj};
+    for (hipEvent_t* e : evs) MRP_HIP(hipEventCreate(e));
     for (size_t i0 = 0; i0 < total && rc == MRP_OK; i0 += chunk_items) {
         const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
         a.item0 = (uint32_t)i0;
@@ -519,7 +527,8 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         if (n_active) *n_active += n_act;
         MRP_HIP(hipEventRecord(ts0, v->stream));
         if (n_act > 0) {
-            shadow_kernel<<<(n_act + 255) / 256, 256, 0, v->stream>>>(a, list, n_act, pos_act, fy, rng_pos, fx);
+            shadow_kernel<<<(n_act + 255) / 256, 256, 0, v->stream>>>(a, list, n_act, pos_act, fy, rng_pos, fx,
+                                                                      single);
             MRP_HIP(hipGetLastError());
         }
         MRP_HIP(hipEventRecord(ts1, v->stream));
@@ -542,16 +551,19 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
             } else {
                 rc = launch_mlp(packed, p->mlp_precision - 1, nullptr, feat, pool, gam, p->g, srp, n_act, pred,
                                 v->stream);
-                if (rc) break;
             }
-            MRP_HIP(hipGetLastError());
+            if (rc == MRP_OK) MRP_HIP(hipGetLastError());
         }
         MRP_HIP(hipEventRecord(t3, v->stream));
-        combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, fx, fy, fz, slot_of, pred, a.light_color,
-                                                                  a.scatter_rate, a.sky, cnt, (float4*)v->samples);
-        MRP_HIP(hipGetLastError());
-        MRP_HIP(hipEventSynchronize(t3));
-        if (times) {
+        if (rc == MRP_OK) {
+            combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, single, fy, fz, slot_of, pred,
+                                                                      a.light_color, a.scatter_rate, a.sky, cnt,
+                                                                      (float4*)v->samples);
+            MRP_HIP(hipGetLastError());
+        }
+        MRP_HIP(hipEventRecord(tj, v->stream));
+        MRP_HIP(hipEventSynchronize(tj));
+        if (rc == MRP_OK && times) {
             float ms;
             MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
             times->first_vertex += ms;
@@ -565,12 +577,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
             times->network += ms;
         }
     }
-    hipEventDestroy(t0);
-    hipEventDestroy(tc);
-    hipEventDestroy(ts0);
-    hipEventDestroy(ts1);
-    hipEventDestroy(t2);
-    hipEventDestroy(t3);
+    for (hipEvent_t* e : evs) hipEventDestroy(*e);
     return rc;
 }
 

@@ -144,42 +144,65 @@ MRP_HD bool free_flight_batched(Rng* rng, const Fetch& F, float inv_smax, float 
     return true;
 }
 
-// transmittance_seq with up to B lookups in flight.
+// State of a batched transmittance walk between rounds.
+struct TrWalk {
+    float t, tr;
+    int loop_num;  // the reference's loop counter (advanced by every loop test)
+    int phase;     // 1: ratio tracking, 2: delta tracking, 0: finished (tr is the result)
+};
+MRP_HD TrWalk tr_walk_start() {
+    TrWalk w;
+    w.t = 0.0f;
+    w.tr = 1.0f;
+    w.loop_num = 0;
+    w.phase = 1;
+    return w;
+}
+
+// One round of transmittance_seq with up to B lookups in flight: the walk
+// `w` advances by one window of steps (or moves to its next phase), and the
+// stream is left at exactly the draws the one-step loops would have used.
+// Both phases share one speculation block (a delta-tracking step draws a
+// second number), so lanes of a wave in different phases issue their
+// lookups together and wait for memory once per round.
 template <int B, class Fetch>
-MRP_HD float transmittance_batched(Rng* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
-                                   float dis, int max_steps, uint32_t* lk) {
-    float tr = 1.0f, t = 0.0f;
-    int loop_num = 0;  // the reference's counter
-    // ---- ratio tracking: one draw per step
-    for (;;) {
-        if (!(loop_num < max_steps && tr > 0.5f)) {
-            ++loop_num;  // the failing test still increments
+MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
+                                float dis, int max_steps, TrWalk* w, uint32_t* lk) {
+    // the loop test at the top of each phase
+    if (w->phase == 1 && !(w->loop_num < max_steps && w->tr > 0.5f)) {
+        ++w->loop_num;  // the failing test still increments
+        w->phase = w->tr > 0.001f ? 2 : 0;
+    }
+    if (w->phase == 2 && w->loop_num >= max_steps) w->phase = 0;
+    if (w->phase == 0) return;
+    const bool delta = w->phase == 2;  // delta tracking: two draws per step
+    const uint32_t n0 = rng->n;
+    Rng sp = *rng;
+    float ts[B], ua[B];
+    typename Fetch::Token tok[B];
+    int m = 0;
+    bool escape = false;
+    float tc = w->t;
+#pragma unroll
+    for (int j = 0; j < B; ++j) {
+        if (escape || w->loop_num + j >= max_steps) break;
+        const float rk = rng_next(&sp);
+        tc -= logf_(1.0f - rk) * inv_smax_tr;
+        if (tc > dis) {
+            escape = true;
             break;
         }
-        const uint32_t n0 = rng->n;
-        Rng sp = *rng;
-        float ts[B];
-        typename Fetch::Token tok[B];
-        int m = 0;
-        bool escape = false;
-        float tc = t;
-#pragma unroll
-        for (int j = 0; j < B; ++j) {
-            if (escape || loop_num + j >= max_steps) break;
-            const float rk = rng_next(&sp);
-            tc -= logf_(1.0f - rk) * inv_smax_tr;
-            if (tc > dis) {
-                escape = true;
-                break;
-            }
-            ts[j] = tc;
-            tok[j] = F.issue(pos + (dir * tc));
-            ++m;
-        }
-        // commit step by step; the loop test precedes each step
-        int done = 0;  // steps committed
+        ts[j] = tc;
+        tok[j] = F.issue(pos + (dir * tc));
+        ua[j] = delta ? rng_next(&sp) : 0.0f;
+        ++m;
+    }
+    if (!delta) {
+        // ---- ratio tracking: commit step by step; the loop test precedes each step
+        int done = 0;
         bool stop = false, restart = false;
-        float tdone = t;  // position of the last committed step
+        float tdone = w->t;
+        float tr = w->tr;
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             if (stop || restart || q >= m) break;
@@ -193,78 +216,78 @@ MRP_HD float transmittance_batched(Rng* rng, const Fetch& F, float inv_smax_tr, 
             done = q + 1;
             tdone = ts[q];
             if (d < 0.0f) {
-                t = ts[q] - d;
+                w->t = ts[q] - d;
                 restart = true;
             }
         }
-        loop_num += done;
+        w->tr = tr;
+        w->loop_num += done;
         if (restart) {
             rng_seek(rng, n0 + (uint32_t)done);
-            continue;
+            return;
         }
-        t = tdone;
+        w->t = tdone;
         if (stop) {
-            ++loop_num;
+            ++w->loop_num;
             rng_seek(rng, n0 + (uint32_t)done);
-            break;
+            w->phase = tr > 0.001f ? 2 : 0;
+            return;
         }
         if (escape) {
             // the escaping step's test: it runs only if the walk is still in ratio tracking
-            if (loop_num < max_steps && tr > 0.5f) return tr;  // t > dis: delta tracking breaks at once too
-            ++loop_num;
+            if (w->loop_num < max_steps && tr > 0.5f) {  // t > dis: delta tracking breaks at once too
+                w->phase = 0;
+                return;
+            }
+            ++w->loop_num;
             rng_seek(rng, n0 + (uint32_t)done);
-            break;
+            w->phase = tr > 0.001f ? 2 : 0;
+            return;
         }
         *rng = sp;  // exactly m draws consumed
+        return;
     }
-    // ---- delta tracking: two draws per step; tr stays until a collision zeroes it
-    if (!(tr > 0.001f)) return tr;
-    for (;;) {
-        if (loop_num >= max_steps) return tr;
-        const uint32_t n0 = rng->n;
-        Rng sp = *rng;
-        float ts[B], ua[B];
-        typename Fetch::Token tok[B];
-        int m = 0;
-        bool escape = false;
-        float tc = t;
+    // ---- delta tracking: tr stays until a collision zeroes it
+    bool restart = false, hit = false;
+    int done = 0;
 #pragma unroll
-        for (int j = 0; j < B; ++j) {
-            if (escape || loop_num + j >= max_steps) break;
-            const float rk = rng_next(&sp);
-            tc -= logf_(1.0f - rk) * inv_smax_tr;
-            if (tc > dis) {
-                escape = true;
-                break;
-            }
-            ts[j] = tc;
-            tok[j] = F.issue(pos + (dir * tc));
-            ua[j] = rng_next(&sp);
-            ++m;
+    for (int q = 0; q < B; ++q) {
+        if (restart || hit || q >= m) break;
+        const float d = F.finish(tok[q]);
+        *lk += 1u;
+        done = q + 1;
+        if (ua[q] < fmax_(0.0f, d * ratio)) {
+            hit = true;
+        } else if (d < 0.0f) {
+            w->t = ts[q] - d;
+            restart = true;
         }
-        bool restart = false;
-        int done = 0;
-#pragma unroll
-        for (int q = 0; q < B; ++q) {
-            if (restart || q >= m) break;
-            const float d = F.finish(tok[q]);
-            *lk += 1u;
-            done = q + 1;
-            if (ua[q] < fmax_(0.0f, d * ratio)) return 0.0f;
-            if (d < 0.0f) {
-                t = ts[q] - d;
-                restart = true;
-            }
-        }
-        loop_num += done;
-        if (restart) {
-            rng_seek(rng, n0 + 2u * (uint32_t)done);
-            continue;
-        }
-        if (escape) return tr;
-        t = tc;  // the last speculated position (t itself when m == 0)
-        *rng = sp;
     }
+    w->loop_num += done;
+    if (hit) {
+        w->tr = 0.0f;
+        w->phase = 0;
+        return;
+    }
+    if (restart) {
+        rng_seek(rng, n0 + 2u * (uint32_t)done);
+        return;
+    }
+    if (escape) {
+        w->phase = 0;
+        return;
+    }
+    w->t = tc;  // the last speculated position (t itself when m == 0)
+    *rng = sp;  // exactly 2m draws consumed
+}
+
+// transmittance_seq with up to B lookups in flight.
+template <int B, class Fetch>
+MRP_HD float transmittance_batched(Rng* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
+                                   float dis, int max_steps, uint32_t* lk) {
+    TrWalk w = tr_walk_start();
+    while (w.phase != 0) transmittance_round<B>(rng, F, inv_smax_tr, ratio, pos, dir, dis, max_steps, &w, lk);
+    return w.tr;
 }
 
 }  // namespace mrp

@@ -93,8 +93,8 @@ int main() {
         for (int cap : caps) {
             const int rays = cap == 10000 ? 20000 : 4000;
             bad += run<1>(F, cap, rays, 11) + run<2>(F, cap, rays, 12) + run<3>(F, cap, rays, 13) +
-                   run<4>(F, cap, rays, 14) + run<8>(F, cap, rays, 15);
-            cases += 5L * rays;
+                   run<4>(F, cap, rays, 14) + run<8>(F, cap, rays, 15) + run<16>(F, cap, rays, 16);
+            cases += 6L * rays;
         }
     std::printf("%ld rays x 2 trackers, %ld mismatches\n", cases, bad);
     return bad == 0 ? 0 : 1;
