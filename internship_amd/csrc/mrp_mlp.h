@@ -205,9 +205,11 @@ MRP_HD inline size_t tile_chunk(uint32_t item, int k, int c, int kg) {
 #if defined(__HIPCC__)
 // MFMA paths (mrp_mlp_mfma.hip); fmt 0: BF16 operands, 1: FP8-E4M3 operands.
 // Descriptors: FP32 rows (feat, `tiled` == nullptr) or wave-tiled BF16 (`tiled`).
+// With n_dev the count is read on the device and n_items only bounds the grid.
 int pack_mlp(const float* weights_host, int fmt, void** packed_dev);
 int launch_mlp(const void* packed, int fmt, const float* feat, const void* tiled, const float* pool,
-               const float* gamma, float g, vec3 srp, uint32_t n_items, float* pred, hipStream_t stream);
+               const float* gamma, float g, vec3 srp, uint32_t n_items, const uint32_t* n_dev, float* pred,
+               hipStream_t stream);
 #endif
 
 }  // namespace mrp

@@ -443,6 +443,7 @@ struct MlpArgs {
     float g;
     vec3 srp;
     uint32_t n_items;
+    const uint32_t* n_dev;  // if set, the descriptor count is read here (n_items bounds the grid)
     float* pred;
 };
 
@@ -472,7 +473,8 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
     const float* __restrict__ feat = args.feat;
     const float* __restrict__ pool = args.pool;
     const float g = args.g;
-    const uint32_t n_items = args.n_items;
+    const uint32_t n_items = args.n_dev ? *args.n_dev : args.n_items;
+    if (n_items == 0u) return;  // uniform; after the staging barrier
     const int lane = (int)__lane_id();
     const int kg = lane >> 5;
     const Packed<F> pk{s_a, s_bias, s_unscale};
@@ -740,10 +742,12 @@ __global__ void pool_kernel(const float* __restrict__ feat, uint32_t n_items, fl
 static int g_num_cus[64] = {};
 
 int launch_mlp(const void* packed, int fmt, const float* feat, const void* tiled, const float* pool,
-               const float* gamma, float g, vec3 srp, uint32_t n_items, float* pred, hipStream_t stream) {
+               const float* gamma, float g, vec3 srp, uint32_t n_items, const uint32_t* n_dev, float* pred,
+               hipStream_t stream) {
     if (fmt != (int)Fmt::BF16 && fmt != (int)Fmt::FP8) return fail(MRP_E_INVALID, "unknown packed network format");
     if (n_items == 0) return MRP_OK;
     if (tiled && !pool) return fail(MRP_E_INVALID, "tiled descriptors need the gather's pooled statistics");
+    if (n_dev && !pool) return fail(MRP_E_INVALID, "a device-side count needs the gather's pooled statistics");
     if (!tiled && !feat) return fail(MRP_E_INVALID, "no descriptors");
     int dev = 0;
     MRP_HIP(hipGetDevice(&dev));
@@ -757,7 +761,7 @@ int launch_mlp(const void* packed, int fmt, const float* feat, const void* tiled
     }
     const uint32_t tiles = (n_items + 31) / 32;
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)cus, (tiles + kMlpWaves - 1) / kMlpWaves));
-    const MlpArgs args{(const uint4*)packed, feat, (const uint4*)tiled, pool, gamma, g, srp, n_items, pred};
+    const MlpArgs args{(const uint4*)packed, feat, (const uint4*)tiled, pool, gamma, g, srp, n_items, n_dev, pred};
     if (fmt == (int)Fmt::FP8) {
         if (tiled)
             mlp_fp8_kernel<<<blocks, kMlpBlock, 0, stream>>>(args);

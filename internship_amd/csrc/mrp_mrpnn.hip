@@ -148,12 +148,14 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 // the next vertex when done (0.78 against 0.27 ms at 1080p), 2-8 lanes per
 // ray sharing one walk (0.35-0.64 ms), and this kernel on a second stream
 // beside the gather (both slowed by about what the overlap saved).
-__global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list, uint32_t n_act,
+__global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list,
+                                                     const uint32_t* __restrict__ n_act_dev,
                                                      const float4* __restrict__ pos_act, const float4* __restrict__ fy,
                                                      const uint32_t* __restrict__ rng_pos,
                                                      const float4* __restrict__ fx, float* __restrict__ single) {
     __shared__ unsigned red[256];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n_act = *n_act_dev;  // the compaction's count (the grid covers the whole chunk)
     uint32_t lk = 0u;
     if (k < n_act) {
         const uint32_t i = list[k];
@@ -187,6 +189,7 @@ struct GatherArgs {
     vec3 light, lx;
     float alpha, g;
     uint32_t n_items;
+    const uint32_t* n_dev;  // if set, the descriptor count is read here (n_items bounds the grid)
 };
 
 // Segments of the 12 squeeze-excitation blocks (`SE`/`SERES` call sites,
@@ -228,9 +231,10 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
     const int tm = mip - 1 > 0 ? mip - 1 : 0;
     const int tr = 128 >> tm;
     const vec3 st = i < kShellTaps ? v3(c_stencil[3 * i], c_stencil[3 * i + 1], c_stencil[3 * i + 2]) : v3(0, 0, 0);
+    const uint32_t n_items = a.n_dev ? *a.n_dev : a.n_items;  // a grid sized for n_items ends early
     for (int it = 0; it < kGatherItems; ++it) {
         const uint32_t item = blockIdx.x * kGatherItems + (uint32_t)it;
-        if (item >= a.n_items) break;  // uniform over the block
+        if (item >= n_items) break;  // uniform over the block
         const uint32_t src = list ? list[item] : item;  // compacted: the vertex this descriptor belongs to
         const float4 pa = pos_act[src], x4 = fx[src], y4 = fy[src], z4 = fz[src];
         const vec3 p = v3(pa.x, pa.y, pa.z);
@@ -301,7 +305,7 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
                 }
                 pool[(size_t)item * 72 + i] = r;
             }
-            if (kTiled && ((it & (kStageItems - 1)) == kStageItems - 1 || item + 1 >= a.n_items)) {
+            if (kTiled && ((it & (kStageItems - 1)) == kStageItems - 1 || item + 1 >= n_items)) {
                 // uniform over the block; items start at multiples of kStageItems (kGatherItems is one)
                 __syncthreads();
                 const int cnt = (it & (kStageItems - 1)) + 1;
@@ -321,9 +325,10 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
 // ---------------------------------------------------------------- stage 3 (FP32)
 __global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ w, const float* __restrict__ feat,
                                                       const float* __restrict__ gamma, float g, vec3 srp,
-                                                      uint32_t n_items, float* __restrict__ pred) {
+                                                      uint32_t n_items, const uint32_t* __restrict__ n_dev,
+                                                      float* __restrict__ pred) {
     const uint32_t item = blockIdx.x * blockDim.x + threadIdx.x;
-    if (item >= n_items) return;
+    if (item >= (n_dev ? *n_dev : n_items)) return;
     float o[3];
     mlp_forward_fp32(w, feat + (size_t)item * kFeatures, gamma[item], g, srp, o);
     pred[3 * item] = o[0];
@@ -385,6 +390,7 @@ __global__ void clear_counters_kernel(unsigned long long* p, int n) {
 }
 
 static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
+
 
 
 static bool g_tables_ready[64] = {};
@@ -521,22 +527,19 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         clear_u32_one_kernel<<<1, 1, 0, v->stream>>>(n_active_dev);
         compact_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, cnt, list, slot_of, n_active_dev);
         MRP_HIP(hipGetLastError());
-        uint32_t n_act = 0;
-        MRP_HIP(hipMemcpyAsync(&n_act, n_active_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
-        MRP_HIP(hipStreamSynchronize(v->stream));
-        if (n_active) *n_active += n_act;
+        // No read-back here: the shadow rays, the gather and the network read the
+        // count on the device, with grids sized for the whole chunk.
         MRP_HIP(hipEventRecord(ts0, v->stream));
-        if (n_act > 0) {
-            shadow_kernel<<<(n_act + 255) / 256, 256, 0, v->stream>>>(a, list, n_act, pos_act, fy, rng_pos, fx,
-                                                                      single);
-            MRP_HIP(hipGetLastError());
-        }
+        shadow_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, rng_pos, fx,
+                                                                  single);
+        MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(ts1, v->stream));
-        ga.n_items = n_act;
-        if (n_act > 0) {
+        ga.n_items = cnt;
+        ga.n_dev = n_active_dev;
+        {
             // the MFMA networks read wave-tiled BF16 descriptors (same region: ceil(n/32) tiles
             // of kTileBytes fit in n FP32 rows for n >= 32); the FP32 network reads FP32 rows
-            const unsigned gblocks = (n_act + kGatherItems - 1) / kGatherItems;
+            const unsigned gblocks = (cnt + kGatherItems - 1) / kGatherItems;  // groups past the count exit at once
             if (p->mlp_precision == 0)
                 gather_kernel<false><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam, pool);
             else
@@ -544,15 +547,13 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
             MRP_HIP(hipGetLastError());
         }
         MRP_HIP(hipEventRecord(t2, v->stream));
-        if (n_act > 0) {
-            if (p->mlp_precision == 0) {
-                mlp_fp32_kernel<<<(n_act + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, n_act,
-                                                                         pred);
-            } else {
-                rc = launch_mlp(packed, p->mlp_precision - 1, nullptr, feat, pool, gam, p->g, srp, n_act, pred,
-                                v->stream);
-            }
-            if (rc == MRP_OK) MRP_HIP(hipGetLastError());
+        if (p->mlp_precision == 0) {
+            mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt,
+                                                                     n_active_dev, pred);
+            MRP_HIP(hipGetLastError());
+        } else {
+            rc = launch_mlp(packed, p->mlp_precision - 1, nullptr, feat, pool, gam, p->g, srp, cnt, n_active_dev, pred,
+                            v->stream);
         }
         MRP_HIP(hipEventRecord(t3, v->stream));
         if (rc == MRP_OK) {
@@ -561,13 +562,16 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
                                                                       (float4*)v->samples);
             MRP_HIP(hipGetLastError());
         }
+        uint32_t n_act = 0;
+        MRP_HIP(hipMemcpyAsync(&n_act, n_active_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
         MRP_HIP(hipEventRecord(tj, v->stream));
         MRP_HIP(hipEventSynchronize(tj));
+        if (n_active) *n_active += n_act;
         if (rc == MRP_OK && times) {
             float ms;
             MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
             times->first_vertex += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));  // compaction and its count read-back
+            MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));
             times->compact += ms;
             MRP_HIP(hipEventElapsedTime(&ms, ts0, ts1));
             times->shadow += ms;
@@ -705,6 +709,7 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     ga.alpha = alpha;
     ga.g = g;
     ga.n_items = (uint32_t)n;
+    ga.n_dev = nullptr;
     gather_kernel<false><<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
         ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev, (const float4*)frame_y_dev,
         (const float4*)frame_z_dev, nullptr, feat_dev, gamma_dev, nullptr);
@@ -721,13 +726,14 @@ int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const 
     MRP_HIP(hipSetDevice(net->device));
     const vec3 s = V(srp);
     if (precision == 0) {
-        mlp_fp32_kernel<<<(n + 63) / 64, 64>>>(net->w, feat_dev, gamma_dev, g, s, (uint32_t)n, out_dev);
+        mlp_fp32_kernel<<<(n + 63) / 64, 64>>>(net->w, feat_dev, gamma_dev, g, s, (uint32_t)n, nullptr, out_dev);
         MRP_HIP(hipGetLastError());
     } else if (precision == 1 || precision == 2) {
         const void* packed = precision == 2 ? net->packed_fp8 : net->packed;
         if (!packed) return fail(MRP_E_STATE, "network has no MFMA packing");
         const int rc =
-            launch_mlp(packed, precision - 1, feat_dev, nullptr, nullptr, gamma_dev, g, s, (uint32_t)n, out_dev, nullptr);
+            launch_mlp(packed, precision - 1, feat_dev, nullptr, nullptr, gamma_dev, g, s, (uint32_t)n, nullptr, out_dev,
+                       nullptr);
         if (rc) return rc;
     } else {
         return fail(MRP_E_INVALID, "precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
