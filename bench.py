@@ -391,6 +391,7 @@ def run_frame(a, ia, sc):
     log("progressive frames: %.2f ms/frame (trace %.2f, histogram + denoise + tone + pack %.3f ms)"
         % (1000 * elapsed / a.steps, tr_ms, post_ms))
     return {
+        "metric": "1080p progressive path-traced frames/s (checkerboard, histograms, denoise, ACES, BGRA8)",
         "value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "f32",
         "config": {"workload": "progressive path-traced frames, 512^3 Perlin-fBm cloud, %dx%d, checkerboard "
                                "(half the pixels per frame), histogram denoiser, ACES, BGRA8" % (W, H),
