@@ -198,7 +198,7 @@ MRP_HD void mlp_forward_fp32(const float* w, const float* feat, float gamma, flo
 // fetches a whole 16-feature block of 32 descriptors.  1152 bytes per descriptor.
 constexpr int kStreamChunks = kTaps / 16;
 constexpr size_t kTileBytes = (size_t)3 * kStreamChunks * 64 * 16;
-MRP_HD inline size_t tile_chunk(uint32_t item, int k, int c, int kg) {
+MRP_HD size_t tile_chunk(uint32_t item, int k, int c, int kg) {
     return (((size_t)(item >> 5) * 3 + (size_t)k) * kStreamChunks + (size_t)c) * 64 + (size_t)(kg * 32) + (item & 31u);
 }
 

@@ -23,6 +23,12 @@ struct TapTable {
     int mip[kTaps];
     float lut_v[kTaps];            // HG-table v of taps 1..191 (tap 0 uses HG directly)
     float stencil[kShellTaps][3];  // unit-frame stencil points
+    // HG-table u of the shell taps.  A shell tap lies at p + (X s0 + Y s1 + Z s2) * off
+    // with (X, Y, Z) orthonormal and off > 0, so its direction from p is
+    // (X s0 + Y s1 + Z s2) / |s|: its cosine to X is s0 / |s| for every vertex, and
+    // its cosine to the light axis LX is (LX.X s0 + LX.Y s1 + LX.Z s2) / |s|.
+    float inv_len[kShellTaps];     // 1 / |s|
+    float u_view[kShellTaps];      // s0 / |s| * 0.5 + 0.5
 };
 
 // reference lerp(a, b, v) = a*(1-v) + b*v in FP32
@@ -89,8 +95,18 @@ static inline void descriptor_schedule(TapTable* t) {
         const float angle = atanf_(0.5f * radius / off);
         t->lut_v[i] = angle / (3.1415926535f * 60.0f / 180.0f);
     }
-    for (int i = 0; i < kShellTaps; ++i)
+    for (int i = 0; i < kShellTaps; ++i) {
         for (int c = 0; c < 3; ++c) t->stencil[i][c] = kStencil[i][c];
+        const float* s = kStencil[i];
+        t->inv_len[i] = 1.0f / sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        t->u_view[i] = s[0] * t->inv_len[i] * 0.5f + 0.5f;
+    }
+}
+
+// HG-table u of a shell tap's direction against the light axis (see TapTable),
+// from lxX = LX.X, lxY = LX.Y, lxZ = LX.Z.
+MRP_HD float tap_u_light(float lxX, float lxY, float lxZ, float s0, float s1, float s2, float inv_len) {
+    return ((lxX * s0 + lxY * s1) + lxZ * s2) * inv_len * 0.5f + 0.5f;
 }
 
 }  // namespace mrp

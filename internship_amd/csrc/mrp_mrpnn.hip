@@ -39,6 +39,8 @@ __constant__ float c_tap_offset[kTaps];
 __constant__ int c_tap_mip[kTaps];
 __constant__ float c_tap_v[kTaps];
 __constant__ float c_stencil[kShellTaps * 3];
+__constant__ float c_tap_inv_len[kShellTaps];
+__constant__ float c_tap_u_view[kShellTaps];
 
 struct NNArgs {
     const float4* cells;
@@ -213,8 +215,10 @@ static_assert(kGatherItems % 8 == 0, "the tiled gather flushes groups of 8 descr
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
+// The gather is bound by memory latency per descriptor, so it is held to 7
+// waves per SIMD (at most 72 VGPRs): at 76 VGPRs and 6 waves it ran 7 % slower.
 template <bool kTiled>
-__global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
+__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
                                                        void* __restrict__ feat_out, float* __restrict__ gamma,
@@ -231,6 +235,14 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
     const int tm = mip - 1 > 0 ? mip - 1 : 0;
     const int tr = 128 >> tm;
     const vec3 st = i < kShellTaps ? v3(c_stencil[3 * i], c_stencil[3 * i + 1], c_stencil[3 * i + 2]) : v3(0, 0, 0);
+    const float s_inv = i < kShellTaps ? c_tap_inv_len[i] : 0.0f;
+    // The HG-table factor that is the same for every descriptor (TapTable): the
+    // view-axis factor of a shell tap, the light-axis factor of a light tap.
+    float h_fixed = 0.0f;
+    if (i > 0 && i < kShellTaps)
+        h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, c_tap_u_view[i], lut_v);
+    else if (i >= kShellTaps)
+        h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, dot(a.lx, a.light) * 0.5f + 0.5f, lut_v);
     const uint32_t n_items = a.n_dev ? *a.n_dev : a.n_items;  // a grid sized for n_items ends early
     for (int it = 0; it < kGatherItems; ++it) {
         const uint32_t item = blockIdx.x * kGatherItems + (uint32_t)it;
@@ -249,16 +261,18 @@ __global__ __launch_bounds__(kTaps) void gather_kernel(GatherArgs a, const float
                 pi = p + a.light * off;
             }
             xd = logf_(1.0f + a.alpha * trilinear_cells(a.mips[mip], 256 >> mip, pi) / 64.0f);
+            float harg;
             if (i == 0) {
-                xh = logf_(hg_phase(dot(X, a.light), a.g) + 1.0f);
+                harg = hg_phase(dot(X, a.light), a.g);
             } else {
-                const vec3 ms = normalize(pi - p);
-                const float u = dot(X, ms) * 0.5f + 0.5f;
-                const float u2 = dot(a.lx, ms) * 0.5f + 0.5f;
-                xh = logf_(bilinear_cells(a.lut, kHgLutSize, kHgLutSize, u, lut_v) *
-                               bilinear_cells(a.lut, kHgLutSize, kHgLutSize, u2, lut_v) +
-                           1.0f);
+                // the factor that varies: light axis for a shell tap, view axis for a light
+                // tap (whose direction is the light's); the product commutes exactly
+                const float uv = i < kShellTaps
+                                     ? tap_u_light(dot(a.lx, X), dot(a.lx, Y), dot(a.lx, Z), st.x, st.y, st.z, s_inv)
+                                     : dot(X, a.light) * 0.5f + 0.5f;
+                harg = h_fixed * bilinear_cells(a.lut, kHgLutSize, kHgLutSize, uv, lut_v);
             }
+            xh = logf_(harg + 1.0f);
             if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
                 const float o = ray_box_offset(pi, a.lx);
                 xt = o >= 0.0f ? trilinear_clamp_cells(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
@@ -405,6 +419,8 @@ static int upload_tap_tables(int device) {
     MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_mip), t.mip, sizeof(t.mip)));
     MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_v), t.lut_v, sizeof(t.lut_v)));
     MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_stencil), &t.stencil[0][0], sizeof(t.stencil)));
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_inv_len), t.inv_len, sizeof(t.inv_len)));
+    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_tap_u_view), t.u_view, sizeof(t.u_view)));
     if (device >= 0 && device < 64) g_tables_ready[device] = true;
     return MRP_OK;
 }

@@ -490,9 +490,19 @@ int oracle_mrpnn_features(const float* mips, const float* trs, const float* lut,
             if (i == 0) {
                 xh[i] = logf_(hg_phase(dot(X, L), g) + 1.0f);
             } else {
-                const vec3 ms = normalize(pi - p);
-                const float u = dot(X, ms) * 0.5f + 0.5f;
-                const float u2 = dot(LX, ms) * 0.5f + 0.5f;
+                // table coordinates: cosines of the tap direction to the view axis X (u)
+                // and to the light axis LX (u2).  A shell tap's direction is
+                // (X s0 + Y s1 + Z s2)/|s| (orthonormal frame, off > 0), so u is the
+                // tap's constant s0/|s|*0.5+0.5; a light tap's direction is L.
+                float u, u2;
+                if (i < kShellTaps) {
+                    const float* s = T.stencil[i];
+                    u = T.u_view[i];
+                    u2 = tap_u_light(dot(LX, X), dot(LX, Y), dot(LX, Z), s[0], s[1], s[2], T.inv_len[i]);
+                } else {
+                    u = dot(X, L) * 0.5f + 0.5f;
+                    u2 = dot(LX, L) * 0.5f + 0.5f;
+                }
                 const float v = T.lut_v[i];
                 const float h0 = bilinear_clamp(lut, kHgLutSize, kHgLutSize, u, v);
                 const float h1 = bilinear_clamp(lut, kHgLutSize, kHgLutSize, u2, v);
