@@ -428,7 +428,8 @@ def run_train(a, ia, sc):
     ab.upload(aux)
     B = min(a.batch, len(feat))
     w0 = ia.load_reference_weights()
-    tr = ia.Trainer(w0, max_batch=B)
+    prec = a.train_precision
+    tr = ia.Trainer(w0, max_batch=B, precision=prec)
 
     def batch_views(i):
         off = (i * B) % (len(feat) - B + 1)
@@ -446,8 +447,22 @@ def run_train(a, ia, sc):
     w1 = tr.weights()
     tr.close()
     sps = a.steps * B / elapsed
+    # the other arithmetic on the same batches (throughput only)
+    other = ia.Trainer(w0, max_batch=B, precision=1 - prec)
+    f, x = batch_views(0)
+    other.step(f, x, B, a.lr)
+    n_other = 3
+    t0 = time.perf_counter()
+    for i in range(n_other):
+        f, x = batch_views(i + 1)
+        other.step(f, x, B, a.lr)
+    other_sps = n_other * B / (time.perf_counter() - t0)
+    other.close()
+    log("training: %.4g samples/s (%s), %.4g samples/s (%s)" % (sps, "BF16 matrix cores" if prec else "FP32",
+                                                                 other_sps, "FP32" if prec else "BF16 matrix cores"))
     flops = 3 * MLP_FLOP_PER_PASS  # forward + backward (dX and dW) per sample
     achieved = sps * flops / 1e12
+    peak = BF16_PEAK_TFLOPS if prec else FP32_PEAK_TFLOPS
     ref_pred = datagen.predict(w0, hfeat, haux)
     new_pred = datagen.predict(w1, hfeat, haux)
     ab_res = {"heldout_samples": len(hfeat),
@@ -458,14 +473,16 @@ def run_train(a, ia, sc):
         % (ab_res["rel_l2_shipped_weights"], ab_res["rel_l2_trained_weights"], losses[0], losses[-1]))
     return {
         "metric": "MRPNN training samples/s (forward + backward + Adam, GPU)", "unit": "samples/s",
-        "value": sps, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "f32",
+        "value": sps, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16" if prec else "f32",
         "config": {"workload": "MRPNN training, batch %d, %d procedural 128^3 clouds x %d points, GetSample targets "
                                "%d spp / %d bounces" % (B, n_train, a.points_per_cloud, a.target_spp,
                                                          a.target_bounces),
-                   "batch": B, "lr": a.lr},
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None, "kernel": "train_fwd_bwd_kernel + wgrad",
+                   "batch": B, "lr": a.lr, "precision": prec},
+        "roofline": {"bound": "mfma" if prec else "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "kernel": ("train_fwd_mfma + train_bwd_mfma" if prec else "train_fwd_bwd_kernel") + " + wgrad",
                      "flop_per_sample": flops},
+        "other_precision": {"precision": 1 - prec, "samples_per_s": other_sps},
         "datagen": {"seconds": gen_s, "target_seconds": gst["target_s"], "paths": gst["paths"],
                     "target_lookups": gst["target_lookups"]},
         "ab_test": ab_res,
@@ -617,6 +634,8 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
     ap.add_argument("--workers", type=int, default=4, help="serve mode: host threads")
     ap.add_argument("--lr", type=float, default=1e-4, help="train mode: Adam learning rate")
+    ap.add_argument("--train-precision", type=int, choices=[0, 1], default=1,
+                    help="train mode: 0 FP32 forward/backward, 1 BF16 matrix cores (FP32 master weights)")
     ap.add_argument("--train-clouds", type=int, default=16)
     ap.add_argument("--points-per-cloud", type=int, default=8192)
     ap.add_argument("--target-spp", type=int, default=32)

@@ -278,6 +278,12 @@ int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const 
 typedef struct mrp_trainer mrp_trainer;
 int mrp_trainer_create(const float* weights_host, size_t count, int max_batch, mrp_trainer** out);
 int mrp_trainer_destroy(mrp_trainer* t);
+/* Arithmetic of the forward and backward products (train/train.py:220-254
+ * trains in FP32): 0 = FP32 (default), 1 = BF16 matrix cores - BF16 copies of
+ * the FP32 master weights made before each gradient, activations rounded to
+ * BF16 at each product, FP32 accumulation and records; weight gradients,
+ * loss and Adam are the same for both. */
+int mrp_trainer_set_precision(mrp_trainer* t, int precision);
 /* Floats of per-sample activation record (HBM: record_floats * max_batch * 4 bytes). */
 int mrp_trainer_record_floats(const mrp_trainer* t, int* out);
 /* Loss and gradient of the current weights on a batch (no update); grad_host

@@ -158,17 +158,23 @@ class Network:
 
 
 class Trainer:
-    """GPU training of the MRPNN weights (mrp_trainer_*): FP32 forward and
-    backward, FP64 weight gradients, Adam on FP32 master weights.  A batch is
-    feat (B x 576 FP32 in HBM, the gather's layout) and aux (B x 4 FP32:
+    """GPU training of the MRPNN weights (mrp_trainer_*): forward and backward
+    in FP32 (precision 0) or on BF16 matrix cores with FP32 accumulation
+    (precision 1), FP64 weight gradients, Adam on FP32 master weights.  A batch
+    is feat (B x 576 FP32 in HBM, the gather's layout) and aux (B x 4 FP32:
     gamma, g, srp, target)."""
 
-    def __init__(self, weights: np.ndarray = None, max_batch: int = 16384):
+    def __init__(self, weights: np.ndarray = None, max_batch: int = 16384, precision: int = 0):
         self.lib = load()
         w = load_reference_weights() if weights is None else np.ascontiguousarray(weights, dtype=np.float32)
         self.h = ctypes.c_void_p()
         self.max_batch = int(max_batch)
         check(self.lib.mrp_trainer_create(_ptr(w), ctypes.c_size_t(w.size), self.max_batch, ctypes.byref(self.h)))
+        if precision:
+            self.set_precision(precision)
+
+    def set_precision(self, precision: int):
+        check(self.lib.mrp_trainer_set_precision(self.h, int(precision)))
 
     def gradient(self, feat: "DeviceBuffer", aux: "DeviceBuffer", batch: int):
         g = np.zeros(49673, dtype=np.float64)

@@ -106,6 +106,7 @@ SIGNATURES = {
     "mrp_tone_map_host": (I, [P, SZ, ctypes.c_int, P]),
     "mrp_trainer_create": (I, [P, SZ, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "mrp_trainer_destroy": (I, [P]),
+    "mrp_trainer_set_precision": (I, [P, ctypes.c_int]),
     "mrp_trainer_record_floats": (I, [P, ctypes.POINTER(ctypes.c_int)]),
     "mrp_trainer_gradient": (I, [P, ctypes.c_int, P, P, P, ctypes.POINTER(ctypes.c_double)]),
     "mrp_trainer_step": (I, [P, ctypes.c_int, P, P, ctypes.c_float, ctypes.POINTER(ctypes.c_double)]),

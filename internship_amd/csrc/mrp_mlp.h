@@ -210,6 +210,18 @@ int pack_mlp(const float* weights_host, int fmt, void** packed_dev);
 int launch_mlp(const void* packed, int fmt, const float* feat, const void* tiled, const float* pool,
                const float* gamma, float g, vec3 srp, uint32_t n_items, const uint32_t* n_dev, float* pred,
                hipStream_t stream);
+// Per-segment means and maxima (72 per descriptor) of n FP32 descriptor rows.
+int launch_pool(const float* feat, uint32_t n_items, float* pool, hipStream_t stream);
+
+// BF16 matrix-core training step (mrp_train_mfma.hip): forward and backward of
+// a batch into the trainer's record (layout of mrp_train.h), loss and p per
+// sample, the batch's loss sum in FP64.
+struct TrainMfma;
+int train_mfma_create(int max_batch, TrainMfma** out);
+void train_mfma_destroy(TrainMfma* m);
+int train_mfma_fwd_bwd(TrainMfma* m, const float* w_dev, const float* feat, const float* aux, uint32_t batch,
+                       float inv_batch, float* rec, float* loss, float* pred, double* loss_sum,
+                       hipStream_t stream);
 #endif
 
 }  // namespace mrp

@@ -12,6 +12,10 @@
 // backward writes dL/dz (z = pre-activation).  The weight gradient of a layer
 // over a batch is then the product dZ^T X of two record slices, computed by a
 // separate GEMM kernel (and by the oracle in FP64).
+//
+// Q rounds the operands of the products that the matrix-core trainer runs in
+// BF16 (layers with lowp set: W and x in the forward, W and dz in dx = W^T dz);
+// NoRound leaves them exact.
 #pragma once
 
 #include "mrp_mlp.h"
@@ -25,6 +29,12 @@ struct TrainLayer {
     int out, in;
     int act, bias_last; // activation; bias added after (1) or before (0) the sum
     int x_off, y_off, dz_off;  // record slots
+    int lowp;           // 1: a matrix-core product in the BF16 trainer (0: LGGS, LX12 on the VALU)
+};
+
+struct NoRound {
+    template <class T>
+    static constexpr T r(T x) { return x; }  // constexpr: host and device
 };
 
 // layer ids
@@ -41,6 +51,7 @@ static inline int build_train_layers(TrainLayer* L) {
     int off = 0, n = 0;
     auto add = [&](int w, int b, int out, int in, int act, int bias_last) {
         TrainLayer& l = L[n++];
+        l.lowp = w != MRP_W_LGGSW && w != MRP_W_LX12W;
         l.w_off = w;
         l.b_off = b;
         l.out = out;
@@ -86,13 +97,17 @@ static inline double t_fma(double a, double b, double c) { return a * b + c; }
 #endif
 
 // Dense layer forward: records x and y.
-template <class T, class Rec>
+template <class T, class Rec, class Q = NoRound>
 MRP_HD void t_lin(const float* w, const TrainLayer& l, const T* in, T* out, Rec& rec) {
     for (int j = 0; j < l.in; ++j) rec.put(l.x_off + j, in[j]);
     for (int i = 0; i < l.out; ++i) {
         T acc = (l.b_off >= 0 && !l.bias_last) ? (T)w[l.b_off + i] : (T)0;
         const float* wr = w + l.w_off + i * l.in;
-        for (int j = 0; j < l.in; ++j) acc = t_fma(in[j], (T)wr[j], acc);
+        if (l.lowp) {
+            for (int j = 0; j < l.in; ++j) acc = t_fma(Q::r(in[j]), Q::r((T)wr[j]), acc);
+        } else {
+            for (int j = 0; j < l.in; ++j) acc = t_fma(in[j], (T)wr[j], acc);
+        }
         if (l.b_off >= 0 && l.bias_last) acc = acc + (T)w[l.b_off + i];
         if (l.act == ACT_RELU) acc = acc >= (T)0 ? acc : (T)0;
         else if (l.act == ACT_SIGMOID) acc = (T)1 / ((T)1 + t_exp(-acc));
@@ -102,7 +117,7 @@ MRP_HD void t_lin(const float* w, const TrainLayer& l, const T* in, T* out, Rec&
 }
 
 // Dense layer backward: dz = dy * act'(y) (recorded); dx = W^T dz (if dx).
-template <class T, class Rec>
+template <class T, class Rec, class Q = NoRound>
 MRP_HD void t_lin_back(const float* w, const TrainLayer& l, const T* dy, T* dx, Rec& rec) {
     T dz[128];
     for (int i = 0; i < l.out; ++i) {
@@ -116,14 +131,18 @@ MRP_HD void t_lin_back(const float* w, const TrainLayer& l, const T* dy, T* dx, 
     if (!dx) return;
     for (int j = 0; j < l.in; ++j) {
         T s = (T)0;
-        for (int i = 0; i < l.out; ++i) s = t_fma((T)w[l.w_off + i * l.in + j], dz[i], s);
+        if (l.lowp) {
+            for (int i = 0; i < l.out; ++i) s = t_fma(Q::r((T)w[l.w_off + i * l.in + j]), Q::r(dz[i]), s);
+        } else {
+            for (int i = 0; i < l.out; ++i) s = t_fma((T)w[l.w_off + i * l.in + j], dz[i], s);
+        }
         dx[j] = s;
     }
 }
 
 // Forward and backward of one sample; returns its loss.  inv_batch scales the
 // recorded gradients (the batch loss is the mean).  *pred receives p.
-template <class T, class Rec>
+template <class T, class Rec, class Q = NoRound>
 MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, float gamma, float g, float srp,
                       float target, T inv_batch, Rec& rec, T* pred) {
     const StreamOp ops[kStreamOps] = {MRP_STREAM_OPS};
@@ -167,8 +186,8 @@ MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, fl
         }
         t[6] = (T)g;
         t[7] = (T)gamma;
-        t_lin(w, L[5 * e], t, h, rec);
-        t_lin(w, L[5 * e + 1], h, gate, rec);
+        t_lin<T, Rec, Q>(w, L[5 * e], t, h, rec);
+        t_lin<T, Rec, Q>(w, L[5 * e + 1], h, gate, rec);
         for (int k = 0; k < 3; ++k) {
             T x[32], out[32];
             const float* in = feat + k * kTaps + op.from;
@@ -177,30 +196,30 @@ MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, fl
                 if (op.op == OP_SERES) v = v + y[k][op.last + i];
                 x[i] = v;
             }
-            t_lin(w, L[5 * e + 2 + k], x, out, rec);
+            t_lin<T, Rec, Q>(w, L[5 * e + 2 + k], x, out, rec);
             for (int i = 0; i < op.size; ++i) y[k][op.to + i] = out[i];
         }
         ++e;
     }
     // ---------------------------------------------------------- head
     T comb[120], gg[8], se1[16], se2[6], s[128], h0[128], h1[64], h2[32], a[16], b[16], c[16], P;
-    for (int k = 0; k < 3; ++k) t_lin(w, L[TL_L81 + k], &y[k][128], comb + 32 * k, rec);
-    for (int k = 0; k < 3; ++k) t_lin(w, L[TL_LD41 + k], &y[k][24], comb + 96 + 8 * k, rec);
-    t_lin(w, L[TL_LGGS], pool + 72, gg, rec);
-    t_lin(w, L[TL_FIN1], pool, se1, rec);
-    t_lin(w, L[TL_FIN2], se1, se2, rec);
+    for (int k = 0; k < 3; ++k) t_lin<T, Rec, Q>(w, L[TL_L81 + k], &y[k][128], comb + 32 * k, rec);
+    for (int k = 0; k < 3; ++k) t_lin<T, Rec, Q>(w, L[TL_LD41 + k], &y[k][24], comb + 96 + 8 * k, rec);
+    t_lin<T, Rec, Q>(w, L[TL_LGGS], pool + 72, gg, rec);
+    t_lin<T, Rec, Q>(w, L[TL_FIN1], pool, se1, rec);
+    t_lin<T, Rec, Q>(w, L[TL_FIN2], se1, se2, rec);
     for (int i = 0; i < 120; ++i) s[i] = comb[i] * se2[i < 96 ? i / 32 : 3 + (i - 96) / 8];
     for (int i = 0; i < 8; ++i) s[120 + i] = gg[i];
-    t_lin(w, L[TL_LC0], s, h0, rec);
-    t_lin(w, L[TL_LC1], h0, h1, rec);
-    t_lin(w, L[TL_LC2], h1, h2, rec);
-    t_lin(w, L[TL_LX], h2, a, rec);
+    t_lin<T, Rec, Q>(w, L[TL_LC0], s, h0, rec);
+    t_lin<T, Rec, Q>(w, L[TL_LC1], h0, h1, rec);
+    t_lin<T, Rec, Q>(w, L[TL_LC2], h1, h2, rec);
+    t_lin<T, Rec, Q>(w, L[TL_LX], h2, a, rec);
     for (int r = 0; r < 6; ++r) {
-        t_lin(w, L[TL_LXR + 2 * r], a, b, rec);
-        t_lin(w, L[TL_LXR + 2 * r + 1], b, c, rec);
+        t_lin<T, Rec, Q>(w, L[TL_LXR + 2 * r], a, b, rec);
+        t_lin<T, Rec, Q>(w, L[TL_LXR + 2 * r + 1], b, c, rec);
         for (int i = 0; i < 16; ++i) a[i] = a[i] + c[i];
     }
-    t_lin(w, L[TL_LX12], a, &P, rec);
+    t_lin<T, Rec, Q>(w, L[TL_LX12], a, &P, rec);
     const T eP = t_exp(P);
     const T p = (eP - (T)1 > (T)0 ? eP - (T)1 : (T)0) * (T)srp;
     *pred = p;
@@ -211,25 +230,25 @@ MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, fl
     T dP = (T)2 * diff / den * inv_batch;
     dP = eP - (T)1 > (T)0 ? dP * (T)srp * eP : (T)0;
     T da[16], dt[16], dc[16], dh2[32], dh1[64], dh0[128], ds[128];
-    t_lin_back(w, L[TL_LX12], &dP, da, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_LX12], &dP, da, rec);
     for (int r = 5; r >= 0; --r) {
-        t_lin_back(w, L[TL_LXR + 2 * r + 1], da, dt, rec);
-        t_lin_back(w, L[TL_LXR + 2 * r], dt, dc, rec);
+        t_lin_back<T, Rec, Q>(w, L[TL_LXR + 2 * r + 1], da, dt, rec);
+        t_lin_back<T, Rec, Q>(w, L[TL_LXR + 2 * r], dt, dc, rec);
         for (int i = 0; i < 16; ++i) da[i] = da[i] + dc[i];
     }
-    t_lin_back(w, L[TL_LX], da, dh2, rec);
-    t_lin_back(w, L[TL_LC2], dh2, dh1, rec);
-    t_lin_back(w, L[TL_LC1], dh1, dh0, rec);
-    t_lin_back(w, L[TL_LC0], dh0, ds, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_LX], da, dh2, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_LC2], dh2, dh1, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_LC1], dh1, dh0, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_LC0], dh0, ds, rec);
     T dcomb[120], dse2[6] = {(T)0, (T)0, (T)0, (T)0, (T)0, (T)0}, dse1[16];
     for (int i = 0; i < 120; ++i) {
         const int seg = i < 96 ? i / 32 : 3 + (i - 96) / 8;
         dcomb[i] = ds[i] * se2[seg];
         dse2[seg] = dse2[seg] + ds[i] * comb[i];
     }
-    t_lin_back(w, L[TL_LGGS], ds + 120, (T*)nullptr, rec);
-    t_lin_back(w, L[TL_FIN2], dse2, dse1, rec);
-    t_lin_back(w, L[TL_FIN1], dse1, (T*)nullptr, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_LGGS], ds + 120, (T*)nullptr, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_FIN2], dse2, dse1, rec);
+    t_lin_back<T, Rec, Q>(w, L[TL_FIN1], dse1, (T*)nullptr, rec);
     // ---------------------------------------------------------- backward: streams
     // G[k][i] = dL/dy[k][i] for the current contents of y; the light stream
     // (ops 16..23) and the main stream (ops 0..15) are separate chains.
@@ -240,10 +259,10 @@ MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, fl
         T dyo[32];
         for (int k = 0; k < 3; ++k) {
             if (chain == 0) {
-                t_lin_back(w, L[TL_LD41 + k], dcomb + 96 + 8 * k, dyo, rec);
+                t_lin_back<T, Rec, Q>(w, L[TL_LD41 + k], dcomb + 96 + 8 * k, dyo, rec);
                 for (int i = 0; i < 8; ++i) G[k][24 + i] = dyo[i];
             } else {
-                t_lin_back(w, L[TL_L81 + k], dcomb + 32 * k, dyo, rec);
+                t_lin_back<T, Rec, Q>(w, L[TL_L81 + k], dcomb + 32 * k, dyo, rec);
                 for (int i = 0; i < 32; ++i) G[k][128 + i] = dyo[i];
             }
         }
@@ -263,7 +282,7 @@ MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, fl
             T dgate[3];
             for (int k = 0; k < 3; ++k) {
                 T dx[32];
-                t_lin_back(w, L[5 * se + 2 + k], &G[k][op.to], dx, rec);
+                t_lin_back<T, Rec, Q>(w, L[5 * se + 2 + k], &G[k][op.to], dx, rec);
                 const float* in = feat + k * kTaps + op.from;
                 T dg = (T)0;
                 for (int i = 0; i < op.size; ++i) {
@@ -275,8 +294,8 @@ MRP_HD T train_sample(const float* w, const TrainLayer* L, const float* feat, fl
                 dgate[k] = dg;
             }
             T dh[8];
-            t_lin_back(w, L[5 * se + 1], dgate, dh, rec);
-            t_lin_back(w, L[5 * se], dh, (T*)nullptr, rec);
+            t_lin_back<T, Rec, Q>(w, L[5 * se + 1], dgate, dh, rec);
+            t_lin_back<T, Rec, Q>(w, L[5 * se], dh, (T*)nullptr, rec);
             --se;
         }
     }

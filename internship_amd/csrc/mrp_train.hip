@@ -10,10 +10,14 @@
 //      structure-of-arrays record rec[slot][sample] (coalesced across lanes);
 //   2. wgrad_kernel - per layer, dW = dZ^T X and db = sum dZ over the batch:
 //      16x16 output tiles, batch chunks of 64 staged through LDS, FP64
-//      accumulation (deterministic: fixed order, no atomics);
+//      accumulation; the batch is split into slices of 1024 samples (one
+//      workgroup per tile and slice) whose partial sums wgrad_reduce_kernel
+//      adds in slice order (deterministic: fixed order, no atomics);
 //   3. loss_sum_kernel - FP64 sum of the per-sample losses in a fixed tree;
 //   4. adam_kernel - Adam (beta1 0.9, beta2 0.999, eps 1e-8) on FP32 master
 //      weights with FP64 gradients.
+// Precision 1 replaces step 1 by the BF16 matrix-core forward and backward of
+// mrp_train_mfma.hip (same record, same steps 2-4).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,6 +26,7 @@
 #include <vector>
 
 #include "mrp_internal.h"
+#include "mrp_mlp.h"
 #include "mrp_train.h"
 
 namespace mrp {
@@ -55,24 +60,30 @@ struct WTile {
 };
 
 constexpr int kWT = 16, kWK = 64;
+constexpr uint32_t kWSlice = 1024;  // samples per gradient slice
+constexpr int kWMaxSlices = 16;
 
+// Slice blockIdx.y covers samples [y * slice, min(B, (y + 1) * slice)); its
+// partial gradient goes to grad + y * MRP_W_TOTAL.
 __global__ __launch_bounds__(256) void wgrad_kernel(const WTile* __restrict__ tiles, const float* __restrict__ rec,
-                                                    uint32_t B, double* __restrict__ grad) {
+                                                    uint32_t B, uint32_t slice, double* __restrict__ grad) {
     __shared__ float sdz[kWT][kWK + 1];
     __shared__ float sx[kWT][kWK + 1];
     const WTile t = tiles[blockIdx.x];
     const TrainLayer l = c_train_layers[t.layer];
     const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // ty -> output row i, tx -> input column j
     const int i = t.i0 + ty, j = t.j0 + tx;
+    const uint32_t s_begin = blockIdx.y * slice, s_end = min(B, s_begin + slice);
+    grad += (size_t)blockIdx.y * MRP_W_TOTAL;
     double acc = 0.0;
-    for (uint32_t s0 = 0; s0 < B; s0 += kWK) {
+    for (uint32_t s0 = s_begin; s0 < s_end; s0 += kWK) {
         // stage dz rows i0..i0+15 and x rows j0..j0+15 for samples s0..s0+63
         for (int e = threadIdx.x; e < kWT * kWK; e += 256) {
             const int r = e / kWK, c = e - r * kWK;
             const uint32_t s = s0 + (uint32_t)c;
             const int ri = t.i0 + r, rj = t.j0 + r;
             float dz = 0.0f, xv = 0.0f;
-            if (s < B) {
+            if (s < s_end) {
                 if (ri < l.out) dz = rec[(size_t)(l.dz_off + ri) * B + s];
                 if (rj < l.in) xv = rec[(size_t)(l.x_off + rj) * B + s];
                 else if (rj == l.in) xv = 1.0f;  // bias column
@@ -88,6 +99,16 @@ __global__ __launch_bounds__(256) void wgrad_kernel(const WTile* __restrict__ ti
         if (j < l.in) grad[l.w_off + i * l.in + j] = acc;
         else if (j == l.in && l.b_off >= 0) grad[l.b_off + i] = acc;
     }
+}
+
+// grad[i] = sum of the slices' partial gradients, in slice order.
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(const double* __restrict__ part, int slices,
+                                                           double* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= MRP_W_TOTAL) return;
+    double a = 0.0;
+    for (int q = 0; q < slices; ++q) a += part[(size_t)q * MRP_W_TOTAL + i];
+    grad[i] = a;
 }
 
 // Sum of n values in FP64 in a fixed order: 256 strided partial sums, then a tree.
@@ -136,6 +157,7 @@ struct mrp_trainer {
     float* m = nullptr;
     float* v = nullptr;
     double* grad = nullptr;
+    double* grad_part = nullptr;  // kWMaxSlices partial gradients
     float* rec = nullptr;
     float* loss = nullptr;
     float* pred = nullptr;
@@ -143,6 +165,8 @@ struct mrp_trainer {
     WTile* tiles = nullptr;
     int n_tiles = 0;
     int steps = 0;
+    int precision = 0;          // 0: FP32 forward/backward, 1: BF16 matrix cores
+    TrainMfma* mfma = nullptr;  // created on first use of precision 1
 };
 
 static void trainer_free(mrp_trainer* t) {
@@ -151,11 +175,13 @@ static void trainer_free(mrp_trainer* t) {
     hipFree(t->m);
     hipFree(t->v);
     hipFree(t->grad);
+    hipFree(t->grad_part);
     hipFree(t->rec);
     hipFree(t->loss);
     hipFree(t->pred);
     hipFree(t->loss_sum);
     hipFree(t->tiles);
+    train_mfma_destroy(t->mfma);
     if (t->stream) hipStreamDestroy(t->stream);
     delete t;
 }
@@ -164,13 +190,26 @@ static int trainer_gradient(mrp_trainer* t, int batch, const float* feat, const 
     if (batch <= 0 || batch > t->max_batch) return fail(MRP_E_INVALID, "batch must be in [1, max_batch]");
     if (!feat || !aux) return fail(MRP_E_INVALID, "NULL argument");
     const uint32_t B = (uint32_t)batch;
-    train_fwd_bwd_kernel<<<(B + 63) / 64, 64, 0, t->stream>>>(t->w, feat, (const float4*)aux, B, 1.0f / (float)B,
-                                                              t->rec, t->loss, t->pred);
+    if (t->precision == 1) {
+        const int rc = train_mfma_fwd_bwd(t->mfma, t->w, feat, aux, B, 1.0f / (float)B, t->rec, t->loss, t->pred,
+                                          t->loss_sum, t->stream);
+        if (rc) return rc;
+    } else {
+        train_fwd_bwd_kernel<<<(B + 63) / 64, 64, 0, t->stream>>>(t->w, feat, (const float4*)aux, B,
+                                                                  1.0f / (float)B, t->rec, t->loss, t->pred);
+        MRP_HIP(hipGetLastError());
+        loss_sum_kernel<<<1, 256, 0, t->stream>>>(t->loss, B, t->loss_sum);
+        MRP_HIP(hipGetLastError());
+    }
+    const int slices = std::min<int>(kWMaxSlices, (int)((B + kWSlice - 1) / kWSlice));
+    const uint32_t slice = ((B + (uint32_t)slices - 1) / (uint32_t)slices + kWK - 1) / kWK * kWK;
+    wgrad_kernel<<<dim3((unsigned)t->n_tiles, (unsigned)slices), 256, 0, t->stream>>>(
+        t->tiles, t->rec, B, slice, slices == 1 ? t->grad : t->grad_part);
     MRP_HIP(hipGetLastError());
-    wgrad_kernel<<<t->n_tiles, 256, 0, t->stream>>>(t->tiles, t->rec, B, t->grad);
-    MRP_HIP(hipGetLastError());
-    loss_sum_kernel<<<1, 256, 0, t->stream>>>(t->loss, B, t->loss_sum);
-    MRP_HIP(hipGetLastError());
+    if (slices > 1) {
+        wgrad_reduce_kernel<<<(MRP_W_TOTAL + 255) / 256, 256, 0, t->stream>>>(t->grad_part, slices, t->grad);
+        MRP_HIP(hipGetLastError());
+    }
     return MRP_OK;
 }
 
@@ -198,6 +237,7 @@ int mrp_trainer_create(const float* weights, size_t count, int max_batch, mrp_tr
     ok = ok && hipMalloc(&t->m, n * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&t->v, n * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&t->grad, n * sizeof(double)) == hipSuccess;
+    ok = ok && hipMalloc(&t->grad_part, (size_t)kWMaxSlices * n * sizeof(double)) == hipSuccess;
     ok = ok && hipMalloc(&t->rec, (size_t)t->rec_floats * max_batch * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&t->loss, (size_t)max_batch * sizeof(float)) == hipSuccess;
     ok = ok && hipMalloc(&t->pred, (size_t)max_batch * sizeof(float)) == hipSuccess;
@@ -225,6 +265,18 @@ int mrp_trainer_create(const float* weights, size_t count, int max_batch, mrp_tr
 
 int mrp_trainer_destroy(mrp_trainer* t) {
     trainer_free(t);
+    return MRP_OK;
+}
+
+int mrp_trainer_set_precision(mrp_trainer* t, int precision) {
+    if (!t) return fail(MRP_E_INVALID, "NULL trainer");
+    if (precision != 0 && precision != 1) return fail(MRP_E_INVALID, "precision must be 0 (FP32) or 1 (BF16)");
+    MRP_HIP(hipSetDevice(t->device));
+    if (precision == 1 && !t->mfma) {
+        const int rc = train_mfma_create(t->max_batch, &t->mfma);
+        if (rc) return rc;
+    }
+    t->precision = precision;
     return MRP_OK;
 }
 

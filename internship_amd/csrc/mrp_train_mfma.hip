@@ -1,0 +1,728 @@
+// mrp_train_mfma.hip - training step of the MRPNN network on gfx950 matrix
+// cores: BF16 copies of the FP32 master weights, BF16 activations at every
+// product, FP32 accumulation, FP32 records, FP64 weight gradients.
+//
+// The step is the one of mrp_train.h (derived from `RadiancePredict`,
+// core/radiancePredict.cu:370-486, one colour channel per sample) with every
+// dense-layer product on v_mfma_f32_32x32x16_bf16, 32 samples per wave in the
+// D-order registers of the inference network (mrp_mfma.h):
+//   train_pack_*_kernel   - before each step: master weights -> BF16 forward
+//                           tiles (the inference layout) and BF16 transposed
+//                           tiles (rows = a layer's inputs, K = its outputs),
+//                           through element maps built once on the host;
+//   train_fwd_mfma_kernel - forward; writes each layer's input x and output y
+//                           (FP32, before BF16 rounding) into the trainer's
+//                           record rec[slot][sample], the loss and p;
+//   train_bwd_mfma_kernel - backward in reverse layer order: dz = dy * act'(y)
+//                           with y read back from the record, dz recorded,
+//                           dx = W^T dz on the matrix cores.
+// The weight gradients dZ^T X then come from the FP64 GEMM of the FP32 path.
+// LGGS (8x3) and LX12 (1x16) stay on the VALU in FP32, as in inference.
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <vector>
+
+#include "mrp_internal.h"
+#include "mrp_mfma.h"
+#include "mrp_mlp.h"
+#include "mrp_train.h"
+
+namespace mrp {
+
+constexpr int kTilesT = 132;          // transposed tiles (layers whose input gradient is needed)
+// One tile of 32 samples per wave: a grid-stride loop over tiles would let the
+// compiler hoist the record addresses of every layer out of the loop (loop
+// invariant), which spills.  Workgroups hold all weight tiles in LDS, so one
+// fits a CU; kTrainBlock threads per workgroup.
+constexpr int kTrainBlock = 128;
+constexpr int kFloatSection = kBiasTiles * 32 + kExtraFloats;
+
+__constant__ LayerDesc c_tf[LY_COUNT];        // forward tiles (the inference table)
+__constant__ LayerDesc c_tb[LY_COUNT];        // transposed tiles (M = inputs, K = outputs; tile < 0: none)
+__constant__ TrainLayer c_tl[kTrainLayers];   // record slots
+
+// record slots of inference layer `ly`: the training layer list orders each SE
+// block as {SE first, SE second, stream A, B, C}
+MRP_HD int tl_of(int ly) {
+    if (ly < LY_SE1) return 5 * (ly - LY_SE0);
+    if (ly < LY_STREAM) return 5 * (ly - LY_SE1) + 1;
+    if (ly < LY_L81) return 5 * ((ly - LY_STREAM) / 3) + 2 + (ly - LY_STREAM) % 3;
+    if (ly < LY_LD41) return TL_L81 + (ly - LY_L81);
+    if (ly < LY_FIN1) return TL_LD41 + (ly - LY_LD41);
+    if (ly == LY_FIN1) return TL_FIN1;
+    if (ly == LY_FIN2) return TL_FIN2;
+    if (ly == LY_LC0) return TL_LC0;
+    if (ly == LY_LC1) return TL_LC0 + 1;
+    if (ly == LY_LC2) return TL_LC0 + 2;
+    if (ly == LY_LX) return TL_LX;
+    return TL_LXR + (ly - LY_LXR);
+}
+
+// ---------------------------------------------------------------- weight copies
+__global__ void train_pack_bf16_kernel(const float* __restrict__ w, const int* __restrict__ map, int n,
+                                       __bf16* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int src = map[i];
+    out[i] = (__bf16)(src >= 0 ? w[src] : 0.0f);
+}
+
+__global__ void train_pack_f32_kernel(const float* __restrict__ w, const int* __restrict__ map, int n,
+                                      float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int src = map[i];
+    out[i] = src >= 0 ? w[src] : 0.0f;
+}
+
+// ---------------------------------------------------------------- LDS staging
+// n 16-byte words from global memory to LDS by one workgroup of kTrainBlock
+// threads, kStageBatch independent loads in flight per thread.
+constexpr int kStageBatch = 16;
+__device__ __forceinline__ void stage16(const uint4* __restrict__ src, uint4* dst, int n) {
+    const int full = n / (kTrainBlock * kStageBatch) * (kTrainBlock * kStageBatch);
+    for (int i0 = (int)threadIdx.x; i0 < full; i0 += kTrainBlock * kStageBatch) {
+        uint4 t[kStageBatch];
+#pragma unroll
+        for (int j = 0; j < kStageBatch; ++j) t[j] = src[i0 + j * kTrainBlock];
+#pragma unroll
+        for (int j = 0; j < kStageBatch; ++j) dst[i0 + j * kTrainBlock] = t[j];
+    }
+    for (int i = full + (int)threadIdx.x; i < n; i += kTrainBlock) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------- record access
+// A lane's view of the record: sample s (clamped for tail lanes, which write nothing).
+struct RecLane {
+    float* rec;
+    uint32_t B, s;
+    bool valid;
+    // rows 32j + vrow(v, kg) < n of tile x -> slots off + row
+    __device__ __forceinline__ void put(int off, const f32x16& x, int j, int n, int kg) const {
+        if (!valid) return;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int r = 32 * j + vrow(v, kg);
+            if (r < n) rec[(size_t)(off + r) * B + s] = x[v];
+        }
+    }
+    __device__ __forceinline__ void put1(int slot, float x) const {
+        if (valid) rec[(size_t)slot * B + s] = x;
+    }
+    __device__ __forceinline__ float get1(int slot) const { return rec[(size_t)slot * B + s]; }
+    __device__ __forceinline__ void get(int off, f32x16& y, int j, int n, int kg) const {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int r = 32 * j + vrow(v, kg);
+            y[v] = r < n ? rec[(size_t)(off + r) * B + s] : 0.0f;
+        }
+    }
+};
+
+struct TrainMfmaArgs {
+    const uint4* packed;    // forward: BF16 tiles, bias tiles, the VALU layers (mrp_mfma.h layout)
+    const uint4* packed_t;  // transposed BF16 tiles
+    const float* feat;      // B x kFeatures
+    const float4* aux;      // gamma, g, srp, target
+    uint32_t B;
+    float inv_batch;
+    float* rec;
+    float* loss;
+    float* pred;
+    double* loss_tiles;     // per 32-sample tile
+};
+
+__constant__ int c_seg_from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
+
+// the block's input features of stream k, in D-order (rows >= S are zero)
+__device__ __forceinline__ void load_seg(const float* frow, int k, int from, int S, int kg, f32x16& xv) {
+    xv = zero16();
+    const float* srow = frow + k * kTaps + from;
+    if (S == 8) {
+        const float4 q = *(const float4*)(srow + 4 * kg);
+        xv[0] = q.x;
+        xv[1] = q.y;
+        xv[2] = q.z;
+        xv[3] = q.w;
+    } else {
+        load16(srow, 0, kg, xv, 0);
+        if (S == 32) load16(srow, 16, kg, xv, 1);
+    }
+}
+
+// dense layer `ly` forward with its x and y recorded
+template <int MT, int KB>
+__device__ __forceinline__ void fwd_layer(const Packed<Fmt::BF16>& pk, const RecLane& R, int ly, const f32x16* X,
+                                          f32x16 (&Y)[MT], bool relu, int lane) {
+    const int kg = lane >> 5;
+    const TrainLayer& l = c_tl[tl_of(ly)];
+#pragma unroll
+    for (int j = 0; j < (KB + 1) / 2; ++j) R.put(l.x_off, X[j], j, l.in, kg);
+    dense_d<Fmt::BF16, MT, KB>(pk, c_tf[ly], ly, X, Y, relu, lane);
+    if (relu) {
+#pragma unroll
+        for (int j = 0; j < MT; ++j) R.put(l.y_off, Y[j], j, l.out, kg);
+    }
+}
+
+// ---------------------------------------------------------------- forward
+__global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaArgs a) {
+    __shared__ bf16x8 s_a[kTiles * 64];
+    __shared__ __align__(16) float s_bias[kBiasTiles * 32];
+    __shared__ __align__(16) float s_extra[kExtraFloats];
+    {
+        constexpr int kA16 = (int)(tile_bytes<Fmt::BF16>() / 16);
+        stage16(a.packed, (uint4*)s_a, kA16);
+        stage16(a.packed + kA16, (uint4*)s_bias, kBiasTiles * 8);
+        stage16(a.packed + kA16 + kBiasTiles * 8, (uint4*)s_extra, kExtraFloats / 4);
+    }
+    __syncthreads();
+    const int lane = (int)__lane_id();
+    const int kg = lane >> 5;
+    const Packed<Fmt::BF16> pk{s_a, s_bias, nullptr};
+    const int waves = (int)(blockDim.x >> 6);
+    const uint32_t n_tiles = (a.B + 31u) / 32u;
+    {
+        const uint32_t tile = blockIdx.x * waves + (threadIdx.x >> 6);
+        if (tile >= n_tiles) return;  // after the staging barrier
+        const uint32_t item0 = tile * 32u;
+        const uint32_t item = min(item0 + (uint32_t)(lane & 31), a.B - 1u);
+        const RecLane R{a.rec, a.B, item, item0 + (uint32_t)(lane & 31) < a.B};
+        const float4 ax = a.aux[item];
+        const float gamma = ax.x, g = ax.y, srp = ax.z, target = ax.w;
+        const float* frow = a.feat + (size_t)item * kFeatures;
+
+        f32x16 prev[3] = {zero16(), zero16(), zero16()};
+        f32x16 comb[3], lout[3];
+        // The head's squeeze-excitation input is pool[3 blk + k] = mean and
+        // pool[36 + 3 blk + k] = max of segment blk of stream k, then g, gamma,
+        // srp at 72..74.  The pool goes to its record slots (that layer's x) as
+        // it is computed, each value by the lane that reads it back at the head.
+        const int fin1_x = c_tl[TL_FIN1].x_off;
+#pragma unroll
+        for (int blk = 0; blk < 12; ++blk) {
+            const int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
+            const bool residual = (blk != 0 && blk != 8);
+            const bool rep = (blk == 1 || blk == 4);
+            // the block's features, their means and maxima (rows < S: S/2 registers per lane group)
+            float mean[3], mx[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                f32x16 xs;
+                load_seg(frow, k, c_seg_from[blk], S, kg, xs);
+                float sum = 0.0f, m = -10000.0f;
+#pragma unroll
+                for (int v = 0; v < S / 2; ++v) {
+                    sum += xs[v];
+                    m = fmax_(m, xs[v]);
+                }
+                sum += __shfl_xor(sum, 32);
+                mean[k] = sum / (float)S;
+                mx[k] = fmax_(m, __shfl_xor(m, 32));
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    const int r = q == 0 ? 3 * blk + k : 36 + 3 * blk + k, rr = r % 32;
+                    if (kg == (rr % 8) / 4) R.put1(fin1_x + r, q == 0 ? mean[k] : mx[k]);
+                }
+            }
+            // squeeze-excitation gates on [means, maxima, g, gamma]
+            f32x16 t = zero16();
+            if (kg == 0) {
+                t[0] = mean[0];
+                t[1] = mean[1];
+                t[2] = mean[2];
+                t[3] = mx[0];
+            } else {
+                t[0] = mx[1];
+                t[1] = mx[2];
+                t[2] = g;
+                t[3] = gamma;
+            }
+            f32x16 h[1], o[1];
+            fwd_layer<1, 1>(pk, R, LY_SE0 + blk, &t, h, true, lane);
+            fwd_layer<1, 1>(pk, R, LY_SE1 + blk, h, o, false, lane);
+            f32x16 gs = zero16();
+            gs[0] = sigmoid_(o[0][0]);
+            gs[1] = sigmoid_(o[0][1]);
+            gs[2] = sigmoid_(o[0][2]);
+            R.put(c_tl[tl_of(LY_SE1 + blk)].y_off, gs, 0, 3, kg);  // rows 0..2: lane group 0
+            const int src = lane & 31;
+            const float gate[3] = {__shfl(gs[0], src), __shfl(gs[1], src), __shfl(gs[2], src)};
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                f32x16 xv;
+                load_seg(frow, k, c_seg_from[blk], S, kg, xv);  // again: keeps 48 registers free
+                f32x16 x;
+#pragma unroll
+                for (int v = 0; v < 16; ++v) x[v] = xv[v] * gate[k] + (residual ? prev[k][v] : 0.0f);
+                f32x16 y[1];
+                if (S == 32) {
+                    fwd_layer<1, 2>(pk, R, LY_STREAM + 3 * blk + k, &x, y, true, lane);
+                } else {
+                    fwd_layer<1, 1>(pk, R, LY_STREAM + 3 * blk + k, &x, y, true, lane);
+                }
+                if (rep) {
+                    f32x16 w = y[0];
+                    if (S == 8) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) w[4 + v] = xv[v];
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 8; ++v) w[8 + v] = xv[v];
+                    }
+                    prev[k] = w;
+                } else {
+#pragma unroll
+                    for (int v = 0; v < 16; ++v) prev[k][v] = y[0][v] + xv[v];
+                }
+            }
+            if (blk == 7) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    f32x16 oc[1];
+                    fwd_layer<1, 2>(pk, R, LY_L81 + k, &prev[k], oc, true, lane);
+                    comb[k] = oc[0];
+                    prev[k] = zero16();
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            f32x16 ol[1];
+            fwd_layer<1, 1>(pk, R, LY_LD41 + k, &prev[k], ol, true, lane);
+            lout[k] = ol[0];
+        }
+        // ------------------------------------------------ head (the sample's channel)
+        f32x16 tp[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) R.get(fin1_x, tp[j], j, 72, kg);
+        if (kg == 0) {  // rows 72, 73, 74
+            tp[2][4] = g;
+            tp[2][5] = gamma;
+            tp[2][6] = srp;
+        }
+        f32x16 s1[1], s2[1];
+        fwd_layer<1, 5>(pk, R, LY_FIN1, tp, s1, true, lane);
+        fwd_layer<1, 1>(pk, R, LY_FIN2, s1, s2, false, lane);
+        f32x16 sg = zero16();
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sg[v] = sigmoid_(s2[0][v]);
+        R.put(c_tl[TL_FIN2].y_off, sg, 0, 6, kg);
+        float s[6];
+        const int src0 = lane & 31, src1 = (lane & 31) + 32;
+        s[0] = __shfl(sg[0], src0);
+        s[1] = __shfl(sg[1], src0);
+        s[2] = __shfl(sg[2], src0);
+        s[3] = __shfl(sg[3], src0);
+        s[4] = __shfl(sg[0], src1);
+        s[5] = __shfl(sg[1], src1);
+        f32x16 x0[4];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) x0[j][v] = comb[j][v] * s[j];
+        }
+        // LGGS (8x3, bias after the sum) on [g, gamma, srp]: rows 4kg..4kg+3
+        {
+            const TrainLayer& l = c_tl[TL_LGGS];
+            if (kg == 0) {
+                R.put1(l.x_off, g);
+                R.put1(l.x_off + 1, gamma);
+                R.put1(l.x_off + 2, srp);
+            }
+            f32x16 lg = zero16();
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = 4 * kg + v;
+                float acc = 0.0f;
+                acc = fmaf(g, s_extra[r * 3 + 0], acc);
+                acc = fmaf(gamma, s_extra[r * 3 + 1], acc);
+                acc = fmaf(srp, s_extra[r * 3 + 2], acc);
+                acc += s_extra[24 + r];
+                lg[v] = acc >= 0.0f ? acc : 0.0f;
+                x0[3][12 + v] = lg[v];
+                x0[3][v] = lout[0][v] * s[3];
+                x0[3][4 + v] = lout[1][v] * s[4];
+                x0[3][8 + v] = lout[2][v] * s[5];
+            }
+            R.put(l.y_off, lg, 0, 8, kg);
+        }
+        f32x16 h0[4], h1[2], h2[1], ha[1];
+        fwd_layer<4, 8>(pk, R, LY_LC0, x0, h0, true, lane);
+        fwd_layer<2, 8>(pk, R, LY_LC1, h0, h1, true, lane);
+        fwd_layer<1, 4>(pk, R, LY_LC2, h1, h2, true, lane);
+        fwd_layer<1, 2>(pk, R, LY_LX, h2, ha, true, lane);
+#pragma unroll
+        for (int r = 0; r < 6; ++r) {
+            f32x16 hb[1], hc[1];
+            fwd_layer<1, 1>(pk, R, LY_LXR + 2 * r, ha, hb, true, lane);
+            fwd_layer<1, 1>(pk, R, LY_LXR + 2 * r + 1, hb, hc, true, lane);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) ha[0][v] = ha[0][v] + hc[0][v];
+        }
+        // LX12 (1x16, bias after the sum)
+        const TrainLayer& l12 = c_tl[TL_LX12];
+        R.put(l12.x_off, ha[0], 0, 16, kg);
+        float part = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) part = fmaf(ha[0][v], s_extra[32 + vrow(v, kg)], part);
+        part += __shfl_xor(part, 32);
+        float P = part + s_extra[48];
+        P = P >= 0.0f ? P : 0.0f;
+        const float eP = expf_(P);
+        const float p = (eP - 1.0f > 0.0f ? eP - 1.0f : 0.0f) * srp;
+        const float den = target * target + 0.01f;
+        const float diff = p - target;
+        const float loss = diff * diff / den;
+        if (kg == 0 && R.valid) {
+            R.put1(l12.y_off, P);
+            a.loss[item] = loss;
+            a.pred[item] = p;
+        }
+        // the tile's loss in FP64, in a fixed order (butterfly over the 32 samples)
+        double ls = (kg == 0 && R.valid) ? (double)loss : 0.0;
+#pragma unroll
+        for (int o = 16; o >= 1; o >>= 1) ls += __shfl_xor(ls, o);
+        if (lane == 0) a.loss_tiles[tile] = ls;
+    }
+}
+
+// Sum of the tiles' FP64 losses in a fixed order: 256 strided partial sums, then a tree.
+__global__ __launch_bounds__(256) void train_loss_tiles_kernel(const double* __restrict__ v, uint32_t n,
+                                                               double* __restrict__ out) {
+    __shared__ double part[256];
+    double s = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 256) s += v[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) part[threadIdx.x] += part[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// ---------------------------------------------------------------- backward
+// dz = dy * act'(y) over rows < out (y read back), recorded.
+template <int MT>
+__device__ __forceinline__ void act_back(const RecLane& R, const TrainLayer& l, const f32x16* dy, f32x16 (&dz)[MT],
+                                         int kg) {
+#pragma unroll
+    for (int j = 0; j < MT; ++j) {
+        f32x16 y;
+        R.get(l.y_off, y, j, l.out, kg);
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int r = 32 * j + vrow(v, kg);
+            float d = r < l.out ? dy[j][v] : 0.0f;
+            if (l.act == ACT_RELU) d = y[v] > 0.0f ? d : 0.0f;
+            else if (l.act == ACT_SIGMOID) d = d * (y[v] * (1.0f - y[v]));
+            dz[j][v] = d;
+        }
+        R.put(l.dz_off, dz[j], j, l.out, kg);
+    }
+}
+
+// backward of inference layer `ly`: records dz; dX (MI tiles over the inputs) = W^T dz
+template <int MO, int MI, int KO>
+__device__ __forceinline__ void back_layer(const Packed<Fmt::BF16>& pk, const RecLane& R, int ly, const f32x16* dy,
+                                           f32x16 (&dx)[MI], int lane) {
+    f32x16 dz[MO];
+    act_back<MO>(R, c_tl[tl_of(ly)], dy, dz, lane >> 5);
+    dense_d<Fmt::BF16, MI, KO>(pk, c_tb[ly], ly, dz, dx, false, lane);
+}
+
+// one SE block backward: Gp = d loss / d (the block's output window) in, the
+// gradient for the previous block's window out (blocks 0 and 8 start a chain)
+__device__ __forceinline__ void block_back(const Packed<Fmt::BF16>& pk, const RecLane& R, const float* frow, int blk,
+                                           f32x16 (&Gp)[3], int lane) {
+    const int kg = lane >> 5;
+    const int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
+    float dgate[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        f32x16 dx[1];
+        if (S == 32) {
+            back_layer<1, 1, 2>(pk, R, LY_STREAM + 3 * blk + k, &Gp[k], dx, lane);
+        } else {
+            back_layer<1, 1, 1>(pk, R, LY_STREAM + 3 * blk + k, &Gp[k], dx, lane);
+        }
+        f32x16 xv;
+        load_seg(frow, k, c_seg_from[blk], S, kg, xv);
+        float part = 0.0f;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) part = fmaf(dx[0][v], xv[v], part);
+        dgate[k] = part + __shfl_xor(part, 32);
+        Gp[k] = dx[0];
+    }
+    // gates: rows 0..2 of the second SE layer (lane group 0)
+    f32x16 dg = zero16();
+    dg[0] = dgate[0];
+    dg[1] = dgate[1];
+    dg[2] = dgate[2];
+    f32x16 dh[1];
+    back_layer<1, 1, 1>(pk, R, LY_SE1 + blk, &dg, dh, lane);
+    f32x16 dz0[1];
+    act_back<1>(R, c_tl[tl_of(LY_SE0 + blk)], dh, dz0, kg);
+}
+
+__global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaArgs a) {
+    __shared__ bf16x8 s_t[kTilesT * 64];
+    __shared__ __align__(16) float s_extra[kExtraFloats];
+    {
+        stage16(a.packed_t, (uint4*)s_t, kTilesT * 64);
+        constexpr int kA16 = (int)(tile_bytes<Fmt::BF16>() / 16);
+        stage16(a.packed + kA16 + kBiasTiles * 8, (uint4*)s_extra, kExtraFloats / 4);
+    }
+    __syncthreads();
+    const int lane = (int)__lane_id();
+    const int kg = lane >> 5;
+    const Packed<Fmt::BF16> pk{s_t, nullptr, nullptr};
+    const int waves = (int)(blockDim.x >> 6);
+    const uint32_t n_tiles = (a.B + 31u) / 32u;
+    {
+        const uint32_t tile = blockIdx.x * waves + (threadIdx.x >> 6);
+        if (tile >= n_tiles) return;  // after the staging barrier
+        const uint32_t item0 = tile * 32u;
+        const uint32_t item = min(item0 + (uint32_t)(lane & 31), a.B - 1u);
+        const RecLane R{a.rec, a.B, item, item0 + (uint32_t)(lane & 31) < a.B};
+        const float4 ax = a.aux[item];
+        const float srp = ax.z, target = ax.w;
+        const float* frow = a.feat + (size_t)item * kFeatures;
+
+        // ------------------------------------------------ loss -> LX12
+        const TrainLayer& l12 = c_tl[TL_LX12];
+        const float P = R.get1(l12.y_off);
+        const float eP = expf_(P);
+        const float p = (eP - 1.0f > 0.0f ? eP - 1.0f : 0.0f) * srp;
+        const float den = target * target + 0.01f;
+        const float diff = p - target;
+        float dP = 2.0f * diff / den * a.inv_batch;
+        dP = eP - 1.0f > 0.0f ? dP * srp * eP : 0.0f;
+        const float dz12 = P > 0.0f ? dP : 0.0f;
+        if (kg == 0) R.put1(l12.dz_off, dz12);
+        f32x16 da[1];
+        da[0] = zero16();
+#pragma unroll
+        for (int v = 0; v < 8; ++v) da[0][v] = s_extra[32 + vrow(v, kg)] * dz12;
+        // ------------------------------------------------ residual tower, head layers
+#pragma unroll
+        for (int r = 5; r >= 0; --r) {
+            f32x16 dt[1], dc[1];
+            back_layer<1, 1, 1>(pk, R, LY_LXR + 2 * r + 1, da, dt, lane);
+            back_layer<1, 1, 1>(pk, R, LY_LXR + 2 * r, dt, dc, lane);
+#pragma unroll
+            for (int v = 0; v < 8; ++v) da[0][v] = da[0][v] + dc[0][v];
+        }
+        f32x16 dh2[1], dh1[2], dh0[4], ds[4];
+        back_layer<1, 1, 1>(pk, R, LY_LX, da, dh2, lane);
+        back_layer<1, 2, 2>(pk, R, LY_LC2, dh2, dh1, lane);
+        back_layer<2, 4, 4>(pk, R, LY_LC1, dh1, dh0, lane);
+        back_layer<4, 4, 8>(pk, R, LY_LC0, dh0, ds, lane);
+        // ------------------------------------------------ head input = [comb * s | LD41 * s | LGGS]
+        float se2[6];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) se2[i] = R.get1(c_tl[TL_FIN2].y_off + i);
+        f32x16 dcomb[3];
+        float dse2[6];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            f32x16 cy;
+            R.get(c_tl[TL_L81 + j].y_off, cy, 0, 32, kg);
+            float part = 0.0f;
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                dcomb[j][v] = ds[j][v] * se2[j];
+                part = fmaf(ds[j][v], cy[v], part);
+            }
+            dse2[j] = part + __shfl_xor(part, 32);
+        }
+        f32x16 dld[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            dld[k] = zero16();
+            float part = 0.0f;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const float yv = R.get1(c_tl[TL_LD41 + k].y_off + 4 * kg + v);
+                dld[k][v] = ds[3][4 * k + v] * se2[3 + k];
+                part = fmaf(ds[3][4 * k + v], yv, part);
+            }
+            dse2[3 + k] = part + __shfl_xor(part, 32);
+        }
+        {
+            f32x16 dyl = zero16(), dzl[1];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dyl[v] = ds[3][12 + v];
+            act_back<1>(R, c_tl[TL_LGGS], &dyl, dzl, kg);
+        }
+        {
+            // FIN2 rows 0..5: lane group 0 registers 0..3, lane group 1 registers 0..1
+            f32x16 dy2 = zero16();
+            dy2[0] = kg == 0 ? dse2[0] : dse2[4];
+            dy2[1] = kg == 0 ? dse2[1] : dse2[5];
+            dy2[2] = kg == 0 ? dse2[2] : 0.0f;
+            dy2[3] = kg == 0 ? dse2[3] : 0.0f;
+            f32x16 dse1[1], dz1[1];
+            back_layer<1, 1, 1>(pk, R, LY_FIN2, &dy2, dse1, lane);
+            act_back<1>(R, c_tl[TL_FIN1], dse1, dz1, kg);
+        }
+        // ------------------------------------------------ light stream (blocks 11..8)
+        f32x16 Gp[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            f32x16 o[1];
+            back_layer<1, 1, 1>(pk, R, LY_LD41 + k, &dld[k], o, lane);
+            Gp[k] = o[0];
+        }
+#pragma unroll
+        for (int blk = 11; blk >= 8; --blk) block_back(pk, R, frow, blk, Gp, lane);
+        // ------------------------------------------------ main stream (blocks 7..0)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            f32x16 o[1];
+            back_layer<1, 1, 2>(pk, R, LY_L81 + k, &dcomb[k], o, lane);
+            Gp[k] = o[0];
+        }
+#pragma unroll
+        for (int blk = 7; blk >= 0; --blk) block_back(pk, R, frow, blk, Gp, lane);
+    }
+}
+
+// ---------------------------------------------------------------- host side
+struct TrainMfma {
+    int* map_tiles = nullptr;   // forward tiles then transposed tiles: weight index or -1
+    int* map_floats = nullptr;  // bias tiles and the VALU layers
+    void* packed = nullptr;     // forward tiles | bias tiles | VALU layers | unscales (mrp_mfma.h layout)
+    void* packed_t = nullptr;   // transposed tiles
+    double* loss_tiles = nullptr;  // max_batch / 32 (rounded up)
+    int max_batch = 0;
+};
+
+static const size_t kPackedBytes = tile_bytes<Fmt::BF16>() + (size_t)kFloatSection * 4 + kScaleFloats * 4;
+
+void train_mfma_destroy(TrainMfma* m) {
+    if (!m) return;
+    (void)hipFree(m->map_tiles);
+    (void)hipFree(m->map_floats);
+    (void)hipFree(m->packed);
+    (void)hipFree(m->packed_t);
+    (void)hipFree(m->loss_tiles);
+    delete m;
+}
+
+int train_mfma_create(int max_batch, TrainMfma** out) {
+    *out = nullptr;
+    LayerDesc F[LY_COUNT], T[LY_COUNT];
+    layer_table(F);
+    int nb = 0;
+    if (total_tiles(F, &nb) != kTiles || nb != kBiasTiles) return fail(MRP_E_STATE, "packed weight tiles do not match the kernel's LDS layout");
+    TrainLayer TL[kTrainLayers];
+    build_train_layers(TL);
+    // the two layer lists must describe the same products
+    for (int ly = 0; ly < LY_COUNT; ++ly) {
+        const TrainLayer& l = TL[tl_of(ly)];
+        if (l.w_off != F[ly].w || l.b_off != F[ly].b || l.out != F[ly].M || l.in != F[ly].K)
+            return fail(MRP_E_STATE, "training layer list does not match the inference layer table");
+    }
+    // transposed tiles for every layer whose input gradient is needed
+    int tt = 0;
+    for (int ly = 0; ly < LY_COUNT; ++ly) {
+        const bool need_dx = !(ly >= LY_SE0 && ly < LY_SE1) && ly != LY_FIN1;
+        T[ly] = {F[ly].w, -1, F[ly].K, F[ly].M, need_dx ? tt : -1, -1};
+        if (need_dx) tt += ((F[ly].K + 31) / 32) * ((F[ly].M + 15) / 16);
+    }
+    if (tt != kTilesT) return fail(MRP_E_STATE, "transposed tile count does not match the kernel's LDS layout");
+    std::vector<int> mt((size_t)(kTiles + kTilesT) * 512, -1), mf(kFloatSection, -1);
+    for (int ly = 0; ly < LY_COUNT; ++ly) {
+        const LayerDesc& d = F[ly];
+        const int mtn = (d.M + 31) / 32, kbn = (d.K + 15) / 16;
+        for (int m0 = 0; m0 < mtn; ++m0)
+            for (int kb = 0; kb < kbn; ++kb)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int m = m0 * 32 + l % 32, f = korder(kb, l / 32, e);
+                        if (m < d.M && f < d.K) mt[((size_t)(d.tile + m0 * kbn + kb)) * 512 + l * 8 + e] = d.w + m * d.K + f;
+                    }
+        if (d.b >= 0)
+            for (int m0 = 0; m0 < mtn; ++m0)
+                for (int kg = 0; kg < 2; ++kg)
+                    for (int v = 0; v < 16; ++v) {
+                        const int r = m0 * 32 + vrow(v, kg);
+                        if (r < d.M) mf[((size_t)(d.btile + m0) * 2 + kg) * 16 + v] = d.b + r;
+                    }
+        const LayerDesc& t = T[ly];
+        if (t.tile < 0) continue;
+        // A^T element: row m = an input of the layer, K index k = an output: W[k][m]
+        const int tmn = (t.M + 31) / 32, tkn = (t.K + 15) / 16;
+        for (int m0 = 0; m0 < tmn; ++m0)
+            for (int kb = 0; kb < tkn; ++kb)
+                for (int l = 0; l < 64; ++l)
+                    for (int e = 0; e < 8; ++e) {
+                        const int m = m0 * 32 + l % 32, k = korder(kb, l / 32, e);
+                        if (m < t.M && k < t.K)
+                            mt[((size_t)(kTiles + t.tile + m0 * tkn + kb)) * 512 + l * 8 + e] = d.w + k * d.K + m;
+                    }
+    }
+    int* extra = mf.data() + kBiasTiles * 32;
+    for (int i = 0; i < 24; ++i) extra[i] = MRP_W_LGGSW + i;
+    for (int i = 0; i < 8; ++i) extra[24 + i] = MRP_W_LGGSB + i;
+    for (int i = 0; i < 16; ++i) extra[32 + i] = MRP_W_LX12W + i;
+    extra[48] = MRP_W_LX12B;
+
+    TrainMfma* m = new (std::nothrow) TrainMfma();
+    if (!m) return fail(MRP_E_NOMEM, "host allocation failed");
+    m->max_batch = max_batch;
+    bool ok = hipMalloc(&m->map_tiles, mt.size() * sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&m->map_floats, mf.size() * sizeof(int)) == hipSuccess;
+    ok = ok && hipMalloc(&m->packed, kPackedBytes) == hipSuccess;
+    ok = ok && hipMalloc(&m->packed_t, (size_t)kTilesT * 64 * 16) == hipSuccess;
+    ok = ok && hipMalloc(&m->loss_tiles, (size_t)(max_batch + 31) / 32 * sizeof(double)) == hipSuccess;
+    if (!ok) {
+        train_mfma_destroy(m);
+        return fail(MRP_E_NOMEM, "trainer allocation failed");
+    }
+    std::vector<float> ones(kScaleFloats, 1.0f);  // per-layer unscales (used by FP8 only)
+    ok = hipMemcpy(m->map_tiles, mt.data(), mt.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy(m->map_floats, mf.data(), mf.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpy((char*)m->packed + tile_bytes<Fmt::BF16>() + (size_t)kFloatSection * 4, ones.data(),
+                         kScaleFloats * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tf), F, sizeof(F)) == hipSuccess;
+    ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tb), T, sizeof(T)) == hipSuccess;
+    ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tl), TL, sizeof(TL)) == hipSuccess;
+    if (!ok) {
+        train_mfma_destroy(m);
+        return fail(MRP_E_HIP, "trainer initialisation failed");
+    }
+    *out = m;
+    return MRP_OK;
+}
+
+int train_mfma_fwd_bwd(TrainMfma* m, const float* w, const float* feat, const float* aux, uint32_t B,
+                       float inv_batch, float* rec, float* loss, float* pred, double* loss_sum,
+                       hipStream_t stream) {
+    if (B == 0 || (int)B > m->max_batch) return fail(MRP_E_INVALID, "batch must be in [1, max_batch]");
+    // BF16 copies of the master weights
+    __bf16* tiles_f = (__bf16*)m->packed;
+    __bf16* tiles_t = (__bf16*)m->packed_t;
+    train_pack_bf16_kernel<<<(kTiles * 512 + 255) / 256, 256, 0, stream>>>(w, m->map_tiles, kTiles * 512, tiles_f);
+    train_pack_bf16_kernel<<<(kTilesT * 512 + 255) / 256, 256, 0, stream>>>(w, m->map_tiles + kTiles * 512,
+                                                                            kTilesT * 512, tiles_t);
+    train_pack_f32_kernel<<<(kFloatSection + 255) / 256, 256, 0, stream>>>(
+        w, m->map_floats, kFloatSection, (float*)((char*)m->packed + tile_bytes<Fmt::BF16>()));
+    MRP_HIP(hipGetLastError());
+    const TrainMfmaArgs args{(const uint4*)m->packed, (const uint4*)m->packed_t, feat, (const float4*)aux,
+                             B, inv_batch, rec, loss, pred, m->loss_tiles};
+    const uint32_t tiles = (B + 31u) / 32u, waves = kTrainBlock / 64;
+    const uint32_t blocks = (tiles + waves - 1) / waves;
+    train_fwd_mfma_kernel<<<blocks, kTrainBlock, 0, stream>>>(args);
+    MRP_HIP(hipGetLastError());
+    train_bwd_mfma_kernel<<<blocks, kTrainBlock, 0, stream>>>(args);
+    MRP_HIP(hipGetLastError());
+    train_loss_tiles_kernel<<<1, 256, 0, stream>>>(m->loss_tiles, tiles, loss_sum);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
+}
+
+}  // namespace mrp

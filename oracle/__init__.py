@@ -276,9 +276,10 @@ def get_samples(grid, max_dens, queries, light_color, sky, multi, spp, seed=0, t
     return out, lk.value
 
 
-def mrpnn_grad(weights, feat, aux):
+def mrpnn_grad(weights, feat, aux, bf16=False):
     """FP64 training loss and gradient: feat (n,576), aux (n,4) = (gamma, g, srp, target).
-    Returns (grad (49673,) float64, mean loss, predictions (n,))."""
+    bf16: the operands of every matrix-core product rounded to BF16 (the BF16
+    trainer's arithmetic).  Returns (grad (49673,) float64, mean loss, predictions (n,))."""
     w = np.ascontiguousarray(weights, dtype=np.float32)
     feat = np.ascontiguousarray(feat, dtype=np.float32)
     aux = np.ascontiguousarray(aux, dtype=np.float32)
@@ -286,7 +287,15 @@ def mrpnn_grad(weights, feat, aux):
     grad = np.zeros(w.size, dtype=np.float64)
     pred = np.zeros(n, dtype=np.float64)
     loss = ctypes.c_double()
-    rc = load().oracle_mrpnn_grad(_ptr(w), ctypes.c_int(n), _ptr(feat), _ptr(aux), _ptr(grad), ctypes.byref(loss),
-                                  _ptr(pred))
+    fn = load().oracle_mrpnn_grad_bf16 if bf16 else load().oracle_mrpnn_grad
+    rc = fn(_ptr(w), ctypes.c_int(n), _ptr(feat), _ptr(aux), _ptr(grad), ctypes.byref(loss), _ptr(pred))
     assert rc == 0
     return grad, loss.value, pred
+
+
+def bf16_round(x):
+    """The oracle's BF16 rounding (via FP32, round to nearest even)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros_like(x)
+    assert load().oracle_bf16_round(_ptr(x), ctypes.c_int(x.size), _ptr(out)) == 0
+    return out

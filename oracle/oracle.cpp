@@ -1166,11 +1166,24 @@ struct HostRec {
 };
 }  // namespace
 
-// Loss and gradient of the training objective in FP64 (mrp_train.h with
-// T = double): n samples, feat n x 576, aux n x 4 (gamma, g, srp, target).
-// grad (49,673) = d mean-loss / d w, accumulated sample by sample in order.
-int oracle_mrpnn_grad(const float* w, int n, const float* feat, const float* aux, double* grad, double* loss_mean,
-                      double* pred) {
+}  // extern "C"
+
+// BF16 rounding (round to nearest even) of a value first rounded to FP32: the
+// conversion the matrix-core trainer applies to its FP32 operands.
+struct Bf16Round {
+    static double r(double x) {
+        float f = (float)x;
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        if ((u & 0x7f800000u) != 0x7f800000u) u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+        memcpy(&f, &u, 4);
+        return (double)f;
+    }
+};
+
+template <class Q>
+static int mrpnn_grad_impl(const float* w, int n, const float* feat, const float* aux, double* grad, double* loss_mean,
+                           double* pred) {
     if (!w || !feat || !aux || !grad || n <= 0) return -1;
     TrainLayer L[kTrainLayers];
     const int R = build_train_layers(L);
@@ -1181,7 +1194,8 @@ int oracle_mrpnn_grad(const float* w, int n, const float* feat, const float* aux
     for (int s = 0; s < n; ++s) {
         double p = 0.0;
         const float* a = aux + 4 * (size_t)s;
-        loss += train_sample<double>(w, L, feat + (size_t)s * kFeatures, a[0], a[1], a[2], a[3], 1.0 / (double)n, r, &p);
+        loss += train_sample<double, HostRec, Q>(w, L, feat + (size_t)s * kFeatures, a[0], a[1], a[2], a[3],
+                                                 1.0 / (double)n, r, &p);
         if (pred) pred[s] = p;
         for (int li = 0; li < kTrainLayers; ++li) {
             const TrainLayer& l = L[li];
@@ -1193,6 +1207,31 @@ int oracle_mrpnn_grad(const float* w, int n, const float* feat, const float* aux
         }
     }
     if (loss_mean) *loss_mean = loss / (double)n;
+    return 0;
+}
+
+extern "C" {
+
+// Loss and gradient of the training objective in FP64 (mrp_train.h with
+// T = double): n samples, feat n x 576, aux n x 4 (gamma, g, srp, target).
+// grad (49,673) = d mean-loss / d w, accumulated sample by sample in order.
+int oracle_mrpnn_grad(const float* w, int n, const float* feat, const float* aux, double* grad, double* loss_mean,
+                      double* pred) {
+    return mrpnn_grad_impl<NoRound>(w, n, feat, aux, grad, loss_mean, pred);
+}
+
+// The same in FP64 with the operands of every matrix-core product (forward
+// W and x, backward W and dz) rounded to BF16: the arithmetic of the BF16
+// matrix-core trainer, up to its FP32 accumulation.
+int oracle_mrpnn_grad_bf16(const float* w, int n, const float* feat, const float* aux, double* grad,
+                           double* loss_mean, double* pred) {
+    return mrpnn_grad_impl<Bf16Round>(w, n, feat, aux, grad, loss_mean, pred);
+}
+
+// out[i] = Bf16Round(in[i])
+int oracle_bf16_round(const double* in, int n, double* out) {
+    if (!in || !out || n < 0) return -1;
+    for (int i = 0; i < n; ++i) out[i] = Bf16Round::r(in[i]);
     return 0;
 }
 

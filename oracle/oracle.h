@@ -24,6 +24,9 @@
  *                                                                    single scattering = FP64 quadrature
  * oracle_mrpnn_grad        training loss and gradient (derived from    FP64 finite differences of an
  *                          core/radiancePredict.cu:370-486)          independent numpy forward
+ * oracle_mrpnn_grad_bf16   the same with BF16 product operands        rounding = numpy BF16 (tests/lowp.py);
+ *                                                                    the BF16 numpy forward's predictions;
+ *                                                                    finite differences of its loss
  * oracle_reproject         core/volume.cu:125-141                    static camera maps to itself; pinhole
  *                                                                    projection into a moved camera (numpy)
  * Later sections of oracle.cpp list their own pins.
@@ -73,6 +76,9 @@ int oracle_denoise_pack(const float* target, mrp_histogram* histo, int W, int H,
 int oracle_tone_pack(const float* rgb, int n, float exposure, int tone, uint32_t* out);
 int oracle_mrpnn_grad(const float* w, int n, const float* feat, const float* aux, double* grad, double* loss_mean,
                       double* pred);
+int oracle_mrpnn_grad_bf16(const float* w, int n, const float* feat, const float* aux, double* grad,
+                           double* loss_mean, double* pred);
+int oracle_bf16_round(const double* in, int n, double* out);
 int oracle_get_samples(const float* grid, int res, float max_density, int n, const mrp_sample_query* q,
                        mrp_vec3 light_color, mrp_vec3 sky, int multi, int spp, uint64_t seed, float* out,
                        uint64_t* lookups_out, int threads);

@@ -69,3 +69,61 @@ def test_gradient_matches_finite_differences():
         assert abs(grad[i] - fd) <= 1e-5 * abs(fd) + 1e-8, (i, grad[i], fd)
         checked += abs(fd) > 1e-6
     assert checked >= len(idx) // 3  # most picks carry a live gradient
+
+
+# ---------------------------------------------------------------- BF16 variant
+# The BF16 matrix-core trainer rounds the operands of every matrix-core product
+# (forward W and x, backward W and dz) from FP32 to BF16; the oracle's
+# oracle_mrpnn_grad_bf16 restates that arithmetic in FP64.
+
+def test_oracle_bf16_rounding_matches_numpy():
+    from tests.lowp import bf16
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.normal(0, 1, 4000), rng.normal(0, 1e-30, 100), rng.normal(0, 1e30, 100)])
+    x = x.astype(np.float32).astype(np.float64)  # FP32 values, as the kernels hold them
+    # exact ties of BF16 (bit 15 set, bits 0..14 clear): round to even
+    ties = np.array([1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(1.0 + 2.0 ** -8), 3.0 * 2.0 ** -9 + 1.0])
+    x = np.concatenate([x, ties])
+    np.testing.assert_array_equal(oracle.bf16_round(x), bf16(x))
+    assert oracle.bf16_round(np.array([1.0 + 2.0 ** -8]))[0] == 1.0  # tie -> even
+    assert oracle.bf16_round(np.array([1.0 + 3 * 2.0 ** -8]))[0] == 1.0 + 2.0 ** -6
+
+
+def _bf16_quant(name, W, x):
+    from tests.lowp import VALU_LAYERS, bf16
+    if name in VALU_LAYERS:
+        return W, x
+    return bf16(W), bf16(np.asarray(x, np.float64).astype(np.float32).astype(np.float64))
+
+
+def test_bf16_loss_matches_numpy_bf16_forward():
+    feat, aux = _batch(12, 4)
+    _, loss, pred = oracle.mrpnn_grad(W32, feat, aux, bf16=True)
+    W = _wdict(W32.astype(np.float64))
+    ref_pred, total = [], 0.0
+    for s in range(feat.shape[0]):
+        gam, g, srp, y = (float(v) for v in aux[s])
+        p = forward64(W, feat[s:s + 1], np.array([gam]), g, (srp, srp, srp), quant=_bf16_quant)[0, 0]
+        ref_pred.append(p)
+        total += (p - y) ** 2 / (y * y + 0.01)
+    np.testing.assert_allclose(pred, ref_pred, rtol=1e-9, atol=1e-12)
+    assert loss == pytest.approx(total / feat.shape[0], rel=1e-9)
+    _, loss_exact, pred_exact = oracle.mrpnn_grad(W32, feat, aux)
+    assert 1e-5 < np.abs(pred - pred_exact).max() / np.abs(pred_exact).max() < 5e-2  # BF16 rounding is live
+
+
+def test_bf16_gradient_is_a_perturbation_of_the_exact_one():
+    # BF16 noise (2^-9 relative per operand, and ReLU boundaries it moves) is
+    # largest in the deep stream layers: about 2 % of the whole gradient at 64
+    # samples, up to 15 % of single layers
+    feat, aux = _batch(64, 6)
+    g16, _, _ = oracle.mrpnn_grad(W32, feat, aux, bf16=True)
+    g64, _, _ = oracle.mrpnn_grad(W32, feat, aux)
+    rel = np.linalg.norm(g16 - g64) / np.linalg.norm(g64)
+    assert 1e-3 < rel < 5e-2, rel
+    # every layer's gradient keeps its direction
+    for a in MAN["arrays"]:
+        s = slice(a["offset"], a["offset"] + a["count"])
+        if np.linalg.norm(g64[s]) > 1e-6 * np.linalg.norm(g64):
+            c = g16[s] @ g64[s] / (np.linalg.norm(g16[s]) * np.linalg.norm(g64[s]))
+            assert c > 0.98, (a["name"], c)
