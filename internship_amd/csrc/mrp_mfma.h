@@ -9,6 +9,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
+#include <utility>
 
 #include "mrp_math.h"
 #include "mrp_weights_gen.h"
@@ -146,6 +148,18 @@ __device__ __forceinline__ long pack8_fp8(float a0, float a1, float a2, float a3
     hi = __builtin_amdgcn_cvt_pk_fp8_f32(a6, a7, hi, true);
     return (long)(((unsigned long)(unsigned)hi << 32) | (unsigned long)(unsigned)lo);
 }
+// f(std::integral_constant<int, B + i * D>) for i = 0..N-1: a loop unrolled by
+// construction, whatever the compiler's unrolling thresholds (a large body
+// left as a loop turns every layer index into a run-time value).
+template <int B, int D, class F, int... I>
+__device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, B + I * D>{}), ...);
+}
+template <int B, int N, int D = 1, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+    static_for_impl<B, D>(f, std::make_integer_sequence<int, N>{});
+}
+
 // ---------------------------------------------------------------- device helpers
 template <Fmt F>
 struct Packed {

@@ -46,7 +46,8 @@ constexpr int TL_LX12 = 85;
 constexpr int kTrainLayers = 86;
 
 // Fill the layer table; returns the record length (floats per sample).
-static inline int build_train_layers(TrainLayer* L) {
+// constexpr, so that kernels can take offsets and sizes as constants.
+static constexpr int build_train_layers(TrainLayer* L) {
     const StreamOp ops[kStreamOps] = {MRP_STREAM_OPS};
     int off = 0, n = 0;
     auto add = [&](int w, int b, int out, int in, int act, int bias_last) {
@@ -87,6 +88,16 @@ static inline int build_train_layers(TrainLayer* L) {
     for (int r = 0; r < 12; ++r) add(MRP_W_LX0W + r * stride, MRP_W_LX0W + r * stride + 256, 16, 16, ACT_RELU, 0);
     add(MRP_W_LX12W, MRP_W_LX12B, 1, 16, ACT_RELU, 1);
     return off;
+}
+
+struct TrainTable {
+    TrainLayer L[kTrainLayers];
+    int rec_floats;
+};
+static constexpr TrainTable make_train_table() {
+    TrainTable t{};
+    t.rec_floats = build_train_layers(t.L);
+    return t;
 }
 
 MRP_HD float t_exp(float x) { return expf_(x); }

@@ -40,7 +40,7 @@ constexpr int kFloatSection = kBiasTiles * 32 + kExtraFloats;
 
 __constant__ LayerDesc c_tf[LY_COUNT];        // forward tiles (the inference table)
 __constant__ LayerDesc c_tb[LY_COUNT];        // transposed tiles (M = inputs, K = outputs; tile < 0: none)
-__constant__ TrainLayer c_tl[kTrainLayers];   // record slots
+constexpr TrainTable kTT = make_train_table();  // record slots and sizes, as constants
 
 // record slots of inference layer `ly`: the training layer list orders each SE
 // block as {SE first, SE second, stream A, B, C}
@@ -93,29 +93,36 @@ __device__ __forceinline__ void stage16(const uint4* __restrict__ src, uint4* ds
 }
 
 // ---------------------------------------------------------------- record access
-// A lane's view of the record: sample s (clamped for tail lanes, which write nothing).
+// A lane's view of the record: sample s (clamped for tail lanes, which write
+// nothing).  Slot `slot` of sample s is rec[slot * B + s] (32-bit index: the
+// record stays below 2^32 floats).  Row 32j + vrow(v, kg) of a tile is slot
+// off + 32j + 8(v/4) + v%4 + 4kg, so the index splits into a part uniform
+// over the wave (scalar unit, a constant offset) and the lane's 4 kg B + s.
 struct RecLane {
     float* rec;
-    uint32_t B, s;
+    uint32_t B, s, base;  // base = 4 kg B + s
     bool valid;
+    __device__ __forceinline__ uint32_t at(int off, int j, int v) const {
+        return (uint32_t)(off + 32 * j + 8 * (v / 4) + v % 4) * B + base;
+    }
     // rows 32j + vrow(v, kg) < n of tile x -> slots off + row
     __device__ __forceinline__ void put(int off, const f32x16& x, int j, int n, int kg) const {
         if (!valid) return;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int r = 32 * j + vrow(v, kg);
-            if (r < n) rec[(size_t)(off + r) * B + s] = x[v];
+            if (r < n) rec[at(off, j, v)] = x[v];
         }
     }
     __device__ __forceinline__ void put1(int slot, float x) const {
-        if (valid) rec[(size_t)slot * B + s] = x;
+        if (valid) rec[(uint32_t)slot * B + s] = x;
     }
-    __device__ __forceinline__ float get1(int slot) const { return rec[(size_t)slot * B + s]; }
+    __device__ __forceinline__ float get1(int slot) const { return rec[(uint32_t)slot * B + s]; }
     __device__ __forceinline__ void get(int off, f32x16& y, int j, int n, int kg) const {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int r = 32 * j + vrow(v, kg);
-            y[v] = r < n ? rec[(size_t)(off + r) * B + s] : 0.0f;
+            y[v] = r < n ? rec[at(off, j, v)] : 0.0f;
         }
     }
 };
@@ -155,8 +162,8 @@ __device__ __forceinline__ void load_seg(const float* frow, int k, int from, int
 template <int MT, int KB>
 __device__ __forceinline__ void fwd_layer(const Packed<Fmt::BF16>& pk, const RecLane& R, int ly, const f32x16* X,
                                           f32x16 (&Y)[MT], bool relu, int lane) {
-    const int kg = lane >> 5;
-    const TrainLayer& l = c_tl[tl_of(ly)];
+    const int kg = (lane >> 5) & 1;  // 0 or 1, visibly to the compiler
+    const TrainLayer& l = kTT.L[tl_of(ly)];
 #pragma unroll
     for (int j = 0; j < (KB + 1) / 2; ++j) R.put(l.x_off, X[j], j, l.in, kg);
     dense_d<Fmt::BF16, MT, KB>(pk, c_tf[ly], ly, X, Y, relu, lane);
@@ -179,7 +186,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
     }
     __syncthreads();
     const int lane = (int)__lane_id();
-    const int kg = lane >> 5;
+    const int kg = (lane >> 5) & 1;  // 0 or 1, visibly to the compiler
     const Packed<Fmt::BF16> pk{s_a, s_bias, nullptr};
     const int waves = (int)(blockDim.x >> 6);
     const uint32_t n_tiles = (a.B + 31u) / 32u;
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
         if (tile >= n_tiles) return;  // after the staging barrier
         const uint32_t item0 = tile * 32u;
         const uint32_t item = min(item0 + (uint32_t)(lane & 31), a.B - 1u);
-        const RecLane R{a.rec, a.B, item, item0 + (uint32_t)(lane & 31) < a.B};
+        const RecLane R{a.rec, a.B, item, 4u * (uint32_t)kg * a.B + item, item0 + (uint32_t)(lane & 31) < a.B};
         const float4 ax = a.aux[item];
         const float gamma = ax.x, g = ax.y, srp = ax.z, target = ax.w;
         const float* frow = a.feat + (size_t)item * kFeatures;
@@ -199,23 +206,30 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
         // pool[36 + 3 blk + k] = max of segment blk of stream k, then g, gamma,
         // srp at 72..74.  The pool goes to its record slots (that layer's x) as
         // it is computed, each value by the lane that reads it back at the head.
-        const int fin1_x = c_tl[TL_FIN1].x_off;
+        const int fin1_x = kTT.L[TL_FIN1].x_off;
+        f32x16 xn[3];  // the next block's features, loaded ahead of this block's stores
 #pragma unroll
-        for (int blk = 0; blk < 12; ++blk) {
-            const int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
-            const bool residual = (blk != 0 && blk != 8);
-            const bool rep = (blk == 1 || blk == 4);
+        for (int k = 0; k < 3; ++k) load_seg(frow, k, c_seg_from[0], 8, kg, xn[k]);
+        static_for<0, 12>([&](auto blk_c) __attribute__((always_inline)) {
+            constexpr int blk = decltype(blk_c)::value;
+            constexpr int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
+            constexpr bool residual = (blk != 0 && blk != 8);
+            constexpr bool rep = (blk == 1 || blk == 4);
             // the block's features, their means and maxima (rows < S: S/2 registers per lane group)
             float mean[3], mx[3];
+            f32x16 xs[3] = {xn[0], xn[1], xn[2]};
+            if (blk < 11) {
+                const int Sn = blk + 1 < 2 ? 8 : (blk + 1 < 5 ? 16 : (blk + 1 < 8 ? 32 : 8));
+#pragma unroll
+                for (int k = 0; k < 3; ++k) load_seg(frow, k, c_seg_from[blk + 1], Sn, kg, xn[k]);
+            }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                f32x16 xs;
-                load_seg(frow, k, c_seg_from[blk], S, kg, xs);
                 float sum = 0.0f, m = -10000.0f;
 #pragma unroll
                 for (int v = 0; v < S / 2; ++v) {
-                    sum += xs[v];
-                    m = fmax_(m, xs[v]);
+                    sum += xs[k][v];
+                    m = fmax_(m, xs[k][v]);
                 }
                 sum += __shfl_xor(sum, 32);
                 mean[k] = sum / (float)S;
@@ -246,13 +260,12 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
             gs[0] = sigmoid_(o[0][0]);
             gs[1] = sigmoid_(o[0][1]);
             gs[2] = sigmoid_(o[0][2]);
-            R.put(c_tl[tl_of(LY_SE1 + blk)].y_off, gs, 0, 3, kg);  // rows 0..2: lane group 0
+            R.put(kTT.L[tl_of(LY_SE1 + blk)].y_off, gs, 0, 3, kg);  // rows 0..2: lane group 0
             const int src = lane & 31;
             const float gate[3] = {__shfl(gs[0], src), __shfl(gs[1], src), __shfl(gs[2], src)};
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                f32x16 xv;
-                load_seg(frow, k, c_seg_from[blk], S, kg, xv);  // again: keeps 48 registers free
+                const f32x16& xv = xs[k];
                 f32x16 x;
 #pragma unroll
                 for (int v = 0; v < 16; ++v) x[v] = xv[v] * gate[k] + (residual ? prev[k][v] : 0.0f);
@@ -286,7 +299,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
                     prev[k] = zero16();
                 }
             }
-        }
+        });
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             f32x16 ol[1];
@@ -308,7 +321,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
         f32x16 sg = zero16();
 #pragma unroll
         for (int v = 0; v < 4; ++v) sg[v] = sigmoid_(s2[0][v]);
-        R.put(c_tl[TL_FIN2].y_off, sg, 0, 6, kg);
+        R.put(kTT.L[TL_FIN2].y_off, sg, 0, 6, kg);
         float s[6];
         const int src0 = lane & 31, src1 = (lane & 31) + 32;
         s[0] = __shfl(sg[0], src0);
@@ -325,7 +338,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
         }
         // LGGS (8x3, bias after the sum) on [g, gamma, srp]: rows 4kg..4kg+3
         {
-            const TrainLayer& l = c_tl[TL_LGGS];
+            const TrainLayer& l = kTT.L[TL_LGGS];
             if (kg == 0) {
                 R.put1(l.x_off, g);
                 R.put1(l.x_off + 1, gamma);
@@ -353,16 +366,16 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
         fwd_layer<2, 8>(pk, R, LY_LC1, h0, h1, true, lane);
         fwd_layer<1, 4>(pk, R, LY_LC2, h1, h2, true, lane);
         fwd_layer<1, 2>(pk, R, LY_LX, h2, ha, true, lane);
-#pragma unroll
-        for (int r = 0; r < 6; ++r) {
+        static_for<0, 6>([&](auto r_c) __attribute__((always_inline)) {
+            constexpr int r = decltype(r_c)::value;
             f32x16 hb[1], hc[1];
             fwd_layer<1, 1>(pk, R, LY_LXR + 2 * r, ha, hb, true, lane);
             fwd_layer<1, 1>(pk, R, LY_LXR + 2 * r + 1, hb, hc, true, lane);
 #pragma unroll
             for (int v = 0; v < 8; ++v) ha[0][v] = ha[0][v] + hc[0][v];
-        }
+        });
         // LX12 (1x16, bias after the sum)
-        const TrainLayer& l12 = c_tl[TL_LX12];
+        const TrainLayer& l12 = kTT.L[TL_LX12];
         R.put(l12.x_off, ha[0], 0, 16, kg);
         float part = 0.0f;
 #pragma unroll
@@ -404,55 +417,87 @@ __global__ __launch_bounds__(256) void train_loss_tiles_kernel(const double* __r
 }
 
 // ---------------------------------------------------------------- backward
-// dz = dy * act'(y) over rows < out (y read back), recorded.
-template <int MT>
-__device__ __forceinline__ void act_back(const RecLane& R, const TrainLayer& l, const f32x16* dy, f32x16 (&dz)[MT],
-                                         int kg) {
+// On CDNA one counter (vmcnt) tracks a wave's loads and stores together, so a
+// load issued after stores cannot be consumed before those stores complete.
+// The backward therefore issues the reads of y ahead of stores: in the head,
+// each layer's y is loaded before the stores of the layer that precedes it;
+// an SE block issues all its reads (five layers' y and its features) at once.
+
+// y tiles of a layer from the record (rows >= out read as 0)
+template <int MO>
+__device__ __forceinline__ void yload(const RecLane& R, int tl, f32x16 (&Y)[MO], int kg) {
 #pragma unroll
-    for (int j = 0; j < MT; ++j) {
-        f32x16 y;
-        R.get(l.y_off, y, j, l.out, kg);
+    for (int j = 0; j < MO; ++j) R.get(kTT.L[tl].y_off, Y[j], j, kTT.L[tl].out, kg);
+}
+
+// dz = dy * act'(y) over rows < out, recorded.
+template <int MO>
+__device__ __forceinline__ void act_back(const RecLane& R, const TrainLayer& l, const f32x16* dy, const f32x16* y,
+                                         f32x16 (&dz)[MO], int kg) {
+#pragma unroll
+    for (int j = 0; j < MO; ++j) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int r = 32 * j + vrow(v, kg);
             float d = r < l.out ? dy[j][v] : 0.0f;
-            if (l.act == ACT_RELU) d = y[v] > 0.0f ? d : 0.0f;
-            else if (l.act == ACT_SIGMOID) d = d * (y[v] * (1.0f - y[v]));
+            if (l.act == ACT_RELU) d = y[j][v] > 0.0f ? d : 0.0f;
+            else if (l.act == ACT_SIGMOID) d = d * (y[j][v] * (1.0f - y[j][v]));
             dz[j][v] = d;
         }
         R.put(l.dz_off, dz[j], j, l.out, kg);
     }
 }
 
-// backward of inference layer `ly`: records dz; dX (MI tiles over the inputs) = W^T dz
+// backward of inference layer `ly` with its y preloaded: records dz; dX (MI
+// tiles over the inputs) = W^T dz
 template <int MO, int MI, int KO>
 __device__ __forceinline__ void back_layer(const Packed<Fmt::BF16>& pk, const RecLane& R, int ly, const f32x16* dy,
-                                           f32x16 (&dx)[MI], int lane) {
+                                           const f32x16* y, f32x16 (&dx)[MI], int lane) {
     f32x16 dz[MO];
-    act_back<MO>(R, c_tl[tl_of(ly)], dy, dz, lane >> 5);
+    act_back<MO>(R, kTT.L[tl_of(ly)], dy, y, dz, (lane >> 5) & 1);
     dense_d<Fmt::BF16, MI, KO>(pk, c_tb[ly], ly, dz, dx, false, lane);
 }
 
-// one SE block backward: Gp = d loss / d (the block's output window) in, the
-// gradient for the previous block's window out (blocks 0 and 8 start a chain)
-__device__ __forceinline__ void block_back(const Packed<Fmt::BF16>& pk, const RecLane& R, const float* frow, int blk,
-                                           f32x16 (&Gp)[3], int lane) {
-    const int kg = lane >> 5;
+// What the backward of SE block blk reads: y of its three stream layers and
+// its two gate layers, and the block's input features.
+struct BlockIn {
+    f32x16 ys[3], xv[3], y1, y0;
+};
+
+__device__ __forceinline__ void block_load(const RecLane& R, const float* frow, int blk, BlockIn& b, int kg) {
     const int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const TrainLayer& l = kTT.L[tl_of(LY_STREAM + 3 * blk + k)];
+        R.get(l.y_off, b.ys[k], 0, l.out, kg);
+        load_seg(frow, k, c_seg_from[blk], S, kg, b.xv[k]);
+    }
+    R.get(kTT.L[tl_of(LY_SE1 + blk)].y_off, b.y1, 0, 3, kg);
+    R.get(kTT.L[tl_of(LY_SE0 + blk)].y_off, b.y0, 0, 8, kg);
+}
+
+// One SE block backward: Gp = d loss / d (the block's output window) in, the
+// gradient for the previous block's window out (blocks 0 and 8 start a chain).
+// All of the block's reads are issued together, before its first store.
+template <int blk>
+__device__ __forceinline__ void block_back(const Packed<Fmt::BF16>& pk, const RecLane& R, const float* frow,
+                                           f32x16 (&Gp)[3], int lane) {
+    const int kg = (lane >> 5) & 1;  // 0 or 1, visibly to the compiler
+    BlockIn b;
+    block_load(R, frow, blk, b, kg);
+    constexpr int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
     float dgate[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         f32x16 dx[1];
         if (S == 32) {
-            back_layer<1, 1, 2>(pk, R, LY_STREAM + 3 * blk + k, &Gp[k], dx, lane);
+            back_layer<1, 1, 2>(pk, R, LY_STREAM + 3 * blk + k, &Gp[k], &b.ys[k], dx, lane);
         } else {
-            back_layer<1, 1, 1>(pk, R, LY_STREAM + 3 * blk + k, &Gp[k], dx, lane);
+            back_layer<1, 1, 1>(pk, R, LY_STREAM + 3 * blk + k, &Gp[k], &b.ys[k], dx, lane);
         }
-        f32x16 xv;
-        load_seg(frow, k, c_seg_from[blk], S, kg, xv);
         float part = 0.0f;
 #pragma unroll
-        for (int v = 0; v < 16; ++v) part = fmaf(dx[0][v], xv[v], part);
+        for (int v = 0; v < 16; ++v) part = fmaf(dx[0][v], b.xv[k][v], part);
         dgate[k] = part + __shfl_xor(part, 32);
         Gp[k] = dx[0];
     }
@@ -462,9 +507,9 @@ __device__ __forceinline__ void block_back(const Packed<Fmt::BF16>& pk, const Re
     dg[1] = dgate[1];
     dg[2] = dgate[2];
     f32x16 dh[1];
-    back_layer<1, 1, 1>(pk, R, LY_SE1 + blk, &dg, dh, lane);
+    back_layer<1, 1, 1>(pk, R, LY_SE1 + blk, &dg, &b.y1, dh, lane);
     f32x16 dz0[1];
-    act_back<1>(R, c_tl[tl_of(LY_SE0 + blk)], dh, dz0, kg);
+    act_back<1>(R, kTT.L[tl_of(LY_SE0 + blk)], dh, &b.y0, dz0, kg);
 }
 
 __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaArgs a) {
@@ -477,7 +522,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaAr
     }
     __syncthreads();
     const int lane = (int)__lane_id();
-    const int kg = lane >> 5;
+    const int kg = (lane >> 5) & 1;  // 0 or 1, visibly to the compiler
     const Packed<Fmt::BF16> pk{s_t, nullptr, nullptr};
     const int waves = (int)(blockDim.x >> 6);
     const uint32_t n_tiles = (a.B + 31u) / 32u;
@@ -486,14 +531,16 @@ __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaAr
         if (tile >= n_tiles) return;  // after the staging barrier
         const uint32_t item0 = tile * 32u;
         const uint32_t item = min(item0 + (uint32_t)(lane & 31), a.B - 1u);
-        const RecLane R{a.rec, a.B, item, item0 + (uint32_t)(lane & 31) < a.B};
+        const RecLane R{a.rec, a.B, item, 4u * (uint32_t)kg * a.B + item, item0 + (uint32_t)(lane & 31) < a.B};
         const float4 ax = a.aux[item];
         const float srp = ax.z, target = ax.w;
         const float* frow = a.feat + (size_t)item * kFeatures;
 
         // ------------------------------------------------ loss -> LX12
-        const TrainLayer& l12 = c_tl[TL_LX12];
+        const TrainLayer& l12 = kTT.L[TL_LX12];
         const float P = R.get1(l12.y_off);
+        f32x16 ya[1], yb[1];
+        yload<1>(R, TL_LXR + 11, ya, kg);
         const float eP = expf_(P);
         const float p = (eP - 1.0f > 0.0f ? eP - 1.0f : 0.0f) * srp;
         const float den = target * target + 0.01f;
@@ -501,40 +548,56 @@ __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaAr
         float dP = 2.0f * diff / den * a.inv_batch;
         dP = eP - 1.0f > 0.0f ? dP * srp * eP : 0.0f;
         const float dz12 = P > 0.0f ? dP : 0.0f;
+        yload<1>(R, TL_LXR + 10, yb, kg);
         if (kg == 0) R.put1(l12.dz_off, dz12);
         f32x16 da[1];
         da[0] = zero16();
 #pragma unroll
         for (int v = 0; v < 8; ++v) da[0][v] = s_extra[32 + vrow(v, kg)] * dz12;
-        // ------------------------------------------------ residual tower, head layers
-#pragma unroll
-        for (int r = 5; r >= 0; --r) {
+        // ------------------------------------------------ residual tower (ya, yb: the pair's y)
+        static_for<5, 6, -1>([&](auto r_c) __attribute__((always_inline)) {
+            constexpr int r = decltype(r_c)::value;
             f32x16 dt[1], dc[1];
-            back_layer<1, 1, 1>(pk, R, LY_LXR + 2 * r + 1, da, dt, lane);
-            back_layer<1, 1, 1>(pk, R, LY_LXR + 2 * r, dt, dc, lane);
+            back_layer<1, 1, 1>(pk, R, LY_LXR + 2 * r + 1, da, ya, dt, lane);
+            yload<1>(R, r > 0 ? TL_LXR + 2 * r - 1 : TL_LX, ya, kg);
+            back_layer<1, 1, 1>(pk, R, LY_LXR + 2 * r, dt, yb, dc, lane);
+            if (r > 0) yload<1>(R, TL_LXR + 2 * r - 2, yb, kg);
 #pragma unroll
             for (int v = 0; v < 8; ++v) da[0][v] = da[0][v] + dc[0][v];
-        }
+        });
+        // ------------------------------------------------ head layers (ya: y of LX)
         f32x16 dh2[1], dh1[2], dh0[4], ds[4];
-        back_layer<1, 1, 1>(pk, R, LY_LX, da, dh2, lane);
-        back_layer<1, 2, 2>(pk, R, LY_LC2, dh2, dh1, lane);
-        back_layer<2, 4, 4>(pk, R, LY_LC1, dh1, dh0, lane);
-        back_layer<4, 4, 8>(pk, R, LY_LC0, dh0, ds, lane);
-        // ------------------------------------------------ head input = [comb * s | LD41 * s | LGGS]
+        f32x16 y2[1], y1[2], y0[4];
+        yload<1>(R, TL_LC0 + 2, y2, kg);
+        back_layer<1, 1, 1>(pk, R, LY_LX, da, ya, dh2, lane);
+        yload<2>(R, TL_LC0 + 1, y1, kg);
+        back_layer<1, 2, 2>(pk, R, LY_LC2, dh2, y2, dh1, lane);
+        yload<4>(R, TL_LC0, y0, kg);
+        // what the head input's backward reads: FIN2's gates, the L81 and LD41
+        // outputs, LGGS's and FIN1's outputs
         float se2[6];
 #pragma unroll
-        for (int i = 0; i < 6; ++i) se2[i] = R.get1(c_tl[TL_FIN2].y_off + i);
+        for (int i = 0; i < 6; ++i) se2[i] = R.get1(kTT.L[TL_FIN2].y_off + i);
+        f32x16 cy[3], ly[3], ylg[1], yf1[1];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            R.get(kTT.L[TL_L81 + k].y_off, cy[k], 0, 32, kg);
+            R.get(kTT.L[TL_LD41 + k].y_off, ly[k], 0, 8, kg);
+        }
+        yload<1>(R, TL_LGGS, ylg, kg);
+        yload<1>(R, TL_FIN1, yf1, kg);
+        back_layer<2, 4, 4>(pk, R, LY_LC1, dh1, y1, dh0, lane);
+        back_layer<4, 4, 8>(pk, R, LY_LC0, dh0, y0, ds, lane);
+        // ------------------------------------------------ head input = [comb * s | LD41 * s | LGGS]
         f32x16 dcomb[3];
         float dse2[6];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            f32x16 cy;
-            R.get(c_tl[TL_L81 + j].y_off, cy, 0, 32, kg);
             float part = 0.0f;
 #pragma unroll
             for (int v = 0; v < 16; ++v) {
                 dcomb[j][v] = ds[j][v] * se2[j];
-                part = fmaf(ds[j][v], cy[v], part);
+                part = fmaf(ds[j][v], cy[j][v], part);
             }
             dse2[j] = part + __shfl_xor(part, 32);
         }
@@ -545,9 +608,8 @@ __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaAr
             float part = 0.0f;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const float yv = R.get1(c_tl[TL_LD41 + k].y_off + 4 * kg + v);
                 dld[k][v] = ds[3][4 * k + v] * se2[3 + k];
-                part = fmaf(ds[3][4 * k + v], yv, part);
+                part = fmaf(ds[3][4 * k + v], ly[k][v], part);
             }
             dse2[3 + k] = part + __shfl_xor(part, 32);
         }
@@ -555,38 +617,44 @@ __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaAr
             f32x16 dyl = zero16(), dzl[1];
 #pragma unroll
             for (int v = 0; v < 4; ++v) dyl[v] = ds[3][12 + v];
-            act_back<1>(R, c_tl[TL_LGGS], &dyl, dzl, kg);
+            act_back<1>(R, kTT.L[TL_LGGS], &dyl, ylg, dzl, kg);
         }
         {
             // FIN2 rows 0..5: lane group 0 registers 0..3, lane group 1 registers 0..1
-            f32x16 dy2 = zero16();
+            f32x16 dy2 = zero16(), yfin2 = zero16();
             dy2[0] = kg == 0 ? dse2[0] : dse2[4];
             dy2[1] = kg == 0 ? dse2[1] : dse2[5];
             dy2[2] = kg == 0 ? dse2[2] : 0.0f;
             dy2[3] = kg == 0 ? dse2[3] : 0.0f;
+            yfin2[0] = kg == 0 ? se2[0] : se2[4];
+            yfin2[1] = kg == 0 ? se2[1] : se2[5];
+            yfin2[2] = kg == 0 ? se2[2] : 0.0f;
+            yfin2[3] = kg == 0 ? se2[3] : 0.0f;
             f32x16 dse1[1], dz1[1];
-            back_layer<1, 1, 1>(pk, R, LY_FIN2, &dy2, dse1, lane);
-            act_back<1>(R, c_tl[TL_FIN1], dse1, dz1, kg);
+            back_layer<1, 1, 1>(pk, R, LY_FIN2, &dy2, &yfin2, dse1, lane);
+            act_back<1>(R, kTT.L[TL_FIN1], dse1, yf1, dz1, kg);
         }
         // ------------------------------------------------ light stream (blocks 11..8)
         f32x16 Gp[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             f32x16 o[1];
-            back_layer<1, 1, 1>(pk, R, LY_LD41 + k, &dld[k], o, lane);
+            back_layer<1, 1, 1>(pk, R, LY_LD41 + k, &dld[k], &ly[k], o, lane);
             Gp[k] = o[0];
         }
-#pragma unroll
-        for (int blk = 11; blk >= 8; --blk) block_back(pk, R, frow, blk, Gp, lane);
+        static_for<11, 4, -1>([&](auto b_c) __attribute__((always_inline)) {
+            block_back<decltype(b_c)::value>(pk, R, frow, Gp, lane);
+        });
         // ------------------------------------------------ main stream (blocks 7..0)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             f32x16 o[1];
-            back_layer<1, 1, 2>(pk, R, LY_L81 + k, &dcomb[k], o, lane);
+            back_layer<1, 1, 2>(pk, R, LY_L81 + k, &dcomb[k], &cy[k], o, lane);
             Gp[k] = o[0];
         }
-#pragma unroll
-        for (int blk = 7; blk >= 0; --blk) block_back(pk, R, frow, blk, Gp, lane);
+        static_for<7, 8, -1>([&](auto b_c) __attribute__((always_inline)) {
+            block_back<decltype(b_c)::value>(pk, R, frow, Gp, lane);
+        });
     }
 }
 
@@ -618,8 +686,9 @@ int train_mfma_create(int max_batch, TrainMfma** out) {
     layer_table(F);
     int nb = 0;
     if (total_tiles(F, &nb) != kTiles || nb != kBiasTiles) return fail(MRP_E_STATE, "packed weight tiles do not match the kernel's LDS layout");
-    TrainLayer TL[kTrainLayers];
-    build_train_layers(TL);
+    const TrainLayer* TL = kTT.L;
+    if ((uint64_t)kTT.rec_floats * (uint64_t)max_batch >= (1ull << 32))
+        return fail(MRP_E_INVALID, "BF16 trainer: max_batch too large for 32-bit record indices (at most 1,041,000)");
     // the two layer lists must describe the same products
     for (int ly = 0; ly < LY_COUNT; ++ly) {
         const TrainLayer& l = TL[tl_of(ly)];
@@ -690,7 +759,6 @@ int train_mfma_create(int max_batch, TrainMfma** out) {
                          kScaleFloats * 4, hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tf), F, sizeof(F)) == hipSuccess;
     ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tb), T, sizeof(T)) == hipSuccess;
-    ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tl), TL, sizeof(TL)) == hipSuccess;
     if (!ok) {
         train_mfma_destroy(m);
         return fail(MRP_E_HIP, "trainer initialisation failed");
