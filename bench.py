@@ -644,8 +644,8 @@ def main(argv=None):
                     help="first run one device-to-device copy of this many MiB (reference kernel for HBM counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(image-tile / sample sharding is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path "
+              "(a frame would shard by image tiles, training by samples)", file=sys.stderr)
         return 2
 
     import internship_amd as ia

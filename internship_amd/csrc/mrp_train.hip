@@ -10,7 +10,7 @@
 //      structure-of-arrays record rec[slot][sample] (coalesced across lanes);
 //   2. wgrad_kernel - per layer, dW = dZ^T X and db = sum dZ over the batch:
 //      16x16 output tiles, batch chunks of 64 staged through LDS, FP64
-//      accumulation; the batch is split into slices of 1024 samples (one
+//      accumulation; the batch is split into slices of 512 samples (one
 //      workgroup per tile and slice) whose partial sums wgrad_reduce_kernel
 //      adds in slice order (deterministic: fixed order, no atomics);
 //   3. loss_sum_kernel - FP64 sum of the per-sample losses in a fixed tree;
@@ -60,8 +60,8 @@ struct WTile {
 };
 
 constexpr int kWT = 16, kWK = 64;
-constexpr uint32_t kWSlice = 1024;  // samples per gradient slice
-constexpr int kWMaxSlices = 16;
+constexpr uint32_t kWSlice = 512;  // samples per gradient slice
+constexpr int kWMaxSlices = 32;
 
 // Slice blockIdx.y covers samples [y * slice, min(B, (y + 1) * slice)); its
 // partial gradient goes to grad + y * MRP_W_TOTAL.
