@@ -160,6 +160,22 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl<B, D>(f, std::make_integer_sequence<int, N>{});
 }
 
+// n 16-byte words from global memory to LDS by a workgroup of kThreads threads,
+// kBatch independent loads in flight per thread (a plain copy loop waits for
+// each load before issuing the next).
+template <int kThreads, int kBatch = 16>
+__device__ __forceinline__ void stage16(const uint4* __restrict__ src, uint4* dst, int n) {
+    const int full = n / (kThreads * kBatch) * (kThreads * kBatch);
+    for (int i0 = (int)threadIdx.x; i0 < full; i0 += kThreads * kBatch) {
+        uint4 t[kBatch];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) t[j] = src[i0 + j * kThreads];
+#pragma unroll
+        for (int j = 0; j < kBatch; ++j) dst[i0 + j * kThreads] = t[j];
+    }
+    for (int i = full + (int)threadIdx.x; i < n; i += kThreads) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------- device helpers
 template <Fmt F>
 struct Packed {

@@ -149,12 +149,9 @@ __device__ __forceinline__ void dense(const Packed<F>& pk, int layer, const f32x
     dense_d<F, MT, KB>(pk, c_layers[layer], layer, X, D, relu, lane);
 }
 
-// Squeeze-excitation gates of block `blk`: returns the 3 sigmoid gates of this lane's item.
-template <Fmt F>
-__device__ __forceinline__ void se_gates(const Packed<F>& pk, int blk, const float* poolrow, float g, float gamma,
-                                         int lane, float* gate) {
-    const int kg = lane >> 5;
-    // t = [mean A, mean B, mean C, max A, max B, max C, g, gamma]; lane group kg holds t[4kg..4kg+3]
+// The squeeze-excitation input of block `blk`: t = [mean A, mean B, mean C,
+// max A, max B, max C, g, gamma]; lane group kg holds t[4kg..4kg+3].
+__device__ __forceinline__ f32x16 se_input(int blk, const float* poolrow, float g, float gamma, int kg) {
     f32x16 t = zero16();
     if (kg == 0) {
         t[0] = poolrow[3 * blk];
@@ -167,6 +164,13 @@ __device__ __forceinline__ void se_gates(const Packed<F>& pk, int blk, const flo
         t[2] = g;
         t[3] = gamma;
     }
+    return t;
+}
+
+// Squeeze-excitation gates of block `blk` from its input t: returns the 3
+// sigmoid gates of this lane's item.
+template <Fmt F>
+__device__ __forceinline__ void se_gates(const Packed<F>& pk, int blk, f32x16 t, int lane, float* gate) {
     f32x16 h[1];
     dense<F, 1, 1>(pk, LY_SE0 + blk, &t, h, true, lane);
     f32x16 o[1];
@@ -205,7 +209,7 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
         const uint4* packed = args.packed;
         constexpr int kA16 = (int)(tile_bytes<F>() / 16);
         uint4* da = (uint4*)s_a;
-        for (int i = threadIdx.x; i < kA16; i += kMlpBlock) da[i] = packed[i];
+        stage16<kMlpBlock>(packed, da, kA16);
         const float4* sb = (const float4*)(packed + kA16);
         float4* db = (float4*)s_bias;
         for (int i = threadIdx.x; i < kBiasTiles * 8; i += kMlpBlock) db[i] = sb[i];
@@ -245,30 +249,41 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
         const int S = size[blk];
         const bool residual = (blk != 0 && blk != 8);
         const bool rep = (blk == 1 || blk == 4);  // REP3 after this block, ADD3 otherwise
+        // The gate input, then the block's three stream chunks, are loaded
+        // before the gate layers: the chunks' latency overlaps those products
+        // (loads complete in order, so the gate input is issued first).
+        const f32x16 t_se = se_input(blk, prow, g, gamma, kg);
+        uint4 cq[3], cr[3];
+        if constexpr (kTiled) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint4* tc = args.tiled + tile_chunk(item, k, from[blk] >> 4, kg);
+                if (S == 8) {
+                    const uint2 h = ((const uint2*)tc)[(from[blk] >> 3) & 1];
+                    cq[k] = make_uint4(h.x, h.y, 0u, 0u);
+                } else {
+                    cq[k] = tc[0];
+                    if (S == 32) cr[k] = tc[64];  // the next chunk of the stream
+                }
+            }
+        }
         float gate[3];
-        se_gates<F>(pk, blk, prow, g, gamma, lane, gate);
+        se_gates<F>(pk, blk, t_se, lane, gate);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             f32x16 xv = zero16();
             if constexpr (kTiled) {
-                const uint4* tc = args.tiled + tile_chunk(item, k, from[blk] >> 4, kg);
-                if (S == 8) {
-                    const uint2 q = ((const uint2*)tc)[(from[blk] >> 3) & 1];
-                    unpack_bf16(q.x, xv, 0);
-                    unpack_bf16(q.y, xv, 2);
-                } else {
-                    const uint4 q = tc[0];
-                    unpack_bf16(q.x, xv, 0);
-                    unpack_bf16(q.y, xv, 2);
-                    unpack_bf16(q.z, xv, 4);
-                    unpack_bf16(q.w, xv, 6);
-                    if (S == 32) {
-                        const uint4 r = tc[64];  // the next chunk of the stream
-                        unpack_bf16(r.x, xv, 8);
-                        unpack_bf16(r.y, xv, 10);
-                        unpack_bf16(r.z, xv, 12);
-                        unpack_bf16(r.w, xv, 14);
-                    }
+                unpack_bf16(cq[k].x, xv, 0);
+                unpack_bf16(cq[k].y, xv, 2);
+                if (S != 8) {
+                    unpack_bf16(cq[k].z, xv, 4);
+                    unpack_bf16(cq[k].w, xv, 6);
+                }
+                if (S == 32) {
+                    unpack_bf16(cr[k].x, xv, 8);
+                    unpack_bf16(cr[k].y, xv, 10);
+                    unpack_bf16(cr[k].z, xv, 12);
+                    unpack_bf16(cr[k].w, xv, 14);
                 }
             } else {
                 const float* srow = frow + k * kTaps + from[blk];

@@ -243,7 +243,10 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
         h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, c_tap_u_view[i], lut_v);
     else if (i >= kShellTaps)
         h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, dot(a.lx, a.light) * 0.5f + 0.5f, lut_v);
-    const uint32_t n_items = a.n_dev ? *a.n_dev : a.n_items;  // a grid sized for n_items ends early
+    // a grid sized for n_items ends early.  Wave-uniform (readfirstlane), so the
+    // loop does not wait at every item for this load, which on CDNA would also
+    // wait for the previous item's stores (one counter for both).
+    const uint32_t n_items = __builtin_amdgcn_readfirstlane(a.n_dev ? *a.n_dev : a.n_items);
     for (int it = 0; it < kGatherItems; ++it) {
         const uint32_t item = blockIdx.x * kGatherItems + (uint32_t)it;
         if (item >= n_items) break;  // uniform over the block
@@ -251,7 +254,6 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
         const float4 pa = pos_act[src], x4 = fx[src], y4 = fy[src], z4 = fz[src];
         const vec3 p = v3(pa.x, pa.y, pa.z);
         const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
-        if (i == 0) gamma[item] = acosf_(dot(X, a.lx));
         float xd = 0.0f, xt = 0.0f, xh = 0.0f;
         if (pa.w != 0.0f) {
             vec3 pi;
@@ -280,6 +282,7 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
                 xt = trilinear_clamp_cells(a.trs[tm], tr, pi + 0.5f);
             }
         }
+        if (i == 0) gamma[item] = acosf_(dot(X, a.lx));  // after the lookups: stores delay later waits
         if (!kTiled) {
             float* out = feat + (size_t)item * kFeatures;
             out[i] = xd;

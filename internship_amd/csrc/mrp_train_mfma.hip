@@ -76,22 +76,6 @@ __global__ void train_pack_f32_kernel(const float* __restrict__ w, const int* __
     out[i] = src >= 0 ? w[src] : 0.0f;
 }
 
-// ---------------------------------------------------------------- LDS staging
-// n 16-byte words from global memory to LDS by one workgroup of kTrainBlock
-// threads, kStageBatch independent loads in flight per thread.
-constexpr int kStageBatch = 16;
-__device__ __forceinline__ void stage16(const uint4* __restrict__ src, uint4* dst, int n) {
-    const int full = n / (kTrainBlock * kStageBatch) * (kTrainBlock * kStageBatch);
-    for (int i0 = (int)threadIdx.x; i0 < full; i0 += kTrainBlock * kStageBatch) {
-        uint4 t[kStageBatch];
-#pragma unroll
-        for (int j = 0; j < kStageBatch; ++j) t[j] = src[i0 + j * kTrainBlock];
-#pragma unroll
-        for (int j = 0; j < kStageBatch; ++j) dst[i0 + j * kTrainBlock] = t[j];
-    }
-    for (int i = full + (int)threadIdx.x; i < n; i += kTrainBlock) dst[i] = src[i];
-}
-
 // ---------------------------------------------------------------- record access
 // A lane's view of the record: sample s (clamped for tail lanes, which write
 // nothing).  Slot `slot` of sample s is rec[slot * B + s] (32-bit index: the
@@ -180,9 +164,9 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
     __shared__ __align__(16) float s_extra[kExtraFloats];
     {
         constexpr int kA16 = (int)(tile_bytes<Fmt::BF16>() / 16);
-        stage16(a.packed, (uint4*)s_a, kA16);
-        stage16(a.packed + kA16, (uint4*)s_bias, kBiasTiles * 8);
-        stage16(a.packed + kA16 + kBiasTiles * 8, (uint4*)s_extra, kExtraFloats / 4);
+        stage16<kTrainBlock>(a.packed, (uint4*)s_a, kA16);
+        stage16<kTrainBlock>(a.packed + kA16, (uint4*)s_bias, kBiasTiles * 8);
+        stage16<kTrainBlock>(a.packed + kA16 + kBiasTiles * 8, (uint4*)s_extra, kExtraFloats / 4);
     }
     __syncthreads();
     const int lane = (int)__lane_id();
@@ -516,9 +500,9 @@ __global__ __launch_bounds__(kTrainBlock) void train_bwd_mfma_kernel(TrainMfmaAr
     __shared__ bf16x8 s_t[kTilesT * 64];
     __shared__ __align__(16) float s_extra[kExtraFloats];
     {
-        stage16(a.packed_t, (uint4*)s_t, kTilesT * 64);
+        stage16<kTrainBlock>(a.packed_t, (uint4*)s_t, kTilesT * 64);
         constexpr int kA16 = (int)(tile_bytes<Fmt::BF16>() / 16);
-        stage16(a.packed + kA16 + kBiasTiles * 8, (uint4*)s_extra, kExtraFloats / 4);
+        stage16<kTrainBlock>(a.packed + kA16 + kBiasTiles * 8, (uint4*)s_extra, kExtraFloats / 4);
     }
     __syncthreads();
     const int lane = (int)__lane_id();
