@@ -153,6 +153,10 @@ int mrp_volume_create(int resolution, mrp_volume** out) {
             return bail(MRP_E_NOMEM, "TR cell allocation failed");
     if (hipMalloc(&v->lut_cells, lut_cell_count(kLutSize, kLutSize) * sizeof(float4)) != hipSuccess)
         return bail(MRP_E_NOMEM, "HG LUT cell allocation failed");
+    if (hipMalloc(&v->mip_cells16, (size_t)mip16_offset(kNumMips) * sizeof(uint4)) != hipSuccess)
+        return bail(MRP_E_NOMEM, "FP16 mip cell allocation failed");
+    if (hipMalloc(&v->tr_cells16, (size_t)tr16_offset(kNumTrMips) * sizeof(uint4)) != hipSuccess)
+        return bail(MRP_E_NOMEM, "FP16 TR cell allocation failed");
     if (hipMalloc(&v->queues, kNumQueues * sizeof(uint32_t)) != hipSuccess) return bail(MRP_E_NOMEM, "queues");
     if (hipMalloc(&v->counters, 4 * sizeof(unsigned long long)) != hipSuccess) return bail(MRP_E_NOMEM, "counters");
     *out = h;
@@ -171,6 +175,8 @@ int mrp_volume_destroy(mrp_volume* h) {
     for (int m = 0; m < kNumMips; ++m) hipFree(v->mip_cells[m]);
     for (int m = 0; m < kNumTrMips; ++m) hipFree(v->tr_cells[m]);
     hipFree(v->lut_cells);
+    hipFree(v->mip_cells16);
+    hipFree(v->tr_cells16);
     hipFree(v->queues);
     hipFree(v->counters);
     hipFree(v->samples);

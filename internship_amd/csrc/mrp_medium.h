@@ -181,6 +181,60 @@ __device__ __forceinline__ float trilinear_clamp_cells(const float4* __restrict_
     return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
                 wx);
 }
+// FP16 cells: the same corners rounded to FP16 (round to nearest even,
+// saturated at +-65504), 16 bytes per cell; the same coordinates, weights and
+// blend order as the FP32 cells.  Level m of the border cells of the density
+// mips (256 >> m)^3 starts at mip16_offset(m) cells, level m of the clamp cells
+// of the transmittance mips (128 >> m)^3 at tr16_offset(m).
+MRP_HD uint32_t mip16_offset(int m) {
+    uint32_t o = 0u;
+    for (int k = 0; k < m; ++k) {
+        const uint32_t c = (uint32_t)(256 >> k) + 1u;
+        o += c * c * c;
+    }
+    return o;
+}
+MRP_HD uint32_t tr16_offset(int m) {
+    uint32_t o = 0u;
+    for (int k = 0; k < m; ++k) {
+        const uint32_t c = (uint32_t)(128 >> k) + 2u;
+        o += c * c * c;
+    }
+    return o;
+}
+__device__ __forceinline__ uint32_t pack_half2(float a, float b) {
+    a = fmax_(-65504.0f, fmin_(a, 65504.0f));
+    b = fmax_(-65504.0f, fmin_(b, 65504.0f));
+    return (uint32_t)__builtin_bit_cast(unsigned short, (_Float16)a) |
+           ((uint32_t)__builtin_bit_cast(unsigned short, (_Float16)b) << 16);
+}
+__device__ __forceinline__ float half_lo(uint32_t w) {
+    return (float)__builtin_bit_cast(_Float16, (unsigned short)(w & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t w) { return (float)__builtin_bit_cast(_Float16, (unsigned short)(w >> 16)); }
+__device__ __forceinline__ float blend_cell16(uint4 q, float wx, float wy, float wz) {
+    return lerp(lerp(lerp(half_lo(q.x), half_hi(q.x), wz), lerp(half_lo(q.y), half_hi(q.y), wz), wy),
+                lerp(lerp(half_lo(q.z), half_hi(q.z), wz), lerp(half_lo(q.w), half_hi(q.w), wz), wy), wx);
+}
+__device__ __forceinline__ float trilinear_cells16(const uint4* __restrict__ cells, int res, vec3 p) {
+    const float fr = (float)res;
+    const float tx = (p.x + 0.5f) * fr - 0.5f;
+    const float ty = (p.y + 0.5f) * fr - 0.5f;
+    const float tz = (p.z + 0.5f) * fr - 0.5f;
+    if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) return 0.0f;
+    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    return blend_cell16(cells[cell_index32(res, (int)fx, (int)fy, (int)fz)], tx - fx, ty - fy, tz - fz);
+}
+__device__ __forceinline__ float trilinear_clamp_cells16(const uint4* __restrict__ cells, int res, vec3 uv) {
+    const float fr = (float)res;
+    float tx = uv.x * fr - 0.5f, ty = uv.y * fr - 0.5f, tz = uv.z * fr - 0.5f;
+    tx = tx >= -1.0f ? fmin_(tx, fr) : -1.0f;
+    ty = ty >= -1.0f ? fmin_(ty, fr) : -1.0f;
+    tz = tz >= -1.0f ? fmin_(tz, fr) : -1.0f;
+    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    return blend_cell16(cells[clamp_cell_index32(res, (int)fx, (int)fy, (int)fz)], tx - fx, ty - fy, tz - fz);
+}
+
 // bilinear_clamp on LUT cells: one 16-byte read.
 __device__ __forceinline__ float bilinear_cells(const float4* __restrict__ cells, int w, int h, float u, float v) {
     float tx = u * (float)w - 0.5f, ty = v * (float)h - 0.5f;

@@ -188,6 +188,8 @@ struct GatherArgs {
     const float4* mips[kNumMips];    // border cells of the density mips
     const float4* trs[kNumTrMips];   // clamp cells of the transmittance mips
     const float4* lut;               // bilinear clamp cells of the HG table
+    const uint4* mips16;             // FP16 cells of every density mip (tiled gather)
+    const uint4* trs16;              // FP16 cells of every transmittance mip (tiled gather)
     vec3 light, lx;
     float alpha, g;
     uint32_t n_items;
@@ -215,10 +217,13 @@ static_assert(kGatherItems % 8 == 0, "the tiled gather flushes groups of 8 descr
 
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
-// The gather is bound by memory latency per descriptor, so it is held to 7
-// waves per SIMD (at most 72 VGPRs): at 76 VGPRs and 6 waves it ran 7 % slower.
+// The gather is bound by memory latency per descriptor, so occupancy matters:
+// it is held to 8 waves per SIMD (at most 64 VGPRs); at 76 VGPRs and 6 waves
+// it ran 7 % slower than at 7.  The tiled (BF16/FP8 renderer) gather reads FP16
+// cells, one 16-byte load per lookup, whose values round below the BF16 the
+// descriptors are stored in; the FP32-row gather reads the FP32 cells.
 template <bool kTiled>
-__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
+__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
                                                        void* __restrict__ feat_out, float* __restrict__ gamma,
@@ -247,11 +252,32 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
     // loop does not wait at every item for this load, which on CDNA would also
     // wait for the previous item's stores (one counter for both).
     const uint32_t n_items = __builtin_amdgcn_readfirstlane(a.n_dev ? *a.n_dev : a.n_items);
+    // The vertex a descriptor belongs to (compacted list) and its record are
+    // uniform over the block (scalar loads).  They are prefetched: the list
+    // entry two items ahead, the record one item ahead, so an item's taps
+    // start without waiting for either.
+    const uint32_t item0 = blockIdx.x * kGatherItems;
+    uint32_t src_next = 0u;
+    float4 pa_n = make_float4(0.f, 0.f, 0.f, 0.f), x4_n = pa_n, y4_n = pa_n, z4_n = pa_n;
+    if (item0 < n_items) {
+        const uint32_t src0 = list ? list[item0] : item0;
+        pa_n = pos_act[src0];
+        x4_n = fx[src0];
+        y4_n = fy[src0];
+        z4_n = fz[src0];
+        if (item0 + 1u < n_items) src_next = list ? list[item0 + 1u] : item0 + 1u;
+    }
     for (int it = 0; it < kGatherItems; ++it) {
-        const uint32_t item = blockIdx.x * kGatherItems + (uint32_t)it;
+        const uint32_t item = item0 + (uint32_t)it;
         if (item >= n_items) break;  // uniform over the block
-        const uint32_t src = list ? list[item] : item;  // compacted: the vertex this descriptor belongs to
-        const float4 pa = pos_act[src], x4 = fx[src], y4 = fy[src], z4 = fz[src];
+        const float4 pa = pa_n, x4 = x4_n, y4 = y4_n, z4 = z4_n;
+        if (it + 1 < kGatherItems && item + 1u < n_items) {
+            pa_n = pos_act[src_next];
+            x4_n = fx[src_next];
+            y4_n = fy[src_next];
+            z4_n = fz[src_next];
+            if (it + 2 < kGatherItems && item + 2u < n_items) src_next = list ? list[item + 2u] : item + 2u;
+        }
         const vec3 p = v3(pa.x, pa.y, pa.z);
         const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
         float xd = 0.0f, xt = 0.0f, xh = 0.0f;
@@ -262,7 +288,9 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
             } else {
                 pi = p + a.light * off;
             }
-            xd = logf_(1.0f + a.alpha * trilinear_cells(a.mips[mip], 256 >> mip, pi) / 64.0f);
+            const float d = kTiled ? trilinear_cells16(a.mips16 + mip16_offset(mip), 256 >> mip, pi)
+                                   : trilinear_cells(a.mips[mip], 256 >> mip, pi);
+            xd = logf_(1.0f + a.alpha * d / 64.0f);
             float harg;
             if (i == 0) {
                 harg = hg_phase(dot(X, a.light), a.g);
@@ -277,9 +305,13 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
             xh = logf_(harg + 1.0f);
             if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
                 const float o = ray_box_offset(pi, a.lx);
-                xt = o >= 0.0f ? trilinear_clamp_cells(a.trs[tm], tr, (pi + a.lx * o) + 0.5f) : 1.0f;
+                const vec3 q = (pi + a.lx * o) + 0.5f;
+                xt = o < 0.0f ? 1.0f
+                              : (kTiled ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, q)
+                                        : trilinear_clamp_cells(a.trs[tm], tr, q));
             } else {
-                xt = trilinear_clamp_cells(a.trs[tm], tr, pi + 0.5f);
+                xt = kTiled ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, pi + 0.5f)
+                            : trilinear_clamp_cells(a.trs[tm], tr, pi + 0.5f);
             }
         }
         if (i == 0) gamma[item] = acosf_(dot(X, a.lx));  // after the lookups: stores delay later waits
@@ -310,16 +342,21 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void
             if (pool && i < 72) {
                 const int q = i % 36, blk = q / 3, k = q % 3;
                 const float* seg = sf + k * kTaps + c_seg_from[blk];
-                const int n = c_seg_size[blk];
-                float r;
-                if (i < 36) {
-                    float sum = 0.0f;
-                    for (int j = 0; j < n; ++j) sum += seg[j];
-                    r = sum / (float)n;
-                } else {
-                    r = -10000.0f;
-                    for (int j = 0; j < n; ++j) r = fmax_(r, seg[j]);
+                const int n = c_seg_size[blk];  // 8, 16 or 32, per thread
+                // 8 reads in flight per step (the loop bound differs across the
+                // wave, so a plain loop would wait for each read); same order of sums
+                float sum = 0.0f, mx = -10000.0f;
+                for (int j0 = 0; j0 < n; j0 += 8) {
+                    float v[8];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) v[e] = seg[j0 + e];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        sum += v[e];
+                        mx = fmax_(mx, v[e]);
+                    }
                 }
+                const float r = i < 36 ? sum / (float)n : mx;
                 pool[(size_t)item * 72 + i] = r;
             }
             if (kTiled && ((it & (kStageItems - 1)) == kStageItems - 1 || item + 1 >= n_items)) {
@@ -525,6 +562,8 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mip_cells[m];
     for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_cells[m];
     ga.lut = v->lut_cells;
+    ga.mips16 = v->mip_cells16;
+    ga.trs16 = v->tr_cells16;
     ga.light = a.light;
     ga.lx = a.lx;
     ga.alpha = p->alpha;
@@ -723,6 +762,8 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mip_cells[m];
     for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_cells[m];
     ga.lut = v->lut_cells;
+    ga.mips16 = v->mip_cells16;
+    ga.trs16 = v->tr_cells16;
     ga.light = normalize(V(light_dir));
     ga.lx = normalize(ga.light);
     ga.alpha = alpha;

@@ -88,6 +88,14 @@ __global__ void build_clamp_cells_kernel(const float* __restrict__ src, int res,
     }
 }
 
+// FP16 copy of n cells (two float4 each) as one uint4 each.
+__global__ void cells_half_kernel(const float4* __restrict__ cells, size_t n, uint4* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 lo = cells[2 * i], hi = cells[2 * i + 1];
+    out[i] = make_uint4(pack_half2(lo.x, lo.y), pack_half2(lo.z, lo.w), pack_half2(hi.x, hi.y), pack_half2(hi.z, hi.w));
+}
+
 // Bilinear clamp cells of a w x h table (x fastest) for cell coordinates in [-1, w] x [-1, h].
 __global__ void build_lut_cells_kernel(const float* __restrict__ lut, int w, int h, float4* __restrict__ cells) {
     const int cw = w + 2, ch = h + 2;
@@ -255,9 +263,13 @@ int launch_update(Volume* v) {
         const int rm = mip_res(m);
         box_kernel<<<blocks_for((size_t)rm * rm * rm), 256, 0, v->stream>>>(v->mips[m - 1], v->mips[m], rm);
     }
-    for (int m = 0; m < kNumMips; ++m)
+    for (int m = 0; m < kNumMips; ++m) {
         build_cells_kernel<<<blocks_for(cell_count(mip_res(m))), 256, 0, v->stream>>>(v->mips[m], mip_res(m),
                                                                                       v->mip_cells[m]);
+        const size_t n = cell_count(mip_res(m));
+        cells_half_kernel<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(v->mip_cells[m], n,
+                                                                               v->mip_cells16 + mip16_offset(m));
+    }
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
     for (float* t : temps) hipFree(t);
@@ -281,6 +293,9 @@ int launch_update_tr(Volume* v, vec3 light, float alpha) {
                                                                              am);
         build_clamp_cells_kernel<<<blocks_for(clamp_cell_count(r)), 256, 0, v->stream>>>(v->tr_mips[m], r,
                                                                                          v->tr_cells[m]);
+        const size_t n = clamp_cell_count(r);
+        cells_half_kernel<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(v->tr_cells[m], n,
+                                                                               v->tr_cells16 + tr16_offset(m));
     }
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
