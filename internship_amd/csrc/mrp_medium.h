@@ -97,18 +97,21 @@ __device__ __forceinline__ uint32_t clamp_cell_index32(int res, int x, int y, in
     const uint32_t c = (uint32_t)res + 2u;
     return ((uint32_t)(x + 1) * c + (uint32_t)(y + 1)) * c + (uint32_t)(z + 1);
 }
+// Outside the cells the result is 0; branch-free (cell (-1,-1,-1) is read and
+// discarded), so lanes of a wave need no divergent branch.
 __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cells, int res, vec3 p) {
     const float fr = (float)res;
     const float tx = (p.x + 0.5f) * fr - 0.5f;
     const float ty = (p.y + 0.5f) * fr - 0.5f;
     const float tz = (p.z + 0.5f) * fr - 0.5f;
-    if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) return 0.0f;
-    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+    const bool in = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
+    const float fx = in ? floorf(tx) : -1.0f, fy = in ? floorf(ty) : -1.0f, fz = in ? floorf(tz) : -1.0f;
     const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
     const float4* c = cells + 2u * cell_index32(res, (int)fx, (int)fy, (int)fz);
     const float4 lo = c[0], hi = c[1];
-    return lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy), lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy),
-                wx);
+    const float r = lerp(lerp(lerp(lo.x, lo.y, wz), lerp(lo.z, lo.w, wz), wy),
+                         lerp(lerp(hi.x, hi.y, wz), lerp(hi.z, hi.w, wz), wy), wx);
+    return in ? r : 0.0f;
 }
 
 // trilinear_cells in two halves, so that several lookups can be in flight:
@@ -186,19 +189,23 @@ __device__ __forceinline__ float trilinear_clamp_cells(const float4* __restrict_
 // blend order as the FP32 cells.  Level m of the border cells of the density
 // mips (256 >> m)^3 starts at mip16_offset(m) cells, level m of the clamp cells
 // of the transmittance mips (128 >> m)^3 at tr16_offset(m).
+// (m is at most 9 / 8; a fixed-bound loop with selects, so a per-thread level
+// costs no divergent loop)
 MRP_HD uint32_t mip16_offset(int m) {
     uint32_t o = 0u;
-    for (int k = 0; k < m; ++k) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
         const uint32_t c = (uint32_t)(256 >> k) + 1u;
-        o += c * c * c;
+        o += k < m ? c * c * c : 0u;
     }
     return o;
 }
 MRP_HD uint32_t tr16_offset(int m) {
     uint32_t o = 0u;
-    for (int k = 0; k < m; ++k) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
         const uint32_t c = (uint32_t)(128 >> k) + 2u;
-        o += c * c * c;
+        o += k < m ? c * c * c : 0u;
     }
     return o;
 }
@@ -221,9 +228,10 @@ __device__ __forceinline__ float trilinear_cells16(const uint4* __restrict__ cel
     const float tx = (p.x + 0.5f) * fr - 0.5f;
     const float ty = (p.y + 0.5f) * fr - 0.5f;
     const float tz = (p.z + 0.5f) * fr - 0.5f;
-    if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) return 0.0f;
-    const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
-    return blend_cell16(cells[cell_index32(res, (int)fx, (int)fy, (int)fz)], tx - fx, ty - fy, tz - fz);
+    const bool in = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
+    const float fx = in ? floorf(tx) : -1.0f, fy = in ? floorf(ty) : -1.0f, fz = in ? floorf(tz) : -1.0f;
+    const float r = blend_cell16(cells[cell_index32(res, (int)fx, (int)fy, (int)fz)], tx - fx, ty - fy, tz - fz);
+    return in ? r : 0.0f;
 }
 __device__ __forceinline__ float trilinear_clamp_cells16(const uint4* __restrict__ cells, int res, vec3 uv) {
     const float fr = (float)res;

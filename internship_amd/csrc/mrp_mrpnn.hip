@@ -303,16 +303,15 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
                 harg = h_fixed * bilinear_cells(a.lut, kHgLutSize, kHgLutSize, uv, lut_v);
             }
             xh = logf_(harg + 1.0f);
-            if (pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f) {
-                const float o = ray_box_offset(pi, a.lx);
-                const vec3 q = (pi + a.lx * o) + 0.5f;
-                xt = o < 0.0f ? 1.0f
-                              : (kTiled ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, q)
-                                        : trilinear_clamp_cells(a.trs[tm], tr, q));
-            } else {
-                xt = kTiled ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, pi + 0.5f)
-                            : trilinear_clamp_cells(a.trs[tm], tr, pi + 0.5f);
-            }
+            // a tap outside the box reads the TR mip where its light ray enters
+            // the box (1 if it misses); branch-free: both cases read one cell
+            const bool outside =
+                pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f;
+            const float o = ray_box_offset(pi, a.lx);
+            const vec3 q = outside ? (pi + a.lx * o) + 0.5f : pi + 0.5f;
+            const float tv = kTiled ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, q)
+                                    : trilinear_clamp_cells(a.trs[tm], tr, q);
+            xt = outside && o < 0.0f ? 1.0f : tv;
         }
         if (i == 0) gamma[item] = acosf_(dot(X, a.lx));  // after the lookups: stores delay later waits
         if (!kTiled) {
