@@ -116,27 +116,24 @@ __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cell
 
 // trilinear_cells in two halves, so that several lookups can be in flight:
 // issue() starts the two loads, finish() blends (the same arithmetic; a point
-// outside the cells gives zero corners and weights, which blend to +0).
+// outside the cells gives +0).
 struct CellsFetch {
     struct Token {
         float4 lo, hi;
         float wx, wy, wz;
+        bool in;
     };
     const float4* __restrict__ cells;
     int res;
+    // branch-free: a point outside reads cell (-1,-1,-1) and finish() discards it
     __device__ __forceinline__ Token issue(vec3 p) const {
         const float fr = (float)res;
         const float tx = (p.x + 0.5f) * fr - 0.5f;
         const float ty = (p.y + 0.5f) * fr - 0.5f;
         const float tz = (p.z + 0.5f) * fr - 0.5f;
         Token k;
-        if (!(tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr)) {
-            k.lo = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            k.hi = k.lo;
-            k.wx = k.wy = k.wz = 0.0f;
-            return k;
-        }
-        const float fx = floorf(tx), fy = floorf(ty), fz = floorf(tz);
+        k.in = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
+        const float fx = k.in ? floorf(tx) : -1.0f, fy = k.in ? floorf(ty) : -1.0f, fz = k.in ? floorf(tz) : -1.0f;
         k.wx = tx - fx;
         k.wy = ty - fy;
         k.wz = tz - fz;
@@ -147,8 +144,9 @@ struct CellsFetch {
     }
     __device__ __forceinline__ float finish(const Token& k) const {
         const float4 lo = k.lo, hi = k.hi;
-        return lerp(lerp(lerp(lo.x, lo.y, k.wz), lerp(lo.z, lo.w, k.wz), k.wy),
-                    lerp(lerp(hi.x, hi.y, k.wz), lerp(hi.z, hi.w, k.wz), k.wy), k.wx);
+        const float r = lerp(lerp(lerp(lo.x, lo.y, k.wz), lerp(lo.z, lo.w, k.wz), k.wy),
+                             lerp(lerp(hi.x, hi.y, k.wz), lerp(hi.z, hi.w, k.wz), k.wy), k.wx);
+        return k.in ? r : 0.0f;
     }
 };
 #endif
