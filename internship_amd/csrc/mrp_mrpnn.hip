@@ -412,18 +412,34 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
     }
 }
 
-// Active first vertices -> a dense list (wave ballot, one atomic per wave).
+// Active first vertices -> a dense list (wave ballots, one atomic per workgroup).
 // The list order depends on scheduling; every descriptor's result does not.
-__global__ __launch_bounds__(256) void compact_kernel(const float4* __restrict__ pos_act, uint32_t n,
-                                                      uint32_t* __restrict__ list, uint32_t* __restrict__ slot_of,
-                                                      uint32_t* __restrict__ count) {
+constexpr int kCompactBlock = 1024;
+__global__ __launch_bounds__(kCompactBlock) void compact_kernel(const float4* __restrict__ pos_act, uint32_t n,
+                                                                uint32_t* __restrict__ list,
+                                                                uint32_t* __restrict__ slot_of,
+                                                                uint32_t* __restrict__ count) {
+    // one atomic per workgroup (a wave ballot per wave, a scan of the wave
+    // counts in LDS): the counter's atomics serialise
+    __shared__ uint32_t wave_off[kCompactBlock / 64];
+    __shared__ uint32_t block_base;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const bool act = i < n && pos_act[i].w != 0.0f;
     const unsigned long long m = __ballot(act);
-    const uint32_t lane = __lane_id();
-    uint32_t base = 0u;
-    if (lane == 0u && m != 0ull) base = atomicAdd(count, (uint32_t)__popcll(m));
-    base = __shfl(base, 0);
+    const uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
+    if (lane == 0u) wave_off[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0u;
+        for (int q = 0; q < kCompactBlock / 64; ++q) {
+            const uint32_t c = wave_off[q];
+            wave_off[q] = tot;
+            tot += c;
+        }
+        block_base = tot ? atomicAdd(count, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = block_base + wave_off[wv];
     if (i < n) {
         if (act) {
             const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
@@ -582,7 +598,8 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(tc, v->stream));
         clear_u32_one_kernel<<<1, 1, 0, v->stream>>>(n_active_dev);
-        compact_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, cnt, list, slot_of, n_active_dev);
+        compact_kernel<<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
+            pos_act, cnt, list, slot_of, n_active_dev);
         MRP_HIP(hipGetLastError());
         // No read-back here: the shadow rays, the gather and the network read the
         // count on the device, with grids sized for the whole chunk.
