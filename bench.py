@@ -100,11 +100,18 @@ class Scene:
         log("max density %.6f" % self.vol.max_density)
         self.target = ia.DeviceBuffer(width * height * 3 * 4)
         self.st = ia.TraceStats()
+        self._params = {}
 
     def render(self, spp_begin, spp_count, seed, rt=0, precision=1, sum64=None, fresh=True):
         if fresh:
             self.target.zero()
-        p = scene_params(self.ia, self.W, self.H, spp_begin, spp_count, self.bounces, seed, rt, precision)
+        # the parameters of a frame differ only in the first sample index: build them once per kind
+        key = (spp_count, seed, rt, precision)
+        p = self._params.get(key)
+        if p is None:
+            p = self._params[key] = scene_params(self.ia, self.W, self.H, spp_begin, spp_count, self.bounces, seed, rt,
+                                                 precision)
+        p.spp_begin = spp_begin
         t0 = time.perf_counter()
         self.vol.render_dev(p, self.target, sum64, self.st)
         return time.perf_counter() - t0

@@ -181,6 +181,8 @@ int mrp_volume_destroy(mrp_volume* h) {
     hipFree(v->counters);
     hipFree(v->samples);
     hipFree(v->nn_scratch);
+    for (hipEvent_t e : v->nn_events)
+        if (e) hipEventDestroy(e);
     if (v->stream) hipStreamDestroy(v->stream);
     delete h;
     return MRP_OK;

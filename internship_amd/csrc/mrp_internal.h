@@ -30,6 +30,7 @@ struct Volume {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
+    hipEvent_t nn_events[7] = {};  // stage timing of MRPNN frames, created on first use
     float* grid = nullptr;           // padded (res+2)^3, border zeros
     float4* cells = nullptr;         // corner-expanded cells (res+1)^3 x 2 float4, the tracer's layout
     float max_density = 0.00001f;

@@ -70,11 +70,15 @@ constexpr int kMaxTrackSteps = 10000;
 // Items run over the logical pixels of `a`'s pixel map; the output slot
 // (s - spp_begin) * P + pixel is 0xffffffff for a logical pixel past the image
 // edge (odd widths under the checkerboard).
+// count_out: the compaction's counter, cleared here (the compaction runs next
+// on the same stream).
 __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __restrict__ pos_act,
                                                            float4* __restrict__ fx, float4* __restrict__ fy,
-                                                           float4* __restrict__ fz, uint32_t* __restrict__ rng_pos) {
+                                                           float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
+                                                           uint32_t* __restrict__ count_out) {
     __shared__ unsigned red[256];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0u) *count_out = 0u;
     uint32_t lk = 0u;
     if (i < a.n_items) {
         const uint32_t P = (uint32_t)(a.width * a.height);
@@ -451,7 +455,6 @@ __global__ __launch_bounds__(kCompactBlock) void compact_kernel(const float4* __
     }
 }
 
-__global__ void clear_u32_one_kernel(uint32_t* p) { *p = 0u; }
 
 __global__ void clear_counters_kernel(unsigned long long* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -586,18 +589,20 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     const vec3 srp = srp_of(a.scatter_rate);
 
     // Stage boundaries: first vertex | compaction | shadow rays | gather | network | combine.
-    hipEvent_t t0, tc, ts0, ts1, t2, t3, tj;
-    hipEvent_t* evs[7] = {&t0, &tc, &ts0, &ts1, &t2, &t3, &tj};
-    for (hipEvent_t* e : evs) MRP_HIP(hipEventCreate(e));
+    // the volume's events, created once (creating them per frame costs host time between frames)
+    for (hipEvent_t& e : v->nn_events)
+        if (!e) MRP_HIP(hipEventCreate(&e));
+    const hipEvent_t t0 = v->nn_events[0], tc = v->nn_events[1], ts0 = v->nn_events[2], ts1 = v->nn_events[3],
+                     t2 = v->nn_events[4], t3 = v->nn_events[5], tj = v->nn_events[6];
     for (size_t i0 = 0; i0 < total && rc == MRP_OK; i0 += chunk_items) {
         const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
         a.item0 = (uint32_t)i0;
         a.n_items = cnt;
         MRP_HIP(hipEventRecord(t0, v->stream));
-        first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos);
+        first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos,
+                                                                       n_active_dev);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(tc, v->stream));
-        clear_u32_one_kernel<<<1, 1, 0, v->stream>>>(n_active_dev);
         compact_kernel<<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
             pos_act, cnt, list, slot_of, n_active_dev);
         MRP_HIP(hipGetLastError());
@@ -655,7 +660,6 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
             times->network += ms;
         }
     }
-    for (hipEvent_t* e : evs) hipEventDestroy(*e);
     return rc;
 }
 
