@@ -153,6 +153,8 @@ int mrp_volume_create(int resolution, mrp_volume** out) {
             return bail(MRP_E_NOMEM, "TR cell allocation failed");
     if (hipMalloc(&v->lut_cells, lut_cell_count(kLutSize, kLutSize) * sizeof(float4)) != hipSuccess)
         return bail(MRP_E_NOMEM, "HG LUT cell allocation failed");
+    if (hipHostMalloc((void**)&v->host_words, 8 * sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+        return bail(MRP_E_NOMEM, "pinned host words");
     if (hipMalloc(&v->mip_cells16, (size_t)mip16_offset(kNumMips) * sizeof(uint4)) != hipSuccess)
         return bail(MRP_E_NOMEM, "FP16 mip cell allocation failed");
     if (hipMalloc(&v->tr_cells16, (size_t)tr16_offset(kNumTrMips) * sizeof(uint4)) != hipSuccess)
@@ -183,6 +185,7 @@ int mrp_volume_destroy(mrp_volume* h) {
     hipFree(v->nn_scratch);
     for (hipEvent_t e : v->nn_events)
         if (e) hipEventDestroy(e);
+    if (v->host_words) hipHostFree(v->host_words);
     if (v->stream) hipStreamDestroy(v->stream);
     delete h;
     return MRP_OK;

@@ -30,7 +30,11 @@ struct Volume {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t nn_events[7] = {};  // stage timing of MRPNN frames, created on first use
+    hipEvent_t nn_events[9] = {};  // stage timing of MRPNN frames, created on first use
+    // pinned host words the device copies into without a blocking copy: [0] the
+    // last MRPNN chunk's descriptor count, [1..4] the lookup counters
+    unsigned long long* host_words = nullptr;
+    bool nn_pending = false;       // the last chunk's count and times await mrpnn_finish
     float* grid = nullptr;           // padded (res+2)^3, border zeros
     float4* cells = nullptr;         // corner-expanded cells (res+1)^3 x 2 float4, the tracer's layout
     float max_density = 0.00001f;
@@ -98,8 +102,12 @@ int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
 struct NNTimes {
     double first_vertex = 0.0, compact = 0.0, shadow = 0.0, gather = 0.0, network = 0.0;
 };
+// With defer_last, the last chunk's count and stage times are left pending
+// (no host synchronisation): after synchronising the stream, the caller adds
+// them with mrpnn_finish.
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
-                  uint64_t* n_active);
+                  uint64_t* n_active, bool defer_last = false);
+int mrpnn_finish(Volume* v, NNTimes* times, uint64_t* n_active);
 int clear_counters(Volume* v);
 int read_counters(Volume* v, unsigned long long out[4]);
 int launch_update(Volume* v);

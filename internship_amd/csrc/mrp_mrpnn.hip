@@ -512,7 +512,7 @@ static vec3 srp_of(vec3 scatter_rate) {
 // First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
 // of the pixels `map` selects, in chunks of at most 4M vertices.
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
-                  uint64_t* n_active) {
+                  uint64_t* n_active, bool defer_last) {
     if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
     if (p->mlp_precision < 0 || p->mlp_precision > 2)
         return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
@@ -641,26 +641,40 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
                                                                       (float4*)v->samples);
             MRP_HIP(hipGetLastError());
         }
-        uint32_t n_act = 0;
-        MRP_HIP(hipMemcpyAsync(&n_act, n_active_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
+        // the count into pinned host memory (a copy into pageable memory would block)
+        MRP_HIP(hipMemcpyAsync(v->host_words, n_active_dev, sizeof(uint32_t), hipMemcpyDeviceToHost, v->stream));
+        if (rc != MRP_OK) break;
+        v->nn_pending = true;
+        if (defer_last && i0 + cnt >= total) break;  // the caller synchronises, then mrpnn_finish
         MRP_HIP(hipEventRecord(tj, v->stream));
         MRP_HIP(hipEventSynchronize(tj));
-        if (n_active) *n_active += n_act;
-        if (rc == MRP_OK && times) {
-            float ms;
-            MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
-            times->first_vertex += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));
-            times->compact += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, ts0, ts1));
-            times->shadow += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, ts1, t2));
-            times->gather += ms;
-            MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
-            times->network += ms;
-        }
+        rc = mrpnn_finish(v, times, n_active);
     }
     return rc;
+}
+
+// Adds the pending chunk's descriptor count and stage times (the stream has
+// been synchronised since that chunk was launched).
+int mrpnn_finish(Volume* v, NNTimes* times, uint64_t* n_active) {
+    if (!v->nn_pending) return MRP_OK;
+    v->nn_pending = false;
+    if (n_active) *n_active += (uint32_t)v->host_words[0];
+    if (times) {
+        const hipEvent_t t0 = v->nn_events[0], tc = v->nn_events[1], ts0 = v->nn_events[2], ts1 = v->nn_events[3],
+                         t2 = v->nn_events[4], t3 = v->nn_events[5];
+        float ms;
+        MRP_HIP(hipEventElapsedTime(&ms, t0, tc));
+        times->first_vertex += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, tc, ts0));
+        times->compact += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, ts0, ts1));
+        times->shadow += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, ts1, t2));
+        times->gather += ms;
+        MRP_HIP(hipEventElapsedTime(&ms, t2, t3));
+        times->network += ms;
+    }
+    return MRP_OK;
 }
 
 int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
@@ -669,9 +683,9 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     const int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16), (size_t)p->spp_count));
     int rc = ensure_samples(v, (size_t)batch * P * 16);
     if (rc) return rc;
-    hipEvent_t e0, e1;
-    MRP_HIP(hipEventCreate(&e0));
-    MRP_HIP(hipEventCreate(&e1));
+    for (hipEvent_t& e : v->nn_events)
+        if (!e) MRP_HIP(hipEventCreate(&e));
+    const hipEvent_t e0 = v->nn_events[7], e1 = v->nn_events[8];
     MRP_HIP(hipEventRecord(e0, v->stream));
     rc = clear_counters(v);
     NNTimes tm;
@@ -679,19 +693,25 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     const PixelMap map;
     for (int s0 = 0; s0 < p->spp_count && rc == MRP_OK; s0 += batch) {
         const int n = std::min(batch, p->spp_count - s0);
-        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &tm, &n_active);
+        const bool last = s0 + n >= p->spp_count;
+        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &tm, &n_active, last);
         if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
     }
     float total_ms = 0.0f;
     unsigned long long cnt4[4] = {0, 0, 0, 0};
     if (rc == MRP_OK) {
+        // one host synchronisation per call: counters into pinned memory, then the end event
+        MRP_HIP(hipMemcpyAsync(v->host_words + 1, v->counters, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               v->stream));
         MRP_HIP(hipEventRecord(e1, v->stream));
         MRP_HIP(hipEventSynchronize(e1));
         MRP_HIP(hipEventElapsedTime(&total_ms, e0, e1));
-        rc = read_counters(v, cnt4);
+        for (int q = 0; q < 4; ++q) cnt4[q] = v->host_words[1 + q];
+        rc = mrpnn_finish(v, &tm, &n_active);
+    } else {
+        hipStreamSynchronize(v->stream);
+        v->nn_pending = false;
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
     if (rc) return rc;
     if (st) {
         st->lookups = cnt4[0] + cnt4[2];
