@@ -294,3 +294,36 @@ def test_gpu_render_mrpnn_mfma_precisions_track_fp32(cloud64, oracle_tables):
         _say("precision %d: image rel L2 %.3g" % (prec, rel))
         assert np.all(np.isfinite(gpu))
         assert rel <= tol
+
+
+def test_gpu_render_mrpnn_chunks_and_batches_agree(cloud64):
+    """An MRPNN render split differently on the device - one 5-spp batch in
+    two chunks (5.2M (sample, pixel) items, above the 4.19M chunk) versus five
+    1-spp batches (a 16 MiB sample budget) - gives the same image bit for bit
+    and the same descriptor and lookup counts: every sample has its own stream
+    and every descriptor is evaluated on its own."""
+    _say("case: MRPNN render 1024x1024, 5 spp, BF16 network: one batch of two chunks vs five batches")
+    w = load_reference_weights()
+    net = Network(w)
+    out = []
+    for budget in (None, 16 << 20):
+        vol = VolumeRender(64)
+        vol.set_datas(cloud64)
+        vol.update()
+        vol.set_network(net)
+        if budget:
+            _lib.check(vol.lib.mrp_set_sample_buffer_budget(vol.h, ctypes.c_size_t(budget)))
+        p = config1_params(size=1024, spp=5, multi=1, seed=7)
+        p.render_type = 2
+        p.mlp_precision = 1
+        img = np.zeros((1024, 1024, 3), np.float32)
+        st = TraceStats()
+        _lib.check(vol.lib.mrp_render_offline(vol.h, ctypes.byref(p), ctypes.c_void_p(img.ctypes.data),
+                                              ctypes.byref(st)))
+        vol.close()
+        out.append((img, st.scatters, st.lookups, st.shadow_lookups))
+    net.close()
+    (a, sa, la, ha), (b, sb, lb, hb) = out
+    _say("descriptors %d / %d, lookups %d / %d, equal pixels %.4f" % (sa, sb, la, lb, np.mean(a == b)))
+    assert sa == sb and la == lb and ha == hb and sa > 0
+    assert np.array_equal(a, b)
