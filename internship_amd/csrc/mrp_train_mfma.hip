@@ -124,7 +124,8 @@ struct TrainMfmaArgs {
     double* loss_tiles;     // per 32-sample tile
 };
 
-__constant__ int c_seg_from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
+// first feature of each SE block's segment (indexed with compile-time block numbers)
+constexpr int kSegFrom[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
 
 // the block's input features of stream k, in D-order (rows >= S are zero)
 __device__ __forceinline__ void load_seg(const float* frow, int k, int from, int S, int kg, f32x16& xv) {
@@ -142,7 +143,8 @@ __device__ __forceinline__ void load_seg(const float* frow, int k, int from, int
     }
 }
 
-// dense layer `ly` forward with its x and y recorded
+// dense layer `ly` forward with its x recorded, and its y for a ReLU layer
+// (the two sigmoid layers record their gates, after the sigmoid, themselves)
 template <int MT, int KB>
 __device__ __forceinline__ void fwd_layer(const Packed<Fmt::BF16>& pk, const RecLane& R, int ly, const f32x16* X,
                                           f32x16 (&Y)[MT], bool relu, int lane) {
@@ -193,7 +195,7 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
         const int fin1_x = kTT.L[TL_FIN1].x_off;
         f32x16 xn[3];  // the next block's features, loaded ahead of this block's stores
 #pragma unroll
-        for (int k = 0; k < 3; ++k) load_seg(frow, k, c_seg_from[0], 8, kg, xn[k]);
+        for (int k = 0; k < 3; ++k) load_seg(frow, k, kSegFrom[0], 8, kg, xn[k]);
         static_for<0, 12>([&](auto blk_c) __attribute__((always_inline)) {
             constexpr int blk = decltype(blk_c)::value;
             constexpr int S = blk < 2 ? 8 : (blk < 5 ? 16 : (blk < 8 ? 32 : 8));
@@ -202,10 +204,10 @@ __global__ __launch_bounds__(kTrainBlock) void train_fwd_mfma_kernel(TrainMfmaAr
             // the block's features, their means and maxima (rows < S: S/2 registers per lane group)
             float mean[3], mx[3];
             f32x16 xs[3] = {xn[0], xn[1], xn[2]};
-            if (blk < 11) {
-                const int Sn = blk + 1 < 2 ? 8 : (blk + 1 < 5 ? 16 : (blk + 1 < 8 ? 32 : 8));
+            if constexpr (blk < 11) {
+                constexpr int Sn = blk + 1 < 2 ? 8 : (blk + 1 < 5 ? 16 : (blk + 1 < 8 ? 32 : 8));
 #pragma unroll
-                for (int k = 0; k < 3; ++k) load_seg(frow, k, c_seg_from[blk + 1], Sn, kg, xn[k]);
+                for (int k = 0; k < 3; ++k) load_seg(frow, k, kSegFrom[blk + 1], Sn, kg, xn[k]);
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -454,7 +456,7 @@ __device__ __forceinline__ void block_load(const RecLane& R, const float* frow, 
     for (int k = 0; k < 3; ++k) {
         const TrainLayer& l = kTT.L[tl_of(LY_STREAM + 3 * blk + k)];
         R.get(l.y_off, b.ys[k], 0, l.out, kg);
-        load_seg(frow, k, c_seg_from[blk], S, kg, b.xv[k]);
+        load_seg(frow, k, kSegFrom[blk], S, kg, b.xv[k]);
     }
     R.get(kTT.L[tl_of(LY_SE1 + blk)].y_off, b.y1, 0, 3, kg);
     R.get(kTT.L[tl_of(LY_SE0 + blk)].y_off, b.y0, 0, 8, kg);
