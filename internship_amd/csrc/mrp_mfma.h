@@ -50,7 +50,8 @@ struct LayerDesc {
     int btile;     // first bias tile (2 lane groups x 16 floats) or -1
 };
 
-static inline void layer_table(LayerDesc* L) {
+// constexpr, so that the kernels take tile and bias offsets as constants
+static constexpr void layer_table(LayerDesc* L) {
     const int se0[12] = {MRP_W_LSE01W, MRP_W_LSE11W, MRP_W_LSE21W, MRP_W_LSE31W, MRP_W_LSE41W, MRP_W_LSE51W,
                          MRP_W_LSE61W, MRP_W_LSE71W, MRP_W_LDSE01W, MRP_W_LDSE11W, MRP_W_LDSE21W, MRP_W_LDSE31W};
     const int se1[12] = {MRP_W_LSE02W, MRP_W_LSE12W, MRP_W_LSE22W, MRP_W_LSE32W, MRP_W_LSE42W, MRP_W_LSE52W,
@@ -98,6 +99,15 @@ static inline void layer_table(LayerDesc* L) {
             btile += mt;
         }
     }
+}
+
+struct LayerTable {
+    LayerDesc L[LY_COUNT];
+};
+static constexpr LayerTable make_layer_table() {
+    LayerTable t{};
+    layer_table(t.L);
+    return t;
 }
 
 static inline int total_tiles(const LayerDesc* L, int* btiles) {
@@ -197,6 +207,11 @@ __device__ __forceinline__ long to_fp8(const f32x16& x, int half, float s) {
                      x[o + 7] * s);
 }
 
+// max(x, +0) as one integer max on the bits: a negative value or -0 (sign bit
+// set: negative as an integer) gives +0.  It differs from x >= 0 ? x : 0 only
+// in the sign of a zero, which no later product or sum of the network sees.
+__device__ __forceinline__ float relu_bits(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
+
 __device__ __forceinline__ f32x16 mfma(bf16x8 a, bf16x8 b, f32x16 c) {
     return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
@@ -240,8 +255,8 @@ __device__ __forceinline__ void dense_d(const Packed<F>& pk, const LayerDesc d, 
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) amax = fmax_(amax, fabsf(X[kb >> 1][8 * (kb & 1) + e]));
-        amax = fmax_(amax, __shfl_xor(amax, 32));  // the item's other lane group
+            for (int e = 0; e < 8; ++e) amax = __builtin_fmaxf(amax, fabsf(X[kb >> 1][8 * (kb & 1) + e]));
+        amax = __builtin_fmaxf(amax, __shfl_xor(amax, 32));  // the item's other lane group
         // amax in [2^(eb-127), 2^(eb-126)): scale 2^(134-eb) puts it in [128, 256)
         uint32_t eb = __float_as_uint(amax) >> 23;
         eb = eb < 8u ? 8u : (eb > 254u ? 254u : eb);
@@ -278,7 +293,7 @@ __device__ __forceinline__ void dense_d(const Packed<F>& pk, const LayerDesc d, 
         }
         if (relu) {
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[v] = acc[v] >= 0.0f ? acc[v] : 0.0f;
+            for (int v = 0; v < 16; ++v) acc[v] = relu_bits(acc[v]);
         }
         D[mt] = acc;
     }

@@ -50,7 +50,7 @@
 
 namespace mrp {
 
-__constant__ LayerDesc c_layers[LY_COUNT];
+constexpr LayerTable kLT = make_layer_table();  // tile and bias offsets, as constants
 
 // Pre-scaled FP32 weight tiles -> E4M3 tiles, with the instruction that also
 // converts the activations.
@@ -138,7 +138,6 @@ int pack_mlp(const float* w, int fmt, void** packed_dev) {
         hipFree(dev);
         return hip_fail(e, "packed weight upload");
     }
-    MRP_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_layers), L, sizeof(L)));
     *packed_dev = dev;
     return MRP_OK;
 }
@@ -146,7 +145,7 @@ int pack_mlp(const float* w, int fmt, void** packed_dev) {
 template <Fmt F, int MT, int KB>
 __device__ __forceinline__ void dense(const Packed<F>& pk, int layer, const f32x16* X, f32x16 (&D)[MT], bool relu,
                                       int lane) {
-    dense_d<F, MT, KB>(pk, c_layers[layer], layer, X, D, relu, lane);
+    dense_d<F, MT, KB>(pk, kLT.L[layer], layer, X, D, relu, lane);
 }
 
 // The squeeze-excitation input of block `blk`: t = [mean A, mean B, mean C,
@@ -167,6 +166,13 @@ __device__ __forceinline__ f32x16 se_input(int blk, const float* poolrow, float 
     return t;
 }
 
+// The sigmoid gates of the BF16 / FP8 networks: 1 / (1 + 2^(-x log2 e)) on the
+// hardware exponential and reciprocal (about 1 ulp each, against the 2^-9
+// rounding of the BF16 products that the gates scale).
+__device__ __forceinline__ float gate_sigmoid(float x) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-1.44269504f * x));
+}
+
 // Squeeze-excitation gates of block `blk` from its input t: returns the 3
 // sigmoid gates of this lane's item.
 template <Fmt F>
@@ -177,9 +183,9 @@ __device__ __forceinline__ void se_gates(const Packed<F>& pk, int blk, f32x16 t,
     dense<F, 1, 1>(pk, LY_SE1 + blk, h, o, false, lane);
     // rows 0..2 live in lane group 0 (registers 0..2)
     const int src = lane & 31;
-    gate[0] = sigmoid_(__shfl(o[0][0], src));
-    gate[1] = sigmoid_(__shfl(o[0][1], src));
-    gate[2] = sigmoid_(__shfl(o[0][2], src));
+    gate[0] = gate_sigmoid(__shfl(o[0][0], src));
+    gate[1] = gate_sigmoid(__shfl(o[0][1], src));
+    gate[2] = gate_sigmoid(__shfl(o[0][2], src));
 }
 
 // ---------------------------------------------------------------- kernel
@@ -201,6 +207,24 @@ struct MlpArgs {
     float* pred;
 };
 
+
+// The three streams' chunks of S features from feature `from` of a
+// wave-tiled descriptor (lane group kg's half): cq the first 16 (8 if S is 8),
+// cr the next 16 when S is 32.
+__device__ __forceinline__ void stream_chunks(const uint4* __restrict__ tiled, uint32_t item, int from, int S, int kg,
+                                              uint4 (&cq)[3], uint4 (&cr)[3]) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint4* tc = tiled + tile_chunk(item, k, from >> 4, kg);
+        if (S == 8) {
+            const uint2 h = ((const uint2*)tc)[(from >> 3) & 1];
+            cq[k] = make_uint4(h.x, h.y, 0u, 0u);
+        } else {
+            cq[k] = tc[0];
+        }
+        cr[k] = S == 32 ? tc[64] : make_uint4(0u, 0u, 0u, 0u);  // the next chunk of the stream
+    }
+}
 
 template <Fmt F, bool kTiled>
 __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpOf<F>::T* s_a, float* s_bias,
@@ -226,11 +250,17 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
     if (n_items == 0u) return;  // uniform; after the staging barrier
     const int lane = (int)__lane_id();
     const int kg = lane >> 5;
-    const Packed<F> pk{s_a, s_bias, s_unscale};
     const float* pextra = s_extra;
     const uint32_t n_tiles = (n_items + 31u) / 32u;
     for (uint32_t tile = blockIdx.x * kMlpWaves + (threadIdx.x >> 6); tile < n_tiles;
          tile += gridDim.x * kMlpWaves) {
+    // The weight and bias tiles are the same for every tile of descriptors, and
+    // their offsets are constants: an opaque zero offset, new each iteration,
+    // keeps their LDS reads inside the loop (all of them read ahead would need
+    // about 600 more registers).
+    uint32_t opaque = 0u;
+    asm volatile("" : "+s"(opaque));
+    const Packed<F> pk{s_a + opaque, s_bias + opaque, s_unscale};
     const uint32_t item0 = tile * 32u;
     const uint32_t item = min(item0 + (uint32_t)(lane & 31), n_items - 1u);  // tail lanes recompute the last item
     const float* frow = kTiled ? nullptr : feat + (size_t)item * kFeatures;
@@ -244,28 +274,29 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
     float lout[3][4];  // LD41 outputs (8 rows: registers 0..3 of each lane group)
     const int from[12] = {0, 8, 16, 32, 48, 64, 96, 128, 160, 168, 176, 184};
     const int size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
+    // Loads run one block ahead: block blk + 1's gate input and stream chunks
+    // are issued before block blk's products, so their latency overlaps a
+    // whole block (loads complete in order; a wait for this block's values
+    // leaves the next block's in flight).
+    f32x16 t_next = se_input(0, prow, g, gamma, kg);
+    uint4 cq_next[3] = {}, cr_next[3] = {};
+    if constexpr (kTiled) stream_chunks(args.tiled, item, from[0], size[0], kg, cq_next, cr_next);
 #pragma unroll
     for (int blk = 0; blk < 12; ++blk) {
         const int S = size[blk];
         const bool residual = (blk != 0 && blk != 8);
         const bool rep = (blk == 1 || blk == 4);  // REP3 after this block, ADD3 otherwise
-        // The gate input, then the block's three stream chunks, are loaded
-        // before the gate layers: the chunks' latency overlaps those products
-        // (loads complete in order, so the gate input is issued first).
-        const f32x16 t_se = se_input(blk, prow, g, gamma, kg);
+        const f32x16 t_se = t_next;
         uint4 cq[3], cr[3];
-        if constexpr (kTiled) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const uint4* tc = args.tiled + tile_chunk(item, k, from[blk] >> 4, kg);
-                if (S == 8) {
-                    const uint2 h = ((const uint2*)tc)[(from[blk] >> 3) & 1];
-                    cq[k] = make_uint4(h.x, h.y, 0u, 0u);
-                } else {
-                    cq[k] = tc[0];
-                    if (S == 32) cr[k] = tc[64];  // the next chunk of the stream
-                }
-            }
+        for (int k = 0; k < 3; ++k) {
+            cq[k] = cq_next[k];
+            cr[k] = cr_next[k];
+        }
+        if (blk + 1 < 12) {
+            t_next = se_input(blk + 1, prow, g, gamma, kg);
+            if constexpr (kTiled)
+                stream_chunks(args.tiled, item, from[blk + 1], size[blk + 1], kg, cq_next, cr_next);
         }
         float gate[3];
         se_gates<F>(pk, blk, t_se, lane, gate);
@@ -383,12 +414,12 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
         dense<F, 1, 1>(pk, LY_FIN2, s1, s2, false, lane);
         float s[6];
         const int src0 = lane & 31, src1 = (lane & 31) + 32;
-        s[0] = sigmoid_(__shfl(s2[0][0], src0));
-        s[1] = sigmoid_(__shfl(s2[0][1], src0));
-        s[2] = sigmoid_(__shfl(s2[0][2], src0));
-        s[3] = sigmoid_(__shfl(s2[0][3], src0));
-        s[4] = sigmoid_(__shfl(s2[0][0], src1));
-        s[5] = sigmoid_(__shfl(s2[0][1], src1));
+        s[0] = gate_sigmoid(__shfl(s2[0][0], src0));
+        s[1] = gate_sigmoid(__shfl(s2[0][1], src0));
+        s[2] = gate_sigmoid(__shfl(s2[0][2], src0));
+        s[3] = gate_sigmoid(__shfl(s2[0][3], src0));
+        s[4] = gate_sigmoid(__shfl(s2[0][0], src1));
+        s[5] = gate_sigmoid(__shfl(s2[0][1], src1));
         // head input: [comb A * s0 | comb B * s1 | comb C * s2 | LD41 x s3..s5, LGGS]
         f32x16 x0[4];
 #pragma unroll
