@@ -38,8 +38,29 @@ constexpr int kTilesT = 132;          // transposed tiles (layers whose input gr
 constexpr int kTrainBlock = 128;
 constexpr int kFloatSection = kBiasTiles * 32 + kExtraFloats;
 
-__constant__ LayerDesc c_tf[LY_COUNT];        // forward tiles (the inference table)
-__constant__ LayerDesc c_tb[LY_COUNT];        // transposed tiles (M = inputs, K = outputs; tile < 0: none)
+// Transposed tiles for the input gradients: layer ly's W^T (M = its inputs,
+// K = its outputs), tile -1 where no input gradient is needed (the first SE
+// layers and LSEFin1, whose inputs are pooled statistics).  n_tiles: the
+// tiles used.
+static constexpr LayerTable make_transposed_table(int* n_tiles) {
+    const LayerTable f = make_layer_table();
+    LayerTable t{};
+    int tt = 0;
+    for (int ly = 0; ly < LY_COUNT; ++ly) {
+        const bool need_dx = !(ly >= LY_SE0 && ly < LY_SE1) && ly != LY_FIN1;
+        t.L[ly] = {f.L[ly].w, -1, f.L[ly].K, f.L[ly].M, need_dx ? tt : -1, -1};
+        if (need_dx) tt += ((f.L[ly].K + 31) / 32) * ((f.L[ly].M + 15) / 16);
+    }
+    *n_tiles = tt;
+    return t;
+}
+static constexpr LayerTable make_transposed_table() {
+    int n = 0;
+    return make_transposed_table(&n);
+}
+// forward tiles (the inference table) and transposed tiles, as constants
+constexpr LayerTable kTF = make_layer_table();
+constexpr LayerTable kTB = make_transposed_table();
 constexpr TrainTable kTT = make_train_table();  // record slots and sizes, as constants
 
 // record slots of inference layer `ly`: the training layer list orders each SE
@@ -152,7 +173,7 @@ __device__ __forceinline__ void fwd_layer(const Packed<Fmt::BF16>& pk, const Rec
     const TrainLayer& l = kTT.L[tl_of(ly)];
 #pragma unroll
     for (int j = 0; j < (KB + 1) / 2; ++j) R.put(l.x_off, X[j], j, l.in, kg);
-    dense_d<Fmt::BF16, MT, KB>(pk, c_tf[ly], ly, X, Y, relu, lane);
+    dense_d<Fmt::BF16, MT, KB>(pk, kTF.L[ly], ly, X, Y, relu, lane);
     if (relu) {
 #pragma unroll
         for (int j = 0; j < MT; ++j) R.put(l.y_off, Y[j], j, l.out, kg);
@@ -441,7 +462,7 @@ __device__ __forceinline__ void back_layer(const Packed<Fmt::BF16>& pk, const Re
                                            const f32x16* y, f32x16 (&dx)[MI], int lane) {
     f32x16 dz[MO];
     act_back<MO>(R, kTT.L[tl_of(ly)], dy, y, dz, (lane >> 5) & 1);
-    dense_d<Fmt::BF16, MI, KO>(pk, c_tb[ly], ly, dz, dx, false, lane);
+    dense_d<Fmt::BF16, MI, KO>(pk, kTB.L[ly], ly, dz, dx, false, lane);
 }
 
 // What the backward of SE block blk reads: y of its three stream layers and
@@ -668,8 +689,7 @@ void train_mfma_destroy(TrainMfma* m) {
 
 int train_mfma_create(int max_batch, TrainMfma** out) {
     *out = nullptr;
-    LayerDesc F[LY_COUNT], T[LY_COUNT];
-    layer_table(F);
+    const LayerDesc* F = kTF.L;
     int nb = 0;
     if (total_tiles(F, &nb) != kTiles || nb != kBiasTiles) return fail(MRP_E_STATE, "packed weight tiles do not match the kernel's LDS layout");
     const TrainLayer* TL = kTT.L;
@@ -683,11 +703,8 @@ int train_mfma_create(int max_batch, TrainMfma** out) {
     }
     // transposed tiles for every layer whose input gradient is needed
     int tt = 0;
-    for (int ly = 0; ly < LY_COUNT; ++ly) {
-        const bool need_dx = !(ly >= LY_SE0 && ly < LY_SE1) && ly != LY_FIN1;
-        T[ly] = {F[ly].w, -1, F[ly].K, F[ly].M, need_dx ? tt : -1, -1};
-        if (need_dx) tt += ((F[ly].K + 31) / 32) * ((F[ly].M + 15) / 16);
-    }
+    (void)make_transposed_table(&tt);
+    const LayerDesc* T = kTB.L;
     if (tt != kTilesT) return fail(MRP_E_STATE, "transposed tile count does not match the kernel's LDS layout");
     std::vector<int> mt((size_t)(kTiles + kTilesT) * 512, -1), mf(kFloatSection, -1);
     for (int ly = 0; ly < LY_COUNT; ++ly) {
@@ -743,8 +760,6 @@ int train_mfma_create(int max_batch, TrainMfma** out) {
     ok = ok && hipMemcpy(m->map_floats, mf.data(), mf.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
     ok = ok && hipMemcpy((char*)m->packed + tile_bytes<Fmt::BF16>() + (size_t)kFloatSection * 4, ones.data(),
                          kScaleFloats * 4, hipMemcpyHostToDevice) == hipSuccess;
-    ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tf), F, sizeof(F)) == hipSuccess;
-    ok = ok && hipMemcpyToSymbol(HIP_SYMBOL(c_tb), T, sizeof(T)) == hipSuccess;
     if (!ok) {
         train_mfma_destroy(m);
         return fail(MRP_E_HIP, "trainer initialisation failed");
