@@ -110,7 +110,7 @@ static constexpr LayerTable make_layer_table() {
     return t;
 }
 
-static inline int total_tiles(const LayerDesc* L, int* btiles) {
+static constexpr int total_tiles(const LayerDesc* L, int* btiles) {
     int t = 0, b = 0;
     for (int i = 0; i < LY_COUNT; ++i) {
         t += ((L[i].M + 31) / 32) * ((L[i].K + 15) / 16);

@@ -51,6 +51,17 @@
 namespace mrp {
 
 constexpr LayerTable kLT = make_layer_table();  // tile and bias offsets, as constants
+static constexpr int bias_tiles_of(const LayerTable& t) {
+    int b = 0;
+    (void)total_tiles(t.L, &b);
+    return b;
+}
+static constexpr int tiles_of(const LayerTable& t) {
+    int b = 0;
+    return total_tiles(t.L, &b);
+}
+static_assert(tiles_of(kLT) == kTiles && bias_tiles_of(kLT) == kBiasTiles,
+              "the layer table's tiles must fill the kernels' LDS layout exactly");
 
 // Pre-scaled FP32 weight tiles -> E4M3 tiles, with the instruction that also
 // converts the activations.

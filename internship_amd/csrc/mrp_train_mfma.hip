@@ -61,6 +61,12 @@ static constexpr LayerTable make_transposed_table() {
 // forward tiles (the inference table) and transposed tiles, as constants
 constexpr LayerTable kTF = make_layer_table();
 constexpr LayerTable kTB = make_transposed_table();
+static constexpr int transposed_tiles() {
+    int n = 0;
+    (void)make_transposed_table(&n);
+    return n;
+}
+static_assert(transposed_tiles() == kTilesT, "the transposed tiles must fill the backward kernel's LDS layout exactly");
 constexpr TrainTable kTT = make_train_table();  // record slots and sizes, as constants
 
 // record slots of inference layer `ly`: the training layer list orders each SE
