@@ -152,8 +152,9 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 // ratio and delta tracking, so lanes in different phases wait for memory
 // together.  Measured alternatives, all slower: persistent lanes that claim
 // the next vertex when done (0.78 against 0.27 ms at 1080p), 2-8 lanes per
-// ray sharing one walk (0.35-0.64 ms), and this kernel on a second stream
-// beside the gather (both slowed by about what the overlap saved).
+// ray sharing one walk (0.35-0.64 ms), this kernel on a second stream
+// beside the gather (both slowed by about what the overlap saved), and 8
+// lookups in flight per lane (170 VGPRs, half the waves: 0.29 ms).
 __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ n_act_dev,
                                                      const float4* __restrict__ pos_act, const float4* __restrict__ fy,
