@@ -59,9 +59,14 @@ struct NNArgs {
 
 // ---------------------------------------------------------------- stage 1
 // The first free flight and the shadow ray use the batched trackers of
-// mrp_track.h: kFVBatch lookups in flight per thread, with the draws and
-// results of the one-lookup-at-a-time loops.
-constexpr int kFVBatch = 4;
+// mrp_track.h: kFVBatch / kShadowBatch lookups in flight per thread, with the
+// draws and results of the one-lookup-at-a-time loops.  The first vertex runs
+// 2M threads, so occupancy times lookups in flight sets its memory
+// parallelism: 8 (153 VGPRs, 3 waves per SIMD) took it from 0.60 to 0.56 ms
+// at 1080p against 4 (100 VGPRs, 5 waves); 12 or more put the batch in
+// scratch.  The shadow rays fill only about 4 waves per SIMD and keep 4.
+constexpr int kFVBatch = 8;
+constexpr int kShadowBatch = 4;
 constexpr int kMaxTrackSteps = 10000;
 
 // pos_act = (pos, active), fx = (X, HG phase to the light; shadow_kernel
@@ -148,7 +153,7 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 // vertex: the stream resumes where first_vertex_kernel stopped, so every draw
 // matches the one-thread-per-pixel order.  single[i] = Tr * phase (phase in
 // fx.w).  Its lookups are counted in counters[2].
-// The walk runs in rounds of kFVBatch lookups whose speculation is shared by
+// The walk runs in rounds of kShadowBatch lookups whose speculation is shared by
 // ratio and delta tracking, so lanes in different phases wait for memory
 // together.  Measured alternatives, all slower: persistent lanes that claim
 // the next vertex when done (0.78 against 0.27 ms at 1080p), 2-8 lanes per
@@ -175,7 +180,7 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
         const vec3 pos = v3(pa.x, pa.y, pa.z);
         const vec3 LX = normalize(a.light);  // the first axis frame_from_normal returns for the light
         const CellsFetch F{a.cells, a.res};
-        const float tr = transmittance_batched<kFVBatch>(&rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX,
+        const float tr = transmittance_batched<kShadowBatch>(&rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX,
                                                          ray_box_distance(pos, LX), kMaxTrackSteps, &lk);
         single[i] = tr * fx[i].w;
     }
