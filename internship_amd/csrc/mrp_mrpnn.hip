@@ -63,8 +63,9 @@ struct NNArgs {
 // draws and results of the one-lookup-at-a-time loops.  The first vertex runs
 // 2M threads, so occupancy times lookups in flight sets its memory
 // parallelism: 8 (153 VGPRs, 3 waves per SIMD) took it from 0.60 to 0.56 ms
-// at 1080p against 4 (100 VGPRs, 5 waves); 12 or more put the batch in
-// scratch.  The shadow rays fill only about 4 waves per SIMD and keep 4.
+// at 1080p against 4 (100 VGPRs, 5 waves) and 6 (127 VGPRs, 4 waves); 10 or
+// more put the batch in scratch.  The shadow rays fill only about 4 waves per
+// SIMD and keep 4.
 constexpr int kFVBatch = 8;
 constexpr int kShadowBatch = 4;
 constexpr int kMaxTrackSteps = 10000;
