@@ -169,6 +169,7 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
     __shared__ unsigned red[256];
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_act = *n_act_dev;  // the compaction's count (the grid covers the whole chunk)
+    if (blockIdx.x * blockDim.x >= n_act) return;  // uniform over the workgroup: no barrier is split
     uint32_t lk = 0u;
     if (k < n_act) {
         const uint32_t i = list[k];
@@ -244,6 +245,14 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
     constexpr int kStageItems = 8, kChunks = 3 * 2 * kStreamChunks;
     __shared__ bf16x8_t stage[kTiled ? kChunks * kStageItems : 1];
     float* __restrict__ feat = (float*)feat_out;
+    // a grid sized for n_items ends early.  Wave-uniform (readfirstlane), so the
+    // loop does not wait at every item for this load, which on CDNA would also
+    // wait for the previous item's stores (one counter for both).  Read first:
+    // the grid covers the whole chunk, and most workgroups have no item, so they
+    // leave before the per-tap set-up (uniform over the workgroup: no barrier
+    // is split).
+    const uint32_t n_items = __builtin_amdgcn_readfirstlane(a.n_dev ? *a.n_dev : a.n_items);
+    if (blockIdx.x * kGatherItems >= n_items) return;
     const int i = threadIdx.x;
     const int mip = c_tap_mip[i];
     const float off = c_tap_offset[i];
@@ -259,10 +268,6 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
         h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, c_tap_u_view[i], lut_v);
     else if (i >= kShellTaps)
         h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, dot(a.lx, a.light) * 0.5f + 0.5f, lut_v);
-    // a grid sized for n_items ends early.  Wave-uniform (readfirstlane), so the
-    // loop does not wait at every item for this load, which on CDNA would also
-    // wait for the previous item's stores (one counter for both).
-    const uint32_t n_items = __builtin_amdgcn_readfirstlane(a.n_dev ? *a.n_dev : a.n_items);
     // The vertex a descriptor belongs to (compacted list) and its record are
     // uniform over the block (scalar loads).  They are prefetched: the list
     // entry two items ahead, the record one item ahead, so an item's taps
