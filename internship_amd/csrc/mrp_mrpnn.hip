@@ -223,7 +223,8 @@ __constant__ int c_seg_size[12] = {8, 8, 16, 16, 16, 32, 32, 32, 8, 8, 8, 8};
 // 96..167 store 16 bytes at a time from the LDS copy of the descriptor.
 // One workgroup (192 threads, one tap each) loops over kGatherItems points, so
 // the dispatcher launches 1/kGatherItems as many workgroups; the LDS row is
-// double-buffered, one barrier per point.
+// double-buffered, one barrier per point.  At 1080p (256k descriptors) 16
+// took 0.59 ms, 8 took 0.61 ms and 32 took 0.62 ms.
 constexpr int kGatherItems = 16;
 static_assert(kGatherItems % 8 == 0, "the tiled gather flushes groups of 8 descriptors");
 
