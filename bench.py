@@ -328,7 +328,9 @@ def mlp_rooflines(k, precision=1, ceiling=None):
     r["gather_kernel"] = {"bound": "hbm", "achieved": ga, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": ga / HBM_PEAK_GBS, "traffic": hbm_traffic("gather_kernel", MLP_WORKLOAD),
                           "kernel": "gather_kernel", "kernel_ms": k["gather_ms"],
-                          "bytes_per_descriptor": GATHER_BYTES_PER_POINT}
+                          "bytes_per_descriptor": GATHER_BYTES_PER_POINT,
+                          "note": "achieved counts the cell bytes the descriptors read; most come from L2 "
+                                  "and the last-level cache, and traffic is the HBM bytes per launch"}
     dominant = max(r, key=lambda n: r[n]["kernel_ms"])
     return r[dominant], r
 
