@@ -259,12 +259,42 @@ int mrp_volume_set_network(mrp_volume* vol, const mrp_network* net);
 int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
                        const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
                        uint64_t seed, float* feat_dev, float* gamma_dev);
+/* The gather as the BF16/FP8 renderer runs it (the same part of
+ * `RadiancePredict`, core/radiancePredict.cu:319-368), on caller-given points:
+ *   tiles  ceil(n/32) x 36,864 bytes: the 576 features of each descriptor
+ *          rounded to BF16 and laid out as the MFMA network's B operands
+ *          (tiles of 32 descriptors; per stream k (density, transmittance,
+ *          phase) and 16-feature chunk c, the 16-byte chunk of lane 32 kg + d
+ *          holds features 16c + 8h + 4kg + e, h, e in 0..1 / 0..3, of
+ *          descriptor d); items past n in the last tile are not written
+ *   pool   n x 72 FP32: the squeeze-excitation blocks' per-segment means
+ *          (0..35) and maxima (36..71), from the FP32 features (`AVGMAX`,
+ *          core/radiancePredict.cu:21-33)
+ *   gamma  n FP32
+ * The density and transmittance lookups read the cell format chosen with
+ * mrp_volume_set_gather_cells (FP32 unless set). */
+int mrp_mrpnn_features_tiled(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
+                             const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha,
+                             float g, uint64_t seed, void* tiles_dev, float* pool_dev, float* gamma_dev);
+/* Cell format of the BF16/FP8 renderers' gather: 32 (default) reads the FP32
+ * corners of the density and transmittance mips, as the reference's FP32
+ * textures hold them (core/volume.cu:1289-1298,1412-1417); 16 reads FP16
+ * copies (one 16-byte load per lookup instead of two). The FP32 renderer always
+ * reads FP32 cells. No counterpart in the reference. */
+int mrp_volume_set_gather_cells(mrp_volume* vol, int bits);
 /* The network on n descriptors (HBM): out[n][3] = max(exp(P_c)-1, 0)*srp_c,
  * srp = (scatter_rate/1.001)^4 per channel.  precision 0 = FP32, 1 = BF16 MFMA,
  * 2 = FP8-E4M3 MFMA (power-of-two scales: per layer for the weights, per layer
  * and descriptor for the activations; FP32 accumulation, FP32 bias). */
 int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
                     mrp_vec3 srp, int precision, float* out_dev);
+
+/* The MFMA network (precision 1 = BF16, 2 = FP8) on descriptors in the tiled
+ * layout of mrp_mrpnn_features_tiled, with their pooled statistics: the
+ * second half of `RadiancePredict` (core/radiancePredict.cu:370-486) as the
+ * renderer runs it. out_dev: n x 3 FP32. */
+int mrp_mlp_forward_tiled(const mrp_network* net, int n, const void* tiles_dev, const float* pool_dev,
+                          const float* gamma_dev, float g, mrp_vec3 srp, int precision, float* out_dev);
 
 /* --------------------------------------------------------------- training */
 /* Training of the MRPNN network on the GPU (forward, backward, Adam), derived

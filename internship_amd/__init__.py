@@ -145,6 +145,13 @@ class Network:
                                        out.ptr))
         return out
 
+    def forward_tiled(self, tiles: "DeviceBuffer", pool: "DeviceBuffer", gamma: "DeviceBuffer", n: int, g: float,
+                      srp, precision=1, out: "DeviceBuffer" = None) -> "DeviceBuffer":
+        out = out or DeviceBuffer(max(1, n) * 12)
+        check(self.lib.mrp_mlp_forward_tiled(self.h, int(n), tiles.ptr, pool.ptr, gamma.ptr, float(g), Vec3.of(srp),
+                                             int(precision), out.ptr))
+        return out
+
     def close(self):
         if self.h is not None and self.h.value:
             self.lib.mrp_network_destroy(self.h)
@@ -207,6 +214,21 @@ class Trainer:
             self.close()
         except Exception:
             pass
+
+
+TILE_BYTES = 3 * 12 * 64 * 16  # one tile: 32 descriptors x 576 BF16 features
+
+
+def untile_descriptors(raw: np.ndarray, n: int) -> np.ndarray:
+    """Wave-tiled BF16 descriptors (mrp_mrpnn_features_tiled) -> (n, 576) float32.
+    Chunk (tile t, stream k, chunk c, lane 32 kg + d) holds the 8 BF16 features
+    16c + 8h + 4kg + e (h in 0..1, e in 0..3) of descriptor 32 t + d."""
+    t = (n + 31) // 32
+    u16 = np.frombuffer(np.ascontiguousarray(raw).tobytes()[:t * TILE_BYTES], dtype=np.uint16)
+    ch = u16.reshape(t, 3, 12, 2, 32, 2, 4)  # tile, stream, chunk, kg, d, h, e
+    f = ch.transpose(0, 4, 1, 2, 5, 3, 6)  # tile, d, stream, chunk, h, kg, e
+    bits = f.reshape(t * 32, 576)[:n].astype(np.uint32) << 16
+    return bits.view(np.float32)
 
 
 def srp_of(scatter_rate):
@@ -362,6 +384,22 @@ class VolumeRender:
         check(self.lib.mrp_mrpnn_features(self.h, int(n), pos_act.ptr, fx.ptr, fy.ptr, fz.ptr, Vec3.of(light_dir),
                                           float(alpha), float(g), ctypes.c_uint64(seed), feat.ptr, gamma.ptr))
         return feat, gamma
+
+    def mrpnn_features_tiled(self, pos_act: DeviceBuffer, fx: DeviceBuffer, fy: DeviceBuffer, fz: DeviceBuffer,
+                             n: int, light_dir, alpha, g, seed=0):
+        """The BF16/FP8 renderer's gather: (tiles, pool, gamma) device buffers;
+        decode the tiles with `untile_descriptors`."""
+        tiles = DeviceBuffer(max(1, (n + 31) // 32) * TILE_BYTES)
+        pool = DeviceBuffer(max(1, n) * 72 * 4)
+        gamma = DeviceBuffer(max(1, n) * 4)
+        check(self.lib.mrp_mrpnn_features_tiled(self.h, int(n), pos_act.ptr, fx.ptr, fy.ptr, fz.ptr,
+                                                Vec3.of(light_dir), float(alpha), float(g), ctypes.c_uint64(seed),
+                                                tiles.ptr, pool.ptr, gamma.ptr))
+        return tiles, pool, gamma
+
+    def set_gather_cells(self, bits: int):
+        """32 (FP32 cells, default) or 16 (FP16 cells) for the BF16/FP8 renderers' gather."""
+        check(self.lib.mrp_volume_set_gather_cells(self.h, int(bits)))
 
     def set_sample_buffer_budget(self, nbytes: int):
         check(self.lib.mrp_set_sample_buffer_budget(self.h, ctypes.c_size_t(int(nbytes))))

@@ -120,6 +120,9 @@ SIGNATURES = {
     "mrp_volume_set_network": (I, [P, P]),
     "mrp_mrpnn_features": (I, [P, I, P, P, P, P, Vec3, F, F, ctypes.c_uint64, P, P]),
     "mrp_mlp_forward": (I, [P, I, P, P, F, Vec3, I, P]),
+    "mrp_mrpnn_features_tiled": (I, [P, I, P, P, P, P, Vec3, F, F, ctypes.c_uint64, P, P, P]),
+    "mrp_volume_set_gather_cells": (I, [P, I]),
+    "mrp_mlp_forward_tiled": (I, [P, I, P, P, P, F, Vec3, I, P]),
 }
 
 _lib = None

@@ -51,10 +51,11 @@ struct Volume {
     float4* tr_cells[kNumTrMips] = {};  // clamp cells of the TR mips, (r+2)^3 x 32 B
     float4* lut_cells = nullptr;        // bilinear clamp cells of the HG table, 514^2 x 16 B
     // FP16 copies of the cells (8 corners in 16 B), every level in one buffer
-    // (level m at mip16_offset / tr16_offset cells), read by the gather of the
-    // BF16/FP8 renderer, whose descriptors are rounded to BF16 anyway
+    // (level m at mip16_offset / tr16_offset cells), read by the BF16/FP8
+    // renderer's gather when the volume opts in to FP16 cells
     uint4* mip_cells16 = nullptr;
     uint4* tr_cells16 = nullptr;
+    bool gather_cells16 = false;  // the tiled gather reads the FP16 cells (mrp_volume_set_gather_cells(16))
     bool lut_valid = false;
     float lut_g = -100.0f;
     uint64_t lut_seed = 0;

@@ -232,10 +232,11 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 
 // The gather is bound by memory latency per descriptor, so occupancy matters:
 // it is held to 8 waves per SIMD (at most 64 VGPRs); at 76 VGPRs and 6 waves
-// it ran 7 % slower than at 7.  The tiled (BF16/FP8 renderer) gather reads FP16
-// cells, one 16-byte load per lookup, whose values round below the BF16 the
-// descriptors are stored in; the FP32-row gather reads the FP32 cells.
-template <bool kTiled>
+// it ran 7 % slower than at 7.  kCells16 reads FP16 copies of the density and
+// transmittance cells (one 16-byte load per lookup) instead of the FP32 cells
+// (two); the FP32 cells are the default of every renderer, the FP16 ones an
+// opt-in variant of the tiled (BF16/FP8) gather.
+template <bool kTiled, bool kCells16>
 __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
@@ -305,8 +306,8 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
             } else {
                 pi = p + a.light * off;
             }
-            const float d = kTiled ? trilinear_cells16(a.mips16 + mip16_offset(mip), 256 >> mip, pi)
-                                   : trilinear_cells(a.mips[mip], 256 >> mip, pi);
+            const float d = kCells16 ? trilinear_cells16(a.mips16 + mip16_offset(mip), 256 >> mip, pi)
+                                     : trilinear_cells(a.mips[mip], 256 >> mip, pi);
             xd = logf_(1.0f + a.alpha * d / 64.0f);
             float harg;
             if (i == 0) {
@@ -326,8 +327,8 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
                 pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f;
             const float o = ray_box_offset(pi, a.lx);
             const vec3 q = outside ? (pi + a.lx * o) + 0.5f : pi + 0.5f;
-            const float tv = kTiled ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, q)
-                                    : trilinear_clamp_cells(a.trs[tm], tr, q);
+            const float tv = kCells16 ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, q)
+                                      : trilinear_clamp_cells(a.trs[tm], tr, q);
             xt = outside && o < 0.0f ? 1.0f : tv;
         }
         if (i == 0) gamma[item] = acosf_(dot(X, a.lx));  // after the lookups: stores delay later waits
@@ -522,6 +523,18 @@ static vec3 srp_of(vec3 scatter_rate) {
     return v3((float)std::pow((double)sr.x, 4.0), (float)std::pow((double)sr.y, 4.0), (float)std::pow((double)sr.z, 4.0));
 }
 
+// The tiled (BF16/FP8 network) gather on the volume's cell format.
+static int launch_gather_tiled(Volume* v, const GatherArgs& ga, unsigned blocks, const float4* pos_act,
+                               const float4* fx, const float4* fy, const float4* fz, const uint32_t* list,
+                               void* tiles, float* gamma, float* pool) {
+    if (v->gather_cells16)
+        gather_kernel<true, true><<<blocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, tiles, gamma, pool);
+    else
+        gather_kernel<true, false><<<blocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, tiles, gamma, pool);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
+}
+
 // First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
 // of the pixels `map` selects, in chunks of at most 4M vertices.
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
@@ -633,10 +646,11 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
             // of kTileBytes fit in n FP32 rows for n >= 32); the FP32 network reads FP32 rows
             const unsigned gblocks = (cnt + kGatherItems - 1) / kGatherItems;  // groups past the count exit at once
             if (p->mlp_precision == 0)
-                gather_kernel<false><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam, pool);
+                gather_kernel<false, false><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam,
+                                                                              pool);
             else
-                gather_kernel<true><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam, pool);
-            MRP_HIP(hipGetLastError());
+                rc = launch_gather_tiled(v, ga, gblocks, pos_act, fx, fy, fz, list, feat, gam, pool);
+            if (rc) return rc;
         }
         MRP_HIP(hipEventRecord(t2, v->stream));
         if (p->mlp_precision == 0) {
@@ -795,13 +809,13 @@ int mrp_network_destroy(mrp_network* n) {
     return MRP_OK;
 }
 
-int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
-                       const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
-                       uint64_t seed, float* feat_dev, float* gamma_dev) {
-    if (!vol || !pos_act_dev || !frame_x_dev || !frame_y_dev || !frame_z_dev || !feat_dev || !gamma_dev)
-        return fail(MRP_E_INVALID, "NULL argument");
+// Shared set-up of the gather entry points on caller-given points: checks,
+// tap tables, transmittance mips and HG table, and the gather's arguments.
+static int gather_setup(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
+                        const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
+                        uint64_t seed, GatherArgs* ga) {
+    if (!vol || !pos_act_dev || !frame_x_dev || !frame_y_dev || !frame_z_dev) return fail(MRP_E_INVALID, "NULL argument");
     if (n < 0) return fail(MRP_E_INVALID, "n must be >= 0");
-    if (n == 0) return MRP_OK;
     Volume* v = &vol->v;
     MRP_HIP(hipSetDevice(v->device));
     if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
@@ -811,23 +825,72 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     if (rc) return rc;
     rc = ensure_hg_lut(v, g, seed);
     if (rc) return rc;
+    for (int m = 0; m < kNumMips; ++m) ga->mips[m] = v->mip_cells[m];
+    for (int m = 0; m < kNumTrMips; ++m) ga->trs[m] = v->tr_cells[m];
+    ga->lut = v->lut_cells;
+    ga->mips16 = v->mip_cells16;
+    ga->trs16 = v->tr_cells16;
+    ga->light = normalize(V(light_dir));
+    ga->lx = normalize(ga->light);
+    ga->alpha = alpha;
+    ga->g = g;
+    ga->n_items = (uint32_t)n;
+    ga->n_dev = nullptr;
+    return MRP_OK;
+}
+
+int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
+                       const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha, float g,
+                       uint64_t seed, float* feat_dev, float* gamma_dev) {
+    if (!feat_dev || !gamma_dev) return fail(MRP_E_INVALID, "NULL argument");
     GatherArgs ga;
-    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mip_cells[m];
-    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_cells[m];
-    ga.lut = v->lut_cells;
-    ga.mips16 = v->mip_cells16;
-    ga.trs16 = v->tr_cells16;
-    ga.light = normalize(V(light_dir));
-    ga.lx = normalize(ga.light);
-    ga.alpha = alpha;
-    ga.g = g;
-    ga.n_items = (uint32_t)n;
-    ga.n_dev = nullptr;
-    gather_kernel<false><<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
+    int rc = gather_setup(vol, n, pos_act_dev, frame_x_dev, frame_y_dev, frame_z_dev, light_dir, alpha, g, seed, &ga);
+    if (rc || n == 0) return rc;
+    Volume* v = &vol->v;
+    gather_kernel<false, false><<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
         ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev, (const float4*)frame_y_dev,
         (const float4*)frame_z_dev, nullptr, feat_dev, gamma_dev, nullptr);
     MRP_HIP(hipGetLastError());
     MRP_HIP(hipStreamSynchronize(v->stream));
+    return MRP_OK;
+}
+
+int mrp_mrpnn_features_tiled(mrp_volume* vol, int n, const float* pos_act_dev, const float* frame_x_dev,
+                             const float* frame_y_dev, const float* frame_z_dev, mrp_vec3 light_dir, float alpha,
+                             float g, uint64_t seed, void* tiles_dev, float* pool_dev, float* gamma_dev) {
+    if (!tiles_dev || !pool_dev || !gamma_dev) return fail(MRP_E_INVALID, "NULL argument");
+    GatherArgs ga;
+    int rc = gather_setup(vol, n, pos_act_dev, frame_x_dev, frame_y_dev, frame_z_dev, light_dir, alpha, g, seed, &ga);
+    if (rc || n == 0) return rc;
+    Volume* v = &vol->v;
+    rc = launch_gather_tiled(v, ga, (unsigned)((n + kGatherItems - 1) / kGatherItems), (const float4*)pos_act_dev,
+                             (const float4*)frame_x_dev, (const float4*)frame_y_dev, (const float4*)frame_z_dev,
+                             nullptr, tiles_dev, gamma_dev, pool_dev);
+    if (rc) return rc;
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    return MRP_OK;
+}
+
+int mrp_volume_set_gather_cells(mrp_volume* vol, int bits) {
+    if (!vol) return fail(MRP_E_INVALID, "NULL volume");
+    if (bits != 16 && bits != 32) return fail(MRP_E_INVALID, "gather cells must be 32 (FP32) or 16 (FP16) bits");
+    vol->v.gather_cells16 = bits == 16;
+    return MRP_OK;
+}
+
+int mrp_mlp_forward_tiled(const mrp_network* net, int n, const void* tiles_dev, const float* pool_dev,
+                          const float* gamma_dev, float g, mrp_vec3 srp, int precision, float* out_dev) {
+    if (!net || !tiles_dev || !pool_dev || !gamma_dev || !out_dev) return fail(MRP_E_INVALID, "NULL argument");
+    if (n < 0) return fail(MRP_E_INVALID, "n must be >= 0");
+    if (precision != 1 && precision != 2) return fail(MRP_E_INVALID, "tiled descriptors: precision 1 (BF16) or 2 (FP8)");
+    if (n == 0) return MRP_OK;
+    MRP_HIP(hipSetDevice(net->device));
+    const void* packed = precision == 2 ? net->packed_fp8 : net->packed;
+    if (!packed) return fail(MRP_E_STATE, "network has no MFMA packing");
+    const int rc = launch_mlp(packed, precision - 1, nullptr, tiles_dev, pool_dev, gamma_dev, g, V(srp), (uint32_t)n,
+                              nullptr, out_dev, nullptr);
+    if (rc) return rc;
+    MRP_HIP(hipDeviceSynchronize());
     return MRP_OK;
 }
 

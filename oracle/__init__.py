@@ -44,6 +44,19 @@ def uniforms(pid, sample, tag, seed, n):
     return out
 
 
+def frame_from_normal(v, sample=0, tag=3, seed=0):
+    """`GetMatrixFromNormal` for n directions (frame i on stream (i, sample, tag, seed)).
+    Returns (frames (n,3,3): rows a1, a2, a3; draws (n,))."""
+    v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+    n = v.shape[0]
+    out = np.zeros((n, 3, 3), dtype=np.float32)
+    draws = np.zeros(n, dtype=np.int32)
+    rc = load().oracle_frame_from_normal(_ptr(v), ctypes.c_int(n), ctypes.c_uint32(sample), ctypes.c_uint32(tag),
+                                         ctypes.c_uint64(seed), _ptr(out), _ptr(draws))
+    assert rc == 0
+    return out, draws
+
+
 MATH_FN = {"log": 0, "exp": 1, "sin": 2, "cos": 3, "atan": 4, "acos": 5, "hg": 6, "hg_cos": 7}
 
 

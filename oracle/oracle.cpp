@@ -276,6 +276,22 @@ int oracle_uniforms(uint32_t id, uint32_t sample, uint32_t tag, uint64_t seed, i
     return 0;
 }
 
+// `GetMatrixFromNormal` (core/render.cuh:355-369) for n directions: frame i
+// draws from Philox stream (i, sample, tag, seed).  out: n x 9 (a1, a2, a3);
+// draws: the number of uniforms each frame consumed (2 per rejection try).
+int oracle_frame_from_normal(const float* v, int n, uint32_t sample, uint32_t tag, uint64_t seed, float* out,
+                             int* draws) {
+    for (int i = 0; i < n; ++i) {
+        Rng r = rng_make((uint32_t)i, sample, tag, seed);
+        vec3 a1, a2, a3;
+        frame_from_normal(&r, v3(v[3 * i], v[3 * i + 1], v[3 * i + 2]), &a1, &a2, &a3);
+        const float f[9] = {a1.x, a1.y, a1.z, a2.x, a2.y, a2.z, a3.x, a3.y, a3.z};
+        for (int k = 0; k < 9; ++k) out[9 * i + k] = f[k];
+        if (draws) draws[i] = (int)r.n;
+    }
+    return 0;
+}
+
 // Evaluate one shared scalar function over an array (pinning hook).
 int oracle_math(int fn, const float* a, const float* b, float* out, int n) {
     for (int i = 0; i < n; ++i) {

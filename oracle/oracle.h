@@ -29,6 +29,12 @@
  *                                                                    finite differences of its loss
  * oracle_reproject         core/volume.cu:125-141                    static camera maps to itself; pinhole
  *                                                                    projection into a moved camera (numpy)
+ * oracle_frame_from_normal core/render.cuh:355-369                   orthonormal, first axis = input, handedness,
+ *                                                                    second axis uniform about the first (KS)
+ * oracle_render_mrpnn      core/render.cuh:469-523, volume.cu:147-179 zero network = single-scatter FP64
+ *                                                                    quadrature per channel (light colour, albedo);
+ *                                                                    channel ratios; static-camera MRPNN frames =
+ *                                                                    closed-form blend of 1-frame renders
  * Later sections of oracle.cpp list their own pins.
  */
 #ifndef MRP_ORACLE_H_
@@ -45,6 +51,8 @@ extern "C" {
 int oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
 int oracle_normalize3(const float* v, float* out);
 int oracle_uniforms(uint32_t id, uint32_t sample, uint32_t tag, uint64_t seed, int n, float* out);
+int oracle_frame_from_normal(const float* v, int n, uint32_t sample, uint32_t tag, uint64_t seed, float* out,
+                             int* draws);
 int oracle_math(int fn, const float* a, const float* b, float* out, int n);
 int oracle_ray_box(const float* p, const float* d, float* offset, float* distance_out, int n);
 int oracle_hg_sample_dir(const float* e0, const float* e1, const float* v, float g, float* out, int n);
