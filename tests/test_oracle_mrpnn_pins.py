@@ -295,11 +295,15 @@ def _relu(x):
     return np.maximum(x, 0)
 
 
-def forward64(W, feat, gamma, g, srp, quant=None):
+def forward64(W, feat, gamma, g, srp, quant=None, pool_feat=None):
     """FP64 restatement of RadiancePredict's network (core/radiancePredict.cu:370-486) in matrix form.
     quant(name, W, x) -> (W', x'), if given, replaces a layer's weights and
-    input before its product (reduced-precision emulation)."""
+    input before its product (reduced-precision emulation).  pool_feat, if
+    given, supplies the features the squeeze-excitation statistics are taken
+    from (the tiled renderer pools the FP32 features but streams BF16 ones)."""
     n = feat.shape[0]
+    pf = feat if pool_feat is None else pool_feat
+    pin = [pf[:, 0:192].astype(np.float64), pf[:, 192:384].astype(np.float64), pf[:, 384:576].astype(np.float64)]
 
     def lin(wname, x, bname=None):
         Wm = W[wname]
@@ -318,8 +322,9 @@ def forward64(W, feat, gamma, g, srp, quant=None):
     def se(nm, sew0, sew1, res_from, src, dst, size, pid, tag):
         seg = [xin[k][:, src:src + size] for k in range(3)]
         for k in range(3):
-            pool[:, pid + k] = seg[k].mean(axis=1)
-            pool[:, 36 + pid + k] = np.maximum(seg[k].max(axis=1), -10000.0)
+            pseg = pin[k][:, src:src + size]
+            pool[:, pid + k] = pseg.mean(axis=1)
+            pool[:, 36 + pid + k] = np.maximum(pseg.max(axis=1), -10000.0)
         t = np.concatenate([pool[:, pid:pid + 3], pool[:, 36 + pid:39 + pid],
                             np.full((n, 1), g), gamma[:, None].astype(np.float64)], axis=1)
         h = _relu(lin(sew0, t))
