@@ -296,6 +296,17 @@ int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const 
 int mrp_mlp_forward_tiled(const mrp_network* net, int n, const void* tiles_dev, const float* pool_dev,
                           const float* gamma_dev, float g, mrp_vec3 srp, int precision, float* out_dev);
 
+/* mrp_mlp_forward without allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The BF16/FP8
+ * networks keep the descriptors' pooled statistics in a fixed 18 MiB device
+ * array of the library (one per GPU, held until the process ends) instead of a
+ * buffer allocated per call; n is limited to 65,536 descriptors (larger n
+ * returns MRP_E_INVALID).  Calls on one GPU share that array: calls that
+ * overlap on one GPU race.  Same kernels and results as mrp_mlp_forward. */
+int mrp_mlp_forward_noalloc(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
+                            mrp_vec3 srp, int precision, float* out_dev);
+
 /* --------------------------------------------------------------- training */
 /* Training of the MRPNN network on the GPU (forward, backward, Adam), derived
  * from the inference network (core/radiancePredict.cu:370-486); the

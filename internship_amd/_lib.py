@@ -123,6 +123,7 @@ SIGNATURES = {
     "mrp_mrpnn_features_tiled": (I, [P, I, P, P, P, P, Vec3, F, F, ctypes.c_uint64, P, P, P]),
     "mrp_volume_set_gather_cells": (I, [P, I]),
     "mrp_mlp_forward_tiled": (I, [P, I, P, P, P, F, Vec3, I, P]),
+    "mrp_mlp_forward_noalloc": (I, [P, I, P, P, F, Vec3, I, P]),
 }
 
 _lib = None

@@ -918,6 +918,63 @@ int mrp_mlp_forward(const mrp_network* net, int n, const float* feat_dev, const 
     return MRP_OK;
 }
 
+// Allocation-free network entry point.  The pooled squeeze-excitation
+// statistics of the descriptors (72 floats each), which mrp_mlp_forward puts
+// in a buffer it allocates per call, go to this fixed device array instead.
+// Every GPU the library runs on holds its own copy (18 MiB) until the process
+// ends; calls on one GPU share it, so calls that overlap on one GPU race.
+// Every entry a call reads is written by the same call first, so nothing is
+// cleared between calls.
+constexpr uint32_t kNoAllocMaxItems = 65536;
+static __device__ __align__(256) float scratch[(size_t)kNoAllocMaxItems * 72];
+static float* g_scratch_dev[64] = {};
+static std::mutex g_scratch_mutex;
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the host-side name is not a device address).
+static int scratch_address(int device, float** out) {
+    if (device < 0 || device >= 64) return fail(MRP_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_dev[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!ptr) return fail(MRP_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+        g_scratch_dev[device] = (float*)ptr;
+    }
+    *out = g_scratch_dev[device];
+    return MRP_OK;
+}
+
+int mrp_mlp_forward_noalloc(const mrp_network* net, int n, const float* feat_dev, const float* gamma_dev, float g,
+                            mrp_vec3 srp, int precision, float* out_dev) {
+    if (!net || !feat_dev || !gamma_dev || !out_dev) return fail(MRP_E_INVALID, "NULL argument");
+    if (n < 0) return fail(MRP_E_INVALID, "n must be >= 0");
+    if ((uint32_t)n > kNoAllocMaxItems)
+        return fail(MRP_E_INVALID, "n exceeds 65536, the descriptors the allocation-free scratch holds");
+    if (precision < 0 || precision > 2) return fail(MRP_E_INVALID, "precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
+    if (n == 0) return MRP_OK;
+    MRP_HIP(hipSetDevice(net->device));
+    const vec3 s = V(srp);
+    if (precision == 0) {
+        mlp_fp32_kernel<<<(n + 63) / 64, 64>>>(net->w, feat_dev, gamma_dev, g, s, (uint32_t)n, nullptr, out_dev);
+        MRP_HIP(hipGetLastError());
+    } else {
+        const void* packed = precision == 2 ? net->packed_fp8 : net->packed;
+        if (!packed) return fail(MRP_E_STATE, "network has no MFMA packing");
+        float* pool = nullptr;
+        int rc = scratch_address(net->device, &pool);
+        if (rc) return rc;
+        rc = launch_pool(feat_dev, (uint32_t)n, pool, nullptr);
+        if (rc) return rc;
+        rc = launch_mlp(packed, precision - 1, feat_dev, nullptr, pool, gamma_dev, g, s, (uint32_t)n, nullptr, out_dev,
+                        nullptr);
+        if (rc) return rc;
+    }
+    MRP_HIP(hipDeviceSynchronize());
+    return MRP_OK;
+}
+
 int mrp_volume_set_network(mrp_volume* vol, const mrp_network* n) {
     if (!vol) return fail(MRP_E_INVALID, "NULL volume");
     vol->v.nn_weights = n ? n->w : nullptr;
