@@ -46,6 +46,7 @@ FP32_PEAK_TFLOPS = 157.3
 BYTES_PER_LOOKUP = 32          # eight FP32 corners of one trilinear density lookup
 MLP_FLOP_PER_PASS = 97072      # 48,536 MACs per descriptor per colour pass
 GATHER_BYTES_PER_POINT = 18400  # 192 x 32 (density) + 192 x 32 (transmittance) + 382 x 16 (HG table)
+GATHER_BYTES_PER_POINT_16 = 12256  # the same with 16-byte FP16 density and transmittance cells
 FRAME_POST_BYTES_PER_PIXEL = 12 + 132 + 132 + 12 + 4  # update: target + histogram r/w; denoise: read both; pack
 MLP_WORKLOAD = "cloud512_1080p_1spp"
 TRACER_WORKLOAD = "cloud512_1080p_%dspp_per_launch"
@@ -174,32 +175,75 @@ def equal_rmse_search(sc, target_rmse, ref, max_spp):
     return hi, hist, True
 
 
-def cpu_baseline(sc, target_s=15.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sc, target_s=15.0, single_s=8.0):
     """Time the CPU oracle (the reference tracer restated in C++/OpenMP) on a
-    bounded sample of the same scene: a band of rows of one 1-spp frame, or
-    whole 1-spp frames when one frame takes less than the target; frames/s."""
+    bounded sample of the same scene: whole 1-spp frames, as many as fit the
+    target time; frames/s.  Threads: every core the process may run on
+    (sched_getaffinity), bounded by OMP_NUM_THREADS when the job sets it (the
+    GPU box gives a one-GPU job 16 cores).  Beside it a single-thread figure
+    from a band of rows: its lookup rate over the lookups of a whole frame."""
     import oracle
-    threads = max(1, min(16, os.cpu_count() or 1))
+    from internship_amd.build import GXX_FLAGS
+    visible = len(os.sched_getaffinity(0))
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = min(visible, omp) if omp > 0 else visible
     grid = sc.grid.download(np.float32, (sc.res, sc.res, sc.res))
     md = sc.vol.max_density
-    r0 = sc.H // 2 - 2
-    t0 = time.perf_counter()
-    oracle.render(grid, md, scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED), threads=threads,
-                  rows=(r0, r0 + 4))
-    probe = time.perf_counter() - t0
-    rows = int(max(4, min(sc.H, 4 * target_s / max(probe, 1e-3))))
-    frames = 1
-    if rows == sc.H:
-        frames = int(max(1, min(64, target_s / max(probe * sc.H / 4, 1e-3))))
-    r0 = max(0, sc.H // 2 - rows // 2)
-    p = scene_params(sc.ia, sc.W, sc.H, 0, frames, sc.bounces, DT_SEED)
-    t0 = time.perf_counter()
-    _, lookups, _ = oracle.render(grid, md, p, threads=threads, rows=(r0, r0 + rows))
-    dt = time.perf_counter() - t0
-    return {"value": frames * (rows / sc.H) / dt, "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": "plain delta-tracking tracer, CPU oracle (C++/OpenMP, %d threads), rows %d..%d of %d "
-                      "1-spp %dx%d frame(s) (%.1f s, %.3g lookups/s), scaled to full frames"
-                      % (threads, r0, r0 + rows, frames, sc.W, sc.H, dt, lookups / dt)}
+
+    def frames(nthreads, budget_s):
+        # whole 1-spp frames: one to size the sample, then enough for the budget
+        t0 = time.perf_counter()
+        _, lk0, _ = oracle.render(grid, md, scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED),
+                                  threads=nthreads)
+        t1 = time.perf_counter() - t0
+        n = int(max(0, min(63, round(budget_s / max(t1, 1e-3)) - 1)))
+        lk = 0
+        if n:
+            t0 = time.perf_counter()
+            _, lk, _ = oracle.render(grid, md, scene_params(sc.ia, sc.W, sc.H, 1, n, sc.bounces, DT_SEED),
+                                     threads=nthreads)
+            t1 += time.perf_counter() - t0
+        return (n + 1) / t1, n + 1, t1, lk0 + lk
+
+    def band(nthreads, budget_s):
+        # rows through the middle of the cloud, sized for the budget
+        r0 = sc.H // 2 - 2
+        t0 = time.perf_counter()
+        oracle.render(grid, md, scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED), threads=nthreads,
+                      rows=(r0, r0 + 4))
+        probe = time.perf_counter() - t0
+        rows = int(max(4, min(sc.H, 4 * budget_s / max(probe, 1e-3))))
+        r0 = max(0, sc.H // 2 - rows // 2)
+        t0 = time.perf_counter()
+        _, lookups, _ = oracle.render(grid, md, scene_params(sc.ia, sc.W, sc.H, 0, 1, sc.bounces, DT_SEED),
+                                      threads=nthreads, rows=(r0, r0 + rows))
+        return r0, rows, time.perf_counter() - t0, lookups
+
+    fps, nf, dt, lookups = frames(threads, target_s)
+    lk_per_frame = lookups / nf
+    r01, rows1, dt1, lookups1 = band(1, single_s)
+    fps1 = lookups1 / dt1 / lk_per_frame
+    return {"value": fps, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": "plain delta-tracking tracer, CPU oracle (C++/OpenMP, %d threads), %d whole 1-spp %dx%d "
+                      "frames, %d bounces (%.1f s, %.3g lookups/s)"
+                      % (threads, nf, sc.W, sc.H, sc.bounces, dt, lookups / dt),
+            "single_thread": {"value": fps1, "unit": "frames/s",
+                              "sample": "1 thread, rows %d..%d of one frame (%.1f s, %.3g lookups/s); frames/s = "
+                                        "that lookup rate over the %.3g lookups of a whole frame"
+                                        % (r01, r01 + rows1, dt1, lookups1 / dt1, lk_per_frame)},
+            "cores_visible": visible, "omp_num_threads": omp or None, "cpu_model": cpu_model(),
+            "compiler": "g++ " + " ".join(f for f in GXX_FLAGS if not f.startswith("-I"))}
 
 
 CEILING_BUFFER = 4 << 30  # about the size of the 512^3 cell array (4.3 GB)
@@ -259,7 +303,7 @@ def run_tracer(a, ia, sc):
     }
 
 
-def hbm_traffic(kernel, workload):
+def hbm_traffic(kernel, workload, field="bytes_per_launch"):
     """HBM bytes per launch of `kernel` on `workload`, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE passes summarised in profiles/hbm_traffic.json
     (each counter scaled by its reading for a copy kernel of known size)."""
@@ -269,7 +313,7 @@ def hbm_traffic(kernel, workload):
     except (OSError, ValueError):
         return None
     rec = table.get(kernel, {}).get(workload)
-    return None if rec is None else rec.get("bytes_per_launch")
+    return None if rec is None else rec.get(field)
 
 
 def time_mlp(a, ia, sc):
@@ -302,14 +346,15 @@ def time_mlp(a, ia, sc):
     return elapsed, k
 
 
-def mlp_rooflines(k, precision=1, ceiling=None):
+def mlp_rooflines(k, precision=1, ceiling=None, cells16=False):
     """Rooflines of the kernels of an MRPNN frame; `roofline` is the one that
     takes longest.  Network: FLOPs of the descriptors evaluated (first
     vertices that scatter; escaping pixels are sky and skip gather and network)
     against the dense BF16 (precision 1) or FP8 (precision 2) MFMA peak.
     First vertex and shadow rays: 32 B per trilinear lookup, also given as a
-    fraction of the measured random-read ceiling.  Gather: 18,400 B per
-    descriptor."""
+    fraction of the measured random-read ceiling.  Gather: measured HBM bytes
+    per launch over its time; its algorithmic 18,400 B per descriptor (12,256 B
+    with FP16 cells) come mostly from the caches and are reported beside."""
     nd = k["descriptors"]
     mlp = MLP_FLOP_PER_PASS * nd / (k["mlp_ms"] * 1e-3) / 1e12
     kname, peak = ("mlp_fp8_kernel", FP8_PEAK_TFLOPS) if precision == 2 else ("mlp_bf16_kernel", BF16_PEAK_TFLOPS)
@@ -324,36 +369,48 @@ def mlp_rooflines(k, precision=1, ceiling=None):
                    "lookups_per_launch": lk, "bytes_per_lookup": BYTES_PER_LOOKUP}
         if ceiling:
             r[name] = with_ceiling(r[name], ceiling, lk / (ms * 1e-3))
-    ga = GATHER_BYTES_PER_POINT * nd / (k["gather_ms"] * 1e-3) / 1e9
-    r["gather_kernel"] = {"bound": "hbm", "achieved": ga, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": ga / HBM_PEAK_GBS, "traffic": hbm_traffic("gather_kernel", MLP_WORKLOAD),
+    # Gather: most of the cell bytes its lookups read are served by L2 and the
+    # last-level cache, so its HBM fraction is the measured HBM traffic over its
+    # time; the algorithmic rate (cell bytes the lookups read) is given beside it.
+    per_desc = GATHER_BYTES_PER_POINT_16 if cells16 else GATHER_BYTES_PER_POINT
+    ga_alg = per_desc * nd / (k["gather_ms"] * 1e-3) / 1e9
+    traffic = hbm_traffic("gather_kernel", MLP_WORKLOAD + ("_cells16" if cells16 else ""))
+    ga_hbm = traffic / (k["gather_ms"] * 1e-3) / 1e9 if traffic else None
+    r["gather_kernel"] = {"bound": "hbm", "achieved": ga_hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": ga_hbm / HBM_PEAK_GBS if ga_hbm else None, "traffic": traffic,
                           "kernel": "gather_kernel", "kernel_ms": k["gather_ms"],
-                          "bytes_per_descriptor": GATHER_BYTES_PER_POINT,
-                          "note": "achieved counts the cell bytes the descriptors read; most come from L2 "
-                                  "and the last-level cache, and traffic is the HBM bytes per launch"}
+                          "cells": "fp16" if cells16 else "fp32",
+                          "achieved_is": "measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
+                                         "scaled by a copy kernel) over the live kernel time",
+                          "algorithmic": {"bytes_per_descriptor": per_desc, "descriptors_per_launch": nd,
+                                          "gbs": ga_alg, "frac_of_hbm_peak": ga_alg / HBM_PEAK_GBS,
+                                          "served_from": "mostly L2 and the last-level cache, not HBM"}}
     dominant = max(r, key=lambda n: r[n]["kernel_ms"])
     return r[dominant], r
 
 
 def headline_config(sc, a):
     net = "FP8-E4M3" if a.mlp_precision == 2 else ("FP32" if a.mlp_precision == 0 else "BF16")
-    return {"workload": "MRPNN renderer (" + net + " MFMA network), 512^3 Perlin-fBm cloud seed 1, %dx%d, 1 spp/frame; "
+    cells = "FP16" if a.gather_cells == 16 and a.mlp_precision != 0 else "FP32"
+    return {"workload": "MRPNN renderer (" + net + " MFMA network, descriptor gather on " + cells + " cells), "
+                        "512^3 Perlin-fBm cloud seed 1, %dx%d, 1 spp/frame; "
                         "vs plain delta tracking at equal RMSE against a %d-spp FP64 ground-truth image"
                         % (sc.W, sc.H, a.truth_spp),
             "width": sc.W, "height": sc.H, "grid": sc.res, "alpha": 256, "albedo": 0.999, "g": 0.857,
-            "bounces": sc.bounces, "truth_spp": a.truth_spp}
+            "bounces": sc.bounces, "truth_spp": a.truth_spp, "gather_cells": cells.lower()}
 
 
 def run_mlp(a, ia, sc):
     """Only the timed MLP-renderer frames (no ground truth, no equal-RMSE search)."""
     net = ia.Network()
     sc.vol.set_network(net)
+    sc.vol.set_gather_cells(a.gather_cells)
     elapsed, k = time_mlp(a, ia, sc)
     net.close()
     cfg = headline_config(sc, a)
     cfg["workload"] = cfg["workload"].split(";")[0]
     ceiling = random_read_ceiling(ia)
-    roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling)
+    roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling, cells16=a.gather_cells == 16)
     return {"value": a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps,
             "dtype": {0: "f32", 1: "bf16", 2: "fp8-e4m3"}[a.mlp_precision],
             "config": cfg, "roofline": roof, "rooflines": roofs, "random_read_ceiling": ceiling, "kernels": k}
@@ -579,6 +636,7 @@ def run_serve(a, ia, sc):
 def run_headline(a, ia, sc):
     net = ia.Network()
     sc.vol.set_network(net)
+    sc.vol.set_gather_cells(a.gather_cells)
     P = sc.W * sc.H
     ref = ground_truth_image(ia, sc, a.truth_spp)  # untimed
     elapsed, k = time_mlp(a, ia, sc)
@@ -595,6 +653,16 @@ def run_headline(a, ia, sc):
     rmse_mlp8 = rmse(sc.image(), ref)
     log("MLP 1-spp RMSE: BF16 %.5g, FP32 %.5g, FP8 %.5g (networks: FP32 %.2f ms, FP8 %.2f ms)"
         % (rmse_mlp, rmse_mlp32, rmse_mlp8, fp32_ms, fp8_ms))
+    # the other cell format of the BF16 gather, timed the same way (a named variant, not the headline)
+    other = 16 if a.gather_cells == 32 else 32
+    sc.vol.set_gather_cells(other)
+    el_o, k_o = time_mlp(a, ia, sc)
+    sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
+    variant = {"gather_cells": "fp%d" % other, "frames_per_s": a.steps / el_o, "ms_per_frame": 1000.0 * el_o / a.steps,
+               "gather_ms": k_o["gather_ms"], "rmse_mlp_bf16": rmse(sc.image(), ref)}
+    sc.vol.set_gather_cells(a.gather_cells)
+    log("BF16 renderer with FP%d gather cells: %.1f frames/s (gather %.3f ms), RMSE %.5g"
+        % (other, variant["frames_per_s"], variant["gather_ms"], variant["rmse_mlp_bf16"]))
     # the plain tracer at equal RMSE
     n_eq, hist, reached = equal_rmse_search(sc, rmse_mlp, ref, a.dt_max_spp)
     t_dt, dt_kernel_ms, dt_lookups = dt_time(sc, n_eq)
@@ -605,12 +673,29 @@ def run_headline(a, ia, sc):
     dt_achieved = dt_lookups * BYTES_PER_LOOKUP / (dt_kernel_ms * 1e-3) / 1e9
     net.close()
     ceiling = random_read_ceiling(ia)
-    roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling)
+    roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling, cells16=a.gather_cells == 16)
+    # the two rooflines the targets name: the tracer against HBM, the network against MFMA
+    tr_traffic = hbm_traffic("trace_kernel", TRACER_WORKLOAD % 16)
+    tr_lookups16 = hbm_traffic("trace_kernel", TRACER_WORKLOAD % 16, "lookups_per_launch")
+    bpl = tr_traffic / tr_lookups16 if tr_traffic and tr_lookups16 else None
+    tracer_hbm = dt_lookups * bpl / (dt_kernel_ms * 1e-3) / 1e9 if bpl else None
+    mlp_key = "mlp_fp8_kernel" if a.mlp_precision == 2 else "mlp_bf16_kernel"
+    north_star = {
+        "tracer_hbm": {"kernel": "trace_kernel", "launch": "%d spp at 1920x1080 (equal RMSE)" % n_eq,
+                       "kernel_ms": dt_kernel_ms, "lookups_per_launch": dt_lookups,
+                       "algorithmic_gbs": dt_achieved, "algorithmic_frac": dt_achieved / HBM_PEAK_GBS,
+                       "hbm_bytes_per_lookup": bpl, "hbm_gbs": tracer_hbm,
+                       "hbm_frac": tracer_hbm / HBM_PEAK_GBS if tracer_hbm else None,
+                       "frac_of_random_read_ceiling": dt_lookups / (dt_kernel_ms * 1e-3) / ceiling["reads_per_s"],
+                       "target_frac": 0.5,
+                       "note": "algorithmic = 32 B per trilinear lookup; hbm = the HBM bytes per lookup measured "
+                               "by rocprofv3 on the 16-spp launch (whole 128-byte lines)"},
+        "mlp_mfma": dict(roofs[mlp_key], target_frac=0.4) if mlp_key in roofs else None}
     return {
         "value": fps_mlp, "ms_per_step": 1000.0 * elapsed / a.steps,
         "dtype": {0: "f32", 1: "bf16", 2: "fp8-e4m3"}[a.mlp_precision],
         "config": headline_config(sc, a),
-        "roofline": roof, "rooflines": roofs, "random_read_ceiling": ceiling,
+        "roofline": roof, "rooflines": roofs, "north_star": north_star, "random_read_ceiling": ceiling,
         "equal_rmse": {"rmse_mlp_bf16": rmse_mlp, "rmse_mlp_fp32": rmse_mlp32, "rmse_mlp_fp8": rmse_mlp8,
                        "dt_spp": n_eq,
                        "dt_reached_target": reached, "rmse_dt": hist.get(n_eq), "fps_mlp": fps_mlp,
@@ -618,9 +703,8 @@ def run_headline(a, ia, sc):
                        "dt_rmse_by_spp": {str(k): v for k, v in sorted(hist.items())},
                        "luma_rmse100_mlp": luma_rmse100(img_mlp, ref),
                        "luma_rmse100_dt1": luma_rmse100(img_dt1, ref)},
-        "kernels": dict(k, mlp_fp32_ms=fp32_ms, mlp_fp8_ms=fp8_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms,
-                        tracer_gbs_algorithmic=dt_achieved, tracer_hbm_frac=dt_achieved / HBM_PEAK_GBS,
-                        tracer_frac_of_random_read_ceiling=dt_lookups / (dt_kernel_ms * 1e-3) / ceiling["reads_per_s"]),
+        "variant": variant,
+        "kernels": dict(k, mlp_fp32_ms=fp32_ms, mlp_fp8_ms=fp8_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms),
     }
 
 
@@ -638,6 +722,8 @@ def main(argv=None):
     ap.add_argument("--dt-max-spp", type=int, default=4096)
     ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--gather-cells", type=int, choices=[32, 16], default=32,
+                    help="headline/mlp modes: cell format of the BF16/FP8 gather (32 FP32, 16 FP16)")
     ap.add_argument("--mlp-precision", type=int, choices=[0, 1, 2], default=1,
                     help="headline/mlp modes: network of the timed frames (0 FP32, 1 BF16, 2 FP8-E4M3)")
     ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
