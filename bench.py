@@ -392,17 +392,20 @@ def mlp_rooflines(k, precision=1, ceiling=None, cells16=False):
 def headline_config(sc, a):
     net = "FP8-E4M3" if a.mlp_precision == 2 else ("FP32" if a.mlp_precision == 0 else "BF16")
     cells = "FP16" if a.gather_cells == 16 and a.mlp_precision != 0 else "FP32"
-    return {"workload": "MRPNN renderer (" + net + " MFMA network, descriptor gather on " + cells + " cells), "
+    main = a.network if (a.network != "native" or __import__("internship_amd").has_native_weights()) else "shipped"
+    return {"workload": "MRPNN renderer (" + net + " MFMA network, " + main + " weights, descriptor gather on " + cells
+                        + " cells), "
                         "512^3 Perlin-fBm cloud seed 1, %dx%d, 1 spp/frame; "
                         "vs plain delta tracking at equal RMSE against a %d-spp FP64 ground-truth image"
                         % (sc.W, sc.H, a.truth_spp),
             "width": sc.W, "height": sc.H, "grid": sc.res, "alpha": 256, "albedo": 0.999, "g": 0.857,
-            "bounces": sc.bounces, "truth_spp": a.truth_spp, "gather_cells": cells.lower()}
+            "bounces": sc.bounces, "truth_spp": a.truth_spp, "gather_cells": cells.lower(), "network": main}
 
 
 def run_mlp(a, ia, sc):
     """Only the timed MLP-renderer frames (no ground truth, no equal-RMSE search)."""
-    net = ia.Network()
+    main, nets = headline_networks(a, ia)
+    net = ia.Network(nets[main])
     sc.vol.set_network(net)
     sc.vol.set_gather_cells(a.gather_cells)
     elapsed, k = time_mlp(a, ia, sc)
@@ -471,88 +474,90 @@ def run_frame(a, ia, sc):
 
 
 def run_train(a, ia, sc):
-    """The learned part: GPU data generation (descriptors + GetSample targets)
-    on config-4 style procedural clouds, then K timed Adam steps of batch
-    --batch on the GPU trainer, starting from the shipped weights; A/B on
-    held-out clouds: relative L2 of the shipped vs the trained weights."""
-    from internship_amd import datagen
-    n_train, n_hold = a.train_clouds, 2
-    specs = datagen.random_specs(n_train + n_hold, seed=100)
-    t0 = time.perf_counter()
-    feat, aux, gst = datagen.make_dataset(specs[:n_train], res=128, n_per_cloud=a.points_per_cloud, spp=a.target_spp,
-                                          multi=a.target_bounces, seed=1, log=log)
-    hfeat, haux, _ = datagen.make_dataset(specs[n_train:], res=128, n_per_cloud=a.points_per_cloud,
-                                          spp=a.target_spp, multi=a.target_bounces, seed=2, log=log)
-    gen_s = time.perf_counter() - t0
-    log("data: %d training + %d held-out samples in %.1f s (targets %.1f s, %.3g paths/s)"
-        % (len(feat), len(hfeat), gen_s, gst["target_s"], gst["paths"] / max(gst["target_s"], 1e-9)))
-    perm = np.random.default_rng(0).permutation(len(feat))
-    feat, aux = np.ascontiguousarray(feat[perm]), np.ascontiguousarray(aux[perm])
-    fb = ia.DeviceBuffer(feat.nbytes)
-    fb.upload(feat)
-    ab = ia.DeviceBuffer(aux.nbytes)
-    ab.upload(aux)
-    B = min(a.batch, len(feat))
-    w0 = ia.load_reference_weights()
-    prec = a.train_precision
-    tr = ia.Trainer(w0, max_batch=B, precision=prec)
-
-    def batch_views(i):
-        off = (i * B) % (len(feat) - B + 1)
-        return datagen.DeviceView(fb, off * 576 * 4), datagen.DeviceView(ab, off * 16)
-
-    losses = []
-    for i in range(a.warmup):
-        f, x = batch_views(i)
-        losses.append(tr.step(f, x, B, a.lr))
-    t0 = time.perf_counter()
-    for i in range(a.warmup, a.warmup + a.steps):
-        f, x = batch_views(i)
-        losses.append(tr.step(f, x, B, a.lr))
-    elapsed = time.perf_counter() - t0
-    w1 = tr.weights()
-    tr.close()
-    sps = a.steps * B / elapsed
-    # the other arithmetic on the same batches (throughput only)
-    other = ia.Trainer(w0, max_batch=B, precision=1 - prec)
-    f, x = batch_views(0)
-    other.step(f, x, B, a.lr)
-    n_other = 3
-    t0 = time.perf_counter()
-    for i in range(n_other):
-        f, x = batch_views(i + 1)
-        other.step(f, x, B, a.lr)
-    other_sps = n_other * B / (time.perf_counter() - t0)
-    other.close()
-    log("training: %.4g samples/s (%s), %.4g samples/s (%s)" % (sps, "BF16 matrix cores" if prec else "FP32",
-                                                                 other_sps, "FP32" if prec else "BF16 matrix cores"))
-    flops = 3 * MLP_FLOP_PER_PASS  # forward + backward (dX and dW) per sample
-    achieved = sps * flops / 1e12
-    peak = BF16_PEAK_TFLOPS if prec else FP32_PEAK_TFLOPS
-    ref_pred = datagen.predict(w0, hfeat, haux)
-    new_pred = datagen.predict(w1, hfeat, haux)
-    ab_res = {"heldout_samples": len(hfeat),
-              "rel_l2_shipped_weights": datagen.relative_l2(ref_pred, haux[:, 3]),
-              "rel_l2_trained_weights": datagen.relative_l2(new_pred, haux[:, 3]),
-              "train_loss_first": losses[0], "train_loss_last": losses[-1]}
-    log("A/B held-out rel L2: shipped %.4g, trained %.4g; loss %.4g -> %.4g"
-        % (ab_res["rel_l2_shipped_weights"], ab_res["rel_l2_trained_weights"], losses[0], losses[-1]))
-    return {
-        "metric": "MRPNN training samples/s (forward + backward + Adam, GPU)", "unit": "samples/s",
-        "value": sps, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16" if prec else "f32",
-        "config": {"workload": "MRPNN training, batch %d, %d procedural 128^3 clouds x %d points, GetSample targets "
-                               "%d spp / %d bounces" % (B, n_train, a.points_per_cloud, a.target_spp,
-                                                         a.target_bounces),
-                   "batch": B, "lr": a.lr, "precision": prec},
-        "roofline": {"bound": "mfma" if prec else "valu", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
-                     "kernel": ("train_fwd_mfma + train_bwd_mfma" if prec else "train_fwd_bwd_kernel") + " + wgrad",
-                     "flop_per_sample": flops},
-        "other_precision": {"precision": 1 - prec, "samples_per_s": other_sps},
-        "datagen": {"seconds": gen_s, "target_seconds": gst["target_s"], "paths": gst["paths"],
-                    "target_lookups": gst["target_lookups"]},
-        "ab_test": ab_res,
+    """Config 4, the learned part: training data generated on the GPU from
+    Perlin-fBm clouds (seeds 100.., thresholds U[0.3,0.5]; per view extinction
+    U[64,512], sun on the upper hemisphere, g U[0.7,0.9]; albedo 0.999), points
+    sampled as `MeanFreePathSample` does, `GetSample` targets; 5 % of the
+    clouds held out.  The network is trained from seeded random weights with
+    Adam (lr, batch, epochs) on the BF16 matrix-core trainer; the held-out loss
+    and relative L2 of the trained and the shipped weights are compared, and
+    (--eval-headline) both networks render the headline scene against its
+    ground truth, with plain delta tracking at equal RMSE beside them."""
+    from internship_amd import datagen, learn
+    views = datagen.config4_views(a.train_clouds, 100, a.views_per_cloud, seed=4)
+    ds, gst = datagen.make_config4(views, res=a.train_res, samples_per_view=a.samples_per_view, spp=a.target_spp,
+                                   bounces=a.target_bounces, albedo=0.999, seed=1, log=log)
+    n_hold = max(1, int(round(0.05 * a.train_clouds)))
+    n_train = int(np.argmax(ds.cloud >= a.train_clouds - n_hold))
+    log("data: %d samples (%d training, %d held out from %d clouds) in %.1f s: targets %.1f s, %.3g paths/s, "
+        "%.3g lookups/s" % (ds.n, n_train, ds.n - n_train, n_hold, gst["total_s"], gst["target_s"],
+                            gst["paths"] / max(gst["target_s"], 1e-9), gst["target_lookups"] / max(gst["target_s"], 1e-9)))
+    srp = float(ds.aux_host[0, 2])
+    mean_y = float(ds.aux_host[:n_train, 3].mean())
+    w0 = learn.init_weights(a.init_seed, mean_y, srp)
+    B = a.batch
+    tr0 = ia.Trainer(w0, max_batch=B, precision=a.train_precision)
+    loss0, rel0, _ = learn.evaluate(tr0, ds, n_train, ds.n, B)
+    tr0.close()
+    shipped = ia.Trainer(ia.load_reference_weights(), max_batch=B, precision=a.train_precision)
+    loss_s, rel_s, _ = learn.evaluate(shipped, ds, n_train, ds.n, B)
+    shipped.close()
+    log("held out before training: random init loss %.5g rel L2 %.4g; shipped weights loss %.5g rel L2 %.4g"
+        % (loss0, rel0, loss_s, rel_s))
+    w1, hist, t_train = learn.train(ds, n_train, w0, epochs=a.epochs, batch=B, lr=a.lr, seed=a.init_seed,
+                                    precision=a.train_precision, log=log, eval_range=(n_train, ds.n),
+                                    eval_every=max(1, a.epochs // 10))
+    steps = a.epochs * (n_train // B)
+    sps = steps * B / t_train
+    if a.save_weights:
+        np.save(a.save_weights, w1)
+    res = {
+        "metric": "MRPNN training from seeded random weights: samples/s (forward + backward + Adam, GPU) and "
+                  "held-out relative L2", "unit": "samples/s",
+        "value": sps, "ms_per_step": 1000.0 * t_train / steps, "dtype": "bf16" if a.train_precision else "f32",
+        "config": {"workload": "config 4: %d Perlin-fBm clouds (seeds 100-%d, %d^3) x %d views, %d samples per view, "
+                               "GetSample targets %d spp / %d bounces, albedo 0.999; %d clouds held out; Adam lr %g, "
+                               "batch %d, %d epochs from seeded random weights (seed %d)"
+                               % (a.train_clouds, 99 + a.train_clouds, a.train_res, a.views_per_cloud,
+                                  a.samples_per_view, a.target_spp, a.target_bounces, n_hold, a.lr, B, a.epochs,
+                                  a.init_seed),
+                   "samples": ds.n, "train_samples": n_train, "heldout_samples": ds.n - n_train,
+                   "batch": B, "lr": a.lr, "epochs": a.epochs, "precision": a.train_precision},
+        "datagen": {k: (float(v) if isinstance(v, (int, float, np.floating)) else v) for k, v in gst.items()},
+        "heldout": {"random_init": {"loss": loss0, "rel_l2": rel0}, "shipped_weights": {"loss": loss_s, "rel_l2": rel_s},
+                    "trained": {"loss": hist[-1][2], "rel_l2": hist[-1][3]}},
+        "history": [{"epoch": h[0], "train_loss": h[1], "heldout_loss": h[2], "heldout_rel_l2": h[3]} for h in hist],
+        "roofline": {"bound": "mfma", "achieved": sps * 3 * MLP_FLOP_PER_PASS / 1e12, "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": sps * 3 * MLP_FLOP_PER_PASS / 1e12 / BF16_PEAK_TFLOPS,
+                     "traffic": None, "kernel": "train_fwd_mfma + train_bwd_mfma + wgrad + adam + gather_rows",
+                     "flop_per_sample": 3 * MLP_FLOP_PER_PASS},
     }
+    if a.eval_headline:
+        res["headline_scene"] = eval_networks_on_headline(a, ia, {"shipped": ia.load_reference_weights(),
+                                                                  "trained": w1})
+    return res
+
+
+def eval_networks_on_headline(a, ia, nets):
+    """RMSE of 1-spp BF16 MRPNN frames of the headline scene with each network
+    against the 4096-spp ground truth, and the delta-tracking spp at equal RMSE."""
+    sc = Scene(ia, 512, 1920, 1080, 512)
+    ref = ground_truth_image(ia, sc, a.truth_spp)
+    out = {}
+    for name, w in nets.items():
+        net = ia.Network(w)
+        sc.vol.set_network(net)
+        el, k = time_mlp(a, ia, sc)
+        sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
+        r = rmse(sc.image(), ref)
+        net.close()
+        n_eq, hist, reached = equal_rmse_search(sc, r, ref, a.dt_max_spp)
+        t_dt, _, _ = dt_time(sc, n_eq)
+        out[name] = {"rmse": r, "frames_per_s": a.steps / el, "dt_spp_at_equal_rmse": n_eq, "dt_reached": reached,
+                     "speedup_vs_dt": (a.steps / el) * t_dt}
+        log("headline scene, %s network: RMSE %.5g, DT at equal RMSE %d spp, speed-up %.1fx"
+            % (name, r, n_eq, out[name]["speedup_vs_dt"]))
+    return out
 
 
 def run_serve(a, ia, sc):
@@ -633,8 +638,20 @@ def run_serve(a, ia, sc):
     }
 
 
+def headline_networks(a, ia):
+    """{name: weights} of the networks the headline renders with, and the one
+    it times: the network this library trained from seeded random weights on
+    its config-4 data (native, when committed) and the shipped NNWeight.cuh
+    weights."""
+    nets = {"shipped": ia.load_reference_weights()}
+    if ia.has_native_weights():
+        nets["native"] = ia.load_native_weights()
+    return (a.network if a.network in nets else "shipped"), nets
+
+
 def run_headline(a, ia, sc):
-    net = ia.Network()
+    main, nets = headline_networks(a, ia)
+    net = ia.Network(nets[main])
     sc.vol.set_network(net)
     sc.vol.set_gather_cells(a.gather_cells)
     P = sc.W * sc.H
@@ -672,6 +689,21 @@ def run_headline(a, ia, sc):
     log("DT %d spp: %.2f ms/frame -> speed-up %.2fx" % (n_eq, 1000 * t_dt, fps_mlp / fps_dt))
     dt_achieved = dt_lookups * BYTES_PER_LOOKUP / (dt_kernel_ms * 1e-3) / 1e9
     net.close()
+    # the other network(s) at equal RMSE (same kernels, so the same frames/s)
+    others = {}
+    for name, w in nets.items():
+        if name == main:
+            continue
+        onet = ia.Network(w)
+        sc.vol.set_network(onet)
+        sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
+        r_o = rmse(sc.image(), ref)
+        onet.close()
+        n_o, hist_o, reached_o = equal_rmse_search(sc, r_o, ref, a.dt_max_spp)
+        t_o, _, _ = dt_time(sc, n_o)
+        others[name] = {"rmse_mlp_bf16": r_o, "dt_spp": n_o, "dt_reached_target": reached_o,
+                        "rmse_dt": hist_o.get(n_o), "fps_dt_at_equal_rmse": 1.0 / t_o, "speedup": fps_mlp * t_o}
+        log("%s network: RMSE %.5g, DT at equal RMSE %d spp, speed-up %.1fx" % (name, r_o, n_o, fps_mlp * t_o))
     ceiling = random_read_ceiling(ia)
     roof, roofs = mlp_rooflines(k, a.mlp_precision, ceiling, cells16=a.gather_cells == 16)
     # the two rooflines the targets name: the tracer against HBM, the network against MFMA
@@ -704,6 +736,9 @@ def run_headline(a, ia, sc):
                        "luma_rmse100_mlp": luma_rmse100(img_mlp, ref),
                        "luma_rmse100_dt1": luma_rmse100(img_dt1, ref)},
         "variant": variant,
+        "network": {"timed": main, "weights": "trained by this library from seeded random weights on its config-4 "
+                                               "data (data/mrpnn_weights_native.*)" if main == "native" else
+                    "core/NNWeight.cuh (shipped)", "other_networks": others},
         "kernels": dict(k, mlp_fp32_ms=fp32_ms, mlp_fp8_ms=fp8_ms, tracer_kernel_ms_at_equal_rmse=dt_kernel_ms),
     }
 
@@ -722,19 +757,29 @@ def main(argv=None):
     ap.add_argument("--dt-max-spp", type=int, default=4096)
     ap.add_argument("--spp-per-step", type=int, default=1, help="tracer mode: 1-spp frames per launch")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--network", choices=["native", "shipped"], default="native",
+                    help="headline/mlp modes: the network trained by this library (native, when its weights are "
+                         "committed) or the shipped NNWeight.cuh weights")
     ap.add_argument("--gather-cells", type=int, choices=[32, 16], default=32,
                     help="headline/mlp modes: cell format of the BF16/FP8 gather (32 FP32, 16 FP16)")
     ap.add_argument("--mlp-precision", type=int, choices=[0, 1, 2], default=1,
                     help="headline/mlp modes: network of the timed frames (0 FP32, 1 BF16, 2 FP8-E4M3)")
     ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
     ap.add_argument("--workers", type=int, default=4, help="serve mode: host threads")
-    ap.add_argument("--lr", type=float, default=1e-4, help="train mode: Adam learning rate")
+    ap.add_argument("--lr", type=float, default=1e-3, help="train mode: Adam learning rate")
+    ap.add_argument("--epochs", type=int, default=20, help="train mode: epochs over the training samples")
+    ap.add_argument("--init-seed", type=int, default=0, help="train mode: seed of the random initial weights")
+    ap.add_argument("--views-per-cloud", type=int, default=4, help="train mode: light/extinction/g draws per cloud")
+    ap.add_argument("--samples-per-view", type=int, default=2048)
+    ap.add_argument("--train-res", type=int, default=256, help="train mode: cloud resolution")
+    ap.add_argument("--save-weights", default="", help="train mode: .npy path for the trained weights")
+    ap.add_argument("--eval-headline", action="store_true",
+                    help="train mode: render the headline scene with the shipped and the trained network")
     ap.add_argument("--train-precision", type=int, choices=[0, 1], default=1,
                     help="train mode: 0 FP32 forward/backward, 1 BF16 matrix cores (FP32 master weights)")
-    ap.add_argument("--train-clouds", type=int, default=16)
-    ap.add_argument("--points-per-cloud", type=int, default=8192)
-    ap.add_argument("--target-spp", type=int, default=32)
-    ap.add_argument("--target-bounces", type=int, default=64)
+    ap.add_argument("--train-clouds", type=int, default=64)
+    ap.add_argument("--target-spp", type=int, default=256)
+    ap.add_argument("--target-bounces", type=int, default=512)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="first run one device-to-device copy of this many MiB (reference kernel for HBM counters)")
     a = ap.parse_args(argv)

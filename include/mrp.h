@@ -110,6 +110,13 @@ int mrp_memcpy_h2d(void* dev_dst, const void* host_src, size_t bytes);
 int mrp_memcpy_d2h(void* host_dst, const void* dev_src, size_t bytes);
 int mrp_memset_zero(void* dev, size_t bytes);
 int mrp_memcpy_d2d(void* dev_dst, const void* dev_src, size_t bytes);
+/* The training data loader's shuffle (no counterpart in the reference, whose
+ * trainer shuffles a CSV on the host, dataGen.cu:335-337): dst row r = src row
+ * idx_dev[r] for r < n_rows, rows of row_bytes (a multiple of 16) in HBM; an
+ * index >= n_src_rows gives a row of zeros.  Queued on the null stream
+ * (asynchronous). */
+int mrp_gather_rows(void* dst_dev, const void* src_dev, const uint32_t* idx_dev, int n_rows, int row_bytes,
+                    int n_src_rows);
 
 /* ------------------------------------------------------------ measurement
  * No counterpart in the reference: the GPU's ceiling for random reads of
@@ -236,6 +243,44 @@ typedef struct mrp_sample_query {
  * Path k of query j draws from Philox stream (j, k, tag 1). */
 int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q_dev, mrp_vec3 light_color, mrp_vec3 sky,
                     int multi_scatter, int spp, uint64_t seed, float* out_dev, mrp_trace_stats* stats);
+
+/* Training points: the sampling of `GetDesiredCountSample` /
+ * `MeanFreePathSample` (dataGen.cu:174-216) on the GPU.  n_rays rays start
+ * uniformly on the sphere of radius 0.5, heading along normalize(r +
+ * normalize(-origin)) for a uniform direction r, and visit up to 4 successive
+ * delta-tracking vertices (extinction scale alpha, HG g); a vertex is kept if
+ * it is the first, lies more than 1/8 from the previous one, or with
+ * probability 0.1; a kept point moves by a uniform offset in the ball of
+ * radius 1/128 with probability 0.5, and its view direction is the ray's
+ * first direction with probability 0.75, else its arrival direction.  Ray r
+ * draws from Philox stream (r, 0, tag 2, seed).  Outputs in HBM, 4 slots per
+ * ray: pos_dev 4 n_rays float4 (position, 1 if the slot holds a point else
+ * 0), view_dev 4 n_rays float4 (unit view direction, 0).  lookups (may be
+ * NULL): trilinear density lookups made. */
+int mrp_sample_training_points(mrp_volume* vol, int n_rays, float alpha, float g, uint64_t seed, float* pos_dev,
+                               float* view_dev, uint64_t* lookups);
+
+/* `GetRadiances` (core/volume.cu:329-402, `CalculateRadianceMulti`,
+ * core/volume.cu:17-34) on n given rays (HBM, n x 3 FP32 origins and
+ * directions).  The render parameters give the light, medium, sky and albedo;
+ * their camera and image fields are ignored.  render_type PT: p->spp_count
+ * paths per ray (`CalculateRadiance`'s sampleNum, core/render.cuh:200-258),
+ * one after another on Philox stream (ray, 0, tag 4, seed).  MRPNN:
+ * `NNPredict` once per ray (the reference ignores sampleNum there) on stream
+ * (ray, 0, tag 0, seed), through the renderer's stages with p->mlp_precision.
+ * out_dev: n x 4 FP32 (radiance, distance to the first scattering vertex or
+ * -1).  stats may be NULL. */
+int mrp_get_radiances(mrp_volume* vol, int n, const float* ori_dev, const float* dir_dev,
+                      const mrp_render_params* p, float* out_dev, mrp_trace_stats* stats);
+/* `GetTr` / `GetTrs` (core/volume.cu:302-323,636-673, `ShadowTerm_TR(s)`,
+ * core/render.cuh:317-341) on n points (HBM, n x 3 FP32 positions and
+ * directions): out_dev[i] = the mean over spp estimates of Tr(pos -> light) *
+ * HG(dir . light, g) (dir as given), drawn one after another on Philox stream
+ * (i, 0, tag 5, seed).  The reference's GetTrs scales this by lightColor * pi
+ * and GetTr rounds sampleNum up to a multiple of 32; the C++ facade does
+ * both.  lookups may be NULL. */
+int mrp_get_shadow_terms(mrp_volume* vol, int n, const float* pos_dev, const float* dir_dev, mrp_vec3 light_dir,
+                         float alpha, float g, int spp, uint64_t seed, float* out_dev, uint64_t* lookups);
 
 /* ---------------------------------------------------------------- network */
 /* Trained MRPNN weights: 49,673 FP32 values, the 146 arrays of

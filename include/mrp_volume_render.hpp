@@ -16,6 +16,7 @@
 //     SetNetwork(weights, count) takes them from memory.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -38,6 +39,8 @@ struct int2 {
 inline float3 make_float3(float x, float y, float z) { return float3{x, y, z}; }
 inline float3 operator*(float3 a, float s) { return {a.x * s, a.y * s, a.z * s}; }
 inline float3 operator-(float3 a) { return {-a.x, -a.y, -a.z}; }
+inline float3 operator+(float3 a, float3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline float3 operator+(float3 a, float s) { return {a.x + s, a.y + s, a.z + s}; }
 
 typedef mrp_histogram Histogram;
 typedef std::function<float(int x, int y, int z, float u, float v, float w)> FillFunc;  // omp.hpp:10
@@ -55,6 +58,50 @@ inline bool ok(int rc, const char* what) {
     if (rc != MRP_OK) std::printf("%s: %s\n", what, mrp_last_error());
     return rc == MRP_OK;
 }
+inline float lerp(float a, float b, float t) { return a + (b - a) * t; }
+// `Sample` / `SampleClamp` (core/volume.cu:675-722): FP32 trilinear at
+// uv * res - 0.5 with border-0 (negative values read as 0) or clamp-to-edge corners.
+inline float sample(const std::vector<float>& d, int res, float3 uv, bool clamp) {
+    auto at = [&](int x, int y, int z) -> float {
+        if (clamp) {
+            x = std::max(std::min(x, res - 1), 0);
+            y = std::max(std::min(y, res - 1), 0);
+            z = std::max(std::min(z, res - 1), 0);
+        } else if (x < 0 || y < 0 || z < 0 || x >= res || y >= res || z >= res) {
+            return 0.0f;
+        }
+        return std::max(0.0f, d[((size_t)x * res + y) * res + z]);
+    };
+    const float px = uv.x * res - 0.5f, py = uv.y * res - 0.5f, pz = uv.z * res - 0.5f;
+    const int x = (int)std::floor(px), y = (int)std::floor(py), z = (int)std::floor(pz);
+    const float wx = px - x, wy = py - y, wz = pz - z;
+    return lerp(lerp(lerp(at(x, y, z), at(x, y, z + 1), wz), lerp(at(x, y + 1, z), at(x, y + 1, z + 1), wz), wy),
+                lerp(lerp(at(x + 1, y, z), at(x + 1, y, z + 1), wz),
+                     lerp(at(x + 1, y + 1, z), at(x + 1, y + 1, z + 1), wz), wy),
+                wx);
+}
+// `RayBoxOffset` (core/vector.cu:41-58): entry distance into the 0.4999 slab box, -1 on a miss.
+inline float ray_box_offset(float3 p, float3 d) {
+    const float id[3] = {d.x == 0 ? 999999999.0f : 1.0f / d.x, d.y == 0 ? 999999999.0f : 1.0f / d.y,
+                         d.z == 0 ? 999999999.0f : 1.0f / d.z};
+    const float pp[3] = {p.x, p.y, p.z};
+    float tmi = -1e30f, tma = 1e30f;
+    for (int k = 0; k < 3; ++k) {
+        const float a = -pp[k] * id[k], b = 0.4999f * id[k];
+        tmi = std::max(tmi, std::min(a + b, a - b));
+        tma = std::min(tma, std::max(a + b, a - b));
+    }
+    return tma >= tmi ? std::max(tmi, 0.0f) : -1.0f;
+}
+// HBM staging for the point queries (the reference allocates per call too).
+struct DevBuf {
+    void* p = nullptr;
+    explicit DevBuf(size_t bytes) { ok(mrp_malloc(&p, bytes ? bytes : 16), "mrp_malloc"); }
+    ~DevBuf() { mrp_free(p); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    float* f() const { return (float*)p; }
+};
 }  // namespace detail
 
 class VolumeRender {
@@ -110,14 +157,146 @@ class VolumeRender {
 
     void Update() {
         if (!h_) return;
+        host_mips_.clear();
+        host_trs_.clear();
+        host_lut_.clear();
         if (!detail::ok(mrp_volume_set_data(h_, datas_.data(), 0), "Update")) return;
         if (!detail::ok(mrp_volume_update(h_), "Update")) return;
         detail::ok(mrp_volume_max_density(h_, &max_density), "Update");
     }
     void Update_TR(float3 lightDir, float alpha = 64.0f, bool /*CPU*/ = false) {
+        host_trs_.clear();
         detail::ok(mrp_volume_update_tr(h_, detail::v(lightDir), alpha), "Update_TR");
     }
-    void UpdateHGLut(float g) { detail::ok(mrp_volume_update_hg_lut(h_, g, seed_), "UpdateHGLut"); }
+    void UpdateHGLut(float g) {
+        host_lut_.clear();
+        detail::ok(mrp_volume_update_hg_lut(h_, g, seed_), "UpdateHGLut");
+    }
+
+    // ---- host queries (core/volume.cu:1184-1201,1495-1531): on host copies of
+    // the density mips, transmittance mips and HG table, fetched on first use
+    // after Update / Update_TR / UpdateHGLut (and after MRPNN renders and
+    // queries, which refresh the light's tables themselves).
+    float DensityAtUV(int mip, float3 uv) {
+        if (mip < 0 || mip > 8 || !fetch_mips()) return 0.0f;
+        return detail::sample(host_mips_[mip], 256 >> mip, uv, false);
+    }
+    float DensityAtUV(float mip, float3 uv) {
+        const int a = (int)mip;
+        return detail::lerp(DensityAtUV(a, uv), DensityAtUV(a + 1, uv), mip - a);
+    }
+    float DensityAtPosition(int mip, float3 pos) { return DensityAtUV(mip, pos + 0.5f); }
+    float DensityAtPosition(float mip, float3 pos) {
+        const int a = (int)mip;
+        return detail::lerp(DensityAtPosition(a, pos), DensityAtPosition(a + 1, pos), mip - a);
+    }
+    // A point outside the box reads the transmittance where its light ray enters
+    // the box, 1 if the ray misses it (`TrAtPosition`, core/volume.cu:1498-1515).
+    float TrAtPosition(int mip, float3 pos, float3 lightDir) {
+        if (mip < 0 || mip > 7 || !fetch_trs()) return 0.0f;
+        const int r = 128 >> mip;
+        if (pos.x < -0.5f || pos.y < -0.5f || pos.z < -0.5f || pos.x > 0.5f || pos.y > 0.5f || pos.z > 0.5f) {
+            const float off = detail::ray_box_offset(pos, lightDir);
+            if (off < 0.0f) return 1.0f;
+            return detail::sample(host_trs_[mip], r, pos + 0.5f + lightDir * off, true);
+        }
+        return detail::sample(host_trs_[mip], r, pos + 0.5f, true);
+    }
+    // The cone-averaged HG table at (cos, cone angle in radians, at most 60
+    // degrees): bilinear on the texel corners u = cos/2 + 1/2, v = angle/60deg as
+    // the reference indexes it (core/volume.cu:1184-1201); each axis is weighted
+    // by its own fraction (the reference weights u by v's fraction and v by u's).
+    float GetHGLut(float cos, float angle) {
+        if (!fetch_lut()) return 0.0f;
+        const int N = 512;
+        const float u = cos * 0.5f + 0.5f, v = angle / (3.1415926535f * 60.0f / 180.0f);
+        auto idx = [&](float t, int k) { return std::max(std::min((int)(t * N) + k, N - 1), 0); };
+        const int u0 = idx(u, 0), u1 = idx(u, 1), v0 = idx(v, 0), v1 = idx(v, 1);
+        const float fu = u * N - std::floor(u * N), fv = v * N - std::floor(v * N);
+        const float a0 = detail::lerp(host_lut_[u0 + v0 * N], host_lut_[u1 + v0 * N], fu);
+        const float a1 = detail::lerp(host_lut_[u0 + v1 * N], host_lut_[u1 + v1 * N], fu);
+        return detail::lerp(a0, a1, fv);
+    }
+
+    // ---- GPU point queries
+    // `GetTr` (core/volume.cu:302-323): the mean of Tr(ori -> light) * HG(dir .
+    // light) over sampleNum estimates rounded up to a multiple of 32.
+    float3 GetTr(float3 ori, float3 dir, float3 lightDir, float alpha = 1, float g = 0, int sampleNum = 1) {
+        const int spp = (std::max(sampleNum, 1) + 31) / 32 * 32;
+        detail::DevBuf po(12), pd(12), out(4);
+        float r = 0.0f;
+        if (detail::ok(mrp_memcpy_h2d(po.p, &ori, 12), "GetTr") && detail::ok(mrp_memcpy_h2d(pd.p, &dir, 12), "GetTr") &&
+            detail::ok(mrp_get_shadow_terms(h_, 1, po.f(), pd.f(), detail::v(lightDir), alpha, g, spp, seed_, out.f(),
+                                            nullptr), "GetTr"))
+            detail::ok(mrp_memcpy_d2h(&r, out.p, 4), "GetTr");
+        return {r, r, r};
+    }
+    // `GetTrs` (core/volume.cu:636-673): lightColor * pi * the mean of Tr * HG per point.
+    std::vector<float3> GetTrs(float alpha, const std::vector<float3>& ori, const std::vector<float3>& dir,
+                               float3 lightDir, float3 lightColor, float g = 0, int sampleNum = 1) {
+        const size_t n = std::min(ori.size(), dir.size());
+        std::vector<float> t(n, 0.0f);
+        std::vector<float3> out(n, float3{0, 0, 0});
+        if (!n) return out;
+        detail::DevBuf po(n * 12), pd(n * 12), pt(n * 4);
+        if (detail::ok(mrp_memcpy_h2d(po.p, ori.data(), n * 12), "GetTrs") &&
+            detail::ok(mrp_memcpy_h2d(pd.p, dir.data(), n * 12), "GetTrs") &&
+            detail::ok(mrp_get_shadow_terms(h_, (int)n, po.f(), pd.f(), detail::v(lightDir), alpha, g,
+                                            std::max(sampleNum, 1), seed_, pt.f(), nullptr), "GetTrs") &&
+            detail::ok(mrp_memcpy_d2h(t.data(), pt.p, n * 4), "GetTrs"))
+            for (size_t i = 0; i < n; ++i) {
+                const float s = t[i] * 3.1415926535f;
+                out[i] = {lightColor.x * s, lightColor.y * s, lightColor.z * s};
+            }
+        return out;
+    }
+    // `GetRadiances` (core/volume.cu:329-402): PT - the mean of sampleNum paths
+    // per ray; MRPNN - one `NNPredict` per ray (the light's tables are refreshed
+    // as the reference's UpdateHGLut / Update_TR do).
+    std::vector<float3> GetRadiances(const std::vector<float3>& ori, const std::vector<float3>& dir, float3 lightDir,
+                                     float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1, float g = 0,
+                                     int sampleNum = 1, RenderType rt = PT) {
+        const size_t n = std::min(ori.size(), dir.size());
+        std::vector<float3> out(n, float3{0, 0, 0});
+        if (!n) return out;
+        mrp_render_params p = params(int2{1, 1}, float3{0, 0, 0}, float3{0, 1, 0}, float3{1, 0, 0}, lightDir, rt, g,
+                                     alpha, lightColor, multiScatter);
+        p.spp_count = std::max(sampleNum, 1);
+        std::vector<float> r(n * 4);
+        detail::DevBuf po(n * 12), pd(n * 12), pr(n * 16);
+        if (rt != PT) {
+            host_trs_.clear();
+            host_lut_.clear();
+        }
+        if (detail::ok(mrp_memcpy_h2d(po.p, ori.data(), n * 12), "GetRadiances") &&
+            detail::ok(mrp_memcpy_h2d(pd.p, dir.data(), n * 12), "GetRadiances") &&
+            detail::ok(mrp_get_radiances(h_, (int)n, po.f(), pd.f(), &p, pr.f(), nullptr), "GetRadiances") &&
+            detail::ok(mrp_memcpy_d2h(r.data(), pr.p, n * 16), "GetRadiances"))
+            for (size_t i = 0; i < n; ++i) out[i] = {r[4 * i], r[4 * i + 1], r[4 * i + 2]};
+        return out;
+    }
+    // `GetSamples` (core/volume.cu:550-626): the training target of `GetSample`
+    // at each point (high-order in-scattering, sampleNum paths of up to
+    // multiScatter bounces), per-point extinction, light, g and scatter.
+    std::vector<float3> GetSamples(const std::vector<float>& alpha, const std::vector<float3>& ori,
+                                   const std::vector<float3>& dir, const std::vector<float3>& lightDir,
+                                   const std::vector<float>& g, const std::vector<float>& scatter,
+                                   float3 lightColor = {1, 1, 1}, int multiScatter = 1, int sampleNum = 1) {
+        const size_t n = std::min({alpha.size(), ori.size(), dir.size(), lightDir.size(), g.size(), scatter.size()});
+        std::vector<float3> out(n, float3{0, 0, 0});
+        if (!n) return out;
+        std::vector<mrp_sample_query> q(n);
+        for (size_t i = 0; i < n; ++i)
+            q[i] = mrp_sample_query{detail::v(ori[i]), detail::v(dir[i]), detail::v(lightDir[i]), alpha[i], g[i],
+                                    scatter[i]};
+        detail::DevBuf pq(n * sizeof(mrp_sample_query)), po(n * 12);
+        if (detail::ok(mrp_memcpy_h2d(pq.p, q.data(), n * sizeof(mrp_sample_query)), "GetSamples") &&
+            detail::ok(mrp_get_samples(h_, (int)n, (const mrp_sample_query*)pq.p, detail::v(lightColor),
+                                       detail::v(sky_), multiScatter, std::max(sampleNum, 1), seed_, po.f(),
+                                       nullptr), "GetSamples"))
+            detail::ok(mrp_memcpy_d2h(out.data(), po.p, n * 12), "GetSamples");
+        return out;
+    }
 
     // setters of the reference's device globals (core/volume.cu:1460-1493)
     void SetCheckboard(bool on) { checkboard_ = on; }
@@ -165,6 +344,10 @@ class VolumeRender {
                                float g = 0.857f, float alpha = 1, float3 lightColor = {1, 1, 1},
                                int multiScatter = 512, int sampleNum = 1024) {
         std::vector<float3> out((size_t)size.x * size.y, float3{0, 0, 0});
+        if (rt != PT) {
+            host_trs_.clear();
+            host_lut_.clear();
+        }
         mrp_render_params p = params(size, ori, up, right, lightDir, rt, g, alpha, lightColor, multiScatter);
         p.spp_begin = 0;
         p.spp_count = sampleNum;
@@ -190,6 +373,36 @@ class VolumeRender {
     }
 
    private:
+    bool fetch_mips() {
+        if (!host_mips_.empty()) return true;
+        std::vector<std::vector<float>> m(9);
+        for (int k = 0; k < 9; ++k) {
+            const size_t r = (size_t)(256 >> k);
+            m[k].resize(r * r * r);
+            if (!detail::ok(mrp_volume_get_mip(h_, k, m[k].data()), "DensityAtUV")) return false;
+        }
+        host_mips_.swap(m);
+        return true;
+    }
+    bool fetch_trs() {
+        if (!host_trs_.empty()) return true;
+        std::vector<std::vector<float>> m(8);
+        for (int k = 0; k < 8; ++k) {
+            const size_t r = (size_t)(128 >> k);
+            m[k].resize(r * r * r);
+            if (!detail::ok(mrp_volume_get_tr_mip(h_, k, m[k].data()), "TrAtPosition")) return false;
+        }
+        host_trs_.swap(m);
+        return true;
+    }
+    bool fetch_lut() {
+        if (!host_lut_.empty()) return true;
+        std::vector<float> t((size_t)512 * 512);
+        if (!detail::ok(mrp_volume_get_hg_lut(h_, t.data()), "GetHGLut")) return false;
+        host_lut_.swap(t);
+        return true;
+    }
+
     mrp_render_params params(int2 size, float3 ori, float3 up, float3 right, float3 lightDir, RenderType rt, float g,
                              float alpha, float3 lightColor, int multiScatter) const {
         mrp_render_params p;
@@ -264,6 +477,8 @@ class VolumeRender {
     float exposure_ = 1.0f;
     uint64_t seed_ = 0;
     int precision_ = 1;
+    std::vector<std::vector<float>> host_mips_, host_trs_;  // host copies for the queries
+    std::vector<float> host_lut_;
 };
 
 typedef std::function<float3(float3 radiance)> ToneMapFunction;
@@ -320,6 +535,9 @@ class Camera {
         }
     }
     float3 GetPosition() const { return ori_; }
+    // this build's extras: the basis SetPosition computed
+    float3 GetUp() const { return up_; }
+    float3 GetRight() const { return right_; }
     void SetVolume(VolumeRender& v) { volume_ = &v; }
 
     void Render(float3 lightDir, float3 lightColor = {1, 1, 1}, float alpha = 1, int multiScatter = 1, float g = 0,

@@ -128,6 +128,36 @@ def weight_manifest():
     return json.load(open(os.path.join(DATA_DIR, "mrpnn_weights.json")))
 
 
+NATIVE_WEIGHTS = "mrpnn_weights_native"
+
+
+def has_native_weights() -> bool:
+    import os
+    return os.path.exists(os.path.join(DATA_DIR, NATIVE_WEIGHTS + ".bin"))
+
+
+def native_weights_manifest():
+    import json
+    import os
+    return json.load(open(os.path.join(DATA_DIR, NATIVE_WEIGHTS + ".json")))
+
+
+def load_native_weights() -> np.ndarray:
+    """The MRPNN network trained by this library from seeded random weights on
+    its own config-4 data (data/mrpnn_weights_native.bin, same layout as the
+    shipped weights; the manifest holds the recipe, the command that produced
+    the file and its held-out figures); checked against the manifest's sha256."""
+    import hashlib
+    import os
+    raw = open(os.path.join(DATA_DIR, NATIVE_WEIGHTS + ".bin"), "rb").read()
+    if hashlib.sha256(raw).hexdigest() != native_weights_manifest()["sha256"]:
+        raise MrpError(NATIVE_WEIGHTS + ".bin does not match its manifest checksum")
+    w = np.frombuffer(raw, dtype="<f4").copy()
+    if w.size != 49673:
+        raise MrpError(NATIVE_WEIGHTS + ".bin must hold 49,673 FP32 values")
+    return w
+
+
 class Network:
     """Trained MRPNN weights in HBM (FP32 + BF16 MFMA packing)."""
 

@@ -124,6 +124,10 @@ SIGNATURES = {
     "mrp_volume_set_gather_cells": (I, [P, I]),
     "mrp_mlp_forward_tiled": (I, [P, I, P, P, P, F, Vec3, I, P]),
     "mrp_mlp_forward_noalloc": (I, [P, I, P, P, F, Vec3, I, P]),
+    "mrp_sample_training_points": (I, [P, I, F, F, ctypes.c_uint64, P, P, P]),
+    "mrp_gather_rows": (I, [P, P, P, I, I, I]),
+    "mrp_get_radiances": (I, [P, I, P, P, P, P, P]),
+    "mrp_get_shadow_terms": (I, [P, I, P, P, Vec3, F, F, I, ctypes.c_uint64, P, P]),
 }
 
 _lib = None

@@ -70,20 +70,22 @@ def build_lib(jobs: int = 4) -> str:
     return LIB
 
 
-FACADE_SRC = os.path.join(ROOT, "tests", "cpp", "facade_render.cpp")
+FACADE_PROGRAMS = ["facade_render", "facade_queries"]
 FACADE_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "facade_render")
 
 
 def build_facade_test() -> str:
-    """The C++ facade's test program, linked against libmrp.so (test infrastructure)."""
+    """The C++ facade's test programs, linked against libmrp.so (test infrastructure)."""
     os.makedirs(os.path.dirname(FACADE_BIN), exist_ok=True)
     gxx = shutil.which("g++") or "g++"
     lib_dir = os.path.dirname(LIB)
     rel = os.path.relpath(lib_dir, os.path.dirname(FACADE_BIN))
-    tmp = FACADE_BIN + ".tmp"
-    run([gxx, "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I" + os.path.join(ROOT, "include"), FACADE_SRC,
-         LIB, "-Wl,-rpath,$ORIGIN/" + rel, "-o", tmp])
-    os.replace(tmp, FACADE_BIN)
+    for name in FACADE_PROGRAMS:
+        out = os.path.join(os.path.dirname(FACADE_BIN), name)
+        tmp = out + ".tmp"
+        run([gxx, "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I" + os.path.join(ROOT, "include"),
+             os.path.join(ROOT, "tests", "cpp", name + ".cpp"), LIB, "-Wl,-rpath,$ORIGIN/" + rel, "-o", tmp])
+        os.replace(tmp, out)
     return FACADE_BIN
 
 

@@ -88,6 +88,32 @@ int mrp_memcpy_d2d(void* dst, const void* src, size_t bytes) {
     MRP_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
     return MRP_OK;
 }
+// dst row r = src row idx[r] (16-byte words); an index past n_src gives a zero row.
+__global__ void gather_rows_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                   const uint32_t* __restrict__ idx, uint32_t n_rows, uint32_t words,
+                                   uint32_t n_src) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (size_t)n_rows * words) return;
+    const uint32_t r = (uint32_t)(t / words), w = (uint32_t)(t - (size_t)r * words);
+    const uint32_t s = idx[r];
+    dst[t] = s < n_src ? src[(size_t)s * words + w] : make_uint4(0u, 0u, 0u, 0u);
+}
+
+int mrp_gather_rows(void* dst_dev, const void* src_dev, const uint32_t* idx_dev, int n_rows, int row_bytes,
+                    int n_src_rows) {
+    if (n_rows < 0 || n_src_rows < 0) return fail(MRP_E_INVALID, "negative row count");
+    if (n_rows == 0) return MRP_OK;
+    if (!dst_dev || !src_dev || !idx_dev) return fail(MRP_E_INVALID, "NULL pointer");
+    if (row_bytes <= 0 || row_bytes % 16) return fail(MRP_E_INVALID, "row_bytes must be a positive multiple of 16");
+    if (((uintptr_t)dst_dev | (uintptr_t)src_dev) % 16) return fail(MRP_E_INVALID, "rows must be 16-byte aligned");
+    const uint32_t words = (uint32_t)(row_bytes / 16);
+    const size_t total = (size_t)n_rows * words;
+    gather_rows_kernel<<<(unsigned)((total + 255) / 256), 256>>>((uint4*)dst_dev, (const uint4*)src_dev, idx_dev,
+                                                                 (uint32_t)n_rows, words, (uint32_t)n_src_rows);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
+}
+
 int mrp_memset_zero(void* dev, size_t bytes) {
     if (bytes == 0) return MRP_OK;
     if (!dev) return fail(MRP_E_INVALID, "NULL pointer");

@@ -84,6 +84,10 @@ struct Volume {
 struct PixelMap {
     int checker = -1;  // -1: every pixel; 0/1: the checkerboard half of checker_col
     int rows_up = 0;   // 1: progressive camera (row 0 at the bottom)
+    // a ray list instead of camera pixels (width = n rays, height 1): ray i
+    // starts at ray_ori[3i..] along ray_dir[3i..] (HBM)
+    const float* ray_ori = nullptr;
+    const float* ray_dir = nullptr;
 };
 
 inline int mip_res(int m) { return 256 >> m; }
@@ -109,6 +113,10 @@ struct NNTimes {
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
                   uint64_t* n_active, bool defer_last = false);
 int mrpnn_finish(Volume* v, NNTimes* times, uint64_t* n_active);
+// `NNPredict<MRPNN>` on n given rays (one sample each): out_dev n x float4
+// (radiance, first-vertex distance or -1).
+int mrpnn_rays(Volume* v, int n, const float* ori, const float* dir, const mrp_render_params* p, float* out,
+               mrp_trace_stats* stats);
 int clear_counters(Volume* v);
 int read_counters(Volume* v, unsigned long long out[4]);
 int launch_update(Volume* v);

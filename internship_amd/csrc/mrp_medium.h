@@ -456,4 +456,125 @@ MRP_HD void frame_from_normal(RNG* rng, vec3 v1, vec3* a1, vec3* a2, vec3* a3) {
     *a3 = v3_;
 }
 
+// Cube root of x in (0, 1] (the radius of a uniform point in a ball).
+MRP_HD float cbrt01(float x) { return expf_(logf_(x) * (1.0f / 3.0f)); }
+
+// Training points of one random ray, the sampling of `GetDesiredCountSample`
+// and `MeanFreePathSample` (dataGen.cu:174-216): the ray starts on the sphere
+// of radius 0.5, uniform, heading along normalize(r + normalize(-ori)) for a
+// uniform direction r; up to 4 successive delta-tracking vertices (free
+// flight, then an HG direction, `next_vertex`) are visited, and vertex i is
+// kept if i = 0, if it lies more than 1/8 from the previous one, or with
+// probability 0.1.  A kept point is moved by a uniform offset in the ball of
+// radius 1/128 with probability 0.5, and its view direction is the ray's
+// first direction with probability 0.75, else the direction it arrived
+// along.  Draw order per ray: origin (2), r (2), then per vertex the free
+// flight's draws, keep, jitter and view (3), and the offset (3) when moved.
+// The reference draws with one host generator and reuses the step's draw as
+// the acceptance test in its free flight (dataGen.cu:150-165); here every
+// ray has its own stream and the free flight is the renderer's.
+struct TrainPoint {
+    vec3 pos, view;
+    int kept;
+};
+template <class RNG, class NextVertex>
+MRP_HD int training_ray_points(RNG* rng, const NextVertex& next_vertex, TrainPoint out[4]) {
+    for (int i = 0; i < 4; ++i) {
+        out[i].pos = v3(0.0f, 0.0f, 0.0f);
+        out[i].view = v3(0.0f, 0.0f, 0.0f);
+        out[i].kept = 0;
+    }
+    const float a0 = rng_next(rng);
+    const float a1 = rng_next(rng);
+    const vec3 ori = uniform_sphere(a0, a1) * 0.5f;
+    const float b0 = rng_next(rng);
+    const float b1 = rng_next(rng);
+    const vec3 dir = normalize(uniform_sphere(b0, b1) + normalize(-ori));
+    const float off = ray_box_offset(ori, dir);
+    if (off < 0.0f) return 0;
+    vec3 sp = ori + dir * off, rd = dir;
+    int kept = 0;
+    for (int i = 0; i < 4; ++i) {
+        vec3 np, nd;
+        if (!next_vertex(rng, sp, rd, ray_box_distance(sp, rd), &np, &nd)) break;
+        const vec3 d = sp - np;
+        const float u_keep = rng_next(rng);
+        const float u_jit = rng_next(rng);
+        const float u_view = rng_next(rng);
+        if (i == 0 || dot(d, d) > 1.0f / 64.0f || u_keep > 0.9f) {
+            vec3 p = np;
+            if (u_jit > 0.5f) {
+                const float j0 = rng_next(rng);
+                const float j1 = rng_next(rng);
+                const float j2 = rng_next(rng);
+                p = np + uniform_sphere(j0, j1) * (cbrt01(j2) * (1.0f / 128.0f));
+            }
+            out[i].pos = p;
+            out[i].view = u_view > 0.25f ? dir : rd;
+            out[i].kept = 1;
+            ++kept;
+        }
+        sp = np;
+        rd = nd;
+    }
+    return kept;
+}
+
+// `CalculateRadiance` (core/render.cuh:200-258) for one given ray with
+// `sampleNum` = spp paths drawn one after another from the same stream: the
+// mean radiance and the distance to the first scattering vertex of the last
+// path that scattered (-1 if none).  next_vertex / transmittance are the
+// caller's free flight and `Tr` (the renderer's cells on the GPU, the grid in
+// the oracle).
+template <class RNG, class NextVertex, class Transmit>
+MRP_HD vec3 radiance_paths(RNG* rng, const NextVertex& next_vertex, const Transmit& transmittance, vec3 ori,
+                           vec3 dir, vec3 light_dir, vec3 light_color, vec3 sky, vec3 scatter_rate, float g,
+                           int multi_scatter, int spp, float* first_t) {
+    dir = normalize(dir);
+    light_dir = normalize(light_dir);
+    float t = -1.0f;
+    const float dis = ray_box_offset(ori, dir);
+    if (dis < 0.0f) {
+        *first_t = t;
+        return sky;
+    }
+    vec3 res = v3(0.0f, 0.0f, 0.0f);
+    for (int i = 0; i < spp; ++i) {
+        vec3 sample_pos = ori + dir * dis;
+        vec3 ray_dir = dir;
+        vec3 csr = v3(1.0f, 1.0f, 1.0f);
+        for (int k = 0; k < multi_scatter; ++k) {
+            vec3 np, nd;
+            if (!next_vertex(rng, sample_pos, ray_dir, ray_box_distance(sample_pos, ray_dir), &np, &nd)) {
+                res = res + sky * csr;
+                break;
+            }
+            if (k == 0) t = distance(ori, np);
+            sample_pos = np;
+            csr = csr * scatter_rate;
+            const float phase = hg_phase(dot(ray_dir, light_dir), g);
+            const float tr = transmittance(rng, sample_pos, light_dir, ray_box_distance(sample_pos, light_dir));
+            res = res + light_color * csr * (tr * phase);
+            ray_dir = nd;
+        }
+    }
+    *first_t = t;
+    return res / (float)spp;
+}
+
+// `ShadowTerm_TR` / `ShadowTerm_TRs` (core/render.cuh:317-341): the mean over
+// spp estimates, drawn one after another from one stream, of Tr(pos -> light)
+// * HG(dir . light, g); dir is used as given (the reference's GetTrs passes
+// an offset direction that need not be unit).
+template <class RNG, class Transmit>
+MRP_HD float shadow_term_paths(RNG* rng, const Transmit& transmittance, vec3 pos, vec3 dir, vec3 light_dir, float g,
+                               int spp) {
+    light_dir = normalize(light_dir);
+    const float dis = ray_box_distance(pos, light_dir);
+    const float phase = hg_phase(dot(dir, light_dir), g);
+    float res = 0.0f;
+    for (int i = 0; i < spp; ++i) res += transmittance(rng, pos, light_dir, dis) * phase;
+    return res / (float)spp;
+}
+
 }  // namespace mrp

@@ -50,6 +50,8 @@ struct NNArgs {
     vec3 ori, forward, right, up, light, lx, light_color, sky, scatter_rate;
     int width, height;
     int checker, rows_up, wl;  // pixel selection (checker_col), camera row convention, logical width
+    const float* ray_ori;      // a ray list instead of camera pixels (n x 3 each), or nullptr
+    const float* ray_dir;
     uint32_t seed_lo, seed_hi;
     int spp_begin;       // absolute sample index of item 0's sample
     uint32_t item0;      // first (sample-major) item of this chunk within the batch
@@ -92,7 +94,12 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
         const uint32_t item = a.item0 + i;
         const uint32_t s = item / Pl, l = item - s * Pl;
         int row, col_l;
-        block_order(l, (uint32_t)a.wl, (uint32_t)a.height, &row, &col_l);  // waves cover 8 x 8 pixel blocks
+        if (a.ray_ori) {  // a ray list: item l is ray l
+            row = 0;
+            col_l = (int)l;
+        } else {
+            block_order(l, (uint32_t)a.wl, (uint32_t)a.height, &row, &col_l);  // waves cover 8 x 8 pixel blocks
+        }
         const int col = checker_col(row, col_l, a.checker);
         if (col >= a.width) {
             pos_act[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -102,16 +109,22 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
         } else {
             const uint32_t pix = (uint32_t)row * (uint32_t)a.width + (uint32_t)col;
             Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
-            const float xu = rng_next(&rng);
-            const float xv = rng_next(&rng);
-            vec3 dir = camera_dir_rows(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv, a.rows_up);
+            vec3 ori = a.ori, dir;
+            if (a.ray_ori) {
+                ori = v3(a.ray_ori[3 * (size_t)pix], a.ray_ori[3 * (size_t)pix + 1], a.ray_ori[3 * (size_t)pix + 2]);
+                dir = v3(a.ray_dir[3 * (size_t)pix], a.ray_dir[3 * (size_t)pix + 1], a.ray_dir[3 * (size_t)pix + 2]);
+            } else {
+                const float xu = rng_next(&rng);
+                const float xv = rng_next(&rng);
+                dir = camera_dir_rows(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv, a.rows_up);
+            }
             dir = normalize(dir);  // NNPredict
             const vec3 gdir = normalize(dir);  // GetSamplePoint
             vec3 pos = v3(0, 0, 0);
             int active = 0;
-            const float off = ray_box_offset(a.ori, gdir);
+            const float off = ray_box_offset(ori, gdir);
             if (off >= 0.0f) {
-                const vec3 sp = a.ori + gdir * off;
+                const vec3 sp = ori + gdir * off;
                 vec3 np;
                 const CellsFetch F{a.cells, a.res};
                 if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
@@ -133,7 +146,7 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
                 phase = hg_phase(dot(X, LX), a.g);
                 rng_pos[i] = rng.n;
             }
-            const float dist = active ? distance(a.ori, pos) : -1.0f;
+            const float dist = active ? distance(ori, pos) : -1.0f;
             pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
             fx[i] = make_float4(X.x, X.y, X.z, phase);
             fy[i] = make_float4(Y.x, Y.y, Y.z, __uint_as_float(s * P + pix));
@@ -596,6 +609,8 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     a.height = H;
     a.checker = map.checker;
     a.rows_up = map.rows_up;
+    a.ray_ori = map.ray_ori;
+    a.ray_dir = map.ray_dir;
     a.wl = wl;
     a.seed_lo = (uint32_t)p->seed;
     a.seed_hi = (uint32_t)(p->seed >> 32);
@@ -751,6 +766,51 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
         st->total_ms = total_ms;
         st->launches = 0;
         st->grid_blocks = 0;
+        st->first_vertex_ms = tm.first_vertex;
+        st->shadow_ms = tm.shadow;
+        st->compact_ms = tm.compact;
+        st->gather_ms = tm.gather;
+    }
+    return MRP_OK;
+}
+
+int mrpnn_rays(Volume* v, int n, const float* ori, const float* dir, const mrp_render_params* p, float* out,
+               mrp_trace_stats* st) {
+    if (!ori || !dir) return fail(MRP_E_INVALID, "NULL argument");
+    if (n < 0 || n > (1 << 28)) return fail(MRP_E_INVALID, "n must be in [0, 2^28]");
+    MRP_HIP(hipSetDevice(v->device));
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() must run first");
+    if (n == 0) return MRP_OK;
+    // `NNPredict` draws one sample per call (core/render.cuh:469-523): sample 0 of ray i's stream
+    mrp_render_params q = *p;
+    q.width = n;
+    q.height = 1;
+    q.spp_begin = 0;
+    q.spp_count = 1;
+    int rc = check_params(v, &q);
+    if (rc) return rc;
+    rc = ensure_samples(v, (size_t)n * 16);
+    if (rc) return rc;
+    rc = clear_counters(v);
+    if (rc) return rc;
+    PixelMap map;
+    map.ray_ori = ori;
+    map.ray_dir = dir;
+    NNTimes tm;
+    uint64_t n_active = 0;
+    rc = mrpnn_samples(v, &q, map, 0, 1, &tm, &n_active, false);
+    if (rc) return rc;
+    MRP_HIP(hipMemcpyAsync(out, v->samples, (size_t)n * 16, hipMemcpyDeviceToDevice, v->stream));
+    unsigned long long cnt4[4] = {0, 0, 0, 0};
+    rc = read_counters(v, cnt4);
+    if (rc) return rc;
+    if (st) {
+        *st = mrp_trace_stats{};
+        st->lookups = cnt4[0] + cnt4[2];
+        st->shadow_lookups = cnt4[2];
+        st->paths = (uint64_t)n;
+        st->scatters = n_active;
+        st->kernel_ms = tm.network;
         st->first_vertex_ms = tm.first_vertex;
         st->shadow_ms = tm.shadow;
         st->compact_ms = tm.compact;

@@ -37,8 +37,8 @@ struct SampleArgs {
 };
 
 // `DeterminateNextVertex` (core/render.cuh:138-172) on the corner-expanded cells.
-__device__ bool gs_next_vertex(Rng* rng, const SampleArgs& a, const Majorant& mj, float g, vec3 pos, vec3 dir,
-                               float dis, vec3* npos, vec3* ndir, uint32_t* lk) {
+__device__ bool gs_next_vertex(Rng* rng, const float4* __restrict__ cells, int res, const Majorant& mj, float g,
+                               vec3 pos, vec3 dir, float dis, vec3* npos, vec3* ndir, uint32_t* lk) {
     float t = 0.0f;
     int loop_num = 0;
     while (loop_num++ < 10000) {
@@ -46,7 +46,7 @@ __device__ bool gs_next_vertex(Rng* rng, const SampleArgs& a, const Majorant& mj
         t -= logf_(1.0f - rk) * mj.inv_smax;
         if (t > dis) return false;
         rk = rng_next(rng);
-        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
+        const float d = trilinear_cells(cells, res, pos + (dir * t));
         *lk += 1u;
         if (d * mj.ratio > rk) break;
         if (d < 0.0f) t -= d;
@@ -59,15 +59,15 @@ __device__ bool gs_next_vertex(Rng* rng, const SampleArgs& a, const Majorant& mj
 }
 
 // `Tr` (core/render.cuh:93-136).
-__device__ float gs_transmittance(Rng* rng, const SampleArgs& a, const Majorant& mj, vec3 pos, vec3 dir, float dis,
-                                  uint32_t* lk) {
+__device__ float gs_transmittance(Rng* rng, const float4* __restrict__ cells, int res, const Majorant& mj, vec3 pos,
+                                  vec3 dir, float dis, uint32_t* lk) {
     float tr = 1.0f, t = 0.0f;
     int loop_num = 0;
     while (loop_num++ < 10000 && tr > 0.5f) {
         const float rk = rng_next(rng);
         t -= logf_(1.0f - rk) * mj.inv_smax_tr;
         if (t > dis) break;
-        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
+        const float d = trilinear_cells(cells, res, pos + (dir * t));
         *lk += 1u;
         tr *= 1.0f - fmax_(0.0f, d * mj.ratio);
         if (d < 0.0f) t -= d;
@@ -76,7 +76,7 @@ __device__ float gs_transmittance(Rng* rng, const SampleArgs& a, const Majorant&
         const float rk = rng_next(rng);
         t -= logf_(1.0f - rk) * mj.inv_smax_tr;
         if (t > dis) break;
-        const float d = trilinear_cells(a.cells, a.res, pos + (dir * t));
+        const float d = trilinear_cells(cells, res, pos + (dir * t));
         *lk += 1u;
         if (rng_next(rng) < fmax_(0.0f, d * mj.ratio)) {
             tr = 0.0f;
@@ -104,14 +104,14 @@ __global__ __launch_bounds__(256) void get_sample_kernel(SampleArgs a) {
         vec3 csr = v3(1, 1, 1), res = v3(0, 0, 0);
         for (int s = 0; s < a.multi_scatter; ++s) {
             vec3 np, nd;
-            if (!gs_next_vertex(&rng, a, mj, q.g, pos, ray, ray_box_distance(pos, ray), &np, &nd, &lk)) {
+            if (!gs_next_vertex(&rng, a.cells, a.res, mj, q.g, pos, ray, ray_box_distance(pos, ray), &np, &nd, &lk)) {
                 res = res + a.sky * csr;
                 break;
             }
             pos = np;
             csr = (csr * 1.001f) * q.scatter;
             const float phase = hg_phase(dot(ray, light), q.g);
-            const float tr = gs_transmittance(&rng, a, mj, pos, light, ray_box_distance(pos, light), &lk);
+            const float tr = gs_transmittance(&rng, a.cells, a.res, mj, pos, light, ray_box_distance(pos, light), &lk);
             res = res + a.light_color * csr * (tr * phase);
             ray = nd;
         }
@@ -150,6 +150,108 @@ __global__ __launch_bounds__(256) void sample_sum_kernel(const float* __restrict
         out[3 * j + 1] = s.y / d;
         out[3 * j + 2] = s.z / d;
     }
+}
+
+// Training points (`MeanFreePathSample`, dataGen.cu:174-216) of n_rays random
+// rays, one thread per ray: slots 4r..4r+3 hold the kept vertices of ray r
+// (pos.w = 1 when kept, 0 otherwise; view.w = 0).  Ray r draws from Philox
+// stream (r, 0, tag 2, seed).
+__global__ __launch_bounds__(256) void training_points_kernel(const float4* __restrict__ cells, int res,
+                                                              Majorant mj, float g, int n_rays, uint32_t seed_lo,
+                                                              uint32_t seed_hi, float4* __restrict__ pos_out,
+                                                              float4* __restrict__ view_out,
+                                                              unsigned long long* __restrict__ counters) {
+    __shared__ unsigned red[256];
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lk = 0u;
+    if (r < (uint32_t)n_rays) {
+        Rng rng = rng_make(r, 0u, 2u, ((uint64_t)seed_hi << 32) | seed_lo);
+        const auto nv = [&](Rng* rg, vec3 p, vec3 d, float dis, vec3* np, vec3* nd) {
+            return gs_next_vertex(rg, cells, res, mj, g, p, d, dis, np, nd, &lk);
+        };
+        TrainPoint tp[4];
+        training_ray_points(&rng, nv, tp);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            pos_out[4 * (size_t)r + i] = make_float4(tp[i].pos.x, tp[i].pos.y, tp[i].pos.z, tp[i].kept ? 1.0f : 0.0f);
+            view_out[4 * (size_t)r + i] = make_float4(tp[i].view.x, tp[i].view.y, tp[i].view.z, 0.0f);
+        }
+    }
+    red[threadIdx.x] = lk;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&counters[0], (unsigned long long)red[0]);
+}
+
+// Lookup counts of a workgroup into counters[0] (one atomic per workgroup).
+__device__ __forceinline__ void add_lookups(unsigned* red, uint32_t lk, unsigned long long* counters) {
+    red[threadIdx.x] = lk;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&counters[0], (unsigned long long)red[0]);
+}
+
+struct RayArgs {
+    const float4* cells;
+    int res;
+    Majorant mj;
+    float g;
+    vec3 light, light_color, sky, scatter_rate;
+    int multi_scatter, spp, n;
+    uint32_t seed_lo, seed_hi;
+    const float* ori;  // n x 3
+    const float* dir;  // n x 3
+    unsigned long long* counters;
+};
+
+// `CalculateRadianceMulti<PT>` (core/volume.cu:17-34): one thread per given
+// ray, spp paths on Philox stream (ray, 0, tag 4, seed); out[i] = (mean
+// radiance, first-scattering distance or -1).
+__global__ __launch_bounds__(256) void radiance_rays_kernel(RayArgs a, float4* __restrict__ out) {
+    __shared__ unsigned red[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lk = 0u;
+    if (i < (uint32_t)a.n) {
+        Rng rng = rng_make(i, 0u, 4u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        const auto nv = [&](Rng* rg, vec3 p, vec3 d, float dis, vec3* np, vec3* nd) {
+            return gs_next_vertex(rg, a.cells, a.res, a.mj, a.g, p, d, dis, np, nd, &lk);
+        };
+        const auto tr = [&](Rng* rg, vec3 p, vec3 d, float dis) {
+            return gs_transmittance(rg, a.cells, a.res, a.mj, p, d, dis, &lk);
+        };
+        const vec3 o = v3(a.ori[3 * (size_t)i], a.ori[3 * (size_t)i + 1], a.ori[3 * (size_t)i + 2]);
+        const vec3 d = v3(a.dir[3 * (size_t)i], a.dir[3 * (size_t)i + 1], a.dir[3 * (size_t)i + 2]);
+        float t;
+        const vec3 r = radiance_paths(&rng, nv, tr, o, d, a.light, a.light_color, a.sky, a.scatter_rate, a.g,
+                                      a.multi_scatter, a.spp, &t);
+        out[i] = make_float4(r.x, r.y, r.z, t);
+    }
+    add_lookups(red, lk, a.counters);
+}
+
+// `GetTrMulti` / `CalculateShadowTerm_TR` (core/volume.cu:297-300,630-636):
+// one thread per point, spp estimates of Tr * HG on Philox stream (point, 0,
+// tag 5, seed).
+__global__ __launch_bounds__(256) void shadow_terms_kernel(RayArgs a, float* __restrict__ out) {
+    __shared__ unsigned red[256];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t lk = 0u;
+    if (i < (uint32_t)a.n) {
+        Rng rng = rng_make(i, 0u, 5u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        const auto tr = [&](Rng* rg, vec3 p, vec3 d, float dis) {
+            return gs_transmittance(rg, a.cells, a.res, a.mj, p, d, dis, &lk);
+        };
+        const vec3 o = v3(a.ori[3 * (size_t)i], a.ori[3 * (size_t)i + 1], a.ori[3 * (size_t)i + 2]);
+        const vec3 d = v3(a.dir[3 * (size_t)i], a.dir[3 * (size_t)i + 1], a.dir[3 * (size_t)i + 2]);
+        out[i] = shadow_term_paths(&rng, tr, o, d, a.light, a.g, a.spp);
+    }
+    add_lookups(red, lk, a.counters);
 }
 
 __global__ void sample_clear_kernel(float* p, size_t n) {
@@ -230,5 +332,108 @@ extern "C" int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q
         stats->kernel_ms = ms;
         stats->total_ms = ms;
     }
+    return MRP_OK;
+}
+
+extern "C" int mrp_sample_training_points(mrp_volume* vol, int n_rays, float alpha, float g, uint64_t seed,
+                                          float* pos_dev, float* view_dev, uint64_t* lookups) {
+    if (!vol || (n_rays > 0 && (!pos_dev || !view_dev))) return fail(MRP_E_INVALID, "NULL argument");
+    if (n_rays < 0 || n_rays > (1 << 28)) return fail(MRP_E_INVALID, "n_rays must be in [0, 2^28]");
+    if (!(alpha > 0.0f)) return fail(MRP_E_INVALID, "alpha must be > 0");
+    Volume* v = &vol->v;
+    MRP_HIP(hipSetDevice(v->device));
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() must run before sampling");
+    if (n_rays == 0) {
+        if (lookups) *lookups = 0;
+        return MRP_OK;
+    }
+    int rc = clear_counters(v);
+    if (rc) return rc;
+    const Majorant mj = make_majorant(v->max_density, alpha, 1.0f);
+    training_points_kernel<<<(unsigned)((n_rays + 255) / 256), 256, 0, v->stream>>>(
+        v->cells, v->res, mj, g, n_rays, (uint32_t)seed, (uint32_t)(seed >> 32), (float4*)pos_dev, (float4*)view_dev,
+        v->counters);
+    MRP_HIP(hipGetLastError());
+    unsigned long long cnt4[4] = {0, 0, 0, 0};
+    rc = read_counters(v, cnt4);
+    if (rc) return rc;
+    if (lookups) *lookups = cnt4[0];
+    return MRP_OK;
+}
+
+static int ray_args(Volume* v, int n, const float* ori, const float* dir, vec3 light, float alpha, float tr_scale,
+                    float g, int spp, uint64_t seed, RayArgs* a) {
+    if (!ori || !dir) return fail(MRP_E_INVALID, "NULL argument");
+    if (n < 0 || n > (1 << 28)) return fail(MRP_E_INVALID, "n must be in [0, 2^28]");
+    if (spp <= 0 || spp > (1 << 24)) return fail(MRP_E_INVALID, "spp must be in [1, 2^24]");
+    if (!(alpha > 0.0f)) return fail(MRP_E_INVALID, "alpha must be > 0");
+    MRP_HIP(hipSetDevice(v->device));
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() must run first");
+    a->cells = v->cells;
+    a->res = v->res;
+    a->mj = make_majorant(v->max_density, alpha, tr_scale);
+    a->g = g;
+    a->light = normalize(light);
+    a->spp = spp;
+    a->n = n;
+    a->seed_lo = (uint32_t)seed;
+    a->seed_hi = (uint32_t)(seed >> 32);
+    a->ori = ori;
+    a->dir = dir;
+    a->counters = v->counters;
+    return MRP_OK;
+}
+
+extern "C" int mrp_get_radiances(mrp_volume* vol, int n, const float* ori_dev, const float* dir_dev,
+                                 const mrp_render_params* p, float* out_dev, mrp_trace_stats* stats) {
+    if (!vol || !p || (n > 0 && !out_dev)) return fail(MRP_E_INVALID, "NULL argument");
+    Volume* v = &vol->v;
+    if (p->render_type == MRP_RENDER_MRPNN) return mrpnn_rays(v, n, ori_dev, dir_dev, p, out_dev, stats);
+    if (p->render_type != MRP_RENDER_PT) return fail(MRP_E_INVALID, "render_type must be PT or MRPNN");
+    if (p->multi_scatter < 0) return fail(MRP_E_INVALID, "multi_scatter must be >= 0");
+    RayArgs a;
+    int rc = ray_args(v, n, ori_dev, dir_dev, v3(p->light_dir.x, p->light_dir.y, p->light_dir.z), p->alpha,
+                      p->tr_scale, p->g, p->spp_count, p->seed, &a);
+    if (rc || n == 0) return rc;
+    a.light_color = v3(p->light_color.x, p->light_color.y, p->light_color.z);
+    a.sky = v3(p->sky.x, p->sky.y, p->sky.z);
+    a.scatter_rate = v3(p->scatter_rate.x, p->scatter_rate.y, p->scatter_rate.z);
+    a.multi_scatter = p->multi_scatter;
+    rc = clear_counters(v);
+    if (rc) return rc;
+    radiance_rays_kernel<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(a, (float4*)out_dev);
+    MRP_HIP(hipGetLastError());
+    unsigned long long cnt4[4] = {0, 0, 0, 0};
+    rc = read_counters(v, cnt4);
+    if (rc) return rc;
+    if (stats) {
+        *stats = mrp_trace_stats{};
+        stats->lookups = cnt4[0];
+        stats->paths = (uint64_t)n * (uint64_t)p->spp_count;
+    }
+    return MRP_OK;
+}
+
+extern "C" int mrp_get_shadow_terms(mrp_volume* vol, int n, const float* pos_dev, const float* dir_dev,
+                                    mrp_vec3 light_dir, float alpha, float g, int spp, uint64_t seed, float* out_dev,
+                                    uint64_t* lookups) {
+    if (!vol || (n > 0 && !out_dev)) return fail(MRP_E_INVALID, "NULL argument");
+    Volume* v = &vol->v;
+    RayArgs a;
+    int rc = ray_args(v, n, pos_dev, dir_dev, v3(light_dir.x, light_dir.y, light_dir.z), alpha, 1.0f, g, spp, seed,
+                      &a);
+    if (rc) return rc;
+    if (n == 0) {
+        if (lookups) *lookups = 0;
+        return MRP_OK;
+    }
+    rc = clear_counters(v);
+    if (rc) return rc;
+    shadow_terms_kernel<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(a, out_dev);
+    MRP_HIP(hipGetLastError());
+    unsigned long long cnt4[4] = {0, 0, 0, 0};
+    rc = read_counters(v, cnt4);
+    if (rc) return rc;
+    if (lookups) *lookups = cnt4[0];
     return MRP_OK;
 }

@@ -265,6 +265,52 @@ def histo_distance(a, b):
     return d.value, wt.value
 
 
+def training_points(grid, max_dens, n_rays, alpha, g, seed=0, threads=0):
+    """`MeanFreePathSample` points of n_rays rays: (pos (n,4,4), view (n,4,4), lookups)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    pos = np.zeros((n_rays, 4, 4), np.float32)
+    view = np.zeros((n_rays, 4, 4), np.float32)
+    lk = ctypes.c_uint64()
+    rc = load().oracle_training_points(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.c_float(max_dens),
+                                       ctypes.c_int(n_rays), ctypes.c_float(alpha), ctypes.c_float(g),
+                                       ctypes.c_uint64(seed), _ptr(pos), _ptr(view), ctypes.byref(lk),
+                                       ctypes.c_int(threads))
+    assert rc == 0
+    return pos, view, lk.value
+
+
+def radiances(grid, max_dens, ori, dirs, params, mips_concat=None, tr_concat=None, lut=None, weights=None, threads=0):
+    """`GetRadiances` on the CPU: (out (n,4) = radiance, first distance; lookups)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    o = np.ascontiguousarray(ori, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((o.shape[0], 4), np.float32)
+    lk = ctypes.c_uint64()
+    c = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+    rc = load().oracle_radiances(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.c_float(max_dens),
+                                 _ptr(c(mips_concat)), _ptr(c(tr_concat)), _ptr(c(lut)), _ptr(c(weights)),
+                                 ctypes.c_int(o.shape[0]), _ptr(o), _ptr(d), ctypes.byref(params), _ptr(out),
+                                 ctypes.byref(lk), ctypes.c_int(threads))
+    assert rc == 0
+    return out, lk.value
+
+
+def shadow_terms(grid, max_dens, pos, dirs, light_dir, alpha, g, spp, seed=0, threads=0):
+    """`GetTr`/`GetTrs` estimator on the CPU: (mean Tr * HG per point (n,), lookups)."""
+    from internship_amd._lib import Vec3
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    p = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+    d = np.ascontiguousarray(dirs, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros(p.shape[0], np.float32)
+    lk = ctypes.c_uint64()
+    rc = load().oracle_shadow_terms(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.c_float(max_dens),
+                                    ctypes.c_int(p.shape[0]), _ptr(p), _ptr(d), Vec3.of(light_dir),
+                                    ctypes.c_float(alpha), ctypes.c_float(g), ctypes.c_int(spp),
+                                    ctypes.c_uint64(seed), _ptr(out), ctypes.byref(lk), ctypes.c_int(threads))
+    assert rc == 0
+    return out, lk.value
+
+
 def reproject(points, last_ori, last_up, last_right, w, h):
     from internship_amd._lib import Vec3
     pts = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)

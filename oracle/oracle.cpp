@@ -1095,6 +1095,152 @@ int oracle_render_frame(const float* grid, int res, float max_density, const flo
                                predict ? 1 : 0, fp->tone, fp->exposure, bgra);
 }
 
+// Training points (`GetDesiredCountSample` + `MeanFreePathSample`,
+// dataGen.cu:174-216) of n_rays rays, ray r on Philox stream (r, 0, tag 2,
+// seed): the shared `training_ray_points` over this oracle's free flight.
+// pos / view: n_rays x 4 x 4 floats (pos.w = 1 when the slot holds a point).
+int oracle_training_points(const float* grid, int res, float max_density, int n_rays, float alpha, float g,
+                           uint64_t seed, float* pos, float* view, uint64_t* lookups_out, int threads) {
+    if (!grid || !pos || !view || n_rays < 0) return -1;
+    Medium m;
+    m.grid = OGrid{grid, res};
+    m.max_density = max_density;
+    m.alpha = alpha;
+    m.g = g;
+    m.tr_scale = 1.0f;
+    m.mj = make_majorant(max_density, alpha, 1.0f);
+    uint64_t total = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : total)
+#endif
+    for (int r = 0; r < n_rays; ++r) {
+        Counters c;
+        Rng rng = rng_make((uint32_t)r, 0u, 2u, seed);
+        const auto nv = [&](Rng* rg, vec3 p, vec3 d, float dis, vec3* np, vec3* nd) {
+            return next_vertex(rg, m, m.g, p, d, dis, np, nd, &c);
+        };
+        TrainPoint tp[4];
+        training_ray_points(&rng, nv, tp);
+        for (int i = 0; i < 4; ++i) {
+            float* pp = pos + 16 * (size_t)r + 4 * i;
+            float* vv = view + 16 * (size_t)r + 4 * i;
+            pp[0] = tp[i].pos.x;
+            pp[1] = tp[i].pos.y;
+            pp[2] = tp[i].pos.z;
+            pp[3] = tp[i].kept ? 1.0f : 0.0f;
+            vv[0] = tp[i].view.x;
+            vv[1] = tp[i].view.y;
+            vv[2] = tp[i].view.z;
+            vv[3] = 0.0f;
+        }
+        total += c.lookups;
+    }
+    if (lookups_out) *lookups_out = total;
+    return 0;
+}
+
+// `GetRadiances` (core/volume.cu:329-402) on n given rays: PT - the shared
+// `radiance_paths` (spp = p->spp_count paths on Philox stream (ray, 0, tag 4,
+// seed)); MRPNN - `NNPredict` (core/render.cuh:469-523) once per ray on stream
+// (ray, 0, tag 0, seed), no camera jitter.  out: n x 4 (radiance, distance to
+// the first scattering vertex or -1).
+int oracle_radiances(const float* grid, int res, float max_density, const float* mips, const float* trs,
+                     const float* lut, const float* weights, int n, const float* ori, const float* dir,
+                     const mrp_render_params* p, float* out, uint64_t* lookups_out, int threads) {
+    if (!grid || !ori || !dir || !p || !out || n < 0) return -1;
+    const bool predict = p->render_type == MRP_RENDER_MRPNN;
+    if (predict && (!mips || !trs || !lut || !weights)) return -1;
+    Medium m;
+    m.grid = OGrid{grid, res};
+    m.max_density = max_density;
+    m.alpha = p->alpha;
+    m.g = p->g;
+    m.tr_scale = p->tr_scale;
+    m.sky = V(p->sky);
+    m.scatter_rate = V(p->scatter_rate);
+    m.light_color = V(p->light_color);
+    m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
+    const vec3 light = V(p->light_dir), Ln = normalize(light);
+    const vec3 sr = v3(m.scatter_rate.x / 1.001f, m.scatter_rate.y / 1.001f, m.scatter_rate.z / 1.001f);
+    mrp_vec3 srp;
+    srp.x = (float)pow((double)sr.x, 4.0);
+    srp.y = (float)pow((double)sr.y, 4.0);
+    srp.z = (float)pow((double)sr.z, 4.0);
+    uint64_t total = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+#endif
+    for (int i = 0; i < n; ++i) {
+        Counters c;
+        const vec3 o = v3(ori[3 * i], ori[3 * i + 1], ori[3 * i + 2]);
+        const vec3 d = v3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]);
+        vec3 rgb;
+        float t;
+        if (!predict) {
+            Rng rng = rng_make((uint32_t)i, 0u, 4u, p->seed);
+            const auto nv = [&](Rng* rg, vec3 pp, vec3 dd, float dis, vec3* np, vec3* nd) {
+                return next_vertex(rg, m, m.g, pp, dd, dis, np, nd, &c);
+            };
+            const auto tr = [&](Rng* rg, vec3 pp, vec3 dd, float dis) { return transmittance(rg, m, pp, dd, dis, &c); };
+            rgb = radiance_paths(&rng, nv, tr, o, d, light, m.light_color, m.sky, m.scatter_rate, m.g, p->multi_scatter,
+                                 p->spp_count, &t);
+        } else {
+            Rng rng = rng_make((uint32_t)i, 0u, 0u, p->seed);
+            const FirstVertex fv = first_vertex(&rng, m, o, d, light, &c);
+            if (fv.active) {
+                float feat[kFeatures], gam, pr[3];
+                mrp_vec3 Lm{Ln.x, Ln.y, Ln.z}, LXm{fv.LX.x, fv.LX.y, fv.LX.z};
+                oracle_mrpnn_features(mips, trs, lut, 1, &fv.pos.x, &fv.X.x, &fv.Y.x, &fv.Z.x, &fv.active, Lm, LXm,
+                                      m.alpha, m.g, feat, &gam);
+                oracle_mrpnn_forward(weights, 1, feat, &gam, m.g, srp, pr);
+                rgb = v3(pr[0] + fv.single, pr[1] + fv.single, pr[2] + fv.single) * m.light_color * m.scatter_rate;
+            } else {
+                rgb = m.sky;
+            }
+            t = fv.dist;
+        }
+        out[4 * i] = rgb.x;
+        out[4 * i + 1] = rgb.y;
+        out[4 * i + 2] = rgb.z;
+        out[4 * i + 3] = t;
+        total += c.lookups;
+    }
+    if (lookups_out) *lookups_out = total;
+    return 0;
+}
+
+// `GetTr` / `GetTrs` (core/volume.cu:302-323,636-673): the shared
+// `shadow_term_paths`, point i on Philox stream (i, 0, tag 5, seed).
+int oracle_shadow_terms(const float* grid, int res, float max_density, int n, const float* pos, const float* dir,
+                        mrp_vec3 light_dir, float alpha, float g, int spp, uint64_t seed, float* out,
+                        uint64_t* lookups_out, int threads) {
+    if (!grid || !pos || !dir || !out || n < 0 || spp <= 0) return -1;
+    Medium m;
+    m.grid = OGrid{grid, res};
+    m.max_density = max_density;
+    m.alpha = alpha;
+    m.g = g;
+    m.tr_scale = 1.0f;
+    m.mj = make_majorant(max_density, alpha, 1.0f);
+    uint64_t total = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total)
+#endif
+    for (int i = 0; i < n; ++i) {
+        Counters c;
+        Rng rng = rng_make((uint32_t)i, 0u, 5u, seed);
+        const auto tr = [&](Rng* rg, vec3 pp, vec3 dd, float dis) { return transmittance(rg, m, pp, dd, dis, &c); };
+        out[i] = shadow_term_paths(&rng, tr, v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]),
+                                   v3(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]), V(light_dir), g, spp);
+        total += c.lookups;
+    }
+    if (lookups_out) *lookups_out = total;
+    return 0;
+}
+
 // Exposure, tone curve and pack of n colours (`Denoise` tail, core/volume.cu:248-263).
 int oracle_tone_pack(const float* rgb, int n, float exposure, int tone, uint32_t* out) {
     for (int i = 0; i < n; ++i) out[i] = tone_pack(v3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]), exposure, tone);

@@ -35,6 +35,16 @@
  *                                                                    quadrature per channel (light colour, albedo);
  *                                                                    channel ratios; static-camera MRPNN frames =
  *                                                                    closed-form blend of 1-frame renders
+ * oracle_training_points   dataGen.cu:174-216 (MeanFreePathSample)    empty volume = no points; dense medium =
+ *                                                                    uniform origins on the r = 0.5 sphere,
+ *                                                                    keep / jitter / view-direction rates
+ * oracle_radiances         core/volume.cu:17-34,329-402 (GetRadiances) PT: ray missing the box = sky, empty
+ *                                                                    volume = sky, albedo 0 = Beer-Lambert,
+ *                                                                    one bounce = single-scatter quadrature;
+ *                                                                    MRPNN: equals oracle_render_mrpnn's
+ *                                                                    composition on the same first vertices
+ * oracle_shadow_terms      core/render.cuh:317-341 (GetTr/GetTrs)   Tr * HG: homogeneous box = exp(-sigma d) * HG
+ *                                                                    within Monte Carlo error; empty = HG
  * Later sections of oracle.cpp list their own pins.
  */
 #ifndef MRP_ORACLE_H_
@@ -90,6 +100,14 @@ int oracle_bf16_round(const double* in, int n, double* out);
 int oracle_get_samples(const float* grid, int res, float max_density, int n, const mrp_sample_query* q,
                        mrp_vec3 light_color, mrp_vec3 sky, int multi, int spp, uint64_t seed, float* out,
                        uint64_t* lookups_out, int threads);
+int oracle_training_points(const float* grid, int res, float max_density, int n_rays, float alpha, float g,
+                           uint64_t seed, float* pos, float* view, uint64_t* lookups_out, int threads);
+int oracle_radiances(const float* grid, int res, float max_density, const float* mips, const float* trs,
+                     const float* lut, const float* weights, int n, const float* ori, const float* dir,
+                     const mrp_render_params* p, float* out, uint64_t* lookups_out, int threads);
+int oracle_shadow_terms(const float* grid, int res, float max_density, int n, const float* pos, const float* dir,
+                        mrp_vec3 light_dir, float alpha, float g, int spp, uint64_t seed, float* out,
+                        uint64_t* lookups_out, int threads);
 int oracle_reproject(const float* pts, int n, mrp_vec3 lori, mrp_vec3 lup, mrp_vec3 lright, int W, int H, int* out);
 int oracle_histo_distance(const mrp_histogram* a, const mrp_histogram* b, float* dist, float* weight);
 

@@ -31,15 +31,8 @@ int main(int argc, char** argv) {
     if (argc > 6)
         camera.RenderToFile(argv[6], light, make_float3(1, 1, 1), 64.0f, multi, 0.857f, spp, Camera::ACES,
                             VolumeRender::PT);
-    // the same camera's rays, the raw image
-    const float3 ori = camera.GetPosition();
-    const float l = std::sqrt(ori.x * ori.x + ori.y * ori.y + ori.z * ori.z);
-    const float3 fwd = make_float3(-ori.x / l, -ori.y / l, -ori.z / l);
-    float3 right = make_float3(fwd.z, 0.0f, -fwd.x);  // cross(Y, forward)
-    const float rl = std::sqrt(right.x * right.x + right.z * right.z);
-    right = make_float3(right.x / rl, 0.0f, right.z / rl);
-    const float3 up = make_float3(fwd.y * right.z - fwd.z * right.y, fwd.z * right.x - fwd.x * right.z,
-                                  fwd.x * right.y - fwd.y * right.x);
+    // the same camera's rays (the basis Camera::SetPosition computed), the raw image
+    const float3 ori = camera.GetPosition(), up = camera.GetUp(), right = camera.GetRight();
     const auto img = volume.Render(int2{size, size}, ori, up, right, light, VolumeRender::PT, 0.857f, 64.0f,
                                    make_float3(1, 1, 1), multi, spp);
     FILE* f = std::fopen(argv[1], "wb");

@@ -63,3 +63,29 @@ def test_gpu_get_samples_match_oracle(spp, budget):
         assert g.max() > 0.01
     finally:
         vol.close()
+
+
+def test_gpu_training_points_match_oracle():
+    """`MeanFreePathSample` points (dataGen.cu:174-216) on the GPU against the
+    oracle on the same streams: positions, view directions, kept slots and the
+    lookup count, bit for bit."""
+    _say("case: training points, cloud 64^3 seed 5, 3000 rays, alpha 128, g 0.8")
+    grid = oracle.generate_cloud(64, seed=5)
+    vol = VolumeRender(64)
+    vol.set_datas(grid)
+    vol.update()
+    n = 3000
+    pos = DeviceBuffer(n * 64)
+    view = DeviceBuffer(n * 64)
+    lk = ctypes.c_uint64()
+    ia.check(vol.lib.mrp_sample_training_points(vol.h, n, ctypes.c_float(128.0), ctypes.c_float(0.8),
+                                                ctypes.c_uint64(21), pos.ptr, view.ptr, ctypes.byref(lk)))
+    gp = pos.download(np.float32, (n, 4, 4))
+    gv = view.download(np.float32, (n, 4, 4))
+    md = vol.max_density
+    vol.close()
+    op, ov, olk = oracle.training_points(grid, md, n, 128.0, 0.8, seed=21)
+    _say("points %d (oracle %d), lookups %d / %d" % (gp[..., 3].sum(), op[..., 3].sum(), lk.value, olk))
+    assert gp[..., 3].sum() > n // 4
+    assert lk.value == olk
+    assert np.array_equal(gp, op) and np.array_equal(gv, ov)
