@@ -51,13 +51,17 @@ int main(int argc, char** argv) {
     volume.Update_TR(light, alpha);
     volume.UpdateHGLut(g);
 
-    // points on a spiral through (and a little beyond) the box, directions about them
+    // 64 points on a spiral inside the cloud, 32 outside the box; directions about them
     const int n = 96;
     std::vector<float3> pos(n), dir(n);
     for (int i = 0; i < n; ++i) {
         const float t = (i + 0.5f) / n, a = 2.399963f * i;
-        const float r = 0.62f * std::sqrt(t);
-        pos[i] = make_float3(r * std::cos(a), 1.1f * (t - 0.5f), r * std::sin(a));
+        if (i < 64) {
+            const float r = 0.33f * (i + 0.5f) / 64.0f;
+            pos[i] = make_float3(r * std::cos(a), 0.25f * std::sin(3.0f * a) * r / 0.33f, 0.02f + r * std::sin(a));
+        } else {
+            pos[i] = make_float3(0.7f * std::cos(a), -0.3f + 0.9f * (t - 0.66f), 0.7f * std::sin(a));
+        }
         const float b = 1.7f * i;
         dir[i] = make_float3(std::cos(b) * std::sin(0.3f + t), std::cos(0.3f + t), std::sin(b) * std::sin(0.3f + t));
     }

@@ -182,7 +182,8 @@ def test_gpu_facade_queries_match_oracle(tmp_path):
     m = 24
     qs = (SampleQuery * m)()
     for i in range(m):
-        qs[i] = SampleQuery(V(pos[i]), V(dirs[i]), V(light), np.float32(32.0 + 4.0 * i), np.float32(0.7 + 0.008 * i),
-                            np.float32(0.99))
+        # the program's FP32 arithmetic: 0.7f + 0.008f * i
+        gi = np.float32(np.float32(0.7) + np.float32(np.float32(0.008) * np.float32(i)))
+        qs[i] = SampleQuery(V(pos[i]), V(dirs[i]), V(light), np.float32(32.0 + 4.0 * i), gi, np.float32(0.99))
     want, _ = oracle.get_samples(grid, md, qs, (2.0, 1.0, 0.5), (0.1, 0.2, 0.3), 16, 8, seed=seed)
     assert np.array_equal(rd("smp", (m, 3)), want) and want.max() > 0
