@@ -104,6 +104,50 @@ struct DevBuf {
 };
 }  // namespace detail
 
+// The volume files of `VolumeRender(string path)` (core/volume.cu:835-911):
+// `path.bin` (int32 resolution + FP32 grid) if present, else `path` as `.txt`
+// (the resolution, then the values) or `.vox` (256^3 byte values as text, 64
+// per line after w/h/d header lines, x 64/255).  No GPU work: the constructor
+// uploads what this reads.
+inline bool LoadVolumeFile(const std::string& path, int* resolution, std::vector<float>* datas, bool* from_bin) {
+    *from_bin = false;
+    if (FILE* f = std::fopen((path + ".bin").c_str(), "rb")) {
+        bool good = std::fread(resolution, sizeof(int), 1, f) == 1 && *resolution > 0 && *resolution <= 1024;
+        if (good) {
+            datas->assign((size_t)*resolution * *resolution * *resolution, 0.0f);
+            good = std::fread(datas->data(), sizeof(float), datas->size(), f) == datas->size();
+        }
+        std::fclose(f);
+        *from_bin = good;
+        return good;
+    }
+    const std::string fmt = path.size() >= 3 ? path.substr(path.size() - 3) : "";
+    if (fmt == "txt") {
+        std::ifstream f(path);
+        if (!(f >> *resolution) || *resolution <= 0 || *resolution > 1024) return false;
+        datas->assign((size_t)*resolution * *resolution * *resolution, 0.0f);
+        for (float& d : *datas)
+            if (!(f >> d)) break;
+        return true;
+    }
+    if (fmt == "vox") {
+        *resolution = 256;
+        datas->assign((size_t)256 * 256 * 256, 0.0f);
+        std::ifstream f(path);
+        std::string line;
+        size_t i = 0;
+        const float inv = (float)(64.0 / 255.0);
+        while (std::getline(f, line) && i < datas->size()) {
+            if (line.empty() || line[0] == 'w' || line[0] == 'h' || line[0] == 'd') continue;
+            std::stringstream s(line);
+            int d;
+            for (int j = 0; j < 64 && (s >> d) && i < datas->size(); ++j) (*datas)[i++] = (float)d * inv;
+        }
+        return true;
+    }
+    return false;
+}
+
 class VolumeRender {
    public:
     enum RenderType { PT, RPNN, MRPNN };
@@ -119,7 +163,7 @@ class VolumeRender {
     // resolution + FP32 grid) if present, else `.txt` (resolution, then the
     // values) or `.vox` (256^3 bytes as text, x 64/255); writes `path.bin`.
     explicit VolumeRender(const std::string& path) {
-        if (!load(path)) {
+        if (!LoadVolumeFile(path, &resolution, &datas_, &have_bin_)) {
             std::printf("File not found!\n");
             return;
         }
@@ -425,44 +469,6 @@ class VolumeRender {
         p.render_type = rt == PT ? MRP_RENDER_PT : MRP_RENDER_MRPNN;
         p.mlp_precision = precision_;
         return p;
-    }
-
-    bool load(const std::string& path) {
-        if (FILE* f = std::fopen((path + ".bin").c_str(), "rb")) {
-            bool good = std::fread(&resolution, sizeof(int), 1, f) == 1 && resolution > 0 && resolution <= 1024;
-            if (good) {
-                datas_.assign((size_t)resolution * resolution * resolution, 0.0f);
-                good = std::fread(datas_.data(), sizeof(float), datas_.size(), f) == datas_.size();
-            }
-            std::fclose(f);
-            have_bin_ = good;
-            return good;
-        }
-        const std::string fmt = path.size() >= 3 ? path.substr(path.size() - 3) : "";
-        if (fmt == "txt") {
-            std::ifstream f(path);
-            if (!(f >> resolution) || resolution <= 0 || resolution > 1024) return false;
-            datas_.assign((size_t)resolution * resolution * resolution, 0.0f);
-            for (float& d : datas_)
-                if (!(f >> d)) break;
-            return true;
-        }
-        if (fmt == "vox") {
-            resolution = 256;
-            datas_.assign((size_t)256 * 256 * 256, 0.0f);
-            std::ifstream f(path);
-            std::string line;
-            size_t i = 0;
-            const float inv = 64.0f / 255.0f;
-            while (std::getline(f, line) && i < datas_.size()) {
-                if (line.empty() || line[0] == 'w' || line[0] == 'h' || line[0] == 'd') continue;
-                std::stringstream s(line);
-                int d;
-                for (int j = 0; j < 64 && (s >> d) && i < datas_.size(); ++j) datas_[i++] = (float)d * inv;
-            }
-            return true;
-        }
-        return false;
     }
 
     mrp_volume* h_ = nullptr;

@@ -56,15 +56,18 @@ def parse_permutations(volume_cu: str, count: int = 28):
     return vals[:count]
 
 
-def parse_weights(nnweight: str):
-    decls = re.findall(r"__device__\s+(?:__constant__\s+)?(?:const\s+)?float\s+(\w+)\[(\d+)\]\s*=\s*\{([^}]*)\}",
+def parse_weights(nnweight: str, with_storage: bool = False):
+    """Arrays of an NNWeight.cuh-format header: [(name, values)], or with
+    with_storage [(name, values, "constant" | "global")] (`__device__
+    __constant__ float` or `__device__ const float`)."""
+    decls = re.findall(r"__device__\s+(__constant__\s+)?(?:const\s+)?float\s+(\w+)\[(\d+)\]\s*=\s*\{([^}]*)\}",
                        nnweight)
     arrays = []
-    for name, n, body in decls:
+    for const, name, n, body in decls:
         vals = [float(v) for v in body.split(",") if v.strip()]
         if len(vals) != int(n):
             raise RuntimeError("%s: %d values, declared %s" % (name, len(vals), n))
-        arrays.append((name, vals))
+        arrays.append((name, vals, "constant" if const else "global") if with_storage else (name, vals))
     return arrays
 
 
@@ -124,8 +127,8 @@ def write_weights_header(arrays, path):
 def write_weights(arrays, bin_path, json_path):
     blob = bytearray()
     manifest = {"format": "little-endian float32, arrays concatenated in order", "arrays": []}
-    for name, vals in arrays:
-        manifest["arrays"].append({"name": name, "offset": len(blob) // 4, "count": len(vals)})
+    for name, vals, storage in arrays:
+        manifest["arrays"].append({"name": name, "offset": len(blob) // 4, "count": len(vals), "storage": storage})
         blob += struct.pack("<%df" % len(vals), *vals)
     manifest["total"] = len(blob) // 4
     manifest["sha256"] = hashlib.sha256(bytes(blob)).hexdigest()
@@ -140,14 +143,14 @@ def main(ref_root: str = "/root/reference"):
     core = os.path.join(ref_root, "core")
     pts = parse_stencils(open(os.path.join(core, "vector.cu")).read())
     perms = parse_permutations(open(os.path.join(core, "volume.cu")).read())
-    arrays = parse_weights(open(os.path.join(core, "NNWeight.cuh")).read())
+    arrays = parse_weights(open(os.path.join(core, "NNWeight.cuh")).read(), with_storage=True)
     os.makedirs(os.path.join(HERE, "data"), exist_ok=True)
     write_tables_header(pts, perms, os.path.join(HERE, "csrc", "mrp_tables_gen.h"))
-    write_weights_header(arrays, os.path.join(HERE, "csrc", "mrp_weights_gen.h"))
+    write_weights_header([a[:2] for a in arrays], os.path.join(HERE, "csrc", "mrp_weights_gen.h"))
     write_weights(arrays, os.path.join(HERE, "data", "mrpnn_weights.bin"),
                   os.path.join(HERE, "data", "mrpnn_weights.json"))
     print("stencil points %d, permutations %d, weight arrays %d (%d values)"
-          % (len(pts), len(perms), len(arrays), sum(len(v) for _, v in arrays)))
+          % (len(pts), len(perms), len(arrays), sum(len(a[1]) for a in arrays)))
 
 
 if __name__ == "__main__":
