@@ -156,6 +156,26 @@ int mrp_volume_get_tr_mip(const mrp_volume* vol, int mip, float* host_out);
 int mrp_volume_update_hg_lut(mrp_volume* vol, float g, uint64_t seed);
 int mrp_volume_get_hg_lut(const mrp_volume* vol, float* host_out /* 512*512 */);
 
+/* `SetHDRI` + `SetEnvExp` (core/volume.cu:1426-1462): an environment map for
+ * `SkyBox` (core/render.cuh:57-59) in place of the constant sky: width x
+ * height RGBA FP32 texels (row-major, row 0 at the top: v = acos(dir.y)/pi,
+ * u = atan2(-dir.z, dir.x)/(2 pi) + 1/2), read bilinearly with wrap
+ * addressing and scaled by exposure.  Escaping paths of every renderer and
+ * `GetSample` read it, and MRPNN renders choose per sample (per call for
+ * progressive frames) between the sun and a direction of the map as the
+ * reference's Render does (core/volume.cu:417-432,471-486): with probability
+ * 4 exposure / (4 exposure + max(lightColor)) the light becomes the direction
+ * of the R2 point of the sample index mapped to the sphere with the map's
+ * colour there times 4 exposure / probability, else the sun's colour is
+ * divided by 1 - probability (the draw: Philox stream (0, sample, tag 6,
+ * seed)).  width = 0 (or rgba_host NULL) removes the map. */
+int mrp_volume_set_hdri(mrp_volume* vol, const float* rgba_host, int width, int height, float exposure);
+/* Radiance RGBE ".hdr" file (the reference's `load_hdr_float4`,
+ * core/hdr_loader.h:249) into width x height RGBA FP32 texels (alpha 1):
+ * with rgba_out NULL only width and height are returned; max_texels bounds
+ * the output.  Flat and run-length-encoded scanlines; host only. */
+int mrp_read_hdr(const char* path, float* rgba_out, size_t max_texels, int* width, int* height);
+
 /* ------------------------------------------------------- synthetic inputs */
 /* Seeded Perlin-fBm cloud (this project's own generator; the reference's
  * procedural volume, genall.cu:32-67, is a non-compiling stub):

@@ -9,8 +9,8 @@
 //     CUDA's float3/int2 and the reference's Histogram);
 //   * errors print the library's message and continue, as the reference's
 //     CheckError does (core/volume.cu:15); LastError() returns it;
-//   * the environment is a constant sky colour (SetEnvExp sets its level;
-//     SetHDRI is not available);
+//   * without SetHDRI the environment is a constant sky colour (SetSky;
+//     SetEnvExp scales it, or the map once one is set);
 //   * MRPNN rendering needs the trained weights: LoadNetwork(path) reads the
 //     49,673-float blob (internship_amd/data/mrpnn_weights.bin), or
 //     SetNetwork(weights, count) takes them from memory.
@@ -344,15 +344,28 @@ class VolumeRender {
 
     // setters of the reference's device globals (core/volume.cu:1460-1493)
     void SetCheckboard(bool on) { checkboard_ = on; }
-    void SetEnvExp(float e) {  // scales the constant sky (the reference scales its HDRI)
+    void SetEnvExp(float e) {  // scales the environment map, or the constant sky when none is set
         env_exp_ = e;
         sky_ = base_sky_ * e;
+        if (hdri_w_ > 0)
+            detail::ok(mrp_volume_set_hdri(h_, hdri_.data(), hdri_w_, hdri_h_, env_exp_), "SetEnvExp");
     }
     void SetTrScale(float s) { tr_scale_ = s; }
     void SetScatterRate(float r) { scatter_ = {r, r, r}; }
     void SetScatterRate(float3 r) { scatter_ = r; }
     void SetExposure(float e) { exposure_ = e; }
-    void SetHDRI(const std::string&) { std::printf("SetHDRI: HDRI environments are not available\n"); }
+    // `SetHDRI` (core/volume.cu:1426-1458): a Radiance .hdr environment map;
+    // MRPNN renders then also sample it as a light (mrp_volume_set_hdri).
+    void SetHDRI(const std::string& path) {
+        int w = 0, h = 0;
+        if (!detail::ok(mrp_read_hdr(path.c_str(), nullptr, 0, &w, &h), "SetHDRI")) return;
+        std::vector<float> px((size_t)w * h * 4);
+        if (!detail::ok(mrp_read_hdr(path.c_str(), px.data(), (size_t)w * h, &w, &h), "SetHDRI")) return;
+        if (!detail::ok(mrp_volume_set_hdri(h_, px.data(), w, h, env_exp_), "SetHDRI")) return;
+        hdri_.swap(px);
+        hdri_w_ = w;
+        hdri_h_ = h;
+    }
     // this build's extras
     void SetSky(float3 c) {
         base_sky_ = c;
@@ -484,6 +497,8 @@ class VolumeRender {
     uint64_t seed_ = 0;
     int precision_ = 1;
     std::vector<std::vector<float>> host_mips_, host_trs_;  // host copies for the queries
+    std::vector<float> hdri_;  // the environment map's texels (re-sent when the exposure changes)
+    int hdri_w_ = 0, hdri_h_ = 0;
     std::vector<float> host_lut_;
 };
 

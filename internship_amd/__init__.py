@@ -92,6 +92,17 @@ def camera_basis(ori, fov_deg=90.0, aspect=1.0):
     return (right * s * aspect).astype(np.float32), (up * s).astype(np.float32)
 
 
+def read_hdr(path: str) -> np.ndarray:
+    """A Radiance RGBE .hdr file as (h, w, 4) FP32 (mrp_read_hdr)."""
+    lib = load()
+    w, h = ctypes.c_int(), ctypes.c_int()
+    check(lib.mrp_read_hdr(path.encode(), None, ctypes.c_size_t(0), ctypes.byref(w), ctypes.byref(h)))
+    out = np.zeros((h.value, w.value, 4), np.float32)
+    check(lib.mrp_read_hdr(path.encode(), _ptr(out), ctypes.c_size_t(out.shape[0] * out.shape[1]), ctypes.byref(w),
+                           ctypes.byref(h)))
+    return out
+
+
 def generate_cloud(res, seed, threshold=0.4, octaves=5, base_freq=4.0, lacunarity=2.0, gain=0.5,
                    device_out=None) -> np.ndarray:
     """Perlin-fBm cloud on the GPU (host array, or into `device_out`)."""
@@ -426,6 +437,15 @@ class VolumeRender:
                                                 Vec3.of(light_dir), float(alpha), float(g), ctypes.c_uint64(seed),
                                                 tiles.ptr, pool.ptr, gamma.ptr))
         return tiles, pool, gamma
+
+    def set_hdri(self, rgba, exposure: float = 1.0):
+        """Environment map (rgba (h, w, 4) FP32, row 0 at the top) for `SkyBox`, or None to use the constant sky."""
+        if rgba is None:
+            check(self.lib.mrp_volume_set_hdri(self.h, None, 0, 0, ctypes.c_float(1.0)))
+            return
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        check(self.lib.mrp_volume_set_hdri(self.h, _ptr(a), int(a.shape[1]), int(a.shape[0]),
+                                           ctypes.c_float(exposure)))
 
     def set_gather_cells(self, bits: int):
         """32 (FP32 cells, default) or 16 (FP16 cells) for the BF16/FP8 renderers' gather."""

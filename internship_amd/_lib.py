@@ -127,6 +127,8 @@ SIGNATURES = {
     "mrp_sample_training_points": (I, [P, I, F, F, ctypes.c_uint64, P, P, P]),
     "mrp_gather_rows": (I, [P, P, P, I, I, I]),
     "mrp_get_radiances": (I, [P, I, P, P, P, P, P]),
+    "mrp_volume_set_hdri": (I, [P, P, I, I, F]),
+    "mrp_read_hdr": (I, [ctypes.c_char_p, P, ctypes.c_size_t, P, P]),
     "mrp_get_shadow_terms": (I, [P, I, P, P, Vec3, F, F, I, ctypes.c_uint64, P, P]),
 }
 

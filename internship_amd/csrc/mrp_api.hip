@@ -1,8 +1,11 @@
 // mrp_api.hip - C ABI: errors, device-memory plumbing, volume lifetime.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <new>
+#include <vector>
 #include <string>
 
 #include "mrp_frame.h"
@@ -19,6 +22,28 @@ int fail(int code, const std::string& msg) {
 }
 int hip_fail(hipError_t e, const char* what) {
     return fail(MRP_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+EnvMap env_map(const Volume* v, vec3 sky) {
+    EnvMap e;
+    e.px = v->hdri;
+    e.w = v->hdri_w;
+    e.h = v->hdri_h;
+    e.exposure = v->hdri_exp;
+    e.sky = sky;
+    return e;
+}
+
+void env_light_of_sample(const Volume* v, int n, uint64_t seed, mrp_vec3* light_dir, mrp_vec3* light_color) {
+    if (!v->hdri) return;
+    EnvMap e = env_map(v, v3(0, 0, 0));
+    e.px = v->hdri_host.data();  // the host copy
+    Rng rng = rng_make(0u, (uint32_t)n, 6u, seed);
+    const float choose = rng_next(&rng);
+    vec3 d = v3(light_dir->x, light_dir->y, light_dir->z), c = v3(light_color->x, light_color->y, light_color->z);
+    env_light(e, n, choose, &d, &c);
+    *light_dir = mrp_vec3{d.x, d.y, d.z};
+    *light_color = mrp_vec3{c.x, c.y, c.z};
 }
 
 int ensure_samples(Volume* v, size_t bytes) {
@@ -191,6 +216,127 @@ int mrp_volume_create(int resolution, mrp_volume** out) {
     return MRP_OK;
 }
 
+int mrp_volume_set_hdri(mrp_volume* vol, const float* rgba_host, int width, int height, float exposure) {
+    if (!vol) return fail(MRP_E_INVALID, "NULL volume");
+    Volume* v = &vol->v;
+    MRP_HIP(hipSetDevice(v->device));
+    if (width <= 0 || height <= 0 || !rgba_host) {  // back to the constant sky
+        if (v->hdri) {
+            MRP_HIP(hipStreamSynchronize(v->stream));
+            MRP_HIP(hipFree(v->hdri));
+        }
+        v->hdri = nullptr;
+        v->hdri_w = v->hdri_h = 0;
+        v->hdri_host.clear();
+        return MRP_OK;
+    }
+    if ((long long)width * height > (1ll << 28)) return fail(MRP_E_INVALID, "environment map too large");
+    if (!(exposure >= 0.0f)) return fail(MRP_E_INVALID, "exposure must be >= 0");
+    const size_t n = 4 * (size_t)width * height;
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    if (v->hdri && (size_t)v->hdri_w * v->hdri_h != (size_t)width * height) {
+        MRP_HIP(hipFree(v->hdri));
+        v->hdri = nullptr;
+    }
+    if (!v->hdri && hipMalloc(&v->hdri, n * sizeof(float)) != hipSuccess) {
+        v->hdri = nullptr;
+        v->hdri_w = v->hdri_h = 0;
+        v->hdri_host.clear();
+        return fail(MRP_E_NOMEM, "environment map allocation failed");
+    }
+    MRP_HIP(hipMemcpy(v->hdri, rgba_host, n * sizeof(float), hipMemcpyHostToDevice));
+    v->hdri_host.assign(rgba_host, rgba_host + n);
+    v->hdri_w = width;
+    v->hdri_h = height;
+    v->hdri_exp = exposure;
+    return MRP_OK;
+}
+
+// Radiance RGBE (".hdr") reader: the header lines up to a blank line (FORMAT
+// must be 32-bit_rle_rgbe if given), the resolution line "-Y h +X w", then h
+// scanlines, each flat (4 bytes per pixel) or new-style run-length encoded
+// (0x02 0x02 then the width, then the 4 channels one after another in runs:
+// a count above 128 repeats the next byte count-128 times, else count bytes
+// follow).  A pixel (r, g, b, e) is (r, g, b) * 2^(e-136) with e = 0 black
+// (G. Ward, "Real pixels", Graphics Gems II, 1991).
+static bool rgbe_scanline(FILE* f, int w, unsigned char* line) {
+    unsigned char head[4];
+    if (std::fread(head, 1, 4, f) != 4) return false;
+    if (w < 8 || w > 0x7fff || head[0] != 2 || head[1] != 2 || (head[2] & 0x80)) {  // flat
+        std::memcpy(line, head, 4);
+        return std::fread(line + 4, 4, (size_t)w - 1, f) == (size_t)w - 1;
+    }
+    if (((int)head[2] << 8 | head[3]) != w) return false;
+    std::vector<unsigned char> ch((size_t)w);
+    for (int c = 0; c < 4; ++c) {
+        int x = 0;
+        while (x < w) {
+            int count = std::fgetc(f);
+            if (count == EOF) return false;
+            if (count > 128) {
+                count -= 128;
+                const int val = std::fgetc(f);
+                if (val == EOF || x + count > w) return false;
+                for (int k = 0; k < count; ++k) ch[x++] = (unsigned char)val;
+            } else {
+                if (count == 0 || x + count > w || std::fread(&ch[x], 1, count, f) != (size_t)count) return false;
+                x += count;
+            }
+        }
+        for (int i = 0; i < w; ++i) line[4 * i + c] = ch[i];
+    }
+    return true;
+}
+
+int mrp_read_hdr(const char* path, float* rgba_out, size_t max_texels, int* width, int* height) {
+    if (!path || !width || !height) return fail(MRP_E_INVALID, "NULL argument");
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(MRP_E_IO, std::string("cannot open ") + path);
+    char buf[512];
+    bool ok = std::fgets(buf, sizeof(buf), f) && buf[0] == '#' && buf[1] == '?';
+    while (ok) {
+        if (!std::fgets(buf, sizeof(buf), f)) {
+            ok = false;
+            break;
+        }
+        if (buf[0] == '\n' || buf[0] == '\r') break;
+        if (std::strncmp(buf, "FORMAT=", 7) == 0 && std::strncmp(buf + 7, "32-bit_rle_rgbe", 15) != 0) ok = false;
+    }
+    int w = 0, h = 0;
+    if (ok) ok = std::fgets(buf, sizeof(buf), f) && std::sscanf(buf, "-Y %d +X %d", &h, &w) == 2 && w > 0 && h > 0 &&
+                 (long long)w * h <= (1ll << 28);
+    if (!ok) {
+        std::fclose(f);
+        return fail(MRP_E_IO, std::string("not a Radiance RGBE file with a -Y h +X w resolution: ") + path);
+    }
+    *width = w;
+    *height = h;
+    if (!rgba_out) {  // size query
+        std::fclose(f);
+        return MRP_OK;
+    }
+    if (max_texels < (size_t)w * h) {
+        std::fclose(f);
+        return fail(MRP_E_INVALID, "output buffer holds fewer than width * height texels");
+    }
+    std::vector<unsigned char> line(4 * (size_t)w);
+    for (int y = 0; y < h && ok; ++y) {
+        ok = rgbe_scanline(f, w, line.data());
+        for (int x = 0; ok && x < w; ++x) {
+            const unsigned char* p = &line[4 * (size_t)x];
+            const float s = p[3] ? std::ldexp(1.0f, (int)p[3] - 136) : 0.0f;
+            float* o = rgba_out + 4 * ((size_t)y * w + x);
+            o[0] = (float)p[0] * s;
+            o[1] = (float)p[1] * s;
+            o[2] = (float)p[2] * s;
+            o[3] = 1.0f;
+        }
+    }
+    std::fclose(f);
+    if (!ok) return fail(MRP_E_IO, std::string("truncated or malformed RGBE scanlines: ") + path);
+    return MRP_OK;
+}
+
 int mrp_volume_destroy(mrp_volume* h) {
     if (!h) return MRP_OK;
     Volume* v = &h->v;
@@ -205,6 +351,7 @@ int mrp_volume_destroy(mrp_volume* h) {
     hipFree(v->lut_cells);
     hipFree(v->mip_cells16);
     hipFree(v->tr_cells16);
+    hipFree(v->hdri);
     hipFree(v->queues);
     hipFree(v->counters);
     hipFree(v->samples);

@@ -225,7 +225,9 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
             clear_histo_kernel<<<(unsigned)((nf + 255) / 256), 256, 0, v->stream>>>(histo, nf);
             MRP_HIP(hipGetLastError());
         }
-        rc = predict ? mrpnn_samples(v, p, map, (int)v->frame_calls, 1, &tm, &n_active)
+        mrp_render_params q = *p;  // MRPNN with an environment map: this call's light
+        if (predict) env_light_of_sample(v, (int)v->frame_calls, p->seed, &q.light_dir, &q.light_color);
+        rc = predict ? mrpnn_samples(v, &q, map, (int)v->frame_calls, 1, &tm, &n_active)
                      : trace_samples(v, p, map, (int)v->frame_calls, 1);
     }
     if (rc == MRP_OK) {

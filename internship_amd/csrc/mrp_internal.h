@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/mrp.h"
 #include "mrp_medium.h"
@@ -70,6 +71,12 @@ struct Volume {
     size_t sample_budget = (size_t)4 << 30;
     void* nn_scratch = nullptr;              // MRPNN stages: vertices, features, pool, predictions
     size_t nn_scratch_bytes = 0;
+    // the environment map (`SetHDRI`, core/volume.cu:1426-1458): RGBA FP32
+    // texels in HBM for the kernels' `SkyBox`, a host copy for the light choice
+    float* hdri = nullptr;
+    int hdri_w = 0, hdri_h = 0;
+    float hdri_exp = 1.0f;
+    std::vector<float> hdri_host;
     // progressive-frame state kept between mrp_render_frame calls, like the
     // reference's globals `flip`, `rand_cpu`, `lori/lup/lright` and
     // `last_predict` (core/volume.cu:50-56,404-407,469-547)
@@ -124,6 +131,12 @@ int launch_update_tr(Volume* v, vec3 light_dir, float alpha);
 int ensure_tr(Volume* v, mrp_vec3 light_dir, float alpha);       // cached Update_TR
 int ensure_hg_lut(Volume* v, float g, uint64_t seed);           // cached UpdateHGLut
 int launch_update_hg_lut(Volume* v, float g, uint64_t seed);
+// The volume's environment (the map if one is set, else the constant sky).
+EnvMap env_map(const Volume* v, vec3 sky);
+// The light of MRPNN sample n with an environment map: the sun or a map
+// direction, chosen by a host draw on Philox stream (0, n, tag 6, seed)
+// (`Render`, core/volume.cu:417-432,471-486); unchanged without a map.
+void env_light_of_sample(const Volume* v, int n, uint64_t seed, mrp_vec3* light_dir, mrp_vec3* light_color);
 
 }  // namespace mrp
 

@@ -215,6 +215,12 @@ MRP_HD float acosf_(float x) {
     x = fmin_(fmax_(x, -1.0f), 1.0f);
     return 2.0f * atanf_(sqrtf((1.0f - x) / (1.0f + x)));
 }
+// atan2 by quadrant from atanf_: (-pi, pi]; atan2(0, 0) = 0, atan2(+-0, x<0) = +pi.
+MRP_HD float atan2f_(float y, float x) {
+    if (x > 0.0f) return atanf_(y / x);
+    if (x < 0.0f) return y >= 0.0f ? atanf_(y / x) + 3.14159274f : atanf_(y / x) - 3.14159274f;
+    return y > 0.0f ? 1.57079637f : (y < 0.0f ? -1.57079637f : 0.0f);
+}
 
 // ---------------------------------------------------------------- Philox4x32-10
 struct u32x4 {

@@ -456,6 +456,84 @@ MRP_HD void frame_from_normal(RNG* rng, vec3 v1, vec3* a1, vec3* a2, vec3* a3) {
     *a3 = v3_;
 }
 
+// ------------------------------------------------------------- environment
+// The environment radiance `SkyBox` (core/render.cuh:57-59): an equirectangular
+// map, u = atan2(-dir.z, dir.x) / (2 pi) + 1/2, v = acos(dir.y) / pi, read as a
+// normalised, linear-filtered, wrap-addressed texture (bilinear on texel
+// centres, exact FP32 weights), times the exposure; the constant sky when no
+// map is set.  Pixels are RGBA FP32, row v, column u.
+struct EnvMap {
+    const float* px;  // w * h texels of 4 floats (RGBA)
+    int w, h;
+    float exposure;
+    vec3 sky;
+};
+MRP_HD float env_u(vec3 d) { return atan2f_(-d.z, d.x) * 0.159154937f + 0.5f; }
+MRP_HD float env_v(vec3 d) { return acosf_(fmax_(fmin_(d.y, 1.0f), -1.0f)) * 0.318309873f; }
+MRP_HD vec3 env_texel(const EnvMap& e, int x, int y) {
+    const float* q = e.px + 4 * ((size_t)y * (size_t)e.w + (size_t)x);
+    return v3(q[0], q[1], q[2]);
+}
+MRP_HD int wrap_index(int i, int n) {
+    const int r = i % n;
+    return r < 0 ? r + n : r;
+}
+MRP_HD vec3 env_radiance(const EnvMap& e, vec3 dir) {
+    if (!e.px) return e.sky;
+    const float tx = env_u(dir) * (float)e.w - 0.5f, ty = env_v(dir) * (float)e.h - 0.5f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float wx = tx - fx, wy = ty - fy;
+    const int x0 = wrap_index((int)fx, e.w), x1 = wrap_index((int)fx + 1, e.w);
+    const int y0 = wrap_index((int)fy, e.h), y1 = wrap_index((int)fy + 1, e.h);
+    const vec3 a = env_texel(e, x0, y0), b = env_texel(e, x1, y0), c = env_texel(e, x0, y1),
+               d = env_texel(e, x1, y1);
+    const vec3 top = v3(lerp(a.x, b.x, wx), lerp(a.y, b.y, wx), lerp(a.z, b.z, wx));
+    const vec3 bot = v3(lerp(c.x, d.x, wx), lerp(c.y, d.y, wx), lerp(c.z, d.z, wx));
+    return v3(lerp(top.x, bot.x, wy), lerp(top.y, bot.y, wy), lerp(top.z, bot.z, wy)) * e.exposure;
+}
+// `Image_host::Sample` (core/volume.hpp:23-41): bilinear on texel centres with
+// clamp addressing, no exposure (the light colour of a sampled direction).
+MRP_HD vec3 env_sample_clamp(const EnvMap& e, float u, float v) {
+    const float tx = u * (float)e.w - 0.5f, ty = v * (float)e.h - 0.5f;
+    const float fx = floorf(tx), fy = floorf(ty);
+    const float wx = tx - fx, wy = ty - fy;
+    auto cl = [](int i, int n) { return i < 0 ? 0 : (i > n - 1 ? n - 1 : i); };
+    const int x0 = cl((int)fx, e.w), x1 = cl((int)fx + 1, e.w), y0 = cl((int)fy, e.h), y1 = cl((int)fy + 1, e.h);
+    const vec3 a = env_texel(e, x0, y0), b = env_texel(e, x1, y0), c = env_texel(e, x0, y1),
+               d = env_texel(e, x1, y1);
+    const vec3 top = v3(lerp(a.x, b.x, wx), lerp(a.y, b.y, wx), lerp(a.z, b.z, wx));
+    const vec3 bot = v3(lerp(c.x, d.x, wx), lerp(c.y, d.y, wx), lerp(c.z, d.z, wx));
+    return v3(lerp(top.x, bot.x, wy), lerp(top.y, bot.y, wy), lerp(top.z, bot.z, wy));
+}
+// `Roberts2` (core/vector.cu:174-178): the R2 low-discrepancy point n,
+// frac((1/g, 1/g^2) n + 1/2) with g the plastic number.
+MRP_HD void roberts2(int n, float* x, float* y) {
+    const float g = 1.32471795724474602596f;  // FP32, as the reference declares it
+    const float a1 = 1.0f / g, a2 = 1.0f / (g * g);
+    const float px = a1 * (float)n + 0.5f, py = a2 * (float)n + 0.5f;
+    *x = px - floorf(px);
+    *y = py - floorf(py);
+}
+// The light of MRPNN sample (or progressive call) n with an environment map
+// (`Render`, core/volume.cu:417-432,471-486): with probability rate = 4 exp /
+// (4 exp + max(lightColor)) the sun is replaced by the map: the direction is
+// the uniform-sphere image of Roberts2(n) and the colour the map there times
+// 4 exp / rate; otherwise the sun's colour is divided by 1 - rate.  choose is
+// the uniform draw of that decision.
+MRP_HD void env_light(const EnvMap& e, int n, float choose, vec3* light_dir, vec3* light_color) {
+    const float m = fmax_(light_color->x, fmax_(light_color->y, light_color->z));
+    const float rate = e.exposure * 4.0f / (e.exposure * 4.0f + m);
+    if (choose < rate) {
+        float rx, ry;
+        roberts2(n, &rx, &ry);
+        const vec3 dir = uniform_sphere(rx, ry);
+        *light_dir = dir;
+        *light_color = env_sample_clamp(e, env_u(dir), env_v(dir)) * (e.exposure * 4.0f / rate);
+    } else {
+        *light_color = *light_color / (1.0f - rate);
+    }
+}
+
 // Cube root of x in (0, 1] (the radius of a uniform point in a ball).
 MRP_HD float cbrt01(float x) { return expf_(logf_(x) * (1.0f / 3.0f)); }
 
@@ -528,7 +606,7 @@ MRP_HD int training_ray_points(RNG* rng, const NextVertex& next_vertex, TrainPoi
 // the oracle).
 template <class RNG, class NextVertex, class Transmit>
 MRP_HD vec3 radiance_paths(RNG* rng, const NextVertex& next_vertex, const Transmit& transmittance, vec3 ori,
-                           vec3 dir, vec3 light_dir, vec3 light_color, vec3 sky, vec3 scatter_rate, float g,
+                           vec3 dir, vec3 light_dir, vec3 light_color, const EnvMap& env, vec3 scatter_rate, float g,
                            int multi_scatter, int spp, float* first_t) {
     dir = normalize(dir);
     light_dir = normalize(light_dir);
@@ -536,7 +614,7 @@ MRP_HD vec3 radiance_paths(RNG* rng, const NextVertex& next_vertex, const Transm
     const float dis = ray_box_offset(ori, dir);
     if (dis < 0.0f) {
         *first_t = t;
-        return sky;
+        return env_radiance(env, dir);
     }
     vec3 res = v3(0.0f, 0.0f, 0.0f);
     for (int i = 0; i < spp; ++i) {
@@ -546,7 +624,7 @@ MRP_HD vec3 radiance_paths(RNG* rng, const NextVertex& next_vertex, const Transm
         for (int k = 0; k < multi_scatter; ++k) {
             vec3 np, nd;
             if (!next_vertex(rng, sample_pos, ray_dir, ray_box_distance(sample_pos, ray_dir), &np, &nd)) {
-                res = res + sky * csr;
+                res = res + env_radiance(env, ray_dir) * csr;
                 break;
             }
             if (k == 0) t = distance(ori, np);

@@ -47,7 +47,8 @@ struct NNArgs {
     int res;
     float smax, alpha, g, tr_scale;
     Majorant mj;
-    vec3 ori, forward, right, up, light, lx, light_color, sky, scatter_rate;
+    vec3 ori, forward, right, up, light, lx, light_color, scatter_rate;
+    EnvMap env;  // `SkyBox` of rays whose first free flight escapes
     int width, height;
     int checker, rows_up, wl;  // pixel selection (checker_col), camera row convention, logical width
     const float* ray_ori;      // a ray list instead of camera pixels (n x 3 each), or nullptr
@@ -73,8 +74,10 @@ constexpr int kShadowBatch = 4;
 constexpr int kMaxTrackSteps = 10000;
 
 // pos_act = (pos, active), fx = (X, HG phase to the light; shadow_kernel
-// multiplies in the transmittance), fy = (Y, output slot as bits), fz = (Z,
-// dist), rng_pos = the stream's draw count where the shadow ray starts.
+// multiplies in the transmittance) or, for a vertex that escaped, (the
+// environment radiance along the ray, 0), fy = (Y, output slot as bits),
+// fz = (Z, dist), rng_pos = the stream's draw count where the shadow ray
+// starts.
 // Items run over the logical pixels of `a`'s pixel map; the output slot
 // (s - spp_begin) * P + pixel is 0xffffffff for a logical pixel past the image
 // edge (odd widths under the checkerboard).
@@ -148,6 +151,7 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
             }
             const float dist = active ? distance(ori, pos) : -1.0f;
             pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
+            if (!active) X = env_radiance(a.env, dir);  // the sky this pixel sees (NNPredict's SkyBox(dir))
             fx[i] = make_float4(X.x, X.y, X.z, phase);
             fy[i] = make_float4(Y.x, Y.y, Y.z, __uint_as_float(s * P + pix));
             fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
@@ -423,10 +427,11 @@ __global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ 
 // ---------------------------------------------------------------- stage 4
 __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act,
                                                       const float* __restrict__ single_in,
+                                                      const float4* __restrict__ fx,
                                                       const float4* __restrict__ fy, const float4* __restrict__ fz,
                                                       const uint32_t* __restrict__ slot_of,
                                                       const float* __restrict__ pred, vec3 light_color,
-                                                      vec3 scatter_rate, vec3 sky, uint32_t n_items,
+                                                      vec3 scatter_rate, uint32_t n_items,
                                                       float4* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
@@ -439,6 +444,7 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
         const vec3 r = p * light_color * scatter_rate;
         out[slot] = make_float4(r.x, r.y, r.z, fz[i].w);
     } else {
+        const float4 sky = fx[i];  // the environment radiance stored by the first vertex
         out[slot] = make_float4(sky.x, sky.y, sky.z, -1.0f);
     }
 }
@@ -603,7 +609,7 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     a.light = normalize(V(p->light_dir));
     a.lx = normalize(a.light);  // first axis of the light frame (GetMatrixFromNormal normalises again)
     a.light_color = V(p->light_color);
-    a.sky = V(p->sky);
+    a.env = env_map(v, V(p->sky));
     a.scatter_rate = V(p->scatter_rate);
     a.width = W;
     a.height = H;
@@ -678,8 +684,8 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         }
         MRP_HIP(hipEventRecord(t3, v->stream));
         if (rc == MRP_OK) {
-            combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, single, fy, fz, slot_of, pred,
-                                                                      a.light_color, a.scatter_rate, a.sky, cnt,
+            combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, single, fx, fy, fz, slot_of, pred,
+                                                                      a.light_color, a.scatter_rate, cnt,
                                                                       (float4*)v->samples);
             MRP_HIP(hipGetLastError());
         }
@@ -722,7 +728,12 @@ int mrpnn_finish(Volume* v, NNTimes* times, uint64_t* n_active) {
 int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum64, mrp_trace_stats* st) {
     const int W = p->width, H = p->height;
     const size_t P = (size_t)W * H;
-    const int batch = (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16), (size_t)p->spp_count));
+    // with an environment map each sample has its own light (and transmittance
+    // mips): one sample per batch
+    const bool env_lights = v->hdri != nullptr;
+    const int batch = env_lights ? 1
+                                 : (int)std::max<size_t>(1, std::min<size_t>(v->sample_budget / (P * 16),
+                                                                             (size_t)p->spp_count));
     int rc = ensure_samples(v, (size_t)batch * P * 16);
     if (rc) return rc;
     for (hipEvent_t& e : v->nn_events)
@@ -736,7 +747,9 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
     for (int s0 = 0; s0 < p->spp_count && rc == MRP_OK; s0 += batch) {
         const int n = std::min(batch, p->spp_count - s0);
         const bool last = s0 + n >= p->spp_count;
-        rc = mrpnn_samples(v, p, map, p->spp_begin + s0, n, &tm, &n_active, last);
+        mrp_render_params q = *p;
+        if (env_lights) env_light_of_sample(v, p->spp_begin + s0, p->seed, &q.light_dir, &q.light_color);
+        rc = mrpnn_samples(v, &q, map, p->spp_begin + s0, n, &tm, &n_active, last);
         if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
     }
     float total_ms = 0.0f;

@@ -25,7 +25,8 @@ struct SampleArgs {
     const float4* cells;
     int res;
     float max_density, tr_scale;
-    vec3 light_color, sky;
+    vec3 light_color;
+    EnvMap env;  // `SkyBox` of escaping paths
     int multi_scatter;
     uint32_t seed_lo, seed_hi;
     int n;           // queries
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void get_sample_kernel(SampleArgs a) {
         for (int s = 0; s < a.multi_scatter; ++s) {
             vec3 np, nd;
             if (!gs_next_vertex(&rng, a.cells, a.res, mj, q.g, pos, ray, ray_box_distance(pos, ray), &np, &nd, &lk)) {
-                res = res + a.sky * csr;
+                res = res + env_radiance(a.env, ray) * csr;
                 break;
             }
             pos = np;
@@ -202,7 +203,8 @@ struct RayArgs {
     int res;
     Majorant mj;
     float g;
-    vec3 light, light_color, sky, scatter_rate;
+    vec3 light, light_color, scatter_rate;
+    EnvMap env;
     int multi_scatter, spp, n;
     uint32_t seed_lo, seed_hi;
     const float* ori;  // n x 3
@@ -228,7 +230,7 @@ __global__ __launch_bounds__(256) void radiance_rays_kernel(RayArgs a, float4* _
         const vec3 o = v3(a.ori[3 * (size_t)i], a.ori[3 * (size_t)i + 1], a.ori[3 * (size_t)i + 2]);
         const vec3 d = v3(a.dir[3 * (size_t)i], a.dir[3 * (size_t)i + 1], a.dir[3 * (size_t)i + 2]);
         float t;
-        const vec3 r = radiance_paths(&rng, nv, tr, o, d, a.light, a.light_color, a.sky, a.scatter_rate, a.g,
+        const vec3 r = radiance_paths(&rng, nv, tr, o, d, a.light, a.light_color, a.env, a.scatter_rate, a.g,
                                       a.multi_scatter, a.spp, &t);
         out[i] = make_float4(r.x, r.y, r.z, t);
     }
@@ -287,7 +289,7 @@ extern "C" int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q
     a.max_density = v->max_density;
     a.tr_scale = 1.0f;
     a.light_color = v3(light_color.x, light_color.y, light_color.z);
-    a.sky = v3(sky.x, sky.y, sky.z);
+    a.env = env_map(v, v3(sky.x, sky.y, sky.z));
     a.multi_scatter = multi_scatter;
     a.seed_lo = (uint32_t)seed;
     a.seed_hi = (uint32_t)(seed >> 32);
@@ -396,7 +398,7 @@ extern "C" int mrp_get_radiances(mrp_volume* vol, int n, const float* ori_dev, c
                       p->tr_scale, p->g, p->spp_count, p->seed, &a);
     if (rc || n == 0) return rc;
     a.light_color = v3(p->light_color.x, p->light_color.y, p->light_color.z);
-    a.sky = v3(p->sky.x, p->sky.y, p->sky.z);
+    a.env = env_map(v, v3(p->sky.x, p->sky.y, p->sky.z));
     a.scatter_rate = v3(p->scatter_rate.x, p->scatter_rate.y, p->scatter_rate.z);
     a.multi_scatter = p->multi_scatter;
     rc = clear_counters(v);

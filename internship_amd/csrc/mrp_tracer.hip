@@ -43,7 +43,8 @@ struct TraceArgs {
     const float4* cells;
     int res;
     float smax, alpha, g, tr_scale;
-    vec3 ori, forward, right, up, light_dir, light_color, sky, scatter_rate;
+    vec3 ori, forward, right, up, light_dir, light_color, scatter_rate;
+    EnvMap env;  // `SkyBox`: the environment map, or the constant sky
     Majorant mj;
     int width, height, multi_scatter;
     int checker, rows_up, wl;  // pixel selection (checker_col) and camera row convention; wl = logical width
@@ -95,6 +96,10 @@ __device__ __forceinline__ float fetch_density(const TraceArgs& a, vec3 p) {
 constexpr int kEventBatch = 16;
 constexpr int kMinStepping = 40;
 
+// kEnvMap: escaping paths read the environment map (a.env.px); without one
+// they add the constant a.env.sky, and the kernel keeps the register budget
+// of 4 waves per SIMD (the map lookup takes it to 136 VGPRs, 3 waves).
+template <bool kEnvMap>
 __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
     const uint32_t lane = __lane_id();
     int q = (int)(blockIdx.x & 7u);
@@ -208,7 +213,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                 const float off = ray_box_offset(a.ori, dir);
                 tfirst = -1.0f;
                 if (off < 0.0f) {
-                    res = a.sky;
+                    res = kEnvMap ? env_radiance(a.env, dir) : a.env.sky;
                     state = ST_DONE;
                 } else {
                     pos = a.ori + dir * off;
@@ -306,7 +311,7 @@ __global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
                 t -= logf_(1.0f - rk) * (state == ST_FF ? a.mj.inv_smax : a.mj.inv_smax_tr);
                 if (t > dis) {
                     if (state == ST_FF) {
-                        res = res + a.sky * csr;  // escaped: environment
+                        res = res + (kEnvMap ? env_radiance(a.env, rdir) : a.env.sky) * csr;  // escaped: environment
                         state = ST_DONE;
                     } else {
                         if (state == ST_TR1) {
@@ -439,7 +444,7 @@ static void fill_trace_args(Volume* v, const mrp_render_params* p, PixelMap map,
     a->up = V(p->up);
     a->light_dir = normalize(V(p->light_dir));
     a->light_color = V(p->light_color);
-    a->sky = V(p->sky);
+    a->env = env_map(v, V(p->sky));
     a->scatter_rate = V(p->scatter_rate);
     a->width = W;
     a->height = H;
@@ -459,11 +464,13 @@ static void fill_trace_args(Volume* v, const mrp_render_params* p, PixelMap map,
 
 static uint64_t trace_items_per_sample(const TraceArgs& a) { return (uint64_t)a.tiles_per_band * 64u; }
 
-static int trace_grid(Volume* v) {
+template <bool kEnvMap>
+static int trace_grid(Volume* v, int* blocks) {
     int occ = 0;
-    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel, 256, 0));
+    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel<kEnvMap>, 256, 0));
     occ = std::max(1, std::min(occ, 8));
-    return std::max(kNumQueues, v->num_cus * occ);
+    *blocks = std::max(kNumQueues, v->num_cus * occ);
+    return MRP_OK;
 }
 
 // One persistent tracer launch over samples [s_begin, s_begin + n) (absolute
@@ -476,7 +483,16 @@ int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     a.spp_count = n;
     a.items_per_queue = (uint32_t)(trace_items_per_sample(a) * (uint64_t)n);
     clear_u32_kernel<<<1, 64, 0, v->stream>>>(v->queues, kNumQueues);
-    trace_kernel<<<trace_grid(v), 256, 0, v->stream>>>(a);
+    int blocks = 0, rc;
+    if (a.env.px) {
+        rc = trace_grid<true>(v, &blocks);
+        if (rc) return rc;
+        trace_kernel<true><<<blocks, 256, 0, v->stream>>>(a);
+    } else {
+        rc = trace_grid<false>(v, &blocks);
+        if (rc) return rc;
+        trace_kernel<false><<<blocks, 256, 0, v->stream>>>(a);
+    }
     MRP_HIP(hipGetLastError());
     return MRP_OK;
 }
@@ -510,7 +526,9 @@ static int trace_dev(Volume* v, const mrp_render_params* p, float* target, doubl
     if (batch < 1) return fail(MRP_E_INVALID, "image too large for one work queue");
     rc = ensure_samples(v, (size_t)batch * per_sample_bytes);
     if (rc) return rc;
-    const int blocks = trace_grid(v);
+    int blocks = 0;
+    rc = probe.env.px ? trace_grid<true>(v, &blocks) : trace_grid<false>(v, &blocks);
+    if (rc) return rc;
 
     hipEvent_t e0, e1, e2, e3;
     MRP_HIP(hipEventCreate(&e0));

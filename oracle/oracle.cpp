@@ -75,12 +75,27 @@ struct Medium {
     float max_density;
     float alpha, g, tr_scale;
     vec3 sky, scatter_rate, light_color;
+    EnvMap env;  // the environment: the map set with oracle_set_hdri, else the constant sky
     Majorant mj;  // reciprocals of SMax = max_density * alpha
 };
 
 struct Counters {
     uint64_t lookups = 0, scatters = 0;
 };
+
+// The environment map of oracle_set_hdri (test state; empty = the constant sky).
+std::vector<float> g_hdri;
+int g_hdri_w = 0, g_hdri_h = 0;
+float g_hdri_exp = 1.0f;
+EnvMap env_of(vec3 sky) {
+    EnvMap e;
+    e.px = g_hdri.empty() ? nullptr : g_hdri.data();
+    e.w = g_hdri_w;
+    e.h = g_hdri_h;
+    e.exposure = g_hdri_exp;
+    e.sky = sky;
+    return e;
+}
 
 // `Tr` (core/render.cuh:93-136): ratio tracking while tr > 0.5, then delta
 // tracking while tr > 0.001, sharing one 10000-iteration cap.
@@ -143,7 +158,7 @@ struct Radiance {
 };
 
 // `CalculateRadiance` with sampleNum = 1 (core/render.cuh:200-258); the
-// environment `SkyBox` is the constant m.sky.
+// environment `SkyBox` is m.env.
 Radiance calculate_radiance(Rng* rng, const Medium& m, vec3 ori, vec3 dir, vec3 light_dir, int multi_scatter,
                             Counters* c) {
     dir = normalize(dir);
@@ -152,7 +167,7 @@ Radiance calculate_radiance(Rng* rng, const Medium& m, vec3 ori, vec3 dir, vec3 
     out.t = -1.0f;
     const float dis = ray_box_offset(ori, dir);
     if (dis < 0.0f) {
-        out.rgb = m.sky;
+        out.rgb = env_radiance(m.env, dir);
         return out;
     }
     vec3 res = v3(0, 0, 0);
@@ -164,7 +179,7 @@ Radiance calculate_radiance(Rng* rng, const Medium& m, vec3 ori, vec3 dir, vec3 
         float max_dis = ray_box_distance(sample_pos, ray_dir);
         const bool in_volume = next_vertex(rng, m, m.g, sample_pos, ray_dir, max_dis, &next_pos, &next_dir, c);
         if (!in_volume) {
-            res = res + m.sky * csr;
+            res = res + env_radiance(m.env, ray_dir) * csr;
             break;
         }
         c->scatters += 1;
@@ -253,6 +268,40 @@ void resample(const float* src, int s, float* dst, int r) {
 }  // namespace
 
 extern "C" {
+
+// The environment map of every later oracle render (`SetHDRI`,
+// core/volume.cu:1426-1458): w x h RGBA FP32 texels, exposure; w = 0 clears it.
+int oracle_set_hdri(const float* rgba, int w, int h, float exposure) {
+    if (w <= 0 || h <= 0 || !rgba) {
+        g_hdri.clear();
+        g_hdri_w = g_hdri_h = 0;
+        return 0;
+    }
+    g_hdri.assign(rgba, rgba + 4 * (size_t)w * (size_t)h);
+    g_hdri_w = w;
+    g_hdri_h = h;
+    g_hdri_exp = exposure;
+    return 0;
+}
+
+// The environment lookup and the per-sample light choice on the current map.
+int oracle_env(const float* dirs, int n, mrp_vec3 sky, float* out) {
+    const EnvMap e = env_of(V(sky));
+    for (int i = 0; i < n; ++i) {
+        const vec3 r = env_radiance(e, v3(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]));
+        out[3 * i] = r.x;
+        out[3 * i + 1] = r.y;
+        out[3 * i + 2] = r.z;
+    }
+    return 0;
+}
+int oracle_env_light(int n, float choose, mrp_vec3 light_dir, mrp_vec3 light_color, float* out6) {
+    vec3 d = V(light_dir), c = V(light_color);
+    env_light(env_of(v3(0, 0, 0)), n, choose, &d, &c);
+    const float o[6] = {d.x, d.y, d.z, c.x, c.y, c.z};
+    for (int k = 0; k < 6; ++k) out6[k] = o[k];
+    return 0;
+}
 
 int oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out) {
     u32x4 c;
@@ -801,6 +850,7 @@ int oracle_render_mrpnn(const float* grid, int res, float max_density, const flo
     m.g = p->g;
     m.tr_scale = p->tr_scale;
     m.sky = V(p->sky);
+    m.env = env_of(m.sky);
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
     m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
@@ -840,7 +890,7 @@ int oracle_render_mrpnn(const float* grid, int res, float max_density, const flo
                 const vec3 pred = v3(pr[0] + fv.single, pr[1] + fv.single, pr[2] + fv.single);
                 rgb = pred * m.light_color * m.scatter_rate;
             } else {
-                rgb = m.sky;
+                rgb = env_radiance(m.env, normalize(dir));
             }
             rgb = vmax(v3(0, 0, 0), rgb);
             tgt = lerp(tgt, rgb, 1.0f / (float)(1 + s));
@@ -878,6 +928,7 @@ int oracle_render(const float* grid, int res, float max_density, const mrp_rende
     m.g = p->g;
     m.tr_scale = p->tr_scale;
     m.sky = V(p->sky);
+    m.env = env_of(m.sky);
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
     m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
@@ -999,6 +1050,7 @@ int oracle_render_frame(const float* grid, int res, float max_density, const flo
     m.g = p->g;
     m.tr_scale = p->tr_scale;
     m.sky = V(p->sky);
+    m.env = env_of(m.sky);
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
     m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
@@ -1050,7 +1102,7 @@ int oracle_render_frame(const float* grid, int res, float max_density, const flo
                 rgb = v3(pr[0] + fv.single, pr[1] + fv.single, pr[2] + fv.single) * m.light_color * m.scatter_rate;
                 t = fv.dist;
             } else {
-                rgb = m.sky;
+                rgb = env_radiance(m.env, normalize(dir));
                 t = -1.0f;
             }
         }
@@ -1158,6 +1210,7 @@ int oracle_radiances(const float* grid, int res, float max_density, const float*
     m.g = p->g;
     m.tr_scale = p->tr_scale;
     m.sky = V(p->sky);
+    m.env = env_of(m.sky);
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
     m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
@@ -1184,7 +1237,7 @@ int oracle_radiances(const float* grid, int res, float max_density, const float*
                 return next_vertex(rg, m, m.g, pp, dd, dis, np, nd, &c);
             };
             const auto tr = [&](Rng* rg, vec3 pp, vec3 dd, float dis) { return transmittance(rg, m, pp, dd, dis, &c); };
-            rgb = radiance_paths(&rng, nv, tr, o, d, light, m.light_color, m.sky, m.scatter_rate, m.g, p->multi_scatter,
+            rgb = radiance_paths(&rng, nv, tr, o, d, light, m.light_color, m.env, m.scatter_rate, m.g, p->multi_scatter,
                                  p->spp_count, &t);
         } else {
             Rng rng = rng_make((uint32_t)i, 0u, 0u, p->seed);
@@ -1197,7 +1250,7 @@ int oracle_radiances(const float* grid, int res, float max_density, const float*
                 oracle_mrpnn_forward(weights, 1, feat, &gam, m.g, srp, pr);
                 rgb = v3(pr[0] + fv.single, pr[1] + fv.single, pr[2] + fv.single) * m.light_color * m.scatter_rate;
             } else {
-                rgb = m.sky;
+                rgb = env_radiance(m.env, normalize(d));
             }
             t = fv.dist;
         }
@@ -1282,6 +1335,7 @@ int oracle_get_samples(const float* grid, int res, float max_density, int n, con
         m.g = q[j].g;
         m.tr_scale = 1.0f;
         m.sky = V(sky);
+        m.env = env_of(m.sky);
         m.scatter_rate = v3(q[j].scatter, q[j].scatter, q[j].scatter);
         m.light_color = V(light_color);
         m.mj = make_majorant(max_density, q[j].alpha, 1.0f);
@@ -1298,7 +1352,7 @@ int oracle_get_samples(const float* grid, int res, float max_density, int n, con
             for (int s = 0; s < multi; ++s) {
                 vec3 np, nd;
                 if (!next_vertex(&rng, m, m.g, pos, ray, ray_box_distance(pos, ray), &np, &nd, &c)) {
-                    res_k = res_k + m.sky * csr;
+                    res_k = res_k + env_radiance(m.env, ray) * csr;
                     break;
                 }
                 pos = np;

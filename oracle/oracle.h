@@ -45,6 +45,11 @@
  *                                                                    composition on the same first vertices
  * oracle_shadow_terms      core/render.cuh:317-341 (GetTr/GetTrs)   Tr * HG: homogeneous box = exp(-sigma d) * HG
  *                                                                    within Monte Carlo error; empty = HG
+ * oracle_env / set_hdri    core/render.cuh:57-59 (SkyBox)            constant map = constant sky; texel values at
+ *                                                                    the equirectangular texel centres; wrap
+ *                                                                    seam; axis directions' (u, v)
+ * oracle_env_light         core/volume.cu:417-432 (Render)           rate formula; Roberts2 R2 points; colour =
+ *                                                                    clamp-sampled map x 4 exp / rate
  * Later sections of oracle.cpp list their own pins.
  */
 #ifndef MRP_ORACLE_H_
@@ -59,6 +64,9 @@ extern "C" {
 #endif
 
 int oracle_philox(const uint32_t* ctr, const uint32_t* key, uint32_t* out);
+int oracle_set_hdri(const float* rgba, int w, int h, float exposure);
+int oracle_env(const float* dirs, int n, mrp_vec3 sky, float* out);
+int oracle_env_light(int n, float choose, mrp_vec3 light_dir, mrp_vec3 light_color, float* out6);
 int oracle_normalize3(const float* v, float* out);
 int oracle_uniforms(uint32_t id, uint32_t sample, uint32_t tag, uint64_t seed, int n, float* out);
 int oracle_frame_from_normal(const float* v, int n, uint32_t sample, uint32_t tag, uint64_t seed, float* out,
