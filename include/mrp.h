@@ -236,6 +236,55 @@ int mrp_volume_reset_frames(mrp_volume* vol);
 /* The progressive state: checkerboard parity of the next call and calls made. */
 int mrp_volume_frame_state(const mrp_volume* vol, int* flip, uint32_t* calls);
 
+/* ------------------------------------------------------ batched serving */
+/* Several scenes and cameras rendered as one batch: the MRPNN image of every
+ * view (`NNPredict<MRPNN>`, core/render.cuh:469-523, as mrp_render_offline
+ * with render_type MRP_RENDER_MRPNN), a delta-tracking fallback for the pixels
+ * whose sample statistics say the prediction is noisy, and the histogram
+ * denoise and BGRA8 pack of the frame path (`Denoise<true>`,
+ * core/volume.cu:208-264).  The reference renders one scene per call; this
+ * entry point has no counterpart there and serves callers that batch scenes
+ * (a render service).
+ *   volumes[k], views[k] : view k's volume (updated, no environment map) and
+ *                parameters; every view has the same width and height
+ *                (width*height a multiple of 64), and the same g, albedo and
+ *                mlp_precision (one network evaluates the batch: volumes[0]'s);
+ *                views of one volume share its light, alpha and seed.
+ *                spp_begin and spp_count are ignored; multi_scatter bounds the
+ *                fallback paths.
+ *   per pixel:  samples 0 .. mrpnn_spp-1 are MRPNN samples (Philox stream
+ *                (pixel, s, 0, seed), as mrp_render_offline), folded into the
+ *                running mean and the pixel histogram of the frame path; if the
+ *                histogram's |E[l^2] - E[l]^2| exceeds var_threshold, the pixel
+ *                is re-estimated from scratch (mean and histogram) with the
+ *                delta-tracking samples mrpnn_spp .. mrpnn_spp+fallback_spp-1
+ *                (`CalculateRadiance`, core/volume.cu:17-34), all fallback
+ *                pixels of all views in one persistent launch.
+ *   rgb_dev    : n_views*width*height*3 FP32, the estimate before the denoise
+ *   bgra_dev   : n_views*width*height 0xAARRGGBB after denoise (if set),
+ *                exposure and tone curve (NULL: no pack)
+ *   flags_dev  : optional n_views*width*height bytes, 1 for fallback pixels
+ * stats may be NULL. */
+typedef struct mrp_serve_params {
+    int mrpnn_spp;        /* MRPNN samples per pixel, >= 1 (2 or more give the variance a meaning) */
+    float var_threshold;  /* fallback when the pixel's histogram variance exceeds this */
+    int fallback_spp;     /* delta-tracking samples of a fallback pixel, >= 1 */
+    int denoise;          /* 1: `Denoise<true>` on the estimate before the pack */
+    int tone;             /* 0 none, 1 gamma 2.2, 2 ACES */
+    float exposure;
+} mrp_serve_params;
+typedef struct mrp_serve_stats {
+    uint64_t mrpnn_lookups;     /* first free flights and shadow rays */
+    uint64_t fallback_lookups;  /* delta-tracking paths of the fallback */
+    uint64_t descriptors;       /* first vertices that scattered */
+    uint64_t fallback_pixels;
+    float total_ms;             /* device time of the call (HIP events) */
+    float mrpnn_ms, accumulate_ms, fallback_ms, denoise_ms;
+} mrp_serve_stats;
+int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_params* views, int n_views,
+                    const mrp_serve_params* sp, float* rgb_dev, uint32_t* bgra_dev, uint8_t* flags_dev,
+                    mrp_serve_stats* stats);
+
 /* Tone curve on host colours (n RGB triples): `Camera::None/Gamma/ACES`
  * (core/camera.cu:66-160, core/render.cuh:640-732); tone 0 none, 1 gamma,
  * 2 ACES.  The same arithmetic as the frame path's device tone map. */

@@ -25,7 +25,8 @@ import math
 import numpy as np
 
 from . import _lib
-from ._lib import CloudParams, FrameParams, Histogram, MrpError, RenderParams, TraceStats, Vec3, check, load
+from ._lib import (CloudParams, FrameParams, Histogram, MrpError, RenderParams, ServeParams, ServeStats, TraceStats,
+                   Vec3, check, load)
 
 PT = 0
 MRPNN = 2
@@ -464,6 +465,41 @@ class VolumeRender:
             self.close()
         except Exception:
             pass
+
+
+class ServeBatch:
+    """Output buffers of mrp_serve_views for n_views views of width x height
+    (HBM): rgb (FP32 estimate before the denoise), bgra (packed output) and
+    the fallback flags."""
+
+    def __init__(self, n_views: int, width: int, height: int):
+        self.n_views, self.width, self.height = int(n_views), int(width), int(height)
+        n = self.n_views * self.width * self.height
+        self.rgb = DeviceBuffer(n * 12)
+        self.bgra = DeviceBuffer(n * 4)
+        self.flags = DeviceBuffer(n)
+
+    def run(self, volumes, views, mrpnn_spp=2, var_threshold=0.01, fallback_spp=16, denoise=True, tone=2,
+            exposure=1.0, bgra=True, flags=True) -> ServeStats:
+        """One batched call: views[k] (RenderParams) on volumes[k] (VolumeRender)."""
+        lib = load()
+        n = len(views)
+        assert n == len(volumes) == self.n_views
+        hs = (ctypes.c_void_p * n)(*[v.h.value for v in volumes])
+        ps = (RenderParams * n)(*views)
+        sp = ServeParams(int(mrpnn_spp), float(var_threshold), int(fallback_spp), int(bool(denoise)), int(tone),
+                         float(exposure))
+        st = ServeStats()
+        check(lib.mrp_serve_views(ctypes.cast(hs, ctypes.c_void_p), ctypes.cast(ps, ctypes.c_void_p), n,
+                                  ctypes.byref(sp), self.rgb.ptr, self.bgra.ptr if bgra else None,
+                                  self.flags.ptr if flags else None, ctypes.byref(st)))
+        return st
+
+    def download(self):
+        """(rgb (n, h, w, 3) FP32, bgra (n, h, w) uint32, flags (n, h, w) uint8)."""
+        shp = (self.n_views, self.height, self.width)
+        return (self.rgb.download(np.float32, shp + (3,)), self.bgra.download(np.uint32, shp),
+                self.flags.download(np.uint8, shp))
 
 
 def aces(c: np.ndarray) -> np.ndarray:

@@ -69,6 +69,22 @@ class TraceStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ServeParams(ctypes.Structure):
+    _fields_ = [("mrpnn_spp", ctypes.c_int), ("var_threshold", ctypes.c_float), ("fallback_spp", ctypes.c_int),
+                ("denoise", ctypes.c_int), ("tone", ctypes.c_int), ("exposure", ctypes.c_float)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [
+        ("mrpnn_lookups", ctypes.c_uint64), ("fallback_lookups", ctypes.c_uint64), ("descriptors", ctypes.c_uint64),
+        ("fallback_pixels", ctypes.c_uint64), ("total_ms", ctypes.c_float), ("mrpnn_ms", ctypes.c_float),
+        ("accumulate_ms", ctypes.c_float), ("fallback_ms", ctypes.c_float), ("denoise_ms", ctypes.c_float),
+    ]
+
+    def asdict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 P = ctypes.c_void_p
 I = ctypes.c_int
 F = ctypes.c_float
@@ -130,6 +146,7 @@ SIGNATURES = {
     "mrp_volume_set_hdri": (I, [P, P, I, I, F]),
     "mrp_read_hdr": (I, [ctypes.c_char_p, P, ctypes.c_size_t, P, P]),
     "mrp_get_shadow_terms": (I, [P, I, P, P, Vec3, F, F, I, ctypes.c_uint64, P, P]),
+    "mrp_serve_views": (I, [P, P, I, ctypes.POINTER(ServeParams), P, P, P, ctypes.POINTER(ServeStats)]),
 }
 
 _lib = None

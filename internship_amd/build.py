@@ -23,7 +23,8 @@ ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 HIP_SOURCES = ["mrp_api.hip", "mrp_volume.hip", "mrp_tracer.hip", "mrp_mrpnn.hip", "mrp_mlp_mfma.hip",
-               "mrp_frame.hip", "mrp_samples.hip", "mrp_train.hip", "mrp_train_mfma.hip", "mrp_probe.hip"]
+               "mrp_frame.hip", "mrp_samples.hip", "mrp_train.hip", "mrp_train_mfma.hip", "mrp_probe.hip",
+               "mrp_serve.hip"]
 ORACLE_SOURCES = ["oracle.cpp"]
 
 HIPCC_FLAGS = [

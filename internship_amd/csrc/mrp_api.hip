@@ -356,7 +356,10 @@ int mrp_volume_destroy(mrp_volume* h) {
     hipFree(v->counters);
     hipFree(v->samples);
     hipFree(v->nn_scratch);
+    hipFree(v->serve_scratch);
     for (hipEvent_t e : v->nn_events)
+        if (e) hipEventDestroy(e);
+    for (hipEvent_t e : v->serve_events)
         if (e) hipEventDestroy(e);
     if (v->host_words) hipHostFree(v->host_words);
     if (v->stream) hipStreamDestroy(v->stream);

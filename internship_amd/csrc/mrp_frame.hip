@@ -82,6 +82,8 @@ __global__ __launch_bounds__(256) void frame_update_kernel(FrameArgs a, const fl
 
 constexpr int kTile = 16, kApron = 5, kSpan = kTile + 2 * kApron, kSpanCells = kSpan * kSpan;
 
+// blockIdx.z selects one of several images of the same size stored one after
+// another (batched serving); 0 for a single frame.
 __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const float* __restrict__ target,
                                                            const float* __restrict__ histo,
                                                            uint32_t* __restrict__ out) {
@@ -89,6 +91,10 @@ __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const fl
     __shared__ float s_total[kSpanCells];
     __shared__ float s_tgt[3][kSpanCells];
     const int W = a.width, H = a.height;
+    const size_t image = (size_t)blockIdx.z * (size_t)W * (size_t)H;
+    target += 3 * image;
+    histo += (size_t)kHistoFloats * image;
+    out += image;
     const int x0 = blockIdx.x * kTile - kApron, y0 = blockIdx.y * kTile - kApron;
     const int tid = threadIdx.y * kTile + threadIdx.x;
     const bool filter = a.frame != 0 && a.denoise;
@@ -169,6 +175,28 @@ __global__ __launch_bounds__(256) void reproject_pack_kernel(FrameArgs a, const 
 __global__ void clear_histo_kernel(float* __restrict__ histo, size_t n_floats) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n_floats) histo[i] = 0.0f;
+}
+
+// The denoise and pack of n_images images of one size (a batch of views):
+// `Denoise<denoise>` past frame 0 on the running means `target` and the
+// histograms `histo`, then exposure, tone curve and BGRA8.
+int launch_denoise_images(hipStream_t stream, int width, int height, int n_images, int denoise, int tone,
+                          float exposure, const float* target, const float* histo, uint32_t* bgra) {
+    FrameArgs a{};
+    a.width = width;
+    a.height = height;
+    a.checker = -1;
+    a.checkerboard = 0;
+    a.frame = 1;
+    a.predict = 0;
+    a.exposure = exposure;
+    a.tone = tone;
+    a.denoise = denoise;
+    const dim3 grid((unsigned)((width + kTile - 1) / kTile), (unsigned)((height + kTile - 1) / kTile),
+                    (unsigned)n_images);
+    denoise_pack_kernel<<<grid, dim3(kTile, kTile), 0, stream>>>(a, target, histo, bgra);
+    MRP_HIP(hipGetLastError());
+    return MRP_OK;
 }
 
 static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }

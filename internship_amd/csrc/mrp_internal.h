@@ -71,6 +71,11 @@ struct Volume {
     size_t sample_budget = (size_t)4 << 30;
     void* nn_scratch = nullptr;              // MRPNN stages: vertices, features, pool, predictions
     size_t nn_scratch_bytes = 0;
+    // batched serving (this volume runs the batch): view tables, MRPNN samples,
+    // histograms and the fallback list; stage events
+    void* serve_scratch = nullptr;
+    size_t serve_scratch_bytes = 0;
+    hipEvent_t serve_events[5] = {};
     // the environment map (`SetHDRI`, core/volume.cu:1426-1458): RGBA FP32
     // texels in HBM for the kernels' `SkyBox`, a host copy for the light choice
     float* hdri = nullptr;
@@ -120,6 +125,18 @@ struct NNTimes {
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
                   uint64_t* n_active, bool defer_last = false);
 int mrpnn_finish(Volume* v, NNTimes* times, uint64_t* n_active);
+// The MRPNN samples [0, spp) of n_views images in one pass of the stages
+// (view k on vols[k] with ps[k], every view of the same size, width * height a
+// multiple of 64): out[(k * spp + s) * P + pixel] = (radiance, first-vertex
+// distance or -1).  vols[0]'s network, stream and scratch run them; tables_dev
+// holds mrpnn_view_table_bytes(n_views) bytes of device memory.
+int mrpnn_views(Volume* const* vols, const mrp_render_params* ps, int n_views, int spp, void* tables_dev,
+                float4* out, NNTimes* times, uint64_t* n_active);
+size_t mrpnn_view_table_bytes(int n_views);
+// Denoise (past frame 0) and BGRA8 pack of n_images images of one size stored
+// one after another: target 3 floats, histo kHistoFloats floats, bgra one word per pixel.
+int launch_denoise_images(hipStream_t stream, int width, int height, int n_images, int denoise, int tone,
+                          float exposure, const float* target, const float* histo, uint32_t* bgra);
 // `NNPredict<MRPNN>` on n given rays (one sample each): out_dev n x float4
 // (radiance, first-vertex distance or -1).
 int mrpnn_rays(Volume* v, int n, const float* ori, const float* dir, const mrp_render_params* p, float* out,

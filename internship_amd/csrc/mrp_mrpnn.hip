@@ -58,6 +58,11 @@ struct NNArgs {
     uint32_t item0;      // first (sample-major) item of this chunk within the batch
     uint32_t n_items;    // items in this chunk
     unsigned long long* counters;
+    // several images in one launch: item k of the chunk belongs to view
+    // (item0 + k) / view_items, whose record (camera, volume, light, seed) is
+    // views[view]; nullptr for one image (this record)
+    const NNArgs* views;
+    uint32_t view_items;
 };
 
 // ---------------------------------------------------------------- stage 1
@@ -83,6 +88,82 @@ constexpr int kMaxTrackSteps = 10000;
 // edge (odd widths under the checkerboard).
 // count_out: the compaction's counter, cleared here (the compaction runs next
 // on the same stream).
+//
+// first_vertex_item: one (sample, pixel) item; `item` counts from the first
+// item of `a`'s image, `slot_base` is added to the output slot.
+__device__ __forceinline__ void first_vertex_item(const NNArgs& a, uint32_t i, uint32_t item, uint32_t slot_base,
+                                                  float4* __restrict__ pos_act, float4* __restrict__ fx,
+                                                  float4* __restrict__ fy, float4* __restrict__ fz,
+                                                  uint32_t* __restrict__ rng_pos, uint32_t* lk) {
+    const uint32_t P = (uint32_t)(a.width * a.height);
+    const uint32_t Pl = (uint32_t)(a.wl * a.height);
+    const uint32_t s = item / Pl, l = item - s * Pl;
+    int row, col_l;
+    if (a.ray_ori) {  // a ray list: item l is ray l
+        row = 0;
+        col_l = (int)l;
+    } else {
+        block_order(l, (uint32_t)a.wl, (uint32_t)a.height, &row, &col_l);  // waves cover 8 x 8 pixel blocks
+    }
+    const int col = checker_col(row, col_l, a.checker);
+    if (col >= a.width) {
+        pos_act[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        fx[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
+        fy[i] = make_float4(0.0f, 1.0f, 0.0f, __uint_as_float(0xffffffffu));
+        fz[i] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
+    } else {
+        const uint32_t pix = (uint32_t)row * (uint32_t)a.width + (uint32_t)col;
+        Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        vec3 ori = a.ori, dir;
+        if (a.ray_ori) {
+            ori = v3(a.ray_ori[3 * (size_t)pix], a.ray_ori[3 * (size_t)pix + 1], a.ray_ori[3 * (size_t)pix + 2]);
+            dir = v3(a.ray_dir[3 * (size_t)pix], a.ray_dir[3 * (size_t)pix + 1], a.ray_dir[3 * (size_t)pix + 2]);
+        } else {
+            const float xu = rng_next(&rng);
+            const float xv = rng_next(&rng);
+            dir = camera_dir_rows(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv, a.rows_up);
+        }
+        dir = normalize(dir);  // NNPredict
+        const vec3 gdir = normalize(dir);  // GetSamplePoint
+        vec3 pos = v3(0, 0, 0);
+        int active = 0;
+        const float off = ray_box_offset(ori, gdir);
+        if (off >= 0.0f) {
+            const vec3 sp = ori + gdir * off;
+            vec3 np;
+            const CellsFetch F{a.cells, a.res};
+            if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
+                                              ray_box_distance(sp, gdir), kMaxTrackSteps, &np, lk)) {
+                pos = np;
+                active = 1;
+            }
+        }
+        // A vertex whose free flight escaped is sky: its frames and shadow ray
+        // (the rest of NNPredict's draws) cannot change any output, so they are skipped.
+        // The shadow ray of a scattering vertex runs in shadow_kernel over the
+        // compacted vertices, from the draw count stored here.
+        vec3 X = v3(1, 0, 0), Y = v3(0, 1, 0), Z = v3(0, 0, 1);
+        float phase = 0.0f;
+        if (active) {
+            vec3 LX, LY, LZ;
+            frame_from_normal(&rng, dir, &X, &Y, &Z);
+            frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
+            phase = hg_phase(dot(X, LX), a.g);
+            rng_pos[i] = rng.n;
+        }
+        const float dist = active ? distance(ori, pos) : -1.0f;
+        pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
+        if (!active) X = env_radiance(a.env, dir);  // the sky this pixel sees (NNPredict's SkyBox(dir))
+        fx[i] = make_float4(X.x, X.y, X.z, phase);
+        fy[i] = make_float4(Y.x, Y.y, Y.z, __uint_as_float(slot_base + s * P + pix));
+        fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
+    }
+}
+
+// kViews: the items run over the images of a.views (view_items items each,
+// view-major); every view's count is a multiple of 64, so a wave's items
+// belong to one view and its record is read with scalar loads.
+template <bool kViews>
 __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __restrict__ pos_act,
                                                            float4* __restrict__ fx, float4* __restrict__ fy,
                                                            float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
@@ -92,69 +173,13 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
     if (i == 0u) *count_out = 0u;
     uint32_t lk = 0u;
     if (i < a.n_items) {
-        const uint32_t P = (uint32_t)(a.width * a.height);
-        const uint32_t Pl = (uint32_t)(a.wl * a.height);
         const uint32_t item = a.item0 + i;
-        const uint32_t s = item / Pl, l = item - s * Pl;
-        int row, col_l;
-        if (a.ray_ori) {  // a ray list: item l is ray l
-            row = 0;
-            col_l = (int)l;
+        if constexpr (kViews) {
+            const uint32_t view = __builtin_amdgcn_readfirstlane(item / a.view_items);
+            const uint32_t base = view * a.view_items;
+            first_vertex_item(a.views[view], i, item - base, base, pos_act, fx, fy, fz, rng_pos, &lk);
         } else {
-            block_order(l, (uint32_t)a.wl, (uint32_t)a.height, &row, &col_l);  // waves cover 8 x 8 pixel blocks
-        }
-        const int col = checker_col(row, col_l, a.checker);
-        if (col >= a.width) {
-            pos_act[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            fx[i] = make_float4(1.0f, 0.0f, 0.0f, 0.0f);
-            fy[i] = make_float4(0.0f, 1.0f, 0.0f, __uint_as_float(0xffffffffu));
-            fz[i] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
-        } else {
-            const uint32_t pix = (uint32_t)row * (uint32_t)a.width + (uint32_t)col;
-            Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
-            vec3 ori = a.ori, dir;
-            if (a.ray_ori) {
-                ori = v3(a.ray_ori[3 * (size_t)pix], a.ray_ori[3 * (size_t)pix + 1], a.ray_ori[3 * (size_t)pix + 2]);
-                dir = v3(a.ray_dir[3 * (size_t)pix], a.ray_dir[3 * (size_t)pix + 1], a.ray_dir[3 * (size_t)pix + 2]);
-            } else {
-                const float xu = rng_next(&rng);
-                const float xv = rng_next(&rng);
-                dir = camera_dir_rows(a.forward, a.right, a.up, row, col, a.width, a.height, xu, xv, a.rows_up);
-            }
-            dir = normalize(dir);  // NNPredict
-            const vec3 gdir = normalize(dir);  // GetSamplePoint
-            vec3 pos = v3(0, 0, 0);
-            int active = 0;
-            const float off = ray_box_offset(ori, gdir);
-            if (off >= 0.0f) {
-                const vec3 sp = ori + gdir * off;
-                vec3 np;
-                const CellsFetch F{a.cells, a.res};
-                if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
-                                                  ray_box_distance(sp, gdir), kMaxTrackSteps, &np, &lk)) {
-                    pos = np;
-                    active = 1;
-                }
-            }
-            // A vertex whose free flight escaped is sky: its frames and shadow ray
-            // (the rest of NNPredict's draws) cannot change any output, so they are skipped.
-            // The shadow ray of a scattering vertex runs in shadow_kernel over the
-            // compacted vertices, from the draw count stored here.
-            vec3 X = v3(1, 0, 0), Y = v3(0, 1, 0), Z = v3(0, 0, 1);
-            float phase = 0.0f;
-            if (active) {
-                vec3 LX, LY, LZ;
-                frame_from_normal(&rng, dir, &X, &Y, &Z);
-                frame_from_normal(&rng, a.light, &LX, &LY, &LZ);
-                phase = hg_phase(dot(X, LX), a.g);
-                rng_pos[i] = rng.n;
-            }
-            const float dist = active ? distance(ori, pos) : -1.0f;
-            pos_act[i] = make_float4(pos.x, pos.y, pos.z, active ? 1.0f : 0.0f);
-            if (!active) X = env_radiance(a.env, dir);  // the sky this pixel sees (NNPredict's SkyBox(dir))
-            fx[i] = make_float4(X.x, X.y, X.z, phase);
-            fy[i] = make_float4(Y.x, Y.y, Y.z, __uint_as_float(s * P + pix));
-            fz[i] = make_float4(Z.x, Z.y, Z.z, dist);
+            first_vertex_item(a, i, item, 0u, pos_act, fx, fy, fz, rng_pos, &lk);
         }
     }
     red[threadIdx.x] = lk;
@@ -178,7 +203,8 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
 // ray sharing one walk (0.35-0.64 ms), this kernel on a second stream
 // beside the gather (both slowed by about what the overlap saved), and 8
 // lookups in flight per lane (170 VGPRs, half the waves: 0.29 ms).
-__global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* __restrict__ list,
+template <bool kViews>
+__global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* __restrict__ list,
                                                      const uint32_t* __restrict__ n_act_dev,
                                                      const float4* __restrict__ pos_act, const float4* __restrict__ fy,
                                                      const uint32_t* __restrict__ rng_pos,
@@ -190,8 +216,16 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
     uint32_t lk = 0u;
     if (k < n_act) {
         const uint32_t i = list[k];
+        uint32_t slot = __float_as_uint(fy[i].w);
+        // the vertex's view (lanes of a wave may belong to different views)
+        const NNArgs* ap = &a0;
+        if constexpr (kViews) {
+            const uint32_t view = slot / a0.view_items;
+            slot -= view * a0.view_items;
+            ap = a0.views + view;
+        }
+        const NNArgs& a = *ap;
         const uint32_t P = (uint32_t)(a.width * a.height);
-        const uint32_t slot = __float_as_uint(fy[i].w);
         const uint32_t s = slot / P, pix = slot - s * P;
         Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
         rng_seek(&rng, rng_pos[i]);
@@ -209,7 +243,7 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a, const uint32_t* _
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && red[0]) atomicAdd(&a.counters[2], (unsigned long long)red[0]);
+    if (threadIdx.x == 0 && red[0]) atomicAdd(&a0.counters[2], (unsigned long long)red[0]);
 }
 
 // ---------------------------------------------------------------- stage 2
@@ -223,6 +257,10 @@ struct GatherArgs {
     float alpha, g;
     uint32_t n_items;
     const uint32_t* n_dev;  // if set, the descriptor count is read here (n_items bounds the grid)
+    // several images in one launch: a descriptor's view is its output slot /
+    // view_items and its tables are views[view]; nullptr for one image
+    const GatherArgs* views;
+    uint32_t view_items;
 };
 
 // Segments of the 12 squeeze-excitation blocks (`SE`/`SERES` call sites,
@@ -253,7 +291,7 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 // transmittance cells (one 16-byte load per lookup) instead of the FP32 cells
 // (two); the FP32 cells are the default of every renderer, the FP16 ones an
 // opt-in variant of the tiled (BF16/FP8) gather.
-template <bool kTiled, bool kCells16>
+template <bool kTiled, bool kCells16, bool kViews>
 __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
@@ -280,13 +318,24 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
     const int tr = 128 >> tm;
     const vec3 st = i < kShellTaps ? v3(c_stencil[3 * i], c_stencil[3 * i + 1], c_stencil[3 * i + 2]) : v3(0, 0, 0);
     const float s_inv = i < kShellTaps ? c_tap_inv_len[i] : 0.0f;
-    // The HG-table factor that is the same for every descriptor (TapTable): the
+    static_assert(!(kViews && kCells16), "the multi-view gather reads the FP32 cells");
+    // The view's tables and light (with kViews, loaded when the view changes:
+    // compacted items of one workgroup mostly share one), and the HG-table
+    // factor that is the same for every descriptor of the view (TapTable): the
     // view-axis factor of a shell tap, the light-axis factor of a light tap.
+    const float4* mip_cells = a.mips[mip];
+    const float4* tr_cells = a.trs[tm];
+    const float4* lut = a.lut;
+    vec3 light = a.light, lx = a.lx;
+    float alpha = a.alpha, g = a.g;
+    const auto fixed_factor = [&]() {
+        if (i > 0 && i < kShellTaps) return bilinear_cells(lut, kHgLutSize, kHgLutSize, c_tap_u_view[i], lut_v);
+        if (i >= kShellTaps) return bilinear_cells(lut, kHgLutSize, kHgLutSize, dot(lx, light) * 0.5f + 0.5f, lut_v);
+        return 0.0f;
+    };
     float h_fixed = 0.0f;
-    if (i > 0 && i < kShellTaps)
-        h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, c_tap_u_view[i], lut_v);
-    else if (i >= kShellTaps)
-        h_fixed = bilinear_cells(a.lut, kHgLutSize, kHgLutSize, dot(a.lx, a.light) * 0.5f + 0.5f, lut_v);
+    if constexpr (!kViews) h_fixed = fixed_factor();
+    uint32_t cur_view = 0xffffffffu;
     // The vertex a descriptor belongs to (compacted list) and its record are
     // uniform over the block (scalar loads).  They are prefetched: the list
     // entry two items ahead, the record one item ahead, so an item's taps
@@ -313,6 +362,21 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
             z4_n = fz[src_next];
             if (it + 2 < kGatherItems && item + 2u < n_items) src_next = list ? list[item + 2u] : item + 2u;
         }
+        if constexpr (kViews) {
+            const uint32_t view = __builtin_amdgcn_readfirstlane(__float_as_uint(y4.w) / a.view_items);
+            if (view != cur_view) {
+                const GatherArgs& va = a.views[view];
+                mip_cells = va.mips[mip];
+                tr_cells = va.trs[tm];
+                lut = va.lut;
+                light = va.light;
+                lx = va.lx;
+                alpha = va.alpha;
+                g = va.g;
+                h_fixed = fixed_factor();
+                cur_view = view;
+            }
+        }
         const vec3 p = v3(pa.x, pa.y, pa.z);
         const vec3 X = v3(x4.x, x4.y, x4.z), Y = v3(y4.x, y4.y, y4.z), Z = v3(z4.x, z4.y, z4.z);
         float xd = 0.0f, xt = 0.0f, xh = 0.0f;
@@ -321,34 +385,34 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
             if (i < kShellTaps) {
                 pi = p + (X * st.x + Y * st.y + Z * st.z) * off;
             } else {
-                pi = p + a.light * off;
+                pi = p + light * off;
             }
             const float d = kCells16 ? trilinear_cells16(a.mips16 + mip16_offset(mip), 256 >> mip, pi)
-                                     : trilinear_cells(a.mips[mip], 256 >> mip, pi);
-            xd = logf_(1.0f + a.alpha * d / 64.0f);
+                                     : trilinear_cells(mip_cells, 256 >> mip, pi);
+            xd = logf_(1.0f + alpha * d / 64.0f);
             float harg;
             if (i == 0) {
-                harg = hg_phase(dot(X, a.light), a.g);
+                harg = hg_phase(dot(X, light), g);
             } else {
                 // the factor that varies: light axis for a shell tap, view axis for a light
                 // tap (whose direction is the light's); the product commutes exactly
                 const float uv = i < kShellTaps
-                                     ? tap_u_light(dot(a.lx, X), dot(a.lx, Y), dot(a.lx, Z), st.x, st.y, st.z, s_inv)
-                                     : dot(X, a.light) * 0.5f + 0.5f;
-                harg = h_fixed * bilinear_cells(a.lut, kHgLutSize, kHgLutSize, uv, lut_v);
+                                     ? tap_u_light(dot(lx, X), dot(lx, Y), dot(lx, Z), st.x, st.y, st.z, s_inv)
+                                     : dot(X, light) * 0.5f + 0.5f;
+                harg = h_fixed * bilinear_cells(lut, kHgLutSize, kHgLutSize, uv, lut_v);
             }
             xh = logf_(harg + 1.0f);
             // a tap outside the box reads the TR mip where its light ray enters
             // the box (1 if it misses); branch-free: both cases read one cell
             const bool outside =
                 pi.x < -0.5f || pi.y < -0.5f || pi.z < -0.5f || pi.x > 0.5f || pi.y > 0.5f || pi.z > 0.5f;
-            const float o = ray_box_offset(pi, a.lx);
-            const vec3 q = outside ? (pi + a.lx * o) + 0.5f : pi + 0.5f;
+            const float o = ray_box_offset(pi, lx);
+            const vec3 q = outside ? (pi + lx * o) + 0.5f : pi + 0.5f;
             const float tv = kCells16 ? trilinear_clamp_cells16(a.trs16 + tr16_offset(tm), tr, q)
-                                      : trilinear_clamp_cells(a.trs[tm], tr, q);
+                                      : trilinear_clamp_cells(tr_cells, tr, q);
             xt = outside && o < 0.0f ? 1.0f : tv;
         }
-        if (i == 0) gamma[item] = acosf_(dot(X, a.lx));  // after the lookups: stores delay later waits
+        if (i == 0) gamma[item] = acosf_(dot(X, lx));  // after the lookups: stores delay later waits
         if (!kTiled) {
             float* out = feat + (size_t)item * kFeatures;
             out[i] = xd;
@@ -425,19 +489,27 @@ __global__ __launch_bounds__(64) void mlp_fp32_kernel(const float* __restrict__ 
 }
 
 // ---------------------------------------------------------------- stage 4
+// kViews: the light colour and albedo are those of the slot's view.
+template <bool kViews>
 __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__ pos_act,
                                                       const float* __restrict__ single_in,
                                                       const float4* __restrict__ fx,
                                                       const float4* __restrict__ fy, const float4* __restrict__ fz,
                                                       const uint32_t* __restrict__ slot_of,
                                                       const float* __restrict__ pred, vec3 light_color,
-                                                      vec3 scatter_rate, uint32_t n_items,
+                                                      vec3 scatter_rate, const NNArgs* __restrict__ views,
+                                                      uint32_t view_items, uint32_t n_items,
                                                       float4* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
     const uint32_t slot = __float_as_uint(fy[i].w);
     if (slot == 0xffffffffu) return;
     if (pos_act[i].w != 0.0f) {
+        if constexpr (kViews) {
+            const NNArgs& va = views[slot / view_items];
+            light_color = va.light_color;
+            scatter_rate = va.scatter_rate;
+        }
         const float single = single_in[i];
         const uint32_t k = slot_of[i];  // the vertex's compacted descriptor
         const vec3 p = v3(pred[3 * k] + single, pred[3 * k + 1] + single, pred[3 * k + 2] + single);
@@ -547,37 +619,96 @@ static int launch_gather_tiled(Volume* v, const GatherArgs& ga, unsigned blocks,
                                const float4* fx, const float4* fy, const float4* fz, const uint32_t* list,
                                void* tiles, float* gamma, float* pool) {
     if (v->gather_cells16)
-        gather_kernel<true, true><<<blocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, tiles, gamma, pool);
+        gather_kernel<true, true, false><<<blocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, tiles, gamma, pool);
     else
-        gather_kernel<true, false><<<blocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, tiles, gamma, pool);
+        gather_kernel<true, false, false><<<blocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, tiles, gamma, pool);
     MRP_HIP(hipGetLastError());
     return MRP_OK;
 }
 
-// First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
-// of the pixels `map` selects, in chunks of at most 4M vertices.
-int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
-                  uint64_t* n_active, bool defer_last) {
-    if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
-    if (p->mlp_precision < 0 || p->mlp_precision > 2)
-        return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
-    const void* packed = p->mlp_precision == 2 ? v->nn_packed_fp8 : v->nn_packed;
-    if (p->mlp_precision != 0 && !packed) return fail(MRP_E_STATE, "network has no MFMA packing");
+// Brings the volume's per-light and per-g tables (TR mips, HG table) up to
+// date on its stream.
+static int prepare_tables(Volume* v, const mrp_render_params* p) {
     int rc = upload_tap_tables(v->device);
     if (rc) return rc;
     rc = ensure_tr(v, p->light_dir, p->alpha);
     if (rc) return rc;
-    rc = ensure_hg_lut(v, p->g, p->seed);
-    if (rc) return rc;
+    return ensure_hg_lut(v, p->g, p->seed);
+}
 
+// Checks a render's network settings, then prepare_tables.
+static int prepare_volume(Volume* v, const mrp_render_params* p, const void** packed) {
+    if (!v->nn_weights) return fail(MRP_E_STATE, "no network set (mrp_volume_set_network)");
+    if (p->mlp_precision < 0 || p->mlp_precision > 2)
+        return fail(MRP_E_INVALID, "mlp_precision must be 0 (FP32), 1 (BF16) or 2 (FP8)");
+    *packed = p->mlp_precision == 2 ? v->nn_packed_fp8 : v->nn_packed;
+    if (p->mlp_precision != 0 && !*packed) return fail(MRP_E_STATE, "network has no MFMA packing");
+    return prepare_tables(v, p);
+}
+
+// The stage records of one image: samples from s_begin of the pixels `map`
+// selects, on volume v.
+static void image_args(const Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, NNArgs* a,
+                       GatherArgs* ga) {
     const int W = p->width, H = p->height;
-    const int wl = map.checker < 0 ? W : (W + 1) / 2;
-    const size_t Pl = (size_t)wl * H;
-    const size_t total = (size_t)n * Pl;
+    *a = NNArgs{};
+    a->cells = v->cells;
+    a->res = v->res;
+    a->alpha = p->alpha;
+    a->smax = v->max_density * p->alpha;
+    a->mj = make_majorant(v->max_density, p->alpha, p->tr_scale);
+    a->g = p->g;
+    a->tr_scale = p->tr_scale;
+    a->ori = V(p->ori);
+    a->forward = normalize(-a->ori);
+    a->right = V(p->right);
+    a->up = V(p->up);
+    a->light = normalize(V(p->light_dir));
+    a->lx = normalize(a->light);  // first axis of the light frame (GetMatrixFromNormal normalises again)
+    a->light_color = V(p->light_color);
+    a->env = env_map(v, V(p->sky));
+    a->scatter_rate = V(p->scatter_rate);
+    a->width = W;
+    a->height = H;
+    a->checker = map.checker;
+    a->rows_up = map.rows_up;
+    a->ray_ori = map.ray_ori;
+    a->ray_dir = map.ray_dir;
+    a->wl = map.checker < 0 ? W : (W + 1) / 2;
+    a->seed_lo = (uint32_t)p->seed;
+    a->seed_hi = (uint32_t)(p->seed >> 32);
+    a->spp_begin = s_begin;
+    a->counters = v->counters;
+    a->views = nullptr;
+    a->view_items = 0u;
+
+    *ga = GatherArgs{};
+    for (int m = 0; m < kNumMips; ++m) ga->mips[m] = v->mip_cells[m];
+    for (int m = 0; m < kNumTrMips; ++m) ga->trs[m] = v->tr_cells[m];
+    ga->lut = v->lut_cells;
+    ga->mips16 = v->mip_cells16;
+    ga->trs16 = v->tr_cells16;
+    ga->light = a->light;
+    ga->lx = a->lx;
+    ga->alpha = p->alpha;
+    ga->g = p->g;
+    ga->views = nullptr;
+    ga->view_items = 0u;
+}
+
+// First vertex -> compaction -> shadow rays -> gather -> network -> combine
+// over `total` items in chunks of at most 4M, on v's stream and scratch; the
+// results go to out[slot].  With a.views set the items run over several
+// images (the kernels' kViews forms); the network settings (precision,
+// weights, g, srp) are shared by all of them.
+static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const void* packed, const float* weights,
+                      float g, vec3 srp, size_t total, float4* out, NNTimes* times, uint64_t* n_active,
+                      bool defer_last) {
+    const bool views = a.views != nullptr;
     const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, total));
     static_assert(2 * kTileBytes == (size_t)32 * kFeatures * sizeof(float), "tiled BF16 descriptors fit the FP32 rows");
     void* scratch = nullptr;
-    rc = ensure_nn_scratch(v, chunk_items * kNNItemBytes + 4096, &scratch);
+    int rc = ensure_nn_scratch(v, chunk_items * kNNItemBytes + 4096, &scratch);
     if (rc) return rc;
     char* sp = (char*)scratch;
     float4* pos_act = (float4*)sp;
@@ -594,47 +725,6 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     float* single = (float*)(rng_pos + chunk_items);
     uint32_t* n_active_dev = (uint32_t*)v->queues;  // the tracer's queue counters are free during MRPNN frames
 
-    NNArgs a;
-    a.cells = v->cells;
-    a.res = v->res;
-    a.alpha = p->alpha;
-    a.smax = v->max_density * p->alpha;
-    a.mj = make_majorant(v->max_density, p->alpha, p->tr_scale);
-    a.g = p->g;
-    a.tr_scale = p->tr_scale;
-    a.ori = V(p->ori);
-    a.forward = normalize(-a.ori);
-    a.right = V(p->right);
-    a.up = V(p->up);
-    a.light = normalize(V(p->light_dir));
-    a.lx = normalize(a.light);  // first axis of the light frame (GetMatrixFromNormal normalises again)
-    a.light_color = V(p->light_color);
-    a.env = env_map(v, V(p->sky));
-    a.scatter_rate = V(p->scatter_rate);
-    a.width = W;
-    a.height = H;
-    a.checker = map.checker;
-    a.rows_up = map.rows_up;
-    a.ray_ori = map.ray_ori;
-    a.ray_dir = map.ray_dir;
-    a.wl = wl;
-    a.seed_lo = (uint32_t)p->seed;
-    a.seed_hi = (uint32_t)(p->seed >> 32);
-    a.spp_begin = s_begin;
-    a.counters = v->counters;
-
-    GatherArgs ga;
-    for (int m = 0; m < kNumMips; ++m) ga.mips[m] = v->mip_cells[m];
-    for (int m = 0; m < kNumTrMips; ++m) ga.trs[m] = v->tr_cells[m];
-    ga.lut = v->lut_cells;
-    ga.mips16 = v->mip_cells16;
-    ga.trs16 = v->tr_cells16;
-    ga.light = a.light;
-    ga.lx = a.lx;
-    ga.alpha = p->alpha;
-    ga.g = p->g;
-    const vec3 srp = srp_of(a.scatter_rate);
-
     // Stage boundaries: first vertex | compaction | shadow rays | gather | network | combine.
     // the volume's events, created once (creating them per frame costs host time between frames)
     for (hipEvent_t& e : v->nn_events)
@@ -645,9 +735,12 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         const uint32_t cnt = (uint32_t)std::min(chunk_items, total - i0);
         a.item0 = (uint32_t)i0;
         a.n_items = cnt;
+        const unsigned blocks = (cnt + 255) / 256;
         MRP_HIP(hipEventRecord(t0, v->stream));
-        first_vertex_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos,
-                                                                       n_active_dev);
+        if (views)
+            first_vertex_kernel<true><<<blocks, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
+        else
+            first_vertex_kernel<false><<<blocks, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(tc, v->stream));
         compact_kernel<<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
@@ -656,8 +749,11 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         // No read-back here: the shadow rays, the gather and the network read the
         // count on the device, with grids sized for the whole chunk.
         MRP_HIP(hipEventRecord(ts0, v->stream));
-        shadow_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, rng_pos, fx,
-                                                                  single);
+        if (views)
+            shadow_kernel<true><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, rng_pos, fx, single);
+        else
+            shadow_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, rng_pos, fx,
+                                                                single);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(ts1, v->stream));
         ga.n_items = cnt;
@@ -666,27 +762,38 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
             // the MFMA networks read wave-tiled BF16 descriptors (same region: ceil(n/32) tiles
             // of kTileBytes fit in n FP32 rows for n >= 32); the FP32 network reads FP32 rows
             const unsigned gblocks = (cnt + kGatherItems - 1) / kGatherItems;  // groups past the count exit at once
-            if (p->mlp_precision == 0)
-                gather_kernel<false, false><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat, gam,
-                                                                              pool);
+            if (views && precision == 0)
+                gather_kernel<false, false, true><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat,
+                                                                                   gam, pool);
+            else if (views)
+                gather_kernel<true, false, true><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list, feat,
+                                                                                  gam, pool);
+            else if (precision == 0)
+                gather_kernel<false, false, false><<<gblocks, kTaps, 0, v->stream>>>(ga, pos_act, fx, fy, fz, list,
+                                                                                    feat, gam, pool);
             else
                 rc = launch_gather_tiled(v, ga, gblocks, pos_act, fx, fy, fz, list, feat, gam, pool);
+            MRP_HIP(hipGetLastError());
             if (rc) return rc;
         }
         MRP_HIP(hipEventRecord(t2, v->stream));
-        if (p->mlp_precision == 0) {
-            mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(v->nn_weights, feat, gam, p->g, srp, cnt,
-                                                                     n_active_dev, pred);
+        if (precision == 0) {
+            mlp_fp32_kernel<<<(cnt + 63) / 64, 64, 0, v->stream>>>(weights, feat, gam, g, srp, cnt, n_active_dev, pred);
             MRP_HIP(hipGetLastError());
         } else {
-            rc = launch_mlp(packed, p->mlp_precision - 1, nullptr, feat, pool, gam, p->g, srp, cnt, n_active_dev, pred,
+            rc = launch_mlp(packed, precision - 1, nullptr, feat, pool, gam, g, srp, cnt, n_active_dev, pred,
                             v->stream);
         }
         MRP_HIP(hipEventRecord(t3, v->stream));
         if (rc == MRP_OK) {
-            combine_kernel<<<(cnt + 255) / 256, 256, 0, v->stream>>>(pos_act, single, fx, fy, fz, slot_of, pred,
-                                                                      a.light_color, a.scatter_rate, cnt,
-                                                                      (float4*)v->samples);
+            if (views)
+                combine_kernel<true><<<blocks, 256, 0, v->stream>>>(pos_act, single, fx, fy, fz, slot_of, pred,
+                                                                    a.light_color, a.scatter_rate, a.views,
+                                                                    a.view_items, cnt, out);
+            else
+                combine_kernel<false><<<blocks, 256, 0, v->stream>>>(pos_act, single, fx, fy, fz, slot_of, pred,
+                                                                     a.light_color, a.scatter_rate, nullptr, 0u, cnt,
+                                                                     out);
             MRP_HIP(hipGetLastError());
         }
         // the count into pinned host memory (a copy into pageable memory would block)
@@ -698,6 +805,80 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
         MRP_HIP(hipEventSynchronize(tj));
         rc = mrpnn_finish(v, times, n_active);
     }
+    return rc;
+}
+
+// First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
+// of the pixels `map` selects, in chunks of at most 4M vertices.
+int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
+                  uint64_t* n_active, bool defer_last) {
+    const void* packed = nullptr;
+    int rc = prepare_volume(v, p, &packed);
+    if (rc) return rc;
+    NNArgs a;
+    GatherArgs ga;
+    image_args(v, p, map, s_begin, &a, &ga);
+    const size_t total = (size_t)n * (size_t)a.wl * (size_t)p->height;
+    return run_stages(v, a, ga, p->mlp_precision, packed, v->nn_weights, p->g, srp_of(a.scatter_rate), total,
+                      (float4*)v->samples, times, n_active, defer_last);
+}
+
+size_t mrpnn_view_table_bytes(int n_views) { return (size_t)n_views * (sizeof(NNArgs) + sizeof(GatherArgs)); }
+
+// The MRPNN samples [0, spp) of n_views images in one pass of the stages
+// (view k: volume vols[k], parameters ps[k]): out[(k * spp + s) * P + pixel].
+// The network of vols[0] and its settings (precision, g, albedo) serve every
+// view; views that share a volume share its light and alpha (its TR mips).
+// tables_dev holds n_views NNArgs then n_views GatherArgs.
+int mrpnn_views(Volume* const* vols, const mrp_render_params* ps, int n_views, int spp, void* tables_dev,
+                float4* out, NNTimes* times, uint64_t* n_active) {
+    Volume* host = vols[0];
+    const mrp_render_params& p0 = ps[0];
+    const size_t P = (size_t)p0.width * p0.height;
+    if (P % 64 != 0) return fail(MRP_E_INVALID, "width * height must be a multiple of 64 for multi-view renders");
+    if ((size_t)n_views * spp * P >= ((size_t)1 << 32) - 1) return fail(MRP_E_INVALID, "too many items in one batch");
+    const void* packed = nullptr;
+    int rc = prepare_volume(host, &p0, &packed);
+    if (rc) return rc;
+    std::vector<NNArgs> nn((size_t)n_views);
+    std::vector<GatherArgs> gv((size_t)n_views);
+    for (int k = 0; k < n_views; ++k) {
+        const mrp_render_params& p = ps[k];
+        if (p.width != p0.width || p.height != p0.height) return fail(MRP_E_INVALID, "views differ in image size");
+        if (p.mlp_precision != p0.mlp_precision || p.g != p0.g || p.scatter_rate.x != p0.scatter_rate.x ||
+            p.scatter_rate.y != p0.scatter_rate.y || p.scatter_rate.z != p0.scatter_rate.z)
+            return fail(MRP_E_INVALID, "views differ in network settings (precision, g or albedo)");
+        Volume* v = vols[k];
+        if (v->device != host->device) return fail(MRP_E_INVALID, "views on different devices");
+        if (v->hdri) return fail(MRP_E_INVALID, "multi-view renders do not take environment-map lights");
+        for (int j = 0; j < k; ++j)
+            if (vols[j] == v && (ps[j].light_dir.x != p.light_dir.x || ps[j].light_dir.y != p.light_dir.y ||
+                                 ps[j].light_dir.z != p.light_dir.z || ps[j].alpha != p.alpha || ps[j].seed != p.seed))
+                return fail(MRP_E_INVALID, "views of one volume differ in light, alpha or seed");
+        if (k > 0) {
+            rc = prepare_tables(v, &p);
+            if (rc) return rc;
+        }
+        image_args(v, &p, PixelMap{}, 0, &nn[k], &gv[k]);
+    }
+    // the tables were brought up to date on each volume's own stream
+    for (int k = 1; k < n_views; ++k)
+        if (vols[k]->stream != host->stream) MRP_HIP(hipStreamSynchronize(vols[k]->stream));
+    NNArgs* nn_dev = (NNArgs*)tables_dev;
+    GatherArgs* gv_dev = (GatherArgs*)(nn_dev + n_views);
+    MRP_HIP(hipMemcpyAsync(nn_dev, nn.data(), nn.size() * sizeof(NNArgs), hipMemcpyHostToDevice, host->stream));
+    MRP_HIP(hipMemcpyAsync(gv_dev, gv.data(), gv.size() * sizeof(GatherArgs), hipMemcpyHostToDevice, host->stream));
+    NNArgs a = nn[0];
+    a.counters = host->counters;
+    a.views = nn_dev;
+    a.view_items = (uint32_t)(spp * P);
+    GatherArgs ga = gv[0];
+    ga.views = gv_dev;
+    ga.view_items = a.view_items;
+    rc = run_stages(host, a, ga, p0.mlp_precision, packed, host->nn_weights, p0.g, srp_of(V(p0.scatter_rate)),
+                    (size_t)n_views * spp * P, out, times, n_active, false);
+    // the host copies of the tables must outlive the copies above
+    MRP_HIP(hipStreamSynchronize(host->stream));
     return rc;
 }
 
@@ -920,7 +1101,7 @@ int mrp_mrpnn_features(mrp_volume* vol, int n, const float* pos_act_dev, const f
     int rc = gather_setup(vol, n, pos_act_dev, frame_x_dev, frame_y_dev, frame_z_dev, light_dir, alpha, g, seed, &ga);
     if (rc || n == 0) return rc;
     Volume* v = &vol->v;
-    gather_kernel<false, false><<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
+    gather_kernel<false, false, false><<<(n + kGatherItems - 1) / kGatherItems, kTaps, 0, v->stream>>>(
         ga, (const float4*)pos_act_dev, (const float4*)frame_x_dev, (const float4*)frame_y_dev,
         (const float4*)frame_z_dev, nullptr, feat_dev, gamma_dev, nullptr);
     MRP_HIP(hipGetLastError());

@@ -235,7 +235,12 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
         }
         if (escape) {
             // the escaping step's test: it runs only if the walk is still in ratio tracking
-            if (w->loop_num < max_steps && tr > 0.5f) {  // t > dis: delta tracking breaks at once too
+            if (w->loop_num < max_steps && tr > 0.5f) {
+                // the step draws and t > dis ends ratio tracking; the delta-tracking
+                // loop's test then passes unless the cap is reached (tr > 0.5 >
+                // 0.001), and its step draws once more and ends too (t only grows)
+                ++w->loop_num;
+                rng_seek(rng, n0 + (uint32_t)done + 1u + (w->loop_num < max_steps ? 1u : 0u));
                 w->phase = 0;
                 return;
             }
@@ -267,6 +272,7 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
     if (hit) {
         w->tr = 0.0f;
         w->phase = 0;
+        rng_seek(rng, n0 + 2u * (uint32_t)done);  // two draws per step, the last one the collision test
         return;
     }
     if (restart) {
@@ -275,6 +281,7 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
     }
     if (escape) {
         w->phase = 0;
+        rng_seek(rng, n0 + 2u * (uint32_t)done + 1u);  // done == m steps, then the escaping step's draw
         return;
     }
     w->t = tc;  // the last speculated position (t itself when m == 0)

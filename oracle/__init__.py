@@ -228,6 +228,24 @@ def render_mrpnn(grid, max_dens, mips_concat, tr_concat, lut, weights, params, t
     return target
 
 
+def serve_view(grid, max_dens, mips_concat, tr_concat, lut, weights, params, spp, threshold, fallback_spp, threads=0):
+    """One view of a serving batch on the CPU: (rgb (h,w,3), histo (h,w,33), flags (h,w) uint8, fallback lookups)."""
+    grid = np.ascontiguousarray(grid, dtype=np.float32)
+    h, w = params.height, params.width
+    rgb = np.zeros((h, w, 3), np.float32)
+    histo = np.zeros((h, w, 33), np.float32)
+    flags = np.zeros((h, w), np.uint8)
+    lk = ctypes.c_uint64()
+    c = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    rc = load().oracle_serve_view(_ptr(grid), ctypes.c_int(grid.shape[0]), ctypes.c_float(max_dens),
+                                  _ptr(c(mips_concat)), _ptr(c(tr_concat)), _ptr(c(lut)), _ptr(c(weights)),
+                                  ctypes.byref(params), ctypes.c_int(spp), ctypes.c_float(threshold),
+                                  ctypes.c_int(fallback_spp), _ptr(rgb), _ptr(histo), _ptr(flags), ctypes.byref(lk),
+                                  ctypes.c_int(threads))
+    assert rc == 0
+    return rgb, histo, flags, lk.value
+
+
 def normalize3(v):
     """FP32 normalize with the shared kernels' operation order."""
     a = np.ascontiguousarray(v, dtype=np.float32)

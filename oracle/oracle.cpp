@@ -837,13 +837,18 @@ static FirstVertex first_vertex(Rng* rng, const Medium& m, vec3 ori, vec3 dir, v
     return fv;
 }
 
-// MRPNN offline render: `RenderCamera<true, MRPNN>` (core/volume.cu:147-179)
-// with NNPredict; out = (predict + single) * lightColor * scatter_rate for
-// pixels whose first free flight scatters, the sky otherwise.
-int oracle_render_mrpnn(const float* grid, int res, float max_density, const float* mips, const float* trs,
-                        const float* lut, const float* weights, const mrp_render_params* p, float* target,
-                        double* sum64, int threads) {
+namespace {
+// The per-image constants of an MRPNN render.
+struct MrpnnImage {
     Medium m;
+    vec3 ori, up, right, light, forward, Ln;
+    mrp_vec3 srp;
+    int W, H;
+    uint64_t seed;
+};
+static MrpnnImage mrpnn_image(const float* grid, int res, float max_density, const mrp_render_params* p) {
+    MrpnnImage im;
+    Medium& m = im.m;
     m.grid = OGrid{grid, res};
     m.max_density = max_density;
     m.alpha = p->alpha;
@@ -854,45 +859,65 @@ int oracle_render_mrpnn(const float* grid, int res, float max_density, const flo
     m.scatter_rate = V(p->scatter_rate);
     m.light_color = V(p->light_color);
     m.mj = make_majorant(max_density, p->alpha, p->tr_scale);
-    const vec3 ori = V(p->ori), up = V(p->up), right = V(p->right), light = V(p->light_dir);
-    const vec3 forward = normalize(-ori);
-    const vec3 Ln = normalize(light);
+    im.ori = V(p->ori);
+    im.up = V(p->up);
+    im.right = V(p->right);
+    im.light = V(p->light_dir);
+    im.forward = normalize(-im.ori);
+    im.Ln = normalize(im.light);
     const vec3 sr = v3(m.scatter_rate.x / 1.001f, m.scatter_rate.y / 1.001f, m.scatter_rate.z / 1.001f);
-    mrp_vec3 srp;
-    srp.x = (float)pow((double)sr.x, 4.0);
-    srp.y = (float)pow((double)sr.y, 4.0);
-    srp.z = (float)pow((double)sr.z, 4.0);
+    im.srp.x = (float)pow((double)sr.x, 4.0);
+    im.srp.y = (float)pow((double)sr.y, 4.0);
+    im.srp.z = (float)pow((double)sr.z, 4.0);
+    im.W = p->width;
+    im.H = p->height;
+    im.seed = p->seed;
+    return im;
+}
+}  // namespace
+
+// One MRPNN sample s of pixel idx (Philox stream (idx, s, 0, seed)):
+// (predict + single) * lightColor * scatter_rate if the first free flight
+// scatters, the sky otherwise; not clamped.  feat: kFeatures floats of scratch.
+static vec3 mrpnn_sample(const MrpnnImage& im, const float* mips, const float* trs, const float* lut,
+                         const float* weights, long long idx, int s, float* feat, Counters* c) {
+    const Medium& m = im.m;
+    const int row = (int)(idx / im.W), col = (int)(idx % im.W);
+    Rng rng = rng_make((uint32_t)idx, (uint32_t)s, 0u, im.seed);
+    const float xu = rng_next(&rng);
+    const float xv = rng_next(&rng);
+    const vec3 dir = camera_dir(im.forward, im.right, im.up, row, col, im.W, im.H, xu, xv);
+    const FirstVertex fv = first_vertex(&rng, m, im.ori, dir, im.light, c);
+    if (!fv.active) return env_radiance(m.env, normalize(dir));
+    float gam;
+    mrp_vec3 Lm{im.Ln.x, im.Ln.y, im.Ln.z}, LXm{fv.LX.x, fv.LX.y, fv.LX.z};
+    oracle_mrpnn_features(mips, trs, lut, 1, &fv.pos.x, &fv.X.x, &fv.Y.x, &fv.Z.x, &fv.active, Lm, LXm, m.alpha, m.g,
+                          feat, &gam);
+    float pr[3];
+    oracle_mrpnn_forward(weights, 1, feat, &gam, m.g, im.srp, pr);
+    const vec3 pred = v3(pr[0] + fv.single, pr[1] + fv.single, pr[2] + fv.single);
+    return pred * m.light_color * m.scatter_rate;
+}
+
+// MRPNN offline render: `RenderCamera<true, MRPNN>` (core/volume.cu:147-179)
+// with NNPredict; out = (predict + single) * lightColor * scatter_rate for
+// pixels whose first free flight scatters, the sky otherwise.
+int oracle_render_mrpnn(const float* grid, int res, float max_density, const float* mips, const float* trs,
+                        const float* lut, const float* weights, const mrp_render_params* p, float* target,
+                        double* sum64, int threads) {
+    const MrpnnImage im = mrpnn_image(grid, res, max_density, p);
     const int W = p->width, H = p->height;
 #ifdef _OPENMP
     if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 8)
 #endif
     for (long long idx = 0; idx < (long long)W * H; ++idx) {
-        const int row = (int)(idx / W), col = (int)(idx % W);
         Counters c;
         vec3 tgt = v3(target[3 * idx], target[3 * idx + 1], target[3 * idx + 2]);
         double s0 = 0, s1 = 0, s2 = 0;
         std::vector<float> feat(kFeatures);
         for (int s = p->spp_begin; s < p->spp_begin + p->spp_count; ++s) {
-            Rng rng = rng_make((uint32_t)idx, (uint32_t)s, 0u, p->seed);
-            const float xu = rng_next(&rng);
-            const float xv = rng_next(&rng);
-            const vec3 dir = camera_dir(forward, right, up, row, col, W, H, xu, xv);
-            const FirstVertex fv = first_vertex(&rng, m, ori, dir, light, &c);
-            vec3 rgb;
-            if (fv.active) {
-                float gam;
-                mrp_vec3 Lm{Ln.x, Ln.y, Ln.z}, LXm{fv.LX.x, fv.LX.y, fv.LX.z};
-                oracle_mrpnn_features(mips, trs, lut, 1, &fv.pos.x, &fv.X.x, &fv.Y.x, &fv.Z.x, &fv.active, Lm, LXm,
-                                      m.alpha, m.g, feat.data(), &gam);
-                float pr[3];
-                oracle_mrpnn_forward(weights, 1, feat.data(), &gam, m.g, srp, pr);
-                const vec3 pred = v3(pr[0] + fv.single, pr[1] + fv.single, pr[2] + fv.single);
-                rgb = pred * m.light_color * m.scatter_rate;
-            } else {
-                rgb = env_radiance(m.env, normalize(dir));
-            }
-            rgb = vmax(v3(0, 0, 0), rgb);
+            const vec3 rgb = vmax(v3(0, 0, 0), mrpnn_sample(im, mips, trs, lut, weights, idx, s, feat.data(), &c));
             tgt = lerp(tgt, rgb, 1.0f / (float)(1 + s));
             s0 += rgb.x;
             s1 += rgb.y;
@@ -907,6 +932,57 @@ int oracle_render_mrpnn(const float* grid, int res, float max_density, const flo
             sum64[3 * idx + 2] += s2;
         }
     }
+    return 0;
+}
+
+// One view of a serving batch: MRPNN samples 0 .. spp-1 of every pixel into
+// the running mean and the frame path's pixel histogram (`RenderCamera`,
+// core/volume.cu:99-123); a pixel whose |E[l^2] - E[l]^2| exceeds threshold
+// is re-estimated from scratch with delta-tracking samples spp ..
+// spp+fallback_spp-1 (`CalculateRadiance`, core/volume.cu:17-34, on stream
+// (pixel, s, 0, seed)).  rgb (P*3), histo (P*kHistoFloats), flags (P) out;
+// *fallback_lookups = the density lookups of the fallback paths.
+int oracle_serve_view(const float* grid, int res, float max_density, const float* mips, const float* trs,
+                      const float* lut, const float* weights, const mrp_render_params* p, int spp, float threshold,
+                      int fallback_spp, float* rgb, float* histo, uint8_t* flags, uint64_t* fallback_lookups,
+                      int threads) {
+    if (!grid || !p || !rgb || !histo || spp < 1 || fallback_spp < 1) return -1;
+    const MrpnnImage im = mrpnn_image(grid, res, max_density, p);
+    const int W = p->width, H = p->height;
+    uint64_t lookups = 0;
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 8) reduction(+ : lookups)
+#endif
+    for (long long idx = 0; idx < (long long)W * H; ++idx) {
+        Counters c, cf;
+        std::vector<float> feat(kFeatures);
+        float tg[3] = {0.0f, 0.0f, 0.0f};
+        float* hp = histo + (size_t)kHistoFloats * idx;
+        for (int s = 0; s < spp; ++s)
+            histo_accumulate(tg, hp, vmax(v3(0, 0, 0), mrpnn_sample(im, mips, trs, lut, weights, idx, s, feat.data(), &c)),
+                             s);
+        const float x = hp[3 * kHistoBins + 2], x2 = hp[3 * kHistoBins + 1];
+        const bool flag = fabs_(x2 - x * x) > threshold;
+        if (flag) {
+            const int row = (int)(idx / W), col = (int)(idx % W);
+            tg[0] = tg[1] = tg[2] = 0.0f;
+            for (int k = 0; k < fallback_spp; ++k) {
+                Rng rng = rng_make((uint32_t)idx, (uint32_t)(spp + k), 0u, p->seed);
+                const float xu = rng_next(&rng);
+                const float xv = rng_next(&rng);
+                const vec3 dir = camera_dir(im.forward, im.right, im.up, row, col, W, H, xu, xv);
+                const Radiance r = calculate_radiance(&rng, im.m, im.ori, dir, im.light, p->multi_scatter, &cf);
+                histo_accumulate(tg, hp, vmax(v3(0, 0, 0), r.rgb), k);
+            }
+        }
+        rgb[3 * idx] = tg[0];
+        rgb[3 * idx + 1] = tg[1];
+        rgb[3 * idx + 2] = tg[2];
+        if (flags) flags[idx] = flag ? 1 : 0;
+        lookups += cf.lookups;
+    }
+    if (fallback_lookups) *fallback_lookups = lookups;
     return 0;
 }
 

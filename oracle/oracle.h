@@ -88,6 +88,12 @@ int oracle_mrpnn_forward(const float* w, int n, const float* feat, const float* 
 int oracle_render_mrpnn(const float* grid, int res, float max_density, const float* mips, const float* trs,
                         const float* lut, const float* weights, const mrp_render_params* p, float* target,
                         double* sum64, int threads);
+// One view of a batched serving call (mrp_serve_views): MRPNN samples, the
+// histogram-variance fallback to delta tracking; see oracle.cpp.
+int oracle_serve_view(const float* grid, int res, float max_density, const float* mips, const float* trs,
+                      const float* lut, const float* weights, const mrp_render_params* p, int spp, float threshold,
+                      int fallback_spp, float* rgb, float* histo, uint8_t* flags, uint64_t* fallback_lookups,
+                      int threads);
 int oracle_render(const float* grid, int res, float max_density, const mrp_render_params* p, float* target,
                   double* sum64, uint64_t* lookups_out, uint64_t* scatters_out, int threads, int row_begin,
                   int row_end);

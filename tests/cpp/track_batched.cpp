@@ -52,10 +52,13 @@ static int run(const Field& F, int max_steps, int rays, uint64_t seed) {
             uint32_t la = 0, lb = 0;
             const float ta = transmittance_seq(&a, F, inv_smax, ratio, pos, dir, dis, max_steps, &la);
             const float tb = transmittance_batched<B>(&b, F, inv_smax, ratio, pos, dir, dis, max_steps, &lb);
-            if (!same(ta, tb) || la != lb) {
-                if (bad < 5) std::printf("transmittance B=%d cap=%d ray %d: %.9g/%.9g lk %u/%u\n", B, max_steps, r, ta, tb, la, lb);
+            if (!same(ta, tb) || la != lb || a.n != b.n) {
+                if (bad < 5)
+                    std::printf("transmittance B=%d cap=%d ray %d: %.9g/%.9g lk %u/%u n %u/%u\n", B, max_steps, r, ta, tb,
+                                la, lb, a.n, b.n);
                 ++bad;
             }
+            if (rng_next(&a) != rng_next(&b)) ++bad;  // the stream continues identically
         }
     }
     return bad;
