@@ -89,6 +89,17 @@ def luma_rmse100(img, ref):
     return float(100.0 * np.sqrt(np.mean(d * d)))
 
 
+def luma_rmse100_packed(bgra, ref):
+    """luma_rmse100 of a packed 0xAARRGGBB image (ACES, exposure 1) against a linear reference."""
+    from internship_amd.tonemap import aces_np
+    b = np.asarray(bgra, np.uint32)
+    v = np.stack([(b >> 16) & 255, (b >> 8) & 255, b & 255], axis=-1).astype(np.float64)
+    la = (0.299 * v[..., 0] + 0.587 * v[..., 1] + 0.114 * v[..., 2]) / 255.0
+    t = np.floor(np.clip(aces_np(np.asarray(ref, np.float32)), 0, 1) * 255.0)
+    lb = (0.299 * t[..., 0] + 0.587 * t[..., 1] + 0.114 * t[..., 2]) / 255.0
+    return float(100.0 * np.sqrt(np.mean((la - lb) ** 2)))
+
+
 class Scene:
     def __init__(self, ia, res, width, height, bounces):
         self.ia, self.res, self.W, self.H, self.bounces = ia, res, width, height, bounces
@@ -561,80 +572,106 @@ def eval_networks_on_headline(a, ia, nets):
 
 
 def run_serve(a, ia, sc):
-    """Batched serving: 16 Perlin-fBm clouds of 128..512^3 (seeds 200-215) x 4
-    orbit cameras at 1280x720, one 1-spp BF16 MRPNN frame per request, served
-    by --workers host threads, each owning a set of scenes (own volume, HIP
-    stream and buffers), so requests of different scenes overlap on the GPU."""
-    import concurrent.futures as cf
-    W, H = 1280, 720
-    sizes = [128, 256, 384, 512]
-    net = ia.Network()
-    scenes = []
+    """Batched serving: 16 Perlin-fBm clouds with resolutions drawn from
+    {128,192,256,384,512}^3 (seeds 200-215) x 4 orbit cameras at 1280x720 =
+    64 views per batch, each call (mrp_serve_views) rendering all of them:
+    BF16 MRPNN (the native network), a delta-tracking fallback for pixels whose
+    sample variance exceeds --serve-threshold (one persistent launch across
+    the scenes), histogram denoise and ACES pack.  A step is one batch; its
+    latency is the per-frame latency of every view in it.  RMSE per scene is
+    over linear RGB of camera 0 against a --serve-truth-spp delta-tracking
+    image with FP64 sums (the GPU tracer, which is bit-exact with the oracle)."""
+    from internship_amd.serving import serving_scenes, view_params
+    W, H = a.serve_width, a.serve_height
+    scenes = serving_scenes(16, 4)
+    which, nets = headline_networks(a, ia)
+    net = ia.Network(nets[which])
+    vols, views, vlist = [], [], []
     t0 = time.perf_counter()
-    for i in range(16):
-        res = sizes[i % 4]
-        grid = ia.DeviceBuffer(res ** 3 * 4)
-        ia.generate_cloud(res, seed=200 + i, device_out=grid)
-        vol = ia.VolumeRender(res)
+    for k, s in enumerate(scenes):
+        grid = ia.DeviceBuffer(s.res ** 3 * 4)
+        ia.generate_cloud(s.res, seed=s.seed, threshold=s.threshold, device_out=grid)
+        vol = ia.VolumeRender(s.res)
         vol.set_datas(grid)
         vol.update()
         grid.free()
         vol.set_network(net)
-        tgt = ia.DeviceBuffer(W * H * 12)
-        tgt.zero()
-        scenes.append((vol, tgt))
-    log("16 scenes prepared in %.1f s" % (time.perf_counter() - t0))
-    light = np.array([0.0, 1.0, 0.3]) / np.linalg.norm([0.0, 1.0, 0.3])
-    cams = []
-    for k in range(4):
-        az = np.radians(45.0 + 90.0 * k)
-        d = np.array([np.cos(az) * np.cos(np.radians(25)), np.sin(np.radians(25)), np.sin(az) * np.cos(np.radians(25))])
-        ori = 2.5 * d
-        right, up = ia.camera_basis(ori, 40.0, W / H)
-        cams.append((ori, up, right))
-
-    def request(si, ci, frame):
-        vol, tgt = scenes[si]
-        ori, up, right = cams[ci]
-        V = ia.Vec3.of
-        p = ia.RenderParams(W, H, V(ori), V(up), V(right), V(light), V((1, 1, 1)), V((0.1, 0.1, 0.1)),
-                            V((0.999, 0.999, 0.999)), 0.857, 256.0, 1.0, 512, frame, 1, 0, ia.MRPNN, 1)
-        t = time.perf_counter()
-        vol.render_dev(p, tgt)
-        return time.perf_counter() - t
-
-    def worker(wi, rounds):
-        lat = []
-        mine = [si for si in range(16) if si % a.workers == wi]
-        for r in range(rounds):
-            for si in mine:
-                for ci in range(4):
-                    lat.append(request(si, ci, r))
-        return lat
-
-    with cf.ThreadPoolExecutor(a.workers) as ex:  # warm-up: per-light tables, scratch
-        list(ex.map(lambda wi: worker(wi, 1), range(a.workers)))
+        vols.append(vol)
+        for cam in s.cameras:
+            views.append(view_params(s, cam, W, H, g=0.857, albedo=0.99, multi=a.bounces, seed=k, precision=1))
+            vlist.append(vol)
+    log("%d scenes (%s) prepared in %.1f s" % (len(scenes), ",".join(str(s.res) for s in scenes),
+                                               time.perf_counter() - t0))
+    out = ia.ServeBatch(len(views), W, H)
+    kw = dict(mrpnn_spp=a.serve_spp, var_threshold=a.serve_threshold, fallback_spp=a.serve_fallback_spp,
+              denoise=True, tone=2)
+    for _ in range(max(1, a.warmup)):
+        st = out.run(vlist, views, **kw)
+    lat, stats = [], []
     t0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(a.workers) as ex:
-        lats = [x for l in ex.map(lambda wi: worker(wi, a.steps), range(a.workers)) for x in l]
+    for _ in range(a.steps):
+        t = time.perf_counter()
+        st = out.run(vlist, views, **kw)
+        lat.append(time.perf_counter() - t)
+        stats.append(st.asdict())
     elapsed = time.perf_counter() - t0
+    lat_ms = np.array(lat) * 1000.0
+    n_views = len(views)
+    mean = {k: float(np.mean([d[k] for d in stats])) for k in stats[0]}
+    log("serving: %d batches of %d views in %.2f s: %.1f frames/s, batch latency p50 %.1f ms, p99 %.1f ms; "
+        "device %.1f ms (MRPNN %.1f, accumulate %.2f, fallback %.1f, denoise %.1f); fallback pixels %.0f (%.2f%%)"
+        % (a.steps, n_views, elapsed, n_views * a.steps / elapsed, np.percentile(lat_ms, 50),
+           np.percentile(lat_ms, 99), mean["total_ms"], mean["mrpnn_ms"], mean["accumulate_ms"], mean["fallback_ms"],
+           mean["denoise_ms"], mean["fallback_pixels"], 100.0 * mean["fallback_pixels"] / (n_views * W * H)))
+    rgb, bgra, flags = out.download()
+    # MRPNN alone (no fallback, no denoise) for comparison, same views
+    out.run(vlist, views, mrpnn_spp=a.serve_spp, var_threshold=1e30, fallback_spp=1, denoise=False, tone=2,
+            flags=False)
+    rgb_nn = out.rgb.download(np.float32, (n_views, H, W, 3))
+    bgra_nn = out.bgra.download(np.uint32, (n_views, H, W))
+    per_scene = []
+    if a.serve_truth_spp > 0:
+        t0 = time.perf_counter()
+        for k, s in enumerate(scenes):
+            i = 4 * k
+            p = ia.RenderParams.from_buffer_copy(views[i])
+            p.render_type, p.spp_begin, p.spp_count = ia.PT, 0, a.serve_truth_spp
+            tgt = ia.DeviceBuffer(W * H * 12)
+            tgt.zero()
+            s64 = ia.DeviceBuffer(W * H * 24)
+            s64.zero()
+            vlist[i].render_dev(p, tgt, s64)
+            ref = s64.download(np.float64, (H, W, 3)) / a.serve_truth_spp
+            tgt.free()
+            s64.free()
+            d = {"scene": k, "res": s.res, "seed": s.seed, "alpha": round(s.alpha, 2),
+                 "rmse_served": rmse(rgb[i], ref), "rmse_mrpnn_only": rmse(rgb_nn[i], ref),
+                 "luma_rmse100_served_denoised": luma_rmse100_packed(bgra[i], ref),
+                 "luma_rmse100_mrpnn_only": luma_rmse100_packed(bgra_nn[i], ref),
+                 "fallback_frac": float(flags[i].mean())}
+            per_scene.append(d)
+            log("  scene %2d (%3d^3, alpha %6.1f): RMSE served %.5f, MRPNN only %.5f; luma RMSEx100 served+denoise "
+                "%.3f, MRPNN only %.3f; fallback %.2f%% (%.1f s)"
+                % (d["scene"], d["res"], d["alpha"], d["rmse_served"], d["rmse_mrpnn_only"],
+                   d["luma_rmse100_served_denoised"], d["luma_rmse100_mrpnn_only"], 100 * d["fallback_frac"],
+                   time.perf_counter() - t0))
     net.close()
-    for vol, tgt in scenes:
-        vol.close()
-        tgt.free()
-    lat_ms = np.array(lats) * 1000.0
-    n_req = len(lats)
-    log("serving: %d requests in %.2f s: %.1f frames/s, latency p50 %.2f ms, p99 %.2f ms"
-        % (n_req, elapsed, n_req / elapsed, np.percentile(lat_ms, 50), np.percentile(lat_ms, 99)))
+    for v in vols:
+        v.close()
     return {
-        "metric": "720p MRPNN frames/s over 16 scenes x 4 cameras (batched serving)", "unit": "frames/s",
-        "value": n_req / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps, "dtype": "bf16",
-        "config": {"workload": "16 Perlin-fBm clouds (128..512^3, seeds 200-215) x 4 orbit cameras, 1280x720, "
-                               "1-spp BF16 MRPNN frames, %d host workers" % a.workers,
-                   "scenes": 16, "cameras": 4, "width": W, "height": H, "workers": a.workers},
+        "metric": "720p served frames/s over 16 scenes x 4 cameras (batched: MRPNN + DT fallback + denoise)",
+        "unit": "frames/s", "value": n_views * a.steps / elapsed, "ms_per_step": 1000.0 * elapsed / a.steps,
+        "dtype": "bf16",
+        "config": {"workload": "16 Perlin-fBm clouds (128..512^3, seeds 200-215) x 4 orbit cameras, %dx%d, %d BF16 "
+                               "MRPNN spp, delta-tracking fallback (%d spp, %d bounces) above variance %g, denoise, "
+                               "%s network; one batch of %d views per step" % (W, H, a.serve_spp,
+                                                                              a.serve_fallback_spp, a.bounces,
+                                                                              a.serve_threshold, which, n_views),
+                   "scenes": len(scenes), "cameras": 4, "width": W, "height": H, "network": which},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p90": float(np.percentile(lat_ms, 90)),
-                       "p99": float(np.percentile(lat_ms, 99)), "max": float(lat_ms.max())},
-        "requests": n_req,
+                       "p99": float(np.percentile(lat_ms, 99)), "max": float(lat_ms.max()),
+                       "note": "per batch = per frame of every view in it"},
+        "device_ms": mean, "per_scene": per_scene, "truth_spp": a.serve_truth_spp,
     }
 
 
@@ -765,7 +802,12 @@ def main(argv=None):
     ap.add_argument("--mlp-precision", type=int, choices=[0, 1, 2], default=1,
                     help="headline/mlp modes: network of the timed frames (0 FP32, 1 BF16, 2 FP8-E4M3)")
     ap.add_argument("--batch", type=int, default=16384, help="train mode: samples per step")
-    ap.add_argument("--workers", type=int, default=4, help="serve mode: host threads")
+    ap.add_argument("--serve-width", type=int, default=1280)
+    ap.add_argument("--serve-height", type=int, default=720)
+    ap.add_argument("--serve-spp", type=int, default=2, help="serve mode: MRPNN samples per pixel")
+    ap.add_argument("--serve-threshold", type=float, default=0.05, help="serve mode: fallback variance threshold")
+    ap.add_argument("--serve-fallback-spp", type=int, default=32, help="serve mode: delta-tracking spp of a fallback")
+    ap.add_argument("--serve-truth-spp", type=int, default=4096, help="serve mode: ground-truth spp (0: none)")
     ap.add_argument("--lr", type=float, default=1e-3, help="train mode: Adam learning rate")
     ap.add_argument("--epochs", type=int, default=20, help="train mode: epochs over the training samples")
     ap.add_argument("--init-seed", type=int, default=0, help="train mode: seed of the random initial weights")

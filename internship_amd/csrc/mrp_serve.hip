@@ -11,9 +11,11 @@
 //      (core/volume.cu:229-231) - exceeds the caller's threshold;
 //   3. fallback_kernel re-estimates the listed pixels of all views with
 //      delta tracking (`CalculateRadiance`, core/volume.cu:17-34): one
-//      persistent launch whose waves claim 64 pixels at a time from one
-//      queue, so scenes of different density and size balance across the
-//      chip; the walks keep several lookups in flight per lane (mrp_track.h);
+//      persistent launch (per 16M-sample pass) whose waves claim 64 (pixel,
+//      sample) paths at a time from one queue, so scenes of different
+//      density and size balance across the chip; the walks keep several
+//      lookups in flight per lane (mrp_track.h); fallback_finish_kernel then
+//      folds each pixel's samples, in order, into its mean and histogram;
 //   4. the frame path's denoise and BGRA8 pack over every view (one launch,
 //      blockIdx.z = view).
 // The reference renders one scene per call and has no such mode; the parts
@@ -33,6 +35,7 @@ namespace mrp {
 constexpr int kFallbackBatch = 4;      // lookups in flight per lane
 constexpr int kFallbackSteps = 10000;  // loop tests per walk (the reference's cap)
 constexpr int kFallbackBlocksPerCU = 4;
+constexpr size_t kFallbackPassItems = (size_t)1 << 24;  // 256 MiB of per-sample results per pass
 
 // What a fallback path needs of its view.
 struct FallbackView {
@@ -59,18 +62,23 @@ __global__ void clear_u32_pair_kernel(uint32_t* p) {
 // One thread per (view, pixel): mean and histogram of the MRPNN samples,
 // the fallback flag, and the list of flagged pixels (a wave ballot and one
 // atomic per wave; the list order depends on scheduling, no result does).
-__global__ __launch_bounds__(256) void serve_accumulate_kernel(ServeArgs a, const float4* __restrict__ samples,
-                                                               float* __restrict__ rgb, float* __restrict__ histo,
-                                                               uint8_t* __restrict__ flags,
-                                                               uint32_t* __restrict__ list,
-                                                               uint32_t* __restrict__ count) {
+// The block's histograms are built in LDS (a stride of 33 floats: no bank
+// conflicts) and stored as one contiguous run of 256 x 33 floats.
+constexpr int kAccBlock = 256;
+__global__ __launch_bounds__(kAccBlock) void serve_accumulate_kernel(ServeArgs a, const float4* __restrict__ samples,
+                                                                     float* __restrict__ rgb, float* __restrict__ histo,
+                                                                     uint8_t* __restrict__ flags,
+                                                                     uint32_t* __restrict__ list,
+                                                                     uint32_t* __restrict__ count) {
+    __shared__ float s_h[kAccBlock * kHistoFloats];
     const size_t total = (size_t)a.n_views * a.P;
-    const size_t gi = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t g0 = (size_t)blockIdx.x * kAccBlock;
+    const size_t gi = g0 + threadIdx.x;
     bool flag = false;
+    float* hp = s_h + kHistoFloats * threadIdx.x;
     if (gi < total) {
         const size_t view = gi / a.P, pix = gi - view * a.P;
         float tg[3] = {0.0f, 0.0f, 0.0f};
-        float* hp = histo + (size_t)kHistoFloats * gi;
         for (int s = 0; s < a.spp; ++s) {
             const float4 r = samples[(view * (size_t)a.spp + (size_t)s) * a.P + pix];
             histo_accumulate(tg, hp, vmax(v3(0, 0, 0), v3(r.x, r.y, r.z)), s);
@@ -88,30 +96,35 @@ __global__ __launch_bounds__(256) void serve_accumulate_kernel(ServeArgs a, cons
     if (lane == 0u && m) base = atomicAdd(count, (uint32_t)__popcll(m));
     base = __builtin_amdgcn_readfirstlane(base);  // lane 0 is active: no thread has left
     if (flag) list[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = (uint32_t)gi;
+    __syncthreads();
+    const size_t n_here = total - g0 < (size_t)kAccBlock ? total - g0 : (size_t)kAccBlock;
+    float* out = histo + (size_t)kHistoFloats * g0;
+    for (size_t i = threadIdx.x; i < n_here * kHistoFloats; i += kAccBlock) out[i] = s_h[i];
 }
 
-// Persistent: each wave claims 64 listed pixels at a time until the list is
-// exhausted (every wave reaches the exit test).  A pixel's samples
-// spp .. spp+fallback_spp-1 run in order on Philox streams (pixel, s, 0,
-// seed), each a `CalculateRadiance` path: its mean and histogram replace the
-// MRPNN ones.  Lookups go to counters[3].
+// Persistent: each wave claims 64 (fallback pixel, sample) items at a time
+// until the pass's items are exhausted (every wave reaches the exit test).
+// Item it of the pass is sample k = it % fallback_spp of listed pixel
+// j = j0 + it / fallback_spp: sample spp + k on Philox stream (pixel, spp + k,
+// 0, seed), a `CalculateRadiance` path, stored at out[it].  Paths of one
+// pixel run in different lanes, so the long and short ones of a wave mix
+// across pixels.  Lookups go to counters[3].
 __global__ __launch_bounds__(256) void fallback_kernel(ServeArgs a, const FallbackView* __restrict__ views,
-                                                       const uint32_t* __restrict__ list,
-                                                       const uint32_t* __restrict__ count,
-                                                       uint32_t* __restrict__ cursor, float* __restrict__ rgb,
-                                                       float* __restrict__ histo,
+                                                       const uint32_t* __restrict__ list, uint32_t j0,
+                                                       uint32_t n_items, uint32_t* __restrict__ cursor,
+                                                       float4* __restrict__ out,
                                                        unsigned long long* __restrict__ counters) {
     __shared__ unsigned red[256];
-    const uint32_t n = *count;
     uint32_t lk = 0u;
     for (;;) {
         uint32_t base = 0u;
         if (__lane_id() == 0u) base = atomicAdd(cursor, 64u);
         base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= n) break;
-        const uint32_t j = base + __lane_id();
-        if (j < n) {
-            const uint32_t gi = list[j];
+        if (base >= n_items) break;
+        const uint32_t it = base + __lane_id();
+        if (it < n_items) {
+            const uint32_t jj = it / (uint32_t)a.fallback_spp, k = it - jj * (uint32_t)a.fallback_spp;
+            const uint32_t gi = list[j0 + jj];
             const uint32_t view = gi / a.P, pix = gi - view * a.P;
             const FallbackView& v = views[view];
             const int row = (int)pix / v.width, col = (int)pix - row * v.width;
@@ -131,21 +144,14 @@ __global__ __launch_bounds__(256) void fallback_kernel(ServeArgs a, const Fallba
                 return transmittance_batched<kFallbackBatch>(rg, F, v.mj.inv_smax_tr, v.mj.ratio, p, d, dis,
                                                              kFallbackSteps, &lk);
             };
-            float tg[3] = {0.0f, 0.0f, 0.0f};
-            float* hp = histo + (size_t)kHistoFloats * gi;
-            for (int k = 0; k < a.fallback_spp; ++k) {
-                Rng rng = rng_make(pix, (uint32_t)(a.spp + k), 0u, seed);
-                const float xu = rng_next(&rng);
-                const float xv = rng_next(&rng);
-                const vec3 dir = camera_dir(v.forward, v.right, v.up, row, col, v.width, v.height, xu, xv);
-                float t;
-                const vec3 r = radiance_paths(&rng, next_vertex, transmit, v.ori, dir, v.light, v.light_color, v.env,
-                                              v.scatter_rate, v.g, v.multi_scatter, 1, &t);
-                histo_accumulate(tg, hp, vmax(v3(0, 0, 0), r), k);
-            }
-            rgb[3 * (size_t)gi] = tg[0];
-            rgb[3 * (size_t)gi + 1] = tg[1];
-            rgb[3 * (size_t)gi + 2] = tg[2];
+            Rng rng = rng_make(pix, (uint32_t)a.spp + k, 0u, seed);
+            const float xu = rng_next(&rng);
+            const float xv = rng_next(&rng);
+            const vec3 dir = camera_dir(v.forward, v.right, v.up, row, col, v.width, v.height, xu, xv);
+            float t;
+            const vec3 r = radiance_paths(&rng, next_vertex, transmit, v.ori, dir, v.light, v.light_color, v.env,
+                                          v.scatter_rate, v.g, v.multi_scatter, 1, &t);
+            out[it] = make_float4(r.x, r.y, r.z, t);
         }
     }
     red[threadIdx.x] = lk;
@@ -155,6 +161,26 @@ __global__ __launch_bounds__(256) void fallback_kernel(ServeArgs a, const Fallba
         __syncthreads();
     }
     if (threadIdx.x == 0 && red[0]) atomicAdd(&counters[3], (unsigned long long)red[0]);
+}
+
+// One thread per fallback pixel of the pass: its samples, in order, replace
+// the MRPNN mean and histogram.
+__global__ __launch_bounds__(256) void fallback_finish_kernel(ServeArgs a, const uint32_t* __restrict__ list,
+                                                              uint32_t j0, uint32_t n_pix,
+                                                              const float4* __restrict__ samples,
+                                                              float* __restrict__ rgb, float* __restrict__ histo) {
+    const uint32_t jj = blockIdx.x * blockDim.x + threadIdx.x;
+    if (jj >= n_pix) return;
+    const size_t gi = list[j0 + jj];
+    float tg[3] = {0.0f, 0.0f, 0.0f};
+    float* hp = histo + (size_t)kHistoFloats * gi;
+    for (int k = 0; k < a.fallback_spp; ++k) {
+        const float4 r = samples[(size_t)jj * a.fallback_spp + k];
+        histo_accumulate(tg, hp, vmax(v3(0, 0, 0), v3(r.x, r.y, r.z)), k);
+    }
+    rgb[3 * gi] = tg[0];
+    rgb[3 * gi + 1] = tg[1];
+    rgb[3 * gi + 2] = tg[2];
 }
 
 static vec3 V(const mrp_vec3& a) { return v3(a.x, a.y, a.z); }
@@ -210,8 +236,11 @@ extern "C" int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_para
     const size_t b_samples = align256(n_pix * (size_t)sp->mrpnn_spp * sizeof(float4));
     const size_t b_histo = align256(n_pix * kHistoFloats * sizeof(float));
     const size_t b_list = align256(n_pix * sizeof(uint32_t));
+    // fallback samples of one pass: at most kFallbackPassItems (pixel, sample) items
+    const size_t pass_items = std::min(n_pix * (size_t)sp->fallback_spp, kFallbackPassItems);
+    const size_t b_fs = align256(pass_items * sizeof(float4));
     void* scratch = nullptr;
-    int rc = ensure_serve_scratch(host, b_tables + b_fb + b_samples + b_histo + b_list + 256, &scratch);
+    int rc = ensure_serve_scratch(host, b_tables + b_fb + b_samples + b_histo + b_list + b_fs + 256, &scratch);
     if (rc) return rc;
     char* sb = (char*)scratch;
     void* tables = sb;
@@ -219,7 +248,8 @@ extern "C" int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_para
     float4* samples = (float4*)(sb + b_tables + b_fb);
     float* histo = (float*)(sb + b_tables + b_fb + b_samples);
     uint32_t* list = (uint32_t*)(sb + b_tables + b_fb + b_samples + b_histo);
-    uint32_t* qc = (uint32_t*)(sb + b_tables + b_fb + b_samples + b_histo + b_list);  // [count, cursor]
+    float4* fsamples = (float4*)(sb + b_tables + b_fb + b_samples + b_histo + b_list);
+    uint32_t* qc = (uint32_t*)(sb + b_tables + b_fb + b_samples + b_histo + b_list + b_fs);  // [count, cursor]
 
     for (hipEvent_t& e : host->serve_events)
         if (!e) MRP_HIP(hipEventCreate(&e));
@@ -268,10 +298,24 @@ extern "C" int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_para
     serve_accumulate_kernel<<<(unsigned)((n_pix + 255) / 256), 256, 0, host->stream>>>(a, samples, rgb_dev, histo,
                                                                                        flags_dev, list, qc);
     MRP_HIP(hipGetLastError());
+    // the fallback count decides the passes: one read-back
+    MRP_HIP(hipMemcpyAsync(host->host_words, qc, sizeof(uint32_t), hipMemcpyDeviceToHost, host->stream));
+    MRP_HIP(hipStreamSynchronize(host->stream));
+    const uint32_t n_flag = (uint32_t)host->host_words[0];
     MRP_HIP(hipEventRecord(ev[2], host->stream));
     const unsigned fb_blocks = (unsigned)std::max(1, host->num_cus * kFallbackBlocksPerCU);
-    fallback_kernel<<<fb_blocks, 256, 0, host->stream>>>(a, fb_dev, list, qc, qc + 1, rgb_dev, histo, host->counters);
-    MRP_HIP(hipGetLastError());
+    const uint32_t pass_pix = (uint32_t)std::max<size_t>(1, pass_items / (size_t)sp->fallback_spp);
+    for (uint32_t j0 = 0; j0 < n_flag; j0 += pass_pix) {
+        const uint32_t np = std::min(pass_pix, n_flag - j0);
+        const uint32_t items = np * (uint32_t)sp->fallback_spp;
+        clear_u32_pair_kernel<<<1, 64, 0, host->stream>>>(qc);  // the cursor (the count was read)
+        MRP_HIP(hipGetLastError());
+        fallback_kernel<<<fb_blocks, 256, 0, host->stream>>>(a, fb_dev, list, j0, items, qc + 1, fsamples,
+                                                             host->counters);
+        MRP_HIP(hipGetLastError());
+        fallback_finish_kernel<<<(np + 255) / 256, 256, 0, host->stream>>>(a, list, j0, np, fsamples, rgb_dev, histo);
+        MRP_HIP(hipGetLastError());
+    }
     MRP_HIP(hipEventRecord(ev[3], host->stream));
     if (bgra_dev) {
         rc = launch_denoise_images(host->stream, W, H, n_views, sp->denoise ? 1 : 0, sp->tone, sp->exposure, rgb_dev,
@@ -279,17 +323,16 @@ extern "C" int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_para
         if (rc) return rc;
     }
     MRP_HIP(hipEventRecord(ev[4], host->stream));
-    // counters and the fallback count into pinned memory, then one synchronisation
+    // counters into pinned memory, then one synchronisation
     MRP_HIP(hipMemcpyAsync(host->host_words + 1, host->counters, 4 * sizeof(unsigned long long),
                            hipMemcpyDeviceToHost, host->stream));
-    MRP_HIP(hipMemcpyAsync(host->host_words, qc, sizeof(uint32_t), hipMemcpyDeviceToHost, host->stream));
     MRP_HIP(hipStreamSynchronize(host->stream));
     if (stats) {
         *stats = mrp_serve_stats{};
         stats->mrpnn_lookups = host->host_words[1] + host->host_words[3];
         stats->fallback_lookups = host->host_words[4];
         stats->descriptors = n_active;
-        stats->fallback_pixels = (uint32_t)host->host_words[0];
+        stats->fallback_pixels = n_flag;
         MRP_HIP(hipEventElapsedTime(&stats->total_ms, ev[0], ev[4]));
         MRP_HIP(hipEventElapsedTime(&stats->mrpnn_ms, ev[0], ev[1]));
         MRP_HIP(hipEventElapsedTime(&stats->accumulate_ms, ev[1], ev[2]));
