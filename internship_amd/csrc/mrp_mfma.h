@@ -255,7 +255,8 @@ __device__ __forceinline__ void dense_d(const Packed<F>& pk, const LayerDesc d, 
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb)
 #pragma unroll
-            for (int e = 0; e < 8; ++e) amax = __builtin_fmaxf(amax, fabsf(X[kb >> 1][8 * (kb & 1) + e]));
+            for (int e = 0; e < 8; ++e)  // element e of K block kb is feature 16*kb + 8*(e/4) + ...: skip padding
+                if (16 * kb + 8 * (e >> 2) < d.K) amax = __builtin_fmaxf(amax, fabsf(X[kb >> 1][8 * (kb & 1) + e]));
         amax = __builtin_fmaxf(amax, __shfl_xor(amax, 32));  // the item's other lane group
         // amax in [2^(eb-127), 2^(eb-126)): scale 2^(134-eb) puts it in [128, 256)
         uint32_t eb = __float_as_uint(amax) >> 23;
@@ -280,20 +281,26 @@ __device__ __forceinline__ void dense_d(const Packed<F>& pk, const LayerDesc d, 
         const typename OpOf<F>::T* at = pk.a + (size_t)(d.tile + mt * KB) * 64 + lane;
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) acc = mfma(at[kb * 64], B[kb], acc);
+        // element-wise work only on registers that hold a row < M: register v
+        // holds rows mt*32 + 8*(v/4) + 4*kg + v%4, and a padded row has zero
+        // weights and bias, so its value is an exact 0 that needs no ReLU
         if constexpr (F == Fmt::FP8) {
             if (d.btile >= 0) {
                 f32x16 b;
                 load_bias(pk.bias, d.btile + mt, kg, b);
 #pragma unroll
-                for (int v = 0; v < 16; ++v) acc[v] = fmaf(acc[v], unscale, b[v]);
+                for (int v = 0; v < 16; ++v)
+                    if (mt * 32 + 8 * (v >> 2) < d.M) acc[v] = fmaf(acc[v], unscale, b[v]);
             } else {
 #pragma unroll
-                for (int v = 0; v < 16; ++v) acc[v] = acc[v] * unscale;
+                for (int v = 0; v < 16; ++v)
+                    if (mt * 32 + 8 * (v >> 2) < d.M) acc[v] = acc[v] * unscale;
             }
         }
         if (relu) {
 #pragma unroll
-            for (int v = 0; v < 16; ++v) acc[v] = relu_bits(acc[v]);
+            for (int v = 0; v < 16; ++v)
+                if (mt * 32 + 8 * (v >> 2) < d.M) acc[v] = relu_bits(acc[v]);
         }
         D[mt] = acc;
     }

@@ -340,9 +340,13 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
                     if (S == 32) load16(srow, 16, kg, xv, 1);
                 }
             }
+            // registers 0..R-1 hold the block's S input features (two lane groups,
+            // 4 rows per register group); the others are padding, kept at 0
+            const int R = S == 8 ? 4 : (S == 16 ? 8 : 16);
             f32x16 x;
 #pragma unroll
-            for (int v = 0; v < 16; ++v) x[v] = xv[v] * gate[k] + (residual ? prev[k][v] : 0.0f);
+            for (int v = 0; v < 16; ++v)  // gate and residual in one rounding (one FMA)
+                x[v] = v < R ? (residual ? fmaf(xv[v], gate[k], prev[k][v]) : xv[v] * gate[k]) : 0.0f;
             f32x16 y[1];
             if (S == 32) {
                 dense<F, 1, 2>(pk, LY_STREAM + 3 * blk + k, &x, y, true, lane);
@@ -362,7 +366,7 @@ __device__ __forceinline__ void mlp_persistent(const MlpArgs& args, typename OpO
                 prev[k] = w;
             } else {
 #pragma unroll
-                for (int v = 0; v < 16; ++v) prev[k][v] = y[0][v] + xv[v];
+                for (int v = 0; v < 16; ++v) prev[k][v] = v < R ? y[0][v] + xv[v] : 0.0f;
             }
         }
         if (blk == 7) {

@@ -297,17 +297,27 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
                                                        void* __restrict__ feat_out, float* __restrict__ gamma,
                                                        float* __restrict__ pool) {
-    __shared__ float sbuf[2][kFeatures];
+    // the descriptor row in LDS, stream k at k * kRowPitch: the pitch is 8
+    // banks past a multiple of 32, so the three streams' segments start in
+    // different banks for the pooling lanes
+    constexpr int kRowPitch = kTaps + 8;
+    __shared__ float sbuf[2][3 * kRowPitch];
     // tiled: BF16 chunks of kStageItems consecutive descriptors, flushed as whole 128-byte lines
     constexpr int kStageItems = 8, kChunks = 3 * 2 * kStreamChunks;
-    __shared__ bf16x8_t stage[kTiled ? kChunks * kStageItems : 1];
+    // staged chunk q of item j at stage[j * kStagePitch + q]: the chunk writes
+    // of one item are consecutive 16-byte words, and the flush's 8 lanes of one
+    // chunk read 8 rows whose pitch of 73 words spreads them over all banks
+    constexpr int kStagePitch = kChunks + 1;
+    __shared__ bf16x8_t stage[kTiled ? kStagePitch * kStageItems : 1];
     float* __restrict__ feat = (float*)feat_out;
     // a grid sized for n_items ends early.  Wave-uniform (readfirstlane), so the
     // loop does not wait at every item for this load, which on CDNA would also
     // wait for the previous item's stores (one counter for both).  Read first:
     // the grid covers the whole chunk, and most workgroups have no item, so they
     // leave before the per-tap set-up (uniform over the workgroup: no barrier
-    // is split).
+    // is split).  A capped grid striding over the groups instead measured
+    // slower (0.598 against 0.580 ms at 1080p): the empty workgroups cost less
+    // than the uneven split and the two registers the loop spills.
     const uint32_t n_items = __builtin_amdgcn_readfirstlane(a.n_dev ? *a.n_dev : a.n_items);
     if (blockIdx.x * kGatherItems >= n_items) return;
     const int i = threadIdx.x;
@@ -422,24 +432,25 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
         if (kTiled || pool) {
             float* sf = sbuf[it & 1];
             sf[i] = xd;
-            sf[kTaps + i] = xt;
-            sf[2 * kTaps + i] = xh;
+            sf[kRowPitch + i] = xt;
+            sf[2 * kRowPitch + i] = xh;
             __syncthreads();
             if (kTiled && i >= 96 && i < 96 + kChunks) {
                 // chunk q = (stream k, 16-feature chunk c, lane group kg)
                 const int q = i - 96, k = q / (2 * kStreamChunks), c = (q >> 1) % kStreamChunks, kg = q & 1;
-                const float* src = sf + k * kTaps + 16 * c + 4 * kg;
+                const float* src = sf + k * kRowPitch + 16 * c + 4 * kg;
                 bf16x8_t v;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     v[e] = (__bf16)src[e];
                     v[4 + e] = (__bf16)src[8 + e];
                 }
-                stage[q * kStageItems + (it & (kStageItems - 1))] = v;
+                stage[(it & (kStageItems - 1)) * kStagePitch + q] = v;
             }
-            if (pool && i < 72) {
-                const int q = i % 36, blk = q / 3, k = q % 3;
-                const float* seg = sf + k * kTaps + c_seg_from[blk];
+            if (pool && i < 36) {
+                // lane (block, stream) computes both statistics of its segment
+                const int q = i, blk = q / 3, k = q % 3;
+                const float* seg = sf + k * kRowPitch + c_seg_from[blk];
                 const int n = c_seg_size[blk];  // 8, 16 or 32, per thread
                 // 8 reads in flight per step (the loop bound differs across the
                 // wave, so a plain loop would wait for each read); same order of sums
@@ -454,8 +465,8 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
                         mx = fmax_(mx, v[e]);
                     }
                 }
-                const float r = i < 36 ? sum / (float)n : mx;
-                pool[(size_t)item * 72 + i] = r;
+                pool[(size_t)item * 72 + i] = sum / (float)n;
+                pool[(size_t)item * 72 + 36 + i] = mx;
             }
             if (kTiled && ((it & (kStageItems - 1)) == kStageItems - 1 || item + 1 >= n_items)) {
                 // uniform over the block; items start at multiples of kStageItems (kGatherItems is one)
@@ -466,7 +477,7 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void
                     const int q = idx / kStageItems, j = idx % kStageItems;
                     if (j < cnt) {
                         const int k = q / (2 * kStreamChunks), c = (q >> 1) % kStreamChunks, kg = q & 1;
-                        ((bf16x8_t*)feat_out)[tile_chunk(base + (uint32_t)j, k, c, kg)] = stage[idx];
+                        ((bf16x8_t*)feat_out)[tile_chunk(base + (uint32_t)j, k, c, kg)] = stage[j * kStagePitch + q];
                     }
                 }
             }
