@@ -246,6 +246,81 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
     if (threadIdx.x == 0 && red[0]) atomicAdd(&a0.counters[2], (unsigned long long)red[0]);
 }
 
+// Shadow-ray order: the listed vertices in buckets of their expected
+// ratio-tracking steps, the exit distance toward the light times the
+// transmittance majorant (log scale), longest first, so a wave's rays are of
+// similar length and the longest start first (0.281 -> 0.261 ms at 1080p,
+// the sort included).  Counting sort in two passes; the order within a
+// bucket depends on scheduling, no result does.
+constexpr int kShadowBuckets = 64;
+template <bool kViews>
+__device__ __forceinline__ int shadow_bucket(const NNArgs& a0, const float4* __restrict__ pos_act,
+                                             const float4* __restrict__ fy, uint32_t i) {
+    vec3 L = a0.light;
+    float inv_maj = a0.mj.inv_smax_tr;
+    if constexpr (kViews) {
+        const NNArgs& av = a0.views[__float_as_uint(fy[i].w) / a0.view_items];
+        L = av.light;
+        inv_maj = av.mj.inv_smax_tr;
+    }
+    const float4 pa = pos_act[i];
+    const float steps = ray_box_distance(v3(pa.x, pa.y, pa.z), normalize(L)) / inv_maj;
+    const int b = (int)(__log2f(1.0f + steps) * 5.0f);  // 2^(64/5) steps and more share the last bucket
+    return kShadowBuckets - 1 - (b < 0 ? 0 : (b > kShadowBuckets - 1 ? kShadowBuckets - 1 : b));
+}
+
+__global__ void clear_buckets_kernel(uint32_t* __restrict__ c) {
+    if (threadIdx.x < 2 * kShadowBuckets) c[threadIdx.x] = 0u;
+}
+
+template <bool kViews>
+__global__ __launch_bounds__(256) void shadow_count_kernel(NNArgs a0, const uint32_t* __restrict__ list,
+                                                           const uint32_t* __restrict__ n_act_dev,
+                                                           const float4* __restrict__ pos_act,
+                                                           const float4* __restrict__ fy, uint32_t* __restrict__ counts) {
+    __shared__ uint32_t hist[kShadowBuckets];
+    const uint32_t n_act = *n_act_dev;
+    if (blockIdx.x * blockDim.x >= n_act) return;  // uniform over the workgroup
+    if (threadIdx.x < kShadowBuckets) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n_act) atomicAdd(&hist[shadow_bucket<kViews>(a0, pos_act, fy, list[k])], 1u);
+    __syncthreads();
+    if (threadIdx.x < kShadowBuckets && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
+}
+
+template <bool kViews>
+__global__ __launch_bounds__(256) void shadow_scatter_kernel(NNArgs a0, const uint32_t* __restrict__ list,
+                                                             const uint32_t* __restrict__ n_act_dev,
+                                                             const float4* __restrict__ pos_act,
+                                                             const float4* __restrict__ fy,
+                                                             const uint32_t* __restrict__ counts,
+                                                             uint32_t* __restrict__ cursors,
+                                                             uint32_t* __restrict__ order) {
+    __shared__ uint32_t hist[kShadowBuckets];
+    __shared__ uint32_t base[kShadowBuckets];
+    const uint32_t n_act = *n_act_dev;
+    if (blockIdx.x * blockDim.x >= n_act) return;  // uniform over the workgroup
+    if (threadIdx.x < kShadowBuckets) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    int b = 0;
+    uint32_t rank = 0u, i = 0u;
+    if (k < n_act) {
+        i = list[k];
+        b = shadow_bucket<kViews>(a0, pos_act, fy, i);
+        rank = atomicAdd(&hist[b], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < kShadowBuckets) {
+        uint32_t start = 0u;  // the bucket's first slot: the counts of the buckets before it
+        for (int u = 0; u < (int)threadIdx.x; ++u) start += counts[u];
+        base[threadIdx.x] = hist[threadIdx.x] ? start + atomicAdd(&cursors[threadIdx.x], hist[threadIdx.x]) : 0u;
+    }
+    __syncthreads();
+    if (k < n_act) order[base[b] + rank] = i;
+}
+
 // ---------------------------------------------------------------- stage 2
 struct GatherArgs {
     const float4* mips[kNumMips];    // border cells of the density mips
@@ -760,11 +835,27 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
         // No read-back here: the shadow rays, the gather and the network read the
         // count on the device, with grids sized for the whole chunk.
         MRP_HIP(hipEventRecord(ts0, v->stream));
-        if (views)
-            shadow_kernel<true><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, rng_pos, fx, single);
-        else
-            shadow_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, rng_pos, fx,
+        // shadow rays in buckets of their length toward the light (the gather
+        // keeps the compaction's pixel-coherent order); the order lives in the
+        // descriptor region, which the gather fills only afterwards
+        uint32_t* sorder = (uint32_t*)feat;
+        uint32_t* bucket_counts = (uint32_t*)(sp + chunk_items * kNNItemBytes);
+        clear_buckets_kernel<<<1, 2 * kShadowBuckets, 0, v->stream>>>(bucket_counts);
+        if (views) {
+            shadow_count_kernel<true><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, bucket_counts);
+            shadow_scatter_kernel<true><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy,
+                                                                       bucket_counts, bucket_counts + kShadowBuckets,
+                                                                       sorder);
+            shadow_kernel<true><<<blocks, 256, 0, v->stream>>>(a, sorder, n_active_dev, pos_act, fy, rng_pos, fx,
+                                                               single);
+        } else {
+            shadow_count_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, bucket_counts);
+            shadow_scatter_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy,
+                                                                        bucket_counts, bucket_counts + kShadowBuckets,
+                                                                        sorder);
+            shadow_kernel<false><<<blocks, 256, 0, v->stream>>>(a, sorder, n_active_dev, pos_act, fy, rng_pos, fx,
                                                                 single);
+        }
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(ts1, v->stream));
         ga.n_items = cnt;
