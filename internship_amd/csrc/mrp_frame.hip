@@ -84,10 +84,15 @@ constexpr int kTile = 16, kApron = 5, kSpan = kTile + 2 * kApron, kSpanCells = k
 
 // blockIdx.z selects one of several images of the same size stored one after
 // another (batched serving); 0 for a single frame.
+// BinT: the LDS type of the staged bin counts.  They are integer sample
+// counts, so FP16 holds them exactly up to 2048: the host picks it when no
+// count can exceed that (40 KB of bins instead of 81 KB: 3 workgroups per CU
+// instead of 1), and FP32 otherwise.
+template <typename BinT>
 __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const float* __restrict__ target,
                                                            const float* __restrict__ histo,
                                                            uint32_t* __restrict__ out) {
-    __shared__ float s_bins[3 * kHistoBins][kSpanCells];  // bin-major: lanes read consecutive cells
+    __shared__ BinT s_bins[3 * kHistoBins][kSpanCells];  // bin-major: lanes read consecutive cells
     __shared__ float s_total[kSpanCells];
     __shared__ float s_tgt[3][kSpanCells];
     const int W = a.width, H = a.height;
@@ -97,7 +102,19 @@ __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const fl
     out += image;
     const int x0 = blockIdx.x * kTile - kApron, y0 = blockIdx.y * kTile - kApron;
     const int tid = threadIdx.y * kTile + threadIdx.x;
-    const bool filter = a.frame != 0 && a.denoise;
+    // the histograms are staged only if a pixel of this tile is filtered
+    // (|E[l^2]-E[l]^2| > 0.01): a quiet tile reads its 16x16 variances and packs
+    bool filter = a.frame != 0 && a.denoise;
+    if (filter) {
+        const int px = blockIdx.x * kTile + threadIdx.x, py = blockIdx.y * kTile + threadIdx.y;
+        bool noisy = false;
+        if (px < W && py < H) {
+            const float* hp = histo + (size_t)kHistoFloats * ((size_t)py * W + px);
+            const float x = hp[3 * kHistoBins + 2], x2 = hp[3 * kHistoBins + 1];
+            noisy = fabs_(x2 - x * x) > 0.01f;
+        }
+        filter = __syncthreads_or(noisy) != 0;
+    }
     for (int c = tid; c < kSpanCells; c += kTile * kTile) {
         const int cy = c / kSpan, cx = c - cy * kSpan;
         const int gy = clampi_(y0 + cy, 0, H - 1), gx = clampi_(x0 + cx, 0, W - 1);
@@ -107,7 +124,7 @@ __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const fl
         s_tgt[2][c] = target[3 * gp + 2];
         if (filter) {
             const float* hp = histo + (size_t)kHistoFloats * gp;
-            for (int b = 0; b < 3 * kHistoBins; ++b) s_bins[b][c] = hp[b];
+            for (int b = 0; b < 3 * kHistoBins; ++b) s_bins[b][c] = (BinT)hp[b];
             s_total[c] = hp[3 * kHistoBins];
         }
     }
@@ -126,13 +143,13 @@ __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const fl
             const vec3 s3 = v3(s_tgt[0][cc + 1], s_tgt[1][cc + 1], s_tgt[2][cc + 1]);
             res = (((s0 + s1) + s2) + s3) / 4.0f;
         }
-    } else if (a.denoise) {
+    } else if (filter) {
         const float* hp = histo + (size_t)kHistoFloats * pix;
         const float x = hp[3 * kHistoBins + 2], x2 = hp[3 * kHistoBins + 1];
         const float variance = fabs_(x2 - x * x);
         if (variance > 0.01f) {
             float cb[3 * kHistoBins];
-            for (int b = 0; b < 3 * kHistoBins; ++b) cb[b] = s_bins[b][cc];
+            for (int b = 0; b < 3 * kHistoBins; ++b) cb[b] = (float)s_bins[b][cc];
             const float nc = s_total[cc];
             vec3 acc = v3(0, 0, 0);
             float ws = 0.0f;
@@ -142,7 +159,7 @@ __global__ __launch_bounds__(256) void denoise_pack_kernel(FrameArgs a, const fl
                     const float syx = sqrtf(s_total[nb] / nc), sxy = sqrtf(nc / s_total[nb]);
                     float p = 0.0f, d = 0.0f;
                     for (int b = 0; b < 3 * kHistoBins; ++b) {
-                        const float hx = cb[b], hy = s_bins[b][nb];
+                        const float hx = cb[b], hy = (float)s_bins[b][nb];
                         if (hx != 0.0f || hy != 0.0f) {
                             p += 1.0f;
                             const float t = syx * hx - sxy * hy;
@@ -181,7 +198,7 @@ __global__ void clear_histo_kernel(float* __restrict__ histo, size_t n_floats) {
 // `Denoise<denoise>` past frame 0 on the running means `target` and the
 // histograms `histo`, then exposure, tone curve and BGRA8.
 int launch_denoise_images(hipStream_t stream, int width, int height, int n_images, int denoise, int tone,
-                          float exposure, const float* target, const float* histo, uint32_t* bgra) {
+                          float exposure, const float* target, const float* histo, uint32_t* bgra, int max_count) {
     FrameArgs a{};
     a.width = width;
     a.height = height;
@@ -194,7 +211,10 @@ int launch_denoise_images(hipStream_t stream, int width, int height, int n_image
     a.denoise = denoise;
     const dim3 grid((unsigned)((width + kTile - 1) / kTile), (unsigned)((height + kTile - 1) / kTile),
                     (unsigned)n_images);
-    denoise_pack_kernel<<<grid, dim3(kTile, kTile), 0, stream>>>(a, target, histo, bgra);
+    if (max_count <= 2048)
+        denoise_pack_kernel<_Float16><<<grid, dim3(kTile, kTile), 0, stream>>>(a, target, histo, bgra);
+    else
+        denoise_pack_kernel<float><<<grid, dim3(kTile, kTile), 0, stream>>>(a, target, histo, bgra);
     MRP_HIP(hipGetLastError());
     return MRP_OK;
 }
@@ -265,7 +285,14 @@ static int render_frame(Volume* v, const mrp_frame_params* fp, float* target, fl
         MRP_HIP(hipGetLastError());
         if (!predict) {
             const dim3 grid((W + kTile - 1) / kTile, (H + kTile - 1) / kTile);
-            denoise_pack_kernel<<<grid, dim3(kTile, kTile), 0, v->stream>>>(a, target, histo, bgra);
+            // a pixel's counts are at most its frames since its histogram was
+            // cleared (frame index 0), and at most the volume's calls since its
+            // last reset (a caller that never passes frame 0)
+            const uint32_t fnum = (uint32_t)frame_index(a.frame, a.checkerboard);
+            if (fnum < 2048u && v->frame_calls < 2048u)
+                denoise_pack_kernel<_Float16><<<grid, dim3(kTile, kTile), 0, v->stream>>>(a, target, histo, bgra);
+            else
+                denoise_pack_kernel<float><<<grid, dim3(kTile, kTile), 0, v->stream>>>(a, target, histo, bgra);
         } else {
             reproject_pack_kernel<<<(unsigned)((P + 255) / 256), 256, 0, v->stream>>>(a, target, histo, bgra);
         }

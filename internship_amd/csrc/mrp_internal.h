@@ -134,9 +134,10 @@ int mrpnn_views(Volume* const* vols, const mrp_render_params* ps, int n_views, i
                 float4* out, NNTimes* times, uint64_t* n_active);
 size_t mrpnn_view_table_bytes(int n_views);
 // Denoise (past frame 0) and BGRA8 pack of n_images images of one size stored
-// one after another: target 3 floats, histo kHistoFloats floats, bgra one word per pixel.
+// one after another: target 3 floats, histo kHistoFloats floats, bgra one word per pixel;
+// max_count bounds every histogram's sample count.
 int launch_denoise_images(hipStream_t stream, int width, int height, int n_images, int denoise, int tone,
-                          float exposure, const float* target, const float* histo, uint32_t* bgra);
+                          float exposure, const float* target, const float* histo, uint32_t* bgra, int max_count);
 // `NNPredict<MRPNN>` on n given rays (one sample each): out_dev n x float4
 // (radiance, first-vertex distance or -1).
 int mrpnn_rays(Volume* v, int n, const float* ori, const float* dir, const mrp_render_params* p, float* out,

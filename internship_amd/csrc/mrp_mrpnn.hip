@@ -367,7 +367,7 @@ typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 // (two); the FP32 cells are the default of every renderer, the FP16 ones an
 // opt-in variant of the tiled (BF16/FP8) gather.
 template <bool kTiled, bool kCells16, bool kViews>
-__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(8))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
+__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(kViews ? 7 : 8))) void gather_kernel(GatherArgs a, const float4* __restrict__ pos_act,
                                                        const float4* __restrict__ fx, const float4* __restrict__ fy,
                                                        const float4* __restrict__ fz, const uint32_t* __restrict__ list,
                                                        void* __restrict__ feat_out, float* __restrict__ gamma,

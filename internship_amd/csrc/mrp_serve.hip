@@ -319,7 +319,7 @@ extern "C" int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_para
     MRP_HIP(hipEventRecord(ev[3], host->stream));
     if (bgra_dev) {
         rc = launch_denoise_images(host->stream, W, H, n_views, sp->denoise ? 1 : 0, sp->tone, sp->exposure, rgb_dev,
-                                   histo, bgra_dev);
+                                   histo, bgra_dev, std::max(sp->mrpnn_spp, sp->fallback_spp));
         if (rc) return rc;
     }
     MRP_HIP(hipEventRecord(ev[4], host->stream));
