@@ -11,11 +11,12 @@
 //      (core/volume.cu:229-231) - exceeds the caller's threshold;
 //   3. fallback_kernel re-estimates the listed pixels of all views with
 //      delta tracking (`CalculateRadiance`, core/volume.cu:17-34): one
-//      persistent launch (per 16M-sample pass) whose waves claim 64 (pixel,
-//      sample) paths at a time from one queue, so scenes of different
-//      density and size balance across the chip; the walks keep several
-//      lookups in flight per lane (mrp_track.h); fallback_finish_kernel then
-//      folds each pixel's samples, in order, into its mean and histogram;
+//      persistent launch (per 16M-sample pass) whose lanes take (pixel,
+//      sample) paths from one queue as their previous path ends, so scenes
+//      of different density and size balance across the chip; the walks keep
+//      several lookups in flight per lane (mrp_track.h);
+//      fallback_finish_kernel then folds each pixel's samples, in order,
+//      into its mean and histogram;
 //   4. the frame path's denoise and BGRA8 pack over every view (one launch,
 //      blockIdx.z = view).
 // The reference renders one scene per call and has no such mode; the parts
@@ -34,7 +35,7 @@ namespace mrp {
 
 constexpr int kFallbackBatch = 4;      // lookups in flight per lane
 constexpr int kFallbackSteps = 10000;  // loop tests per walk (the reference's cap)
-constexpr int kFallbackBlocksPerCU = 4;
+constexpr int kFallbackBlocksPerCU = 3;  // 3 waves per SIMD at <= 168 VGPRs
 constexpr size_t kFallbackPassItems = (size_t)1 << 24;  // 256 MiB of per-sample results per pass
 
 // What a fallback path needs of its view.
@@ -102,56 +103,105 @@ __global__ __launch_bounds__(kAccBlock) void serve_accumulate_kernel(ServeArgs a
     for (size_t i = threadIdx.x; i < n_here * kHistoFloats; i += kAccBlock) out[i] = s_h[i];
 }
 
-// Persistent: each wave claims 64 (fallback pixel, sample) items at a time
-// until the pass's items are exhausted (every wave reaches the exit test).
-// Item it of the pass is sample k = it % fallback_spp of listed pixel
-// j = j0 + it / fallback_spp: sample spp + k on Philox stream (pixel, spp + k,
-// 0, seed), a `CalculateRadiance` path, stored at out[it].  Paths of one
-// pixel run in different lanes, so the long and short ones of a wave mix
-// across pixels.  Lookups go to counters[3].
-__global__ __launch_bounds__(256) void fallback_kernel(ServeArgs a, const FallbackView* __restrict__ views,
-                                                       const uint32_t* __restrict__ list, uint32_t j0,
-                                                       uint32_t n_items, uint32_t* __restrict__ cursor,
-                                                       float4* __restrict__ out,
-                                                       unsigned long long* __restrict__ counters) {
+// The fallback paths.  Item it of the pass is sample k = it % fallback_spp of
+// listed pixel j = j0 + it / fallback_spp: sample spp + k on Philox stream
+// (pixel, spp + k, 0, seed), a `CalculateRadiance` path, stored at out[it];
+// lookups go to counters[3].  Persistent, with lanes that refill: a lane runs
+// one bounce of its path per iteration (free flight, then the shadow ray),
+// and when at least kFallbackRefill lanes of a wave have finished their
+// paths the wave claims that many items with one atomic and starts them, so
+// paths of up to 512 bounces and paths that escape at once share waves
+// without idling them (25.6 -> 17.0 ms against one path per lane).  Each
+// path is `radiance_paths` with spp = 1 step by step (same draws, same
+// arithmetic, same lookups); only which lane runs it, and when, differ.
+// Every wave leaves once the pass's items are claimed and its lanes are idle.
+constexpr int kFallbackRefill = 16;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void fallback_kernel(ServeArgs a, const FallbackView* __restrict__ views,
+                                                              const uint32_t* __restrict__ list, uint32_t j0,
+                                                              uint32_t n_items, uint32_t* __restrict__ cursor,
+                                                              float4* __restrict__ out,
+                                                              unsigned long long* __restrict__ counters) {
     __shared__ unsigned red[256];
+    const uint32_t lane = __lane_id();
     uint32_t lk = 0u;
+    bool active = false, queue_done = false;
+    uint32_t it = 0u, view = 0u;
+    int bounce = 0;
+    Rng rng = rng_make(0u, 0u, 0u, 0ull);
+    vec3 ori = v3(0, 0, 0), pos = v3(0, 0, 0), dir = v3(0, 0, 1), csr = v3(1, 1, 1), res = v3(0, 0, 0);
+    float t_first = -1.0f;
     for (;;) {
-        uint32_t base = 0u;
-        if (__lane_id() == 0u) base = atomicAdd(cursor, 64u);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if (base >= n_items) break;
-        const uint32_t it = base + __lane_id();
-        if (it < n_items) {
-            const uint32_t jj = it / (uint32_t)a.fallback_spp, k = it - jj * (uint32_t)a.fallback_spp;
-            const uint32_t gi = list[j0 + jj];
-            const uint32_t view = gi / a.P, pix = gi - view * a.P;
+        const unsigned long long idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!queue_done && n_idle >= (uint32_t)kFallbackRefill) {  // uniform over the wave
+            uint32_t base = 0u;
+            if (lane == 0u) base = atomicAdd(cursor, n_idle);
+            base = __shfl(base, 0);
+            if (base >= n_items) queue_done = true;
+            if (!active) {
+                const uint32_t item = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (item < n_items) {
+                    it = item;
+                    const uint32_t jj = it / (uint32_t)a.fallback_spp, k = it - jj * (uint32_t)a.fallback_spp;
+                    const uint32_t gi = list[j0 + jj];
+                    view = gi / a.P;
+                    const uint32_t pix = gi - view * a.P;
+                    const FallbackView& v = views[view];
+                    const int row = (int)pix / v.width, col = (int)pix - row * v.width;
+                    rng = rng_make(pix, (uint32_t)a.spp + k, 0u, ((uint64_t)v.seed_hi << 32) | v.seed_lo);
+                    const float xu = rng_next(&rng);
+                    const float xv = rng_next(&rng);
+                    // radiance_paths: normalised direction, box entry, or the sky
+                    ori = v.ori;
+                    dir = normalize(camera_dir(v.forward, v.right, v.up, row, col, v.width, v.height, xu, xv));
+                    const float dis = ray_box_offset(ori, dir);
+                    t_first = -1.0f;
+                    if (dis < 0.0f) {
+                        const vec3 sky = env_radiance(v.env, dir);
+                        out[it] = make_float4(sky.x, sky.y, sky.z, -1.0f);
+                    } else {
+                        pos = ori + dir * dis;
+                        csr = v3(1.0f, 1.0f, 1.0f);
+                        res = v3(0.0f, 0.0f, 0.0f);
+                        bounce = 0;
+                        active = v.multi_scatter > 0;
+                        if (!active) out[it] = make_float4(0.0f, 0.0f, 0.0f, -1.0f);
+                    }
+                }
+            }
+        }
+        if (queue_done && !__any(active)) break;  // uniform: every wave reaches it once its lanes are idle
+        if (active) {
             const FallbackView& v = views[view];
-            const int row = (int)pix / v.width, col = (int)pix - row * v.width;
             const CellsFetch F{v.cells, v.res};
-            const uint64_t seed = ((uint64_t)v.seed_hi << 32) | v.seed_lo;
-            const auto next_vertex = [&](Rng* rg, vec3 p, vec3 d, float dis, vec3* np, vec3* nd) {
-                if (!free_flight_batched<kFallbackBatch>(rg, F, v.mj.inv_smax, v.mj.ratio, p, d, dis, kFallbackSteps, np,
-                                                         &lk))
-                    return false;
-                rng_seek(rg, rg->n - 2u);  // the two direction draws the free flight consumed
-                const float e0 = rng_next(rg);
-                const float e1 = rng_next(rg);
-                *nd = hg_sample_dir(e0, e1, d, v.g);
-                return true;
-            };
-            const auto transmit = [&](Rng* rg, vec3 p, vec3 d, float dis) {
-                return transmittance_batched<kFallbackBatch>(rg, F, v.mj.inv_smax_tr, v.mj.ratio, p, d, dis,
-                                                             kFallbackSteps, &lk);
-            };
-            Rng rng = rng_make(pix, (uint32_t)a.spp + k, 0u, seed);
-            const float xu = rng_next(&rng);
-            const float xv = rng_next(&rng);
-            const vec3 dir = camera_dir(v.forward, v.right, v.up, row, col, v.width, v.height, xu, xv);
-            float t;
-            const vec3 r = radiance_paths(&rng, next_vertex, transmit, v.ori, dir, v.light, v.light_color, v.env,
-                                          v.scatter_rate, v.g, v.multi_scatter, 1, &t);
-            out[it] = make_float4(r.x, r.y, r.z, t);
+            const vec3 light = normalize(v.light);
+            vec3 np;
+            bool done = false;
+            if (!free_flight_batched<kFallbackBatch>(&rng, F, v.mj.inv_smax, v.mj.ratio, pos, dir,
+                                                     ray_box_distance(pos, dir), kFallbackSteps, &np, &lk)) {
+                res = res + env_radiance(v.env, dir) * csr;
+                done = true;
+            } else {
+                rng_seek(&rng, rng.n - 2u);  // the two direction draws the free flight consumed
+                const float e0 = rng_next(&rng);
+                const float e1 = rng_next(&rng);
+                const vec3 nd = hg_sample_dir(e0, e1, dir, v.g);
+                if (bounce == 0) t_first = distance(ori, np);
+                pos = np;
+                csr = csr * v.scatter_rate;
+                const float phase = hg_phase(dot(dir, light), v.g);
+                const float tr = transmittance_batched<kFallbackBatch>(&rng, F, v.mj.inv_smax_tr, v.mj.ratio, pos,
+                                                                       light, ray_box_distance(pos, light),
+                                                                       kFallbackSteps, &lk);
+                res = res + v.light_color * csr * (tr * phase);
+                dir = nd;
+                done = ++bounce >= v.multi_scatter;
+            }
+            if (done) {
+                const vec3 r = res / 1.0f;  // radiance_paths' mean over its one sample
+                out[it] = make_float4(r.x, r.y, r.z, t_first);
+                active = false;
+            }
         }
     }
     red[threadIdx.x] = lk;
@@ -311,7 +361,7 @@ extern "C" int mrp_serve_views(mrp_volume* const* volumes, const mrp_render_para
         clear_u32_pair_kernel<<<1, 64, 0, host->stream>>>(qc);  // the cursor (the count was read)
         MRP_HIP(hipGetLastError());
         fallback_kernel<<<fb_blocks, 256, 0, host->stream>>>(a, fb_dev, list, j0, items, qc + 1, fsamples,
-                                                             host->counters);
+                                                                    host->counters);
         MRP_HIP(hipGetLastError());
         fallback_finish_kernel<<<(np + 255) / 256, 256, 0, host->stream>>>(a, list, j0, np, fsamples, rgb_dev, histo);
         MRP_HIP(hipGetLastError());
