@@ -18,6 +18,7 @@
 #include <algorithm>
 
 #include "mrp_internal.h"
+#include "mrp_track.h"
 
 namespace mrp {
 
@@ -88,38 +89,97 @@ __device__ float gs_transmittance(Rng* rng, const float4* __restrict__ cells, in
     return tr;
 }
 
-__global__ __launch_bounds__(256) void get_sample_kernel(SampleArgs a) {
+// `GetSample` paths, item i = (sample k = i / n, query j = i % n) on Philox
+// stream (j, s_begin + k, tag 1, seed); paths[i] = the path's radiance.
+// Persistent, with lanes that refill: a lane runs one bounce of its path per
+// iteration (free flight with kSampleBatch lookups in flight, then the shadow
+// ray), and when at least kSampleRefill lanes of a wave are idle the wave
+// claims that many items with one atomic.  The draws, arithmetic and lookups
+// of each path are those of the one-lookup-at-a-time loop (mrp_track.h);
+// only which lane runs it, and when, differ.  Every wave leaves once the
+// items are claimed and its lanes are idle.
+constexpr int kTrackSteps = 10000;  // loop tests per walk (the reference's cap)
+constexpr int kSampleRefill = 16;
+constexpr int kSampleBatch = 4;
+constexpr int kSampleBlocksPerCU = 3;  // 3 waves per SIMD at <= 168 VGPRs
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void get_sample_kernel(SampleArgs a,
+                                                                                                uint32_t* cursor) {
     __shared__ unsigned red[256];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint32_t n_items = (uint32_t)a.n * (uint32_t)a.s_count;
     uint32_t lk = 0u;
-    if (i < (uint32_t)a.n * (uint32_t)a.s_count) {
-        const uint32_t k = i / (uint32_t)a.n, j = i - k * (uint32_t)a.n;
-        const mrp_sample_query q = a.q[j];
-        const Majorant mj = make_majorant(a.max_density, q.alpha, a.tr_scale);
-        const vec3 light = v3(q.light_dir.x, q.light_dir.y, q.light_dir.z);
-        Rng rng = rng_make(j, (uint32_t)(a.s_begin + (int)k), 1u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
-        vec3 pos = v3(q.pos.x, q.pos.y, q.pos.z);
-        const float e0 = rng_next(&rng);
-        const float e1 = rng_next(&rng);
-        vec3 ray = hg_sample_dir(e0, e1, v3(q.dir.x, q.dir.y, q.dir.z), q.g);
-        vec3 csr = v3(1, 1, 1), res = v3(0, 0, 0);
-        for (int s = 0; s < a.multi_scatter; ++s) {
-            vec3 np, nd;
-            if (!gs_next_vertex(&rng, a.cells, a.res, mj, q.g, pos, ray, ray_box_distance(pos, ray), &np, &nd, &lk)) {
-                res = res + env_radiance(a.env, ray) * csr;
-                break;
+    bool active = false, queue_done = false;
+    uint32_t i = 0u;
+    int bounce = 0;
+    Rng rng = rng_make(0u, 0u, 0u, 0ull);
+    vec3 pos = v3(0, 0, 0), ray = v3(0, 0, 1), csr = v3(1, 1, 1), res = v3(0, 0, 0), light = v3(0, 1, 0);
+    float g = 0.0f, scatter = 1.0f;
+    Majorant mj = make_majorant(1.0f, 1.0f, 1.0f);
+    const CellsFetch F{a.cells, a.res};
+    for (;;) {
+        const unsigned long long idle = __ballot(!active);
+        const uint32_t n_idle = (uint32_t)__popcll(idle);
+        if (!queue_done && n_idle >= (uint32_t)kSampleRefill) {  // uniform over the wave
+            uint32_t base = 0u;
+            if (lane == 0u) base = atomicAdd(cursor, n_idle);
+            base = __shfl(base, 0);
+            if (base >= n_items) queue_done = true;
+            if (!active) {
+                const uint32_t item = base + (uint32_t)__popcll(idle & ((1ull << lane) - 1ull));
+                if (item < n_items) {
+                    i = item;
+                    const uint32_t k = i / (uint32_t)a.n, j = i - k * (uint32_t)a.n;
+                    const mrp_sample_query q = a.q[j];
+                    mj = make_majorant(a.max_density, q.alpha, a.tr_scale);
+                    light = v3(q.light_dir.x, q.light_dir.y, q.light_dir.z);
+                    g = q.g;
+                    scatter = q.scatter;
+                    rng = rng_make(j, (uint32_t)(a.s_begin + (int)k), 1u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+                    pos = v3(q.pos.x, q.pos.y, q.pos.z);
+                    const float e0 = rng_next(&rng);
+                    const float e1 = rng_next(&rng);
+                    ray = hg_sample_dir(e0, e1, v3(q.dir.x, q.dir.y, q.dir.z), q.g);
+                    csr = v3(1, 1, 1);
+                    res = v3(0, 0, 0);
+                    bounce = 0;
+                    active = a.multi_scatter > 0;
+                    if (!active) {
+                        float* o = a.paths + 3 * (size_t)i;
+                        o[0] = o[1] = o[2] = 0.0f;
+                    }
+                }
             }
-            pos = np;
-            csr = (csr * 1.001f) * q.scatter;
-            const float phase = hg_phase(dot(ray, light), q.g);
-            const float tr = gs_transmittance(&rng, a.cells, a.res, mj, pos, light, ray_box_distance(pos, light), &lk);
-            res = res + a.light_color * csr * (tr * phase);
-            ray = nd;
         }
-        float* o = a.paths + 3 * (size_t)i;
-        o[0] = res.x;
-        o[1] = res.y;
-        o[2] = res.z;
+        if (queue_done && !__any(active)) break;  // uniform: every wave reaches it once its lanes are idle
+        if (active) {
+            vec3 np;
+            bool done = false;
+            if (!free_flight_batched<kSampleBatch>(&rng, F, mj.inv_smax, mj.ratio, pos, ray, ray_box_distance(pos, ray),
+                                                   kTrackSteps, &np, &lk)) {
+                res = res + env_radiance(a.env, ray) * csr;
+                done = true;
+            } else {
+                rng_seek(&rng, rng.n - 2u);  // the two direction draws the free flight consumed
+                const float e0 = rng_next(&rng);
+                const float e1 = rng_next(&rng);
+                const vec3 nd = hg_sample_dir(e0, e1, ray, g);
+                pos = np;
+                csr = (csr * 1.001f) * scatter;
+                const float phase = hg_phase(dot(ray, light), g);
+                const float tr = transmittance_batched<kSampleBatch>(&rng, F, mj.inv_smax_tr, mj.ratio, pos, light,
+                                                                     ray_box_distance(pos, light), kTrackSteps, &lk);
+                res = res + a.light_color * csr * (tr * phase);
+                ray = nd;
+                done = ++bounce >= a.multi_scatter;
+            }
+            if (done) {
+                float* o = a.paths + 3 * (size_t)i;
+                o[0] = res.x;
+                o[1] = res.y;
+                o[2] = res.z;
+                active = false;
+            }
+        }
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -128,6 +188,10 @@ __global__ __launch_bounds__(256) void get_sample_kernel(SampleArgs a) {
         __syncthreads();
     }
     if (threadIdx.x == 0 && red[0]) atomicAdd(&a.counters[0], (unsigned long long)red[0]);
+}
+
+__global__ void clear_cursor_kernel(uint32_t* c) {
+    if (threadIdx.x == 0) *c = 0u;
 }
 
 // sums[j] += paths[k][j] for k = 0..count-1 in order; on the last chunk
@@ -309,7 +373,10 @@ extern "C" int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q
             a.s_begin = s0;
             a.s_count = cnt;
             const size_t items = (size_t)cnt * n;
-            get_sample_kernel<<<(unsigned)((items + 255) / 256), 256, 0, v->stream>>>(a);
+            clear_cursor_kernel<<<1, 64, 0, v->stream>>>(v->queues);  // the tracer's queue words are free here
+            const unsigned blocks = (unsigned)std::min<size_t>((items + 255) / 256,
+                                                               (size_t)std::max(1, v->num_cus * kSampleBlocksPerCU));
+            get_sample_kernel<<<blocks, 256, 0, v->stream>>>(a, v->queues);
             MRP_HIP(hipGetLastError());
             sample_sum_kernel<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(paths, n, cnt, sums, spp,
                                                                                  s0 + cnt >= spp ? 1 : 0, out_dev);
