@@ -44,6 +44,8 @@ __constant__ float c_tap_u_view[kShellTaps];
 
 struct NNArgs {
     const float4* cells;
+    const float4* quads;  // the same grid in the quad layout (QuadsFetch), read by the first free flight
+    QuadGeom qg;
     int res;
     float smax, alpha, g, tr_scale;
     Majorant mj;
@@ -131,7 +133,7 @@ __device__ __forceinline__ void first_vertex_item(const NNArgs& a, uint32_t i, u
         if (off >= 0.0f) {
             const vec3 sp = ori + gdir * off;
             vec3 np;
-            const CellsFetch F{a.cells, a.res};
+            const QuadsFetch F{a.quads, a.res, a.qg};
             if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
                                               ray_box_distance(sp, gdir), kMaxTrackSteps, &np, lk)) {
                 pos = np;
@@ -739,6 +741,8 @@ static void image_args(const Volume* v, const mrp_render_params* p, PixelMap map
     const int W = p->width, H = p->height;
     *a = NNArgs{};
     a->cells = v->cells;
+    a->quads = v->quads;
+    a->qg = quad_geom(v->res);
     a->res = v->res;
     a->alpha = p->alpha;
     a->smax = v->max_density * p->alpha;

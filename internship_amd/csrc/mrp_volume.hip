@@ -68,6 +68,27 @@ __global__ void build_cells_kernel(const float* __restrict__ grid, int res, floa
     }
 }
 
+// quads = the padded grid in the quad layout (quad_x/y/z); slots past the
+// last quad in the last bricks are 0.
+__global__ void build_quads_kernel(const float* __restrict__ grid, int res, QuadGeom g, size_t n,
+                                   float4* __restrict__ quads) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const uint32_t nbz = g.sy / kQuadBrick, nby = g.sx / g.sy;
+    const size_t P = (size_t)res + 2;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint32_t w = (uint32_t)(i % kQuadBrick);
+        const size_t b = i / kQuadBrick;
+        const uint32_t bz = (uint32_t)(b % nbz), by = (uint32_t)((b / nbz) % nby), ux = (uint32_t)(b / ((size_t)nbz * nby));
+        const uint32_t uy = 4u * by + (w >> 1), uz = 2u * bz + (w & 1u);
+        float4 q = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        if (ux <= (uint32_t)res && uy <= (uint32_t)res + 1u && uz <= (uint32_t)res) {
+            const float* c = grid + padded_index(res, (int)ux - 1, (int)uy - 1, (int)uz - 1);
+            q = make_float4(c[0], c[1], c[P * P], c[P * P + 1]);
+        }
+        quads[i] = q;
+    }
+}
+
 // Clamp cells of an unpadded res^3 grid (trilinear_clamp's corners) for cell
 // coordinates in [-1, res].
 __global__ void build_clamp_cells_kernel(const float* __restrict__ src, int res, float4* __restrict__ cells) {
@@ -239,6 +260,8 @@ int launch_update(Volume* v) {
     hipFree(dmax);
     v->max_density = u2f(bits);
     build_cells_kernel<<<blocks_for(cell_count(res)), 256, 0, v->stream>>>(v->grid, res, v->cells);
+    build_quads_kernel<<<blocks_for(quad_count(res)), 256, 0, v->stream>>>(v->grid, res, quad_geom(res),
+                                                                            quad_count(res), v->quads);
 
     // source chain: halve (by trilinear resampling) while above 2x256
     int r = 256;
