@@ -383,13 +383,15 @@ def mlp_rooflines(k, precision=1, ceiling=None, cells16=False):
     # Gather: most of the cell bytes its lookups read are served by L2 and the
     # last-level cache, so its HBM fraction is the measured HBM traffic over its
     # time; the algorithmic rate (cell bytes the lookups read) is given beside it.
+    # (the MFMA networks read the wave-tiled gather; the FP32 network the row gather)
+    gname = "gather_kernel" if precision == 0 else "gather_tiled_kernel"
     per_desc = GATHER_BYTES_PER_POINT_16 if cells16 else GATHER_BYTES_PER_POINT
     ga_alg = per_desc * nd / (k["gather_ms"] * 1e-3) / 1e9
-    traffic = hbm_traffic("gather_kernel", MLP_WORKLOAD + ("_cells16" if cells16 else ""))
+    traffic = hbm_traffic(gname, MLP_WORKLOAD + ("_cells16" if cells16 else ""))
     ga_hbm = traffic / (k["gather_ms"] * 1e-3) / 1e9 if traffic else None
-    r["gather_kernel"] = {"bound": "hbm", "achieved": ga_hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    r[gname] = {"bound": "hbm", "achieved": ga_hbm, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": ga_hbm / HBM_PEAK_GBS if ga_hbm else None, "traffic": traffic,
-                          "kernel": "gather_kernel", "kernel_ms": k["gather_ms"],
+                          "kernel": gname, "kernel_ms": k["gather_ms"],
                           "cells": "fp16" if cells16 else "fp32",
                           "achieved_is": "measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, "
                                          "scaled by a copy kernel) over the live kernel time",
