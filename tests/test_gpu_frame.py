@@ -92,6 +92,14 @@ def test_gpu_pt_frames_full_odd_size_match_oracle():
     _run_pt(sphere_grid(32), 21, 13, 3, False)
 
 
+def test_gpu_pt_frames_denoise_across_tiles_match_oracle():
+    # 45 x 38 pixels: the denoiser's 16 x 16 LDS tiles form a 3 x 3 grid with
+    # partial tiles on the right and bottom, so horizontal and vertical tile
+    # seams (and the 5-pixel halo across them) are compared with the oracle
+    his = _run_pt(sphere_grid(32), 45, 38, 4, True)
+    assert np.all(his[..., 30] >= 1)
+
+
 def test_gpu_mrpnn_frames_reprojection_match_oracle():
     grid = oracle.generate_cloud(32, seed=3)
     w, h = 17, 12  # odd width: the checkerboard's last logical column is partly outside

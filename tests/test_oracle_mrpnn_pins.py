@@ -266,6 +266,49 @@ def test_gather_hg_table_coordinates(gather_inputs):
     np.testing.assert_allclose(feat_u[0, 384 + np.array(idx)], expect, atol=1e-3)
 
 
+def _stencil_table():
+    """The 160 unit-frame stencil points, read from the generated header
+    (parsed from core/vector.cu:77-172 by gen_tables.py)."""
+    import os
+    import re
+    path = os.path.join(os.path.dirname(ia.__file__), "csrc", "mrp_tables_gen.h")
+    text = open(path).read()
+    body = text[text.index("#define MRP_STENCIL_INIT"):]
+    body = body[:body.index("\n\n")]
+    nums = [float(v) for v in re.findall(r"(-?\d+\.\d+(?:e-?\d+)?)f", body)]
+    pts = np.array(nums, np.float64).reshape(-1, 3)
+    assert pts.shape == (160, 3)
+    return pts
+
+
+def test_gather_mip7_shell_taps_against_border_trilinear(gather_inputs):
+    """Taps 128..159 (the SpVn7 shell, mip 7 = 2^3 cells) reach the border of
+    their level, where the linear-density read-back above does not hold.
+    Instead each tap's density feature is compared with an independent FP64
+    border-0 trilinear fetch of the 2^3 level at the position the schedule
+    puts the tap: vertex + (X s0 + Y s1 + Z s2) * offset."""
+    mcat, tcat = gather_inputs
+    lut = np.full((512, 512), 0.25, np.float32)
+    feat, _ = _features(mcat, tcat, lut, FRAMES)
+    st = _stencil_table()[128:160]
+    off = np.array([o for _, o in _schedule()[128:160]])
+    assert all(m == 7 for m, _ in _schedule()[128:160])
+    xc = (np.arange(2) + 0.5) / 2 - 0.5
+    g7 = np.broadcast_to((0.5 + SLOPE[7] * xc)[:, None, None], (2, 2, 2)).astype(np.float32).astype(np.float64)
+    for k, F in enumerate(FRAMES):
+        pos = (st @ F) * off[:, None]  # rows of F are X, Y, Z
+        d = _trilinear_border0(g7, pos)
+        expect = np.log1p(ALPHA * d / 64.0)
+        np.testing.assert_allclose(feat[k, 128:160], expect, rtol=2e-5, atol=2e-6)
+        # the border is reached: not a plain linear read-out
+        assert np.any(np.abs(pos) > 0.25)
+    # the same taps read from level 6 instead would differ by far more than the tolerance
+    xc6 = (np.arange(4) + 0.5) / 4 - 0.5
+    g6 = np.broadcast_to((0.5 + SLOPE[6] * xc6)[:, None, None], (4, 4, 4)).astype(np.float64)
+    wrong = np.log1p(ALPHA * _trilinear_border0(g6, (st @ FRAMES[0]) * off[:, None]) / 64.0)
+    assert np.abs(wrong - feat[0, 128:160]).max() > 1e-2
+
+
 def test_gather_inactive_vertex_is_all_zero(gather_inputs):
     mcat, tcat = gather_inputs
     feat, _ = _features(mcat, tcat, np.full((512, 512), 0.25, np.float32), FRAMES, active=[0, 1, 0])
