@@ -146,6 +146,15 @@ int mrp_volume_update(mrp_volume* vol);
 int mrp_volume_max_density(const mrp_volume* vol, float* out);
 /* Copy density mip `mip` ((256>>mip)^3 FP32, z fastest) to host memory. */
 int mrp_volume_get_mip(const mrp_volume* vol, int mip, float* host_out);
+/* No counterpart in the reference (whose tracer reads every density): the
+ * brick occupancy Update builds for the trackers.  Cells c in [-1, res-1]^3
+ * (corners c..c+1 of the zero-bordered grid) are grouped in bricks of
+ * 2^shift cells per axis, u = c + 1, brick (ux>>shift, uy>>shift, uz>>shift),
+ * n = (res >> shift) + 1 per axis, z fastest; bit b of word b/32 is set when
+ * some corner of the brick's cells is not zero.  A lookup in a clear brick is
+ * an exact zero and reads no grid memory.  Copies ceil(n^3/32) words to
+ * host_out (NULL: sizes only). */
+int mrp_volume_get_occupancy(const mrp_volume* vol, uint32_t* host_out, size_t max_words, int* shift, int* n);
 /* `VolumeRender::Update_TR` (core/volume.cu:1320-1424), GPU path: 8
  * transmittance mips (128>>m)^3 marched 128 steps toward the light through
  * density mip m+1 with extinction alpha/1.73^(m+1). */

@@ -332,6 +332,17 @@ class VolumeRender:
         check(self.lib.mrp_volume_get_mip(self.h, m, _ptr(out)))
         return out
 
+    def get_occupancy(self):
+        """The trackers' brick occupancy: (bool array n x n x n over bricks of
+        2^shift cells per axis, shift), see mrp_volume_get_occupancy."""
+        sh, n = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.mrp_volume_get_occupancy(self.h, None, 0, ctypes.byref(sh), ctypes.byref(n)))
+        nb = n.value ** 3
+        words = np.zeros((nb + 31) // 32, np.uint32)
+        check(self.lib.mrp_volume_get_occupancy(self.h, _ptr(words), words.size, ctypes.byref(sh), ctypes.byref(n)))
+        bits = (words[:, None] >> np.arange(32, dtype=np.uint32)[None, :]) & 1
+        return bits.ravel()[:nb].astype(bool).reshape(n.value, n.value, n.value), sh.value
+
     def update_tr(self, light_dir, alpha=64.0):
         check(self.lib.mrp_volume_update_tr(self.h, Vec3.of(light_dir), float(alpha)))
 

@@ -109,6 +109,7 @@ SIGNATURES = {
     "mrp_volume_update": (I, [P]),
     "mrp_volume_max_density": (I, [P, FP]),
     "mrp_volume_get_mip": (I, [P, I, P]),
+    "mrp_volume_get_occupancy": (I, [P, P, SZ, ctypes.POINTER(I), ctypes.POINTER(I)]),
     "mrp_volume_update_tr": (I, [P, Vec3, F]),
     "mrp_volume_get_tr_mip": (I, [P, I, P]),
     "mrp_volume_update_hg_lut": (I, [P, F, ctypes.c_uint64]),

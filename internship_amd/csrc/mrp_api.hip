@@ -188,6 +188,8 @@ int mrp_volume_create(int resolution, mrp_volume** out) {
         return bail(MRP_E_NOMEM, "cell grid allocation failed");
     if (hipMalloc(&v->quads, quad_count(resolution) * sizeof(float4)) != hipSuccess)
         return bail(MRP_E_NOMEM, "quad grid allocation failed");
+    if (hipMalloc(&v->occ, occupancy_words(resolution) * sizeof(uint32_t)) != hipSuccess)
+        return bail(MRP_E_NOMEM, "occupancy allocation failed");
     for (int m = 0; m < kNumMips; ++m)
         if (hipMalloc(&v->mips[m], padded_count(mip_res(m)) * sizeof(float)) != hipSuccess)
             return bail(MRP_E_NOMEM, "mip allocation failed");
@@ -346,6 +348,7 @@ int mrp_volume_destroy(mrp_volume* h) {
     hipFree(v->grid);
     hipFree(v->cells);
     hipFree(v->quads);
+    hipFree(v->occ);
     for (int m = 0; m < kNumMips; ++m) hipFree(v->mips[m]);
     for (int m = 0; m < kNumTrMips; ++m) hipFree(v->tr_mips[m]);
     hipFree(v->hg_lut);

@@ -39,6 +39,7 @@ struct Volume {
     float* grid = nullptr;           // padded (res+2)^3, border zeros
     float4* cells = nullptr;         // corner-expanded cells (res+1)^3 x 2 float4, the tracer's layout
     float4* quads = nullptr;         // the padded grid as 16-byte quads in 1x4x2 bricks (quad_count), the first free flight's layout
+    uint32_t* occ = nullptr;         // brick occupancy bits of the cells (Occupancy, occupancy_words)
     float max_density = 0.00001f;
     bool has_data = false;
     bool updated = false;

@@ -79,6 +79,43 @@ MRP_HD size_t cell_index(int res, int x, int y, int z) {
     const size_t c = (size_t)res + 1;
     return ((size_t)(x + 1) * c + (size_t)(y + 1)) * c + (size_t)(z + 1);
 }
+
+// Occupancy of the tracer's trilinear cells, by bricks: bit b of `bits`
+// (bricks z fastest, n per axis) is set when some cell u = cell + 1 in brick
+// b = u >> shift has a corner that is not zero.  A cell in a clear brick has
+// eight zero corners, so its trilinear value is an exact zero, and the
+// trackers use a density only through d * ratio > u (false for +-0 and
+// u in (0,1]), d < 0 (false) and 1 - max(0, d * ratio) (exactly 1): a lookup
+// there is answered without reading the grid, with the same draws, results
+// and lookup count.  Over the 512^3 benchmark cloud at 1080p about 72 % of
+// the first free flights' lookups and 45 % of the shadow rays' fall in clear
+// 16^3 bricks.
+// The kernels read the bits from LDS: staged once per workgroup (4.5 KB at
+// 512^3), a test costs an LDS read; from global memory the extra dependent
+// read before each cell load made the path tracer slower (36.5 against
+// 33.1 ms per 16-spp 1080p launch) while at most halving its HBM traffic.
+struct Occupancy {
+    const uint32_t* bits;  // nullptr: every cell counts as occupied
+    uint32_t shift, n;
+};
+MRP_HD uint32_t occupancy_shift(int res) { return res >= 256 ? 4u : (res >= 64 ? 3u : 2u); }
+MRP_HD uint32_t occupancy_n(int res) { return ((uint32_t)res >> occupancy_shift(res)) + 1u; }
+MRP_HD size_t occupancy_words(int res) {
+    const size_t n = occupancy_n(res);
+    return (n * n * n + 31) / 32;
+}
+#if defined(__HIPCC__)
+// Copies the bits of `o` into the workgroup's LDS array `lds` (occupancy_words
+// words) and returns an Occupancy that reads them there.  Every thread of the
+// workgroup must call it (it ends with a barrier).
+__device__ __forceinline__ Occupancy stage_occupancy(const Occupancy& o, int res, uint32_t* lds) {
+    if (!o.bits) return o;  // uniform
+    const uint32_t nw = (uint32_t)occupancy_words(res);
+    for (uint32_t w = threadIdx.x; w < nw; w += blockDim.x) lds[w] = o.bits[w];
+    __syncthreads();
+    return Occupancy{lds, o.shift, o.n};
+}
+#endif
 MRP_HD size_t cell_count(int res) {
     const size_t c = (size_t)res + 1;
     return c * c * c;
@@ -99,13 +136,27 @@ __device__ __forceinline__ uint32_t clamp_cell_index32(int res, int x, int y, in
 }
 // Outside the cells the result is 0; branch-free (cell (-1,-1,-1) is read and
 // discarded), so lanes of a wave need no divergent branch.
-__device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cells, int res, vec3 p) {
+// Whether the cell with floor coordinates (fx, fy, fz) in [-1, res-1] may be
+// nonzero (its brick's bit); a uniform test of the pointer skips the mask.
+__device__ __forceinline__ bool cell_occupied(const Occupancy& o, float fx, float fy, float fz) {
+    if (!o.bits) return true;
+    const uint32_t ux = (uint32_t)(fx + 1.0f) >> o.shift, uy = (uint32_t)(fy + 1.0f) >> o.shift,
+                   uz = (uint32_t)(fz + 1.0f) >> o.shift;
+    const uint32_t b = (ux * o.n + uy) * o.n + uz;
+    return (o.bits[b >> 5] >> (b & 31u)) & 1u;
+}
+__device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cells, int res, vec3 p,
+                                                 const Occupancy& occ = Occupancy{nullptr, 0u, 0u}) {
     const float fr = (float)res;
     const float tx = (p.x + 0.5f) * fr - 0.5f;
     const float ty = (p.y + 0.5f) * fr - 0.5f;
     const float tz = (p.z + 0.5f) * fr - 0.5f;
-    const bool in = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
-    const float fx = in ? floorf(tx) : -1.0f, fy = in ? floorf(ty) : -1.0f, fz = in ? floorf(tz) : -1.0f;
+    const bool inr = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
+    // a point outside, or in a clear brick, reads cell (-1,-1,-1) and discards
+    // it; the brick test sees only in-range coordinates
+    const float gx = inr ? floorf(tx) : -1.0f, gy = inr ? floorf(ty) : -1.0f, gz = inr ? floorf(tz) : -1.0f;
+    const bool in = inr && cell_occupied(occ, gx, gy, gz);
+    const float fx = in ? gx : -1.0f, fy = in ? gy : -1.0f, fz = in ? gz : -1.0f;
     const float wx = tx - fx, wy = ty - fy, wz = tz - fz;
     const float4* c = cells + 2u * cell_index32(res, (int)fx, (int)fy, (int)fz);
     const float4 lo = c[0], hi = c[1];
@@ -125,15 +176,19 @@ struct CellsFetch {
     };
     const float4* __restrict__ cells;
     int res;
-    // branch-free: a point outside reads cell (-1,-1,-1) and finish() discards it
+    Occupancy occ;
+    // branch-free: a point outside, or in a clear brick, reads cell (-1,-1,-1)
+    // (one line that stays in cache) and finish() discards it
     __device__ __forceinline__ Token issue(vec3 p) const {
         const float fr = (float)res;
         const float tx = (p.x + 0.5f) * fr - 0.5f;
         const float ty = (p.y + 0.5f) * fr - 0.5f;
         const float tz = (p.z + 0.5f) * fr - 0.5f;
         Token k;
-        k.in = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
-        const float fx = k.in ? floorf(tx) : -1.0f, fy = k.in ? floorf(ty) : -1.0f, fz = k.in ? floorf(tz) : -1.0f;
+        const bool inr = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
+        const float gx = inr ? floorf(tx) : -1.0f, gy = inr ? floorf(ty) : -1.0f, gz = inr ? floorf(tz) : -1.0f;
+        k.in = inr && cell_occupied(occ, gx, gy, gz);  // the brick test sees only in-range coordinates
+        const float fx = k.in ? gx : -1.0f, fy = k.in ? gy : -1.0f, fz = k.in ? gz : -1.0f;
         k.wx = tx - fx;
         k.wy = ty - fy;
         k.wz = tz - fz;
@@ -192,15 +247,19 @@ struct QuadsFetch {
     const float4* __restrict__ quads;
     int res;
     QuadGeom g;
-    // branch-free: a point outside reads cell (-1,-1,-1) and finish() discards it
+    Occupancy occ;
+    // branch-free: a point outside, or in a clear brick, reads cell (-1,-1,-1)
+    // and finish() discards it
     __device__ __forceinline__ Token issue(vec3 p) const {
         const float fr = (float)res;
         const float tx = (p.x + 0.5f) * fr - 0.5f;
         const float ty = (p.y + 0.5f) * fr - 0.5f;
         const float tz = (p.z + 0.5f) * fr - 0.5f;
         Token k;
-        k.in = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
-        const float fx = k.in ? floorf(tx) : -1.0f, fy = k.in ? floorf(ty) : -1.0f, fz = k.in ? floorf(tz) : -1.0f;
+        const bool inr = tx >= -1.0f && tx < fr && ty >= -1.0f && ty < fr && tz >= -1.0f && tz < fr;
+        const float gx = inr ? floorf(tx) : -1.0f, gy = inr ? floorf(ty) : -1.0f, gz = inr ? floorf(tz) : -1.0f;
+        k.in = inr && cell_occupied(occ, gx, gy, gz);  // the brick test sees only in-range coordinates
+        const float fx = k.in ? gx : -1.0f, fy = k.in ? gy : -1.0f, fz = k.in ? gz : -1.0f;
         k.wx = tx - fx;
         k.wy = ty - fy;
         k.wz = tz - fz;

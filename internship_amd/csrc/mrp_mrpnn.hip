@@ -46,6 +46,7 @@ struct NNArgs {
     const float4* cells;
     const float4* quads;  // the same grid in the quad layout (QuadsFetch), read by the first free flight
     QuadGeom qg;
+    Occupancy occ;        // brick occupancy of the cells: lookups in clear bricks read no grid line
     int res;
     float smax, alpha, g, tr_scale;
     Majorant mj;
@@ -93,10 +94,11 @@ constexpr int kMaxTrackSteps = 10000;
 //
 // first_vertex_item: one (sample, pixel) item; `item` counts from the first
 // item of `a`'s image, `slot_base` is added to the output slot.
-__device__ __forceinline__ void first_vertex_item(const NNArgs& a, uint32_t i, uint32_t item, uint32_t slot_base,
-                                                  float4* __restrict__ pos_act, float4* __restrict__ fx,
-                                                  float4* __restrict__ fy, float4* __restrict__ fz,
-                                                  uint32_t* __restrict__ rng_pos, uint32_t* lk) {
+__device__ __forceinline__ void first_vertex_item(const NNArgs& a, const Occupancy& occ, uint32_t i, uint32_t item,
+                                                  uint32_t slot_base, float4* __restrict__ pos_act,
+                                                  float4* __restrict__ fx, float4* __restrict__ fy,
+                                                  float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
+                                                  uint32_t* lk) {
     const uint32_t P = (uint32_t)(a.width * a.height);
     const uint32_t Pl = (uint32_t)(a.wl * a.height);
     const uint32_t s = item / Pl, l = item - s * Pl;
@@ -133,7 +135,7 @@ __device__ __forceinline__ void first_vertex_item(const NNArgs& a, uint32_t i, u
         if (off >= 0.0f) {
             const vec3 sp = ori + gdir * off;
             vec3 np;
-            const QuadsFetch F{a.quads, a.res, a.qg};
+            const QuadsFetch F{a.quads, a.res, a.qg, occ};
             if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
                                               ray_box_distance(sp, gdir), kMaxTrackSteps, &np, lk)) {
                 pos = np;
@@ -171,18 +173,22 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
                                                            float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
                                                            uint32_t* __restrict__ count_out) {
     __shared__ unsigned red[256];
+    extern __shared__ uint32_t s_occ[];  // occupancy_words(a.res) words (one image), none (views)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0u) *count_out = 0u;
     uint32_t lk = 0u;
-    if (i < a.n_items) {
-        const uint32_t item = a.item0 + i;
-        if constexpr (kViews) {
+    if constexpr (kViews) {
+        // several volumes: no occupancy (read from global memory it costs more than it saves)
+        if (i < a.n_items) {
+            const uint32_t item = a.item0 + i;
             const uint32_t view = __builtin_amdgcn_readfirstlane(item / a.view_items);
             const uint32_t base = view * a.view_items;
-            first_vertex_item(a.views[view], i, item - base, base, pos_act, fx, fy, fz, rng_pos, &lk);
-        } else {
-            first_vertex_item(a, i, item, 0u, pos_act, fx, fy, fz, rng_pos, &lk);
+            first_vertex_item(a.views[view], Occupancy{nullptr, 0u, 0u}, i, item - base, base, pos_act, fx, fy, fz,
+                              rng_pos, &lk);
         }
+    } else {
+        const Occupancy occ = stage_occupancy(a.occ, a.res, s_occ);
+        if (i < a.n_items) first_vertex_item(a, occ, i, a.item0 + i, 0u, pos_act, fx, fy, fz, rng_pos, &lk);
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -212,9 +218,11 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
                                                      const uint32_t* __restrict__ rng_pos,
                                                      const float4* __restrict__ fx, float* __restrict__ single) {
     __shared__ unsigned red[256];
+    extern __shared__ uint32_t s_occ[];  // occupancy_words(a0.res) words (one image), none (views)
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_act = *n_act_dev;  // the compaction's count (the grid covers the whole chunk)
     if (blockIdx.x * blockDim.x >= n_act) return;  // uniform over the workgroup: no barrier is split
+    const Occupancy occ = kViews ? Occupancy{nullptr, 0u, 0u} : stage_occupancy(a0.occ, a0.res, s_occ);
     uint32_t lk = 0u;
     if (k < n_act) {
         const uint32_t i = list[k];
@@ -234,7 +242,7 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
         const float4 pa = pos_act[i];
         const vec3 pos = v3(pa.x, pa.y, pa.z);
         const vec3 LX = normalize(a.light);  // the first axis frame_from_normal returns for the light
-        const CellsFetch F{a.cells, a.res};
+        const CellsFetch F{a.cells, a.res, occ};
         const float tr = transmittance_batched<kShadowBatch>(&rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX,
                                                          ray_box_distance(pos, LX), kMaxTrackSteps, &lk);
         single[i] = tr * fx[i].w;
@@ -990,6 +998,7 @@ static void image_args(const Volume* v, const mrp_render_params* p, PixelMap map
     a->cells = v->cells;
     a->quads = v->quads;
     a->qg = quad_geom(v->res);
+    a->occ = Occupancy{v->occ, occupancy_shift(v->res), occupancy_n(v->res)};
     a->res = v->res;
     a->alpha = p->alpha;
     a->smax = v->max_density * p->alpha;
@@ -1077,7 +1086,8 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
         if (views)
             first_vertex_kernel<true><<<blocks, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
         else
-            first_vertex_kernel<false><<<blocks, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
+            first_vertex_kernel<false><<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(
+                a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(tc, v->stream));
         compact_kernel<<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
@@ -1104,7 +1114,8 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
             shadow_scatter_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy,
                                                                         bucket_counts, bucket_counts + kShadowBuckets,
                                                                         sorder);
-            shadow_kernel<false><<<blocks, 256, 0, v->stream>>>(a, sorder, n_active_dev, pos_act, fy, rng_pos, fx,
+            shadow_kernel<false><<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(
+                a, sorder, n_active_dev, pos_act, fy, rng_pos, fx,
                                                                 single);
         }
         MRP_HIP(hipGetLastError());

@@ -24,6 +24,7 @@ namespace mrp {
 
 struct SampleArgs {
     const float4* cells;
+    Occupancy occ;  // brick occupancy of the cells
     int res;
     float max_density, tr_scale;
     vec3 light_color;
@@ -105,6 +106,8 @@ constexpr int kSampleBlocksPerCU = 3;  // 3 waves per SIMD at <= 168 VGPRs
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void get_sample_kernel(SampleArgs a,
                                                                                                 uint32_t* cursor) {
     __shared__ unsigned red[256];
+    extern __shared__ uint32_t s_occ[];  // occupancy_words(a.res) words
+    const Occupancy occ = stage_occupancy(a.occ, a.res, s_occ);
     const uint32_t lane = __lane_id();
     const uint32_t n_items = (uint32_t)a.n * (uint32_t)a.s_count;
     uint32_t lk = 0u;
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     vec3 pos = v3(0, 0, 0), ray = v3(0, 0, 1), csr = v3(1, 1, 1), res = v3(0, 0, 0), light = v3(0, 1, 0);
     float g = 0.0f, scatter = 1.0f;
     Majorant mj = make_majorant(1.0f, 1.0f, 1.0f);
-    const CellsFetch F{a.cells, a.res};
+    const CellsFetch F{a.cells, a.res, occ};
     for (;;) {
         const unsigned long long idle = __ballot(!active);
         const uint32_t n_idle = (uint32_t)__popcll(idle);
@@ -349,6 +352,7 @@ extern "C" int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q
     float* sums = paths + (size_t)chunk * n * 3;
     SampleArgs a;
     a.cells = v->cells;
+    a.occ = Occupancy{v->occ, occupancy_shift(v->res), occupancy_n(v->res)};
     a.res = v->res;
     a.max_density = v->max_density;
     a.tr_scale = 1.0f;
@@ -376,7 +380,7 @@ extern "C" int mrp_get_samples(mrp_volume* vol, int n, const mrp_sample_query* q
             clear_cursor_kernel<<<1, 64, 0, v->stream>>>(v->queues);  // the tracer's queue words are free here
             const unsigned blocks = (unsigned)std::min<size_t>((items + 255) / 256,
                                                                (size_t)std::max(1, v->num_cus * kSampleBlocksPerCU));
-            get_sample_kernel<<<blocks, 256, 0, v->stream>>>(a, v->queues);
+            get_sample_kernel<<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(a, v->queues);
             MRP_HIP(hipGetLastError());
             sample_sum_kernel<<<(unsigned)((n + 255) / 256), 256, 0, v->stream>>>(paths, n, cnt, sums, spp,
                                                                                  s0 + cnt >= spp ? 1 : 0, out_dev);

@@ -173,7 +173,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
         if (queue_done && !__any(active)) break;  // uniform: every wave reaches it once its lanes are idle
         if (active) {
             const FallbackView& v = views[view];
-            const CellsFetch F{v.cells, v.res};
+            // several volumes: no brick occupancy (read from global memory it costs more than it saves)
+            const CellsFetch F{v.cells, v.res, Occupancy{nullptr, 0u, 0u}};
             const vec3 light = normalize(v.light);
             vec3 np;
             bool done = false;

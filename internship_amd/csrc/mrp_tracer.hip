@@ -41,6 +41,7 @@ enum : int { ST_IDLE = 0, ST_INIT = 1, ST_FF = 2, ST_TR1 = 3, ST_TR2 = 4, ST_ACC
 
 struct TraceArgs {
     const float4* cells;
+    Occupancy occ;  // brick occupancy of the cells: lookups in clear bricks read no grid line
     int res;
     float smax, alpha, g, tr_scale;
     vec3 ori, forward, right, up, light_dir, light_color, scatter_rate;
@@ -82,7 +83,7 @@ __device__ __forceinline__ float ring_take(Ring& r) {
 }
 
 __device__ __forceinline__ float fetch_density(const TraceArgs& a, vec3 p) {
-    return trilinear_cells(a.cells, a.res, p);
+    return trilinear_cells(a.cells, a.res, p, a.occ);
 }
 
 // Pending-event batching.  An event (path start, real collision, end of a
@@ -97,10 +98,13 @@ constexpr int kEventBatch = 16;
 constexpr int kMinStepping = 40;
 
 // kEnvMap: escaping paths read the environment map (a.env.px); without one
-// they add the constant a.env.sky, and the kernel keeps the register budget
-// of 4 waves per SIMD (the map lookup takes it to 136 VGPRs, 3 waves).
+// they add the constant a.env.sky.  Both forms are held to 4 waves per SIMD
+// (at most 128 VGPRs, no spills; left alone the compiler took 130 with the
+// brick-occupancy test and 136 with the map, 3 waves).
 template <bool kEnvMap>
-__global__ __launch_bounds__(256) void trace_kernel(TraceArgs a) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void trace_kernel(TraceArgs a) {
+    extern __shared__ uint32_t s_occ[];  // occupancy_words(a.res) words
+    a.occ = stage_occupancy(a.occ, a.res, s_occ);
     const uint32_t lane = __lane_id();
     int q = (int)(blockIdx.x & 7u);
     int qtries = 0;
@@ -432,6 +436,7 @@ static void fill_trace_args(Volume* v, const mrp_render_params* p, PixelMap map,
     const int wl = map.checker < 0 ? W : (W + 1) / 2;
     const int band_rows = (H + kNumQueues - 1) / kNumQueues;
     a->cells = v->cells;
+    a->occ = Occupancy{v->occ, occupancy_shift(v->res), occupancy_n(v->res)};
     a->res = v->res;
     a->alpha = p->alpha;
     a->smax = v->max_density * p->alpha;
@@ -467,7 +472,8 @@ static uint64_t trace_items_per_sample(const TraceArgs& a) { return (uint64_t)a.
 template <bool kEnvMap>
 static int trace_grid(Volume* v, int* blocks) {
     int occ = 0;
-    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel<kEnvMap>, 256, 0));
+    MRP_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trace_kernel<kEnvMap>, 256,
+                                                         occupancy_words(v->res) * sizeof(uint32_t)));
     occ = std::max(1, std::min(occ, 8));
     *blocks = std::max(kNumQueues, v->num_cus * occ);
     return MRP_OK;
@@ -487,11 +493,11 @@ int trace_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     if (a.env.px) {
         rc = trace_grid<true>(v, &blocks);
         if (rc) return rc;
-        trace_kernel<true><<<blocks, 256, 0, v->stream>>>(a);
+        trace_kernel<true><<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(a);
     } else {
         rc = trace_grid<false>(v, &blocks);
         if (rc) return rc;
-        trace_kernel<false><<<blocks, 256, 0, v->stream>>>(a);
+        trace_kernel<false><<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(a);
     }
     MRP_HIP(hipGetLastError());
     return MRP_OK;

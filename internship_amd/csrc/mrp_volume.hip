@@ -238,6 +238,33 @@ __global__ void cloud_kernel(const int* __restrict__ perm, CloudParams c, int re
     }
 }
 
+// Brick occupancy (Occupancy, mrp_medium.h): one workgroup per brick scans
+// the corners of the brick's cells, padded indices [b << shift, (b + 1) <<
+// shift] per axis (cell u has its corners at u and u + 1), clipped to the
+// padded grid, and sets the brick's bit if any corner is not zero (a -0 counts
+// as zero: it blends to +-0 like +0).
+__global__ __launch_bounds__(256) void occupancy_kernel(const float* __restrict__ grid, int res, uint32_t shift,
+                                                        uint32_t n, uint32_t* __restrict__ bits) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t bz = b % n, by = (b / n) % n, bx = b / (n * n);
+    const uint32_t P = (uint32_t)res + 2u, B = 1u << shift;
+    const uint32_t x0 = bx * B, y0 = by * B, z0 = bz * B;
+    const uint32_t ex = min(x0 + B, P - 1u) + 1u - x0, ey = min(y0 + B, P - 1u) + 1u - y0,
+                   ez = min(z0 + B, P - 1u) + 1u - z0;  // extents (>= 1: every brick starts inside the grid)
+    const uint32_t cnt = ex * ey * ez;
+    int any = 0;
+    for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) {
+        const uint32_t z = z0 + i % ez, y = y0 + (i / ez) % ey, x = x0 + i / (ez * ey);
+        any |= grid[((size_t)x * P + y) * P + z] != 0.0f;
+    }
+    if (__syncthreads_or(any) && threadIdx.x == 0) atomicOr(&bits[b >> 5], 1u << (b & 31u));
+}
+
+__global__ void clear_words_kernel(uint32_t* __restrict__ w, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = 0u;
+}
+
 static unsigned blocks_for(size_t n) {
     size_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -262,6 +289,12 @@ int launch_update(Volume* v) {
     build_cells_kernel<<<blocks_for(cell_count(res)), 256, 0, v->stream>>>(v->grid, res, v->cells);
     build_quads_kernel<<<blocks_for(quad_count(res)), 256, 0, v->stream>>>(v->grid, res, quad_geom(res),
                                                                             quad_count(res), v->quads);
+    {
+        const uint32_t on = occupancy_n(res);
+        const size_t words = occupancy_words(res);
+        clear_words_kernel<<<(unsigned)((words + 255) / 256), 256, 0, v->stream>>>(v->occ, words);
+        occupancy_kernel<<<on * on * on, 256, 0, v->stream>>>(v->grid, res, occupancy_shift(res), on, v->occ);
+    }
 
     // source chain: halve (by trilinear resampling) while above 2x256
     int r = 256;
@@ -455,6 +488,21 @@ int mrp_volume_get_mip(const mrp_volume* h, int mip, float* host_out) {
     if (e == hipSuccess) e = hipStreamSynchronize(v->stream);
     hipFree(tmp);
     if (e != hipSuccess) return hip_fail(e, "get_mip");
+    return MRP_OK;
+}
+
+int mrp_volume_get_occupancy(const mrp_volume* h, uint32_t* host_out, size_t max_words, int* shift, int* n) {
+    if (!h || !shift || !n) return fail(MRP_E_INVALID, "NULL argument");
+    const Volume* v = &h->v;
+    if (!v->updated) return fail(MRP_E_STATE, "mrp_volume_update() has not run");
+    *shift = (int)occupancy_shift(v->res);
+    *n = (int)occupancy_n(v->res);
+    if (!host_out) return MRP_OK;  // size query
+    const size_t words = occupancy_words(v->res);
+    if (max_words < words) return fail(MRP_E_INVALID, "output buffer holds fewer than ceil(n^3/32) words");
+    MRP_HIP(hipSetDevice(v->device));
+    MRP_HIP(hipStreamSynchronize(v->stream));
+    MRP_HIP(hipMemcpy(host_out, v->occ, words * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return MRP_OK;
 }
 
