@@ -73,12 +73,13 @@ struct NNArgs {
 // mrp_track.h: kFVBatch / kShadowBatch lookups in flight per thread, with the
 // draws and results of the one-lookup-at-a-time loops.  The first vertex runs
 // 2M threads, so occupancy times lookups in flight sets its memory
-// parallelism: 8 (153 VGPRs, 3 waves per SIMD) took it from 0.60 to 0.56 ms
-// at 1080p against 4 (100 VGPRs, 5 waves) and 6 (127 VGPRs, 4 waves); 10 or
-// more put the batch in scratch.  The shadow rays fill only about 4 waves per
-// SIMD and keep 4.
-constexpr int kFVBatch = 8;
-constexpr int kShadowBatch = 4;
+// parallelism.  With the brick occupancy in LDS most of its lookups in the
+// cloud's hull hit L2, and more waves beat deeper batches: at 1080p 3 (93
+// VGPRs, 5 waves per SIMD) took 0.36 ms against 4: 0.37, 6: 0.41, 8 (153
+// VGPRs, 3 waves): 0.42; 12 put the batch in scratch (1.29 ms).  The shadow
+// rays run about 256K threads, 4 waves per SIMD, so depth pays there: 6 (148 VGPRs) took 0.23 ms against 2: 0.26, 4: 0.25, 8: 0.24.
+constexpr int kFVBatch = 3;
+constexpr int kShadowBatch = 6;
 constexpr int kMaxTrackSteps = 10000;
 
 // pos_act = (pos, active), fx = (X, HG phase to the light; shadow_kernel
