@@ -124,15 +124,24 @@ MRP_HD size_t cell_count(int res) {
 // Density at box position p from the cell layout: the same arithmetic as
 // trilinear_border (same corners, same weights, same blend order).
 #if defined(__HIPCC__)
-// 32-bit cell indices on the device: resolutions are at most 1024, so twice
-// the largest index (2 * 1025^3) stays below 2^32.
+// a * b + c with v_mad_u32_u24 (full rate; a, b < 2^24).  Left to itself the
+// compiler turns a multiply-add of small values into the quarter-rate 64-bit
+// v_mad_u64_u32.
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+// 32-bit cell indices on the device: resolutions are at most 1024
+// (mrp_volume_create), so twice the largest index (2 * 1025^3) stays below
+// 2^32 and every factor is under 2^24.
 __device__ __forceinline__ uint32_t cell_index32(int res, int x, int y, int z) {
     const uint32_t c = (uint32_t)res + 1u;
-    return ((uint32_t)(x + 1) * c + (uint32_t)(y + 1)) * c + (uint32_t)(z + 1);
+    return mad_u24(mad_u24((uint32_t)(x + 1), c, (uint32_t)(y + 1)), c, (uint32_t)(z + 1));
 }
 __device__ __forceinline__ uint32_t clamp_cell_index32(int res, int x, int y, int z) {
     const uint32_t c = (uint32_t)res + 2u;
-    return ((uint32_t)(x + 1) * c + (uint32_t)(y + 1)) * c + (uint32_t)(z + 1);
+    return mad_u24(mad_u24((uint32_t)(x + 1), c, (uint32_t)(y + 1)), c, (uint32_t)(z + 1));
 }
 // Outside the cells the result is 0; branch-free (cell (-1,-1,-1) is read and
 // discarded), so lanes of a wave need no divergent branch.
@@ -142,7 +151,7 @@ __device__ __forceinline__ bool cell_occupied(const Occupancy& o, float fx, floa
     if (!o.bits) return true;
     const uint32_t ux = (uint32_t)(fx + 1.0f) >> o.shift, uy = (uint32_t)(fy + 1.0f) >> o.shift,
                    uz = (uint32_t)(fz + 1.0f) >> o.shift;
-    const uint32_t b = (ux * o.n + uy) * o.n + uz;
+    const uint32_t b = mad_u24(mad_u24(ux, o.n, uy), o.n, uz);
     return (o.bits[b >> 5] >> (b & 31u)) & 1u;
 }
 __device__ __forceinline__ float trilinear_cells(const float4* __restrict__ cells, int res, vec3 p,

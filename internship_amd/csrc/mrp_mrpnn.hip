@@ -570,13 +570,6 @@ struct GatherView {
     float alpha, g, h_fixed;
 };
 
-// v_mad_u32_u24 (full rate); left to itself the compiler picks the quarter-rate
-// 64-bit multiply-add for the first step
-__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t r;
-    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
 __device__ __forceinline__ uint32_t cell_index24(uint32_t c, uint32_t ux, uint32_t uy, uint32_t uz) {
     return mad_u24(mad_u24(ux, c, uy), c, uz);
 }
