@@ -125,3 +125,39 @@ def test_gpu_small_sample_budget_batches_identically(sphere128):
     _lib.check(vol.lib.mrp_render_offline(vol.h, ctypes.byref(p), ctypes.c_void_p(out.ctypes.data), ctypes.byref(st)))
     vol.close()
     assert np.array_equal(out, ref)
+
+
+def test_gpu_tracer_negative_densities_match_oracle():
+    """Negative densities never scatter and move a walk on by -d (t -= d), in
+    free flight and in both shadow-ray loops; the tracker's next step, drawn
+    ahead of the density, must then be dropped."""
+    _say("case: cloud 64^3 seed 4 minus 0.08 (negative voxels), 40x40, 4 spp, 16 bounces, alpha 256")
+    grid = (oracle.generate_cloud(64, seed=4) - np.float32(0.08)).astype(np.float32)
+    assert (grid < 0).mean() > 0.3 and (grid > 0.2).any()
+    p = config1_params(size=40, spp=4, multi=16, seed=21)
+    p.alpha = 256.0
+    gpu, st, _ = _gpu_render(grid, p)
+    cpu, lookups, scatters = oracle.render(grid, oracle.max_density(grid), p)
+    ok, exact, dmax = _compare(gpu, cpu)
+    _say("exact fraction %.6f, max abs diff %.3g, lookups gpu %d cpu %d" % (exact, dmax, st.lookups, lookups))
+    assert ok.all()
+    assert st.lookups == lookups and st.scatters == scatters
+    assert scatters > 0
+
+
+def test_gpu_tracer_loop_caps_match_oracle():
+    """A thin medium under a huge majorant: free flights and shadow rays run
+    into the reference's cap of 10000 loop tests (the free flight then
+    scatters where it stands, the shadow ray ends with its transmittance)."""
+    _say("case: 32^3 thin medium (1e-5, one voxel 1.0), 16x16, 1 spp, 3 bounces, alpha 1e5")
+    grid = np.full((32, 32, 32), 1e-5, np.float32)
+    grid[3, 5, 7] = 1.0
+    p = config1_params(size=16, spp=1, multi=3, seed=8)
+    p.alpha = 1.0e5
+    gpu, st, _ = _gpu_render(grid, p)
+    cpu, lookups, scatters = oracle.render(grid, oracle.max_density(grid), p)
+    ok, exact, dmax = _compare(gpu, cpu)
+    _say("exact fraction %.6f, max abs diff %.3g, lookups gpu %d cpu %d" % (exact, dmax, st.lookups, lookups))
+    assert lookups > 16 * 16 * 2 * 10000  # most bounces reach both caps
+    assert ok.all()
+    assert st.lookups == lookups and st.scatters == scatters
