@@ -78,8 +78,13 @@ struct NNArgs {
 // VGPRs, 5 waves per SIMD) took 0.36 ms against 4: 0.37, 6: 0.41, 8 (153
 // VGPRs, 3 waves): 0.42; 12 put the batch in scratch (1.29 ms).  The shadow
 // rays run about 256K threads, 4 waves per SIMD, so depth pays there: 6 (148 VGPRs) took 0.23 ms against 2: 0.26, 4: 0.25, 8: 0.24.
+// The multi-view forms (serving: 64 views of 16 volumes, no brick occupancy)
+// measured their batch at 6 and 4 (MRPNN stages 73.9 ms per 720p batch
+// against 74.9 with 3 and 6, or with 8 and 4).
 constexpr int kFVBatch = 3;
 constexpr int kShadowBatch = 6;
+constexpr int kFVBatchViews = 6;
+constexpr int kShadowBatchViews = 4;
 constexpr int kMaxTrackSteps = 10000;
 
 // pos_act = (pos, active), fx = (X, HG phase to the light; shadow_kernel
@@ -95,6 +100,7 @@ constexpr int kMaxTrackSteps = 10000;
 //
 // first_vertex_item: one (sample, pixel) item; `item` counts from the first
 // item of `a`'s image, `slot_base` is added to the output slot.
+template <int kBatch>
 __device__ __forceinline__ void first_vertex_item(const NNArgs& a, const Occupancy& occ, uint32_t i, uint32_t item,
                                                   uint32_t slot_base, float4* __restrict__ pos_act,
                                                   float4* __restrict__ fx, float4* __restrict__ fy,
@@ -137,7 +143,7 @@ __device__ __forceinline__ void first_vertex_item(const NNArgs& a, const Occupan
             const vec3 sp = ori + gdir * off;
             vec3 np;
             const QuadsFetch F{a.quads, a.res, a.qg, occ};
-            if (free_flight_batched<kFVBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
+            if (free_flight_batched<kBatch>(&rng, F, a.mj.inv_smax, a.mj.ratio, sp, gdir,
                                               ray_box_distance(sp, gdir), kMaxTrackSteps, &np, lk)) {
                 pos = np;
                 active = 1;
@@ -184,12 +190,12 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
             const uint32_t item = a.item0 + i;
             const uint32_t view = __builtin_amdgcn_readfirstlane(item / a.view_items);
             const uint32_t base = view * a.view_items;
-            first_vertex_item(a.views[view], Occupancy{nullptr, 0u, 0u}, i, item - base, base, pos_act, fx, fy, fz,
-                              rng_pos, &lk);
+            first_vertex_item<kFVBatchViews>(a.views[view], Occupancy{nullptr, 0u, 0u}, i, item - base, base, pos_act,
+                                             fx, fy, fz, rng_pos, &lk);
         }
     } else {
         const Occupancy occ = stage_occupancy(a.occ, a.res, s_occ);
-        if (i < a.n_items) first_vertex_item(a, occ, i, a.item0 + i, 0u, pos_act, fx, fy, fz, rng_pos, &lk);
+        if (i < a.n_items) first_vertex_item<kFVBatch>(a, occ, i, a.item0 + i, 0u, pos_act, fx, fy, fz, rng_pos, &lk);
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -244,8 +250,8 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
         const vec3 pos = v3(pa.x, pa.y, pa.z);
         const vec3 LX = normalize(a.light);  // the first axis frame_from_normal returns for the light
         const CellsFetch F{a.cells, a.res, occ};
-        const float tr = transmittance_batched<kShadowBatch>(&rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX,
-                                                         ray_box_distance(pos, LX), kMaxTrackSteps, &lk);
+        const float tr = transmittance_batched<kViews ? kShadowBatchViews : kShadowBatch>(
+            &rng, F, a.mj.inv_smax_tr, a.mj.ratio, pos, LX, ray_box_distance(pos, LX), kMaxTrackSteps, &lk);
         single[i] = tr * fx[i].w;
     }
     red[threadIdx.x] = lk;
@@ -574,9 +580,11 @@ __device__ __forceinline__ uint32_t cell_index24(uint32_t c, uint32_t ux, uint32
     return mad_u24(mad_u24(ux, c, uy), c, uz);
 }
 
-// 7 waves per SIMD (73 VGPRs): at 8 (64 VGPRs) about ten values spill to scratch.
+// 7 waves per SIMD (73 VGPRs): at 8 (64 VGPRs) about ten values spill to
+// scratch.  The multi-view form keeps the view's constants for a change of
+// view as well and spills 52 bytes per lane at 7, so it runs at 6.
 template <bool kCells16, bool kViews>
-__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(7))) void gather_tiled_kernel(
+__global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(kViews ? 5 : 7))) void gather_tiled_kernel(
     GatherArgs a, const float4* __restrict__ pos_act, const float4* __restrict__ fx, const float4* __restrict__ fy,
     const float4* __restrict__ fz, const uint32_t* __restrict__ list, void* __restrict__ feat_out,
     float* __restrict__ gamma, float* __restrict__ pool) {
