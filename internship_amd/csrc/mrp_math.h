@@ -299,4 +299,78 @@ MRP_HD float rng_next(Rng* r) {
     return u01_(pick4_(r->buf, w));
 }
 
+// The same stream read through a ring of kRingWords generated words, so that a
+// walk can make all the words of a window of draws at one site (rng_fill) and
+// read them by index (rng_word): a wave then runs the cipher once per missing
+// block, not once per draw site that some lane reaches at a block boundary,
+// and a rewind inside the window (the batched trackers' speculation) costs
+// nothing.  Word i sits at w[(i % kRingWords) * kStride]: kStride 64 places
+// the 64 lanes of a wave side by side in LDS (a wave's reads of any words hit
+// 64 different banks), 1 is a private array.  hi, a multiple of 4, ends the
+// generated words; the ring holds words [hi - kRingWords, hi).  A seek may go
+// forward anywhere, or back over words taken since the ring last restarted.
+constexpr uint32_t kRingWords = 16u;
+template <uint32_t kStride>
+struct RingRng {
+    uint32_t* w;
+    uint32_t id, sample, tag, k0, k1;
+    uint32_t n, hi;
+};
+template <uint32_t kStride>
+MRP_HD RingRng<kStride> ring_make(uint32_t* w, uint32_t id, uint32_t sample, uint32_t tag, uint64_t seed) {
+    RingRng<kStride> r;
+    r.w = w;
+    r.id = id;
+    r.sample = sample;
+    r.tag = tag;
+    r.k0 = (uint32_t)seed;
+    r.k1 = (uint32_t)(seed >> 32);
+    r.n = 0u;
+    r.hi = 0u;
+    return r;
+}
+// Makes the words below `end`; with end - n <= kRingWords - 3 every word not
+// yet taken stays in the ring.
+template <uint32_t kStride>
+MRP_HD void rng_fill(RingRng<kStride>* r, uint32_t end) {
+    while (r->hi < end) {
+        u32x4 c;
+        c.v[0] = r->id;
+        c.v[1] = r->sample;
+        c.v[2] = r->hi >> 2;
+        c.v[3] = r->tag;
+        const u32x4 o = philox4x32_10(c, r->k0, r->k1);
+        uint32_t* d = r->w + (r->hi & (kRingWords - 1u)) * kStride;
+        d[0] = o.v[0];
+        d[kStride] = o.v[1];
+        d[2 * kStride] = o.v[2];
+        d[3 * kStride] = o.v[3];
+        r->hi += 4u;
+    }
+}
+// Draw i (made by rng_fill, still in the ring), without taking it.
+template <uint32_t kStride>
+MRP_HD float rng_word(const RingRng<kStride>& r, uint32_t i) {
+    return u01_(r.w[(i & (kRingWords - 1u)) * kStride]);
+}
+template <uint32_t kStride>
+MRP_HD float rng_next(RingRng<kStride>* r) {
+    rng_fill(r, r->n + 1u);
+    const float u = rng_word(*r, r->n);
+    r->n += 1u;
+    return u;
+}
+template <uint32_t kStride>
+MRP_HD void rng_seek(RingRng<kStride>* r, uint32_t n) {
+    if (n > r->hi || n + kRingWords < r->hi) r->hi = n & ~3u;  // outside the ring: restart at n's block
+    r->n = n;
+}
+#if defined(__HIPCC__)
+// This thread's ring in a workgroup's LDS array of blockDim.x * kRingWords
+// words: word i of lane l of wave v at [(v * kRingWords + i % kRingWords) * 64 + l].
+__device__ __forceinline__ uint32_t* lds_ring(uint32_t* ring) {
+    return ring + (threadIdx.x >> 6) * (kRingWords * 64u) + (threadIdx.x & 63u);
+}
+#endif
+
 }  // namespace mrp

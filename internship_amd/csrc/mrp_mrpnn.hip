@@ -105,7 +105,7 @@ __device__ __forceinline__ void first_vertex_item(const NNArgs& a, const Occupan
                                                   uint32_t slot_base, float4* __restrict__ pos_act,
                                                   float4* __restrict__ fx, float4* __restrict__ fy,
                                                   float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
-                                                  uint32_t* lk) {
+                                                  uint32_t* ring, uint32_t* lk) {
     const uint32_t P = (uint32_t)(a.width * a.height);
     const uint32_t Pl = (uint32_t)(a.wl * a.height);
     const uint32_t s = item / Pl, l = item - s * Pl;
@@ -124,7 +124,8 @@ __device__ __forceinline__ void first_vertex_item(const NNArgs& a, const Occupan
         fz[i] = make_float4(0.0f, 0.0f, 1.0f, -1.0f);
     } else {
         const uint32_t pix = (uint32_t)row * (uint32_t)a.width + (uint32_t)col;
-        Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        RingRng<64> rng =
+            ring_make<64>(ring, pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
         vec3 ori = a.ori, dir;
         if (a.ray_ori) {
             ori = v3(a.ray_ori[3 * (size_t)pix], a.ray_ori[3 * (size_t)pix + 1], a.ray_ori[3 * (size_t)pix + 2]);
@@ -180,7 +181,8 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
                                                            float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
                                                            uint32_t* __restrict__ count_out) {
     __shared__ unsigned red[256];
-    extern __shared__ uint32_t s_occ[];  // occupancy_words(a.res) words (one image), none (views)
+    __shared__ uint32_t s_ring[256 * kRingWords];  // the threads' Philox rings (16 KB)
+    extern __shared__ uint32_t s_occ[];             // occupancy_words(a.res) words (one image), none (views)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0u) *count_out = 0u;
     uint32_t lk = 0u;
@@ -191,11 +193,12 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
             const uint32_t view = __builtin_amdgcn_readfirstlane(item / a.view_items);
             const uint32_t base = view * a.view_items;
             first_vertex_item<kFVBatchViews>(a.views[view], Occupancy{nullptr, 0u, 0u}, i, item - base, base, pos_act,
-                                             fx, fy, fz, rng_pos, &lk);
+                                             fx, fy, fz, rng_pos, lds_ring(s_ring), &lk);
         }
     } else {
         const Occupancy occ = stage_occupancy(a.occ, a.res, s_occ);
-        if (i < a.n_items) first_vertex_item<kFVBatch>(a, occ, i, a.item0 + i, 0u, pos_act, fx, fy, fz, rng_pos, &lk);
+        if (i < a.n_items)
+            first_vertex_item<kFVBatch>(a, occ, i, a.item0 + i, 0u, pos_act, fx, fy, fz, rng_pos, lds_ring(s_ring), &lk);
     }
     red[threadIdx.x] = lk;
     __syncthreads();
@@ -225,7 +228,8 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
                                                      const uint32_t* __restrict__ rng_pos,
                                                      const float4* __restrict__ fx, float* __restrict__ single) {
     __shared__ unsigned red[256];
-    extern __shared__ uint32_t s_occ[];  // occupancy_words(a0.res) words (one image), none (views)
+    __shared__ uint32_t s_ring[256 * kRingWords];  // the threads' Philox rings (16 KB)
+    extern __shared__ uint32_t s_occ[];             // occupancy_words(a0.res) words (one image), none (views)
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t n_act = *n_act_dev;  // the compaction's count (the grid covers the whole chunk)
     if (blockIdx.x * blockDim.x >= n_act) return;  // uniform over the workgroup: no barrier is split
@@ -244,7 +248,8 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
         const NNArgs& a = *ap;
         const uint32_t P = (uint32_t)(a.width * a.height);
         const uint32_t s = slot / P, pix = slot - s * P;
-        Rng rng = rng_make(pix, (uint32_t)a.spp_begin + s, 0u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+        RingRng<64> rng = ring_make<64>(lds_ring(s_ring), pix, (uint32_t)a.spp_begin + s, 0u,
+                                        ((uint64_t)a.seed_hi << 32) | a.seed_lo);
         rng_seek(&rng, rng_pos[i]);
         const float4 pa = pos_act[i];
         const vec3 pos = v3(pa.x, pa.y, pa.z);

@@ -106,7 +106,8 @@ constexpr int kSampleBlocksPerCU = 3;  // 3 waves per SIMD at <= 168 VGPRs
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void get_sample_kernel(SampleArgs a,
                                                                                                 uint32_t* cursor) {
     __shared__ unsigned red[256];
-    extern __shared__ uint32_t s_occ[];  // occupancy_words(a.res) words
+    __shared__ uint32_t s_ring[256 * kRingWords];  // the threads' Philox rings (16 KB)
+    extern __shared__ uint32_t s_occ[];             // occupancy_words(a.res) words
     const Occupancy occ = stage_occupancy(a.occ, a.res, s_occ);
     const uint32_t lane = __lane_id();
     const uint32_t n_items = (uint32_t)a.n * (uint32_t)a.s_count;
@@ -114,7 +115,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
     bool active = false, queue_done = false;
     uint32_t i = 0u;
     int bounce = 0;
-    Rng rng = rng_make(0u, 0u, 0u, 0ull);
+    uint32_t* const ring = lds_ring(s_ring);
+    RingRng<64> rng = ring_make<64>(ring, 0u, 0u, 0u, 0ull);
     vec3 pos = v3(0, 0, 0), ray = v3(0, 0, 1), csr = v3(1, 1, 1), res = v3(0, 0, 0), light = v3(0, 1, 0);
     float g = 0.0f, scatter = 1.0f;
     Majorant mj = make_majorant(1.0f, 1.0f, 1.0f);
@@ -137,7 +139,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void g
                     light = v3(q.light_dir.x, q.light_dir.y, q.light_dir.z);
                     g = q.g;
                     scatter = q.scatter;
-                    rng = rng_make(j, (uint32_t)(a.s_begin + (int)k), 1u, ((uint64_t)a.seed_hi << 32) | a.seed_lo);
+                    rng = ring_make<64>(ring, j, (uint32_t)(a.s_begin + (int)k), 1u,
+                                        ((uint64_t)a.seed_hi << 32) | a.seed_lo);
                     pos = v3(q.pos.x, q.pos.y, q.pos.z);
                     const float e0 = rng_next(&rng);
                     const float e1 = rng_next(&rng);

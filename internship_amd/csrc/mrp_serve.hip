@@ -122,12 +122,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
                                                               float4* __restrict__ out,
                                                               unsigned long long* __restrict__ counters) {
     __shared__ unsigned red[256];
+    __shared__ uint32_t s_ring[256 * kRingWords];  // the threads' Philox rings (16 KB)
     const uint32_t lane = __lane_id();
     uint32_t lk = 0u;
     bool active = false, queue_done = false;
     uint32_t it = 0u, view = 0u;
     int bounce = 0;
-    Rng rng = rng_make(0u, 0u, 0u, 0ull);
+    uint32_t* const ring = lds_ring(s_ring);
+    RingRng<64> rng = ring_make<64>(ring, 0u, 0u, 0u, 0ull);
     vec3 ori = v3(0, 0, 0), pos = v3(0, 0, 0), dir = v3(0, 0, 1), csr = v3(1, 1, 1), res = v3(0, 0, 0);
     float t_first = -1.0f;
     for (;;) {
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void f
                     const uint32_t pix = gi - view * a.P;
                     const FallbackView& v = views[view];
                     const int row = (int)pix / v.width, col = (int)pix - row * v.width;
-                    rng = rng_make(pix, (uint32_t)a.spp + k, 0u, ((uint64_t)v.seed_hi << 32) | v.seed_lo);
+                    rng = ring_make<64>(ring, pix, (uint32_t)a.spp + k, 0u, ((uint64_t)v.seed_hi << 32) | v.seed_lo);
                     const float xu = rng_next(&rng);
                     const float xv = rng_next(&rng);
                     // radiance_paths: normalised direction, box entry, or the sky

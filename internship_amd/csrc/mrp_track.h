@@ -4,12 +4,13 @@
 // In both trackers the next tentative position depends only on the random
 // draws (t -= log(1 - u) / majorant); a density value only decides whether
 // the walk stops, or moves t when it is negative (t -= d).  So a thread can
-// draw the next kTrackBatch steps ahead, issue all their lookups, and then
-// consume the results in order.  The counter-based stream makes the look-ahead
-// free: the draws are taken from a copy of the stream, and the committed state
-// is set back (rng_seek) to exactly the draws the one-step-at-a-time loop
-// would have consumed.  Results, draw counts and lookup counts are identical
-// to the sequential loops below, which state the reference semantics:
+// draw the next B steps ahead, issue all their lookups, and then consume the
+// results in order.  The counter-based stream makes the look-ahead free: a
+// round makes the words of its window in the stream's ring at one site
+// (rng_fill), reads them by index, and then sets the stream to exactly the
+// draws the one-step-at-a-time loop would have consumed.  Results, draw
+// counts and lookup counts are identical to the sequential loops below, which
+// state the reference semantics:
 //   free_flight_seq   : `DeterminateNextVertex` with an unused direction
 //                       (core/render.cuh:138-172) as NNPredict's first vertex uses it;
 //   transmittance_seq : `Tr` (core/render.cuh:93-136): ratio tracking while
@@ -25,8 +26,8 @@
 
 namespace mrp {
 
-template <class Fetch>
-MRP_HD bool free_flight_seq(Rng* rng, const Fetch& F, float inv_smax, float ratio, vec3 pos, vec3 dir, float dis,
+template <class R, class Fetch>
+MRP_HD bool free_flight_seq(R* rng, const Fetch& F, float inv_smax, float ratio, vec3 pos, vec3 dir, float dis,
                             int max_steps, vec3* npos, uint32_t* lk) {
     float t = 0.0f;
     int loop_num = 0;
@@ -46,8 +47,8 @@ MRP_HD bool free_flight_seq(Rng* rng, const Fetch& F, float inv_smax, float rati
     return true;
 }
 
-template <class Fetch>
-MRP_HD float transmittance_seq(Rng* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
+template <class R, class Fetch>
+MRP_HD float transmittance_seq(R* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
                                float dis, int max_steps, uint32_t* lk) {
     float tr = 1.0f, t = 0.0f;
     int loop_num = 0;
@@ -76,15 +77,16 @@ MRP_HD float transmittance_seq(Rng* rng, const Fetch& F, float inv_smax_tr, floa
 }
 
 // free_flight_seq with up to B lookups in flight.
-template <int B, class Fetch>
-MRP_HD bool free_flight_batched(Rng* rng, const Fetch& F, float inv_smax, float ratio, vec3 pos, vec3 dir, float dis,
-                                int max_steps, vec3* npos, uint32_t* lk) {
+template <int B, uint32_t kStride, class Fetch>
+MRP_HD bool free_flight_batched(RingRng<kStride>* rng, const Fetch& F, float inv_smax, float ratio, vec3 pos,
+                                vec3 dir, float dis, int max_steps, vec3* npos, uint32_t* lk) {
+    static_assert(2 * B + 3 <= (int)kRingWords, "a round's draws must stay in the ring");
     float t = 0.0f;
     int steps = 0;  // loop tests passed so far (iterations started)
     for (;;) {
         if (steps >= max_steps) break;  // the loop test fails: accept at t
         const uint32_t n0 = rng->n;
-        Rng sp = *rng;
+        rng_fill(rng, n0 + 2u * (uint32_t)B);  // step j draws words n0 + 2j (distance) and n0 + 2j + 1
         float ts[B], ua[B];
         typename Fetch::Token tok[B];
         int m = 0;
@@ -93,13 +95,12 @@ MRP_HD bool free_flight_batched(Rng* rng, const Fetch& F, float inv_smax, float 
 #pragma unroll
         for (int j = 0; j < B; ++j) {
             if (escape || steps + j >= max_steps) break;
-            const float rk = rng_next(&sp);
-            tc -= logf_(1.0f - rk) * inv_smax;
+            tc -= logf_(1.0f - rng_word(*rng, n0 + 2u * (uint32_t)j)) * inv_smax;
             if (tc > dis) {
                 escape = true;
                 break;
             }
-            ua[j] = rng_next(&sp);
+            ua[j] = rng_word(*rng, n0 + 2u * (uint32_t)j + 1u);
             ts[j] = tc;
             tok[j] = F.issue(pos + (dir * tc));
             ++m;
@@ -122,24 +123,23 @@ MRP_HD bool free_flight_batched(Rng* rng, const Fetch& F, float inv_smax, float 
             }
         }
         if (event == 1) {
-            rng_seek(rng, n0 + 2u * (uint32_t)(j + 1));
+            rng->n = n0 + 2u * (uint32_t)(j + 1);
             break;
         }
         if (event == 2) {
             steps += j + 1;
-            rng_seek(rng, n0 + 2u * (uint32_t)(j + 1));
+            rng->n = n0 + 2u * (uint32_t)(j + 1);
             continue;
         }
         steps += m;
         if (escape) {
-            rng_seek(rng, n0 + 2u * (uint32_t)m + 1u);
+            rng->n = n0 + 2u * (uint32_t)m + 1u;
             return false;
         }
-        t = tc;     // the last speculated position (t itself when m == 0)
-        *rng = sp;  // exactly 2m draws consumed
+        t = tc;                             // the last speculated position (t itself when m == 0)
+        rng->n = n0 + 2u * (uint32_t)m;  // exactly 2m draws consumed
     }
-    (void)rng_next(rng);
-    (void)rng_next(rng);
+    rng->n += 2u;  // the HG direction draws (direction unused)
     *npos = (dir * t) + pos;
     return true;
 }
@@ -165,9 +165,10 @@ MRP_HD TrWalk tr_walk_start() {
 // Both phases share one speculation block (a delta-tracking step draws a
 // second number), so lanes of a wave in different phases issue their
 // lookups together and wait for memory once per round.
-template <int B, class Fetch>
-MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
-                                float dis, int max_steps, TrWalk* w, uint32_t* lk) {
+template <int B, uint32_t kStride, class Fetch>
+MRP_HD void transmittance_round(RingRng<kStride>* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos,
+                                vec3 dir, float dis, int max_steps, TrWalk* w, uint32_t* lk) {
+    static_assert(2 * B + 3 <= (int)kRingWords, "a round's draws must stay in the ring");
     // the loop test at the top of each phase
     if (w->phase == 1 && !(w->loop_num < max_steps && w->tr > 0.5f)) {
         ++w->loop_num;  // the failing test still increments
@@ -177,7 +178,8 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
     if (w->phase == 0) return;
     const bool delta = w->phase == 2;  // delta tracking: two draws per step
     const uint32_t n0 = rng->n;
-    Rng sp = *rng;
+    const uint32_t per = delta ? 2u : 1u;  // step j draws words n0 + per * j (distance) and, delta, the next
+    rng_fill(rng, n0 + per * (uint32_t)B);
     float ts[B], ua[B];
     typename Fetch::Token tok[B];
     int m = 0;
@@ -186,15 +188,14 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
 #pragma unroll
     for (int j = 0; j < B; ++j) {
         if (escape || w->loop_num + j >= max_steps) break;
-        const float rk = rng_next(&sp);
-        tc -= logf_(1.0f - rk) * inv_smax_tr;
+        tc -= logf_(1.0f - rng_word(*rng, n0 + per * (uint32_t)j)) * inv_smax_tr;
         if (tc > dis) {
             escape = true;
             break;
         }
         ts[j] = tc;
         tok[j] = F.issue(pos + (dir * tc));
-        ua[j] = delta ? rng_next(&sp) : 0.0f;
+        ua[j] = delta ? rng_word(*rng, n0 + 2u * (uint32_t)j + 1u) : 0.0f;
         ++m;
     }
     if (!delta) {
@@ -223,13 +224,13 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
         w->tr = tr;
         w->loop_num += done;
         if (restart) {
-            rng_seek(rng, n0 + (uint32_t)done);
+            rng->n = n0 + (uint32_t)done;
             return;
         }
         w->t = tdone;
         if (stop) {
             ++w->loop_num;
-            rng_seek(rng, n0 + (uint32_t)done);
+            rng->n = n0 + (uint32_t)done;
             w->phase = tr > 0.001f ? 2 : 0;
             return;
         }
@@ -240,16 +241,16 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
                 // loop's test then passes unless the cap is reached (tr > 0.5 >
                 // 0.001), and its step draws once more and ends too (t only grows)
                 ++w->loop_num;
-                rng_seek(rng, n0 + (uint32_t)done + 1u + (w->loop_num < max_steps ? 1u : 0u));
+                rng->n = n0 + (uint32_t)done + 1u + (w->loop_num < max_steps ? 1u : 0u);
                 w->phase = 0;
                 return;
             }
             ++w->loop_num;
-            rng_seek(rng, n0 + (uint32_t)done);
+            rng->n = n0 + (uint32_t)done;
             w->phase = tr > 0.001f ? 2 : 0;
             return;
         }
-        *rng = sp;  // exactly m draws consumed
+        rng->n = n0 + (uint32_t)m;  // exactly m draws consumed
         return;
     }
     // ---- delta tracking: tr stays until a collision zeroes it
@@ -272,26 +273,26 @@ MRP_HD void transmittance_round(Rng* rng, const Fetch& F, float inv_smax_tr, flo
     if (hit) {
         w->tr = 0.0f;
         w->phase = 0;
-        rng_seek(rng, n0 + 2u * (uint32_t)done);  // two draws per step, the last one the collision test
+        rng->n = n0 + 2u * (uint32_t)done;  // two draws per step, the last one the collision test
         return;
     }
     if (restart) {
-        rng_seek(rng, n0 + 2u * (uint32_t)done);
+        rng->n = n0 + 2u * (uint32_t)done;
         return;
     }
     if (escape) {
         w->phase = 0;
-        rng_seek(rng, n0 + 2u * (uint32_t)done + 1u);  // done == m steps, then the escaping step's draw
+        rng->n = n0 + 2u * (uint32_t)done + 1u;  // done == m steps, then the escaping step's draw
         return;
     }
     w->t = tc;  // the last speculated position (t itself when m == 0)
-    *rng = sp;  // exactly 2m draws consumed
+    rng->n = n0 + 2u * (uint32_t)m;  // exactly 2m draws consumed
 }
 
 // transmittance_seq with up to B lookups in flight.
-template <int B, class Fetch>
-MRP_HD float transmittance_batched(Rng* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos, vec3 dir,
-                                   float dis, int max_steps, uint32_t* lk) {
+template <int B, uint32_t kStride, class Fetch>
+MRP_HD float transmittance_batched(RingRng<kStride>* rng, const Fetch& F, float inv_smax_tr, float ratio, vec3 pos,
+                                   vec3 dir, float dis, int max_steps, uint32_t* lk) {
     TrWalk w = tr_walk_start();
     while (w.phase != 0) transmittance_round<B>(rng, F, inv_smax_tr, ratio, pos, dir, dis, max_steps, &w, lk);
     return w.tr;

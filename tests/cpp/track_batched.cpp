@@ -1,7 +1,10 @@
-// The batched trackers (several lookups in flight) against the sequential
-// loops that state the reference semantics, on random rays through a density
-// field with negative regions, for several batch widths and step caps: the
-// result, the lookup count and the stream position must be identical.
+// The batched trackers (several lookups in flight, reading their draws from
+// the stream's ring of generated words) against the sequential loops that
+// state the reference semantics (one draw at a time from the plain stream),
+// on random rays through a density field with negative regions, for several
+// batch widths and step caps: the result, the lookup count, the stream
+// position and the draws that follow must be identical; a seek back over the
+// free flight's last two draws must give them again.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -35,8 +38,10 @@ static int run(const Field& F, int max_steps, int rays, uint64_t seed) {
         const vec3 dir = normalize(v3(rng_next(&g) - 0.5f, rng_next(&g) - 0.5f, rng_next(&g) - 0.5f));
         const float dis = 0.2f + 1.5f * rng_next(&g);
         const float inv_smax = 1.0f / (8.0f + 200.0f * rng_next(&g)), ratio = 0.5f + 2.0f * rng_next(&g);
+        uint32_t wb[kRingWords];
         {
-            Rng a = rng_make((uint32_t)r, 1u, 0u, seed), b = a;
+            Rng a = rng_make((uint32_t)r, 1u, 0u, seed);
+            RingRng<1> b = ring_make<1>(wb, (uint32_t)r, 1u, 0u, seed);
             vec3 pa = v3(0, 0, 0), pb = v3(0, 0, 0);
             uint32_t la = 0, lb = 0;
             const bool ra = free_flight_seq(&a, F, inv_smax, ratio, pos, dir, dis, max_steps, &pa, &la);
@@ -45,10 +50,23 @@ static int run(const Field& F, int max_steps, int rays, uint64_t seed) {
                 if (bad < 5) std::printf("free flight B=%d cap=%d ray %d: %d/%d lk %u/%u n %u/%u\n", B, max_steps, r, ra, rb, la, lb, a.n, b.n);
                 ++bad;
             }
-            if (ra && rng_next(&a) != rng_next(&b)) ++bad;  // the stream continues identically
+            if (ra) {
+                Rng c = a;  // the free flight's two direction draws, again
+                rng_seek(&c, a.n - 2u);
+                rng_seek(&b, b.n - 2u);
+                for (int k = 0; k < 2; ++k)
+                    if (rng_next(&c) != rng_next(&b)) ++bad;
+                if (rng_next(&a) != rng_next(&b)) ++bad;  // the stream continues identically
+            }
         }
         {
-            Rng a = rng_make((uint32_t)r, 2u, 0u, seed), b = a;
+            Rng a = rng_make((uint32_t)r, 2u, 0u, seed);
+            RingRng<1> b = ring_make<1>(wb, (uint32_t)r, 2u, 0u, seed);
+            if (r & 1) {  // a stream resumed part-way, as the shadow rays resume the first vertex's
+                const uint32_t start = 5u + (uint32_t)r % 11u;
+                rng_seek(&a, start);
+                rng_seek(&b, start);
+            }
             uint32_t la = 0, lb = 0;
             const float ta = transmittance_seq(&a, F, inv_smax, ratio, pos, dir, dis, max_steps, &la);
             const float tb = transmittance_batched<B>(&b, F, inv_smax, ratio, pos, dir, dis, max_steps, &lb);
@@ -96,7 +114,7 @@ int main() {
         for (int cap : caps) {
             const int rays = cap == 10000 ? 20000 : 4000;
             bad += run<1>(F, cap, rays, 11) + run<2>(F, cap, rays, 12) + run<3>(F, cap, rays, 13) +
-                   run<4>(F, cap, rays, 14) + run<8>(F, cap, rays, 15) + run<16>(F, cap, rays, 16);
+                   run<4>(F, cap, rays, 14) + run<5>(F, cap, rays, 15) + run<6>(F, cap, rays, 16);
             cases += 6L * rays;
         }
     std::printf("%ld rays x 2 trackers, %ld mismatches\n", cases, bad);

@@ -1,9 +1,11 @@
 """The batched delta / ratio trackers (several lookups in flight per thread,
-internship_amd/csrc/mrp_track.h) against the one-lookup-at-a-time loops that
+internship_amd/csrc/mrp_track.h, reading their draws by index from the
+stream's ring of generated words) against the one-lookup-at-a-time loops that
 state the reference semantics (`DeterminateNextVertex`, `Tr`,
-core/render.cuh:93-172): a host build of the same templates on random rays
-through a field with negative regions, batch widths 1-8 and step caps 1..10000.
-Result bits, lookup counts and stream positions must be identical.  The same
+core/render.cuh:93-172) on the plain stream: a host build of the same
+templates on random rays through a field with negative regions, batch widths
+1-6, step caps 1..10000, fresh and resumed streams.  Result bits, lookup
+counts, stream positions and the draws that follow must be identical.  The same
 program checks that block_order (the first-vertex stage's 8 x 8 work order)
 visits every pixel exactly once."""
 import os
