@@ -79,11 +79,11 @@ struct NNArgs {
 // VGPRs, 3 waves): 0.42; 12 put the batch in scratch (1.29 ms).  The shadow
 // rays run about 256K threads, 4 waves per SIMD, so depth pays there: 6 (148 VGPRs) took 0.23 ms against 2: 0.26, 4: 0.25, 8: 0.24.
 // The multi-view forms (serving: 64 views of 16 volumes, no brick occupancy)
-// measured their batch at 6 and 4 (MRPNN stages 73.9 ms per 720p batch
-// against 74.9 with 3 and 6, or with 8 and 4).
+// measured their batch at 4 and 4 (MRPNN stages 73.3 ms per 720p batch
+// against 74.0 with 6 and 4 and 74.9 with 3 and 6 or 8 and 4).
 constexpr int kFVBatch = 3;
 constexpr int kShadowBatch = 6;
-constexpr int kFVBatchViews = 6;
+constexpr int kFVBatchViews = 4;
 constexpr int kShadowBatchViews = 4;
 constexpr int kMaxTrackSteps = 10000;
 
