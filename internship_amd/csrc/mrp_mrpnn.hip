@@ -66,6 +66,7 @@ struct NNArgs {
     // views[view]; nullptr for one image (this record)
     const NNArgs* views;
     uint32_t view_items;
+    uint32_t occ_words;  // views: the largest view's occupancy words (the first vertex's LDS)
 };
 
 // ---------------------------------------------------------------- stage 1
@@ -182,18 +183,27 @@ __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __r
                                                            uint32_t* __restrict__ count_out) {
     __shared__ unsigned red[256];
     __shared__ uint32_t s_ring[256 * kRingWords];  // the threads' Philox rings (16 KB)
-    extern __shared__ uint32_t s_occ[];             // occupancy_words(a.res) words (one image), none (views)
+    extern __shared__ uint32_t s_occ[];             // occupancy_words(a.res) words (one image), a.occ_words (views)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0u) *count_out = 0u;
     uint32_t lk = 0u;
     if constexpr (kViews) {
-        // several volumes: no occupancy (read from global memory it costs more than it saves)
+        // several volumes: a workgroup whose items all belong to one view (all
+        // but those that straddle two views) stages that view's brick
+        // occupancy; the others read every cell (from global memory the bits
+        // cost more than they save)
+        const uint32_t first = a.item0 + blockIdx.x * blockDim.x;
+        const uint32_t last = a.item0 + min((blockIdx.x + 1u) * blockDim.x, a.n_items) - 1u;
+        const uint32_t v0 = first / a.view_items;
+        const Occupancy occ = v0 == last / a.view_items  // uniform
+                                  ? stage_occupancy(a.views[v0].occ, a.views[v0].res, s_occ)
+                                  : Occupancy{nullptr, 0u, 0u};
         if (i < a.n_items) {
             const uint32_t item = a.item0 + i;
             const uint32_t view = __builtin_amdgcn_readfirstlane(item / a.view_items);
             const uint32_t base = view * a.view_items;
-            first_vertex_item<kFVBatchViews>(a.views[view], Occupancy{nullptr, 0u, 0u}, i, item - base, base, pos_act,
-                                             fx, fy, fz, rng_pos, lds_ring(s_ring), &lk);
+            first_vertex_item<kFVBatchViews>(a.views[view], occ, i, item - base, base, pos_act, fx, fy, fz, rng_pos,
+                                             lds_ring(s_ring), &lk);
         }
     } else {
         const Occupancy occ = stage_occupancy(a.occ, a.res, s_occ);
@@ -1034,6 +1044,7 @@ static void image_args(const Volume* v, const mrp_render_params* p, PixelMap map
     a->counters = v->counters;
     a->views = nullptr;
     a->view_items = 0u;
+    a->occ_words = 0u;
 
     *ga = GatherArgs{};
     for (int m = 0; m < kNumMips; ++m) ga->mips[m] = v->mip_cells[m];
@@ -1091,7 +1102,8 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
         const unsigned blocks = (cnt + 255) / 256;
         MRP_HIP(hipEventRecord(t0, v->stream));
         if (views)
-            first_vertex_kernel<true><<<blocks, 256, 0, v->stream>>>(a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
+            first_vertex_kernel<true><<<blocks, 256, a.occ_words * sizeof(uint32_t), v->stream>>>(a, pos_act, fx, fy, fz,
+                                                                                                rng_pos, n_active_dev);
         else
             first_vertex_kernel<false><<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(
                 a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
@@ -1243,6 +1255,8 @@ int mrpnn_views(Volume* const* vols, const mrp_render_params* ps, int n_views, i
     a.counters = host->counters;
     a.views = nn_dev;
     a.view_items = (uint32_t)(spp * P);
+    a.occ_words = 0u;
+    for (int k = 0; k < n_views; ++k) a.occ_words = std::max(a.occ_words, (uint32_t)occupancy_words(nn[k].res));
     GatherArgs ga = gv[0];
     ga.views = gv_dev;
     ga.view_items = a.view_items;
