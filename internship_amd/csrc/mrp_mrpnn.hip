@@ -361,6 +361,8 @@ struct GatherArgs {
     const uint4* mips16;             // FP16 cells of every density mip (tiled gather)
     const uint4* trs16;              // FP16 cells of every transmittance mip (tiled gather)
     vec3 light, lx;
+    vec3 id, tf;  // inv3(lx) and 0.4999 * inv3(lx), RayBoxOffset's per-direction terms (made on the host: IEEE
+                  // division and multiplication round alike there)
     float alpha, g;
     uint32_t n_items;
     const uint32_t* n_dev;  // if set, the descriptor count is read here (n_items bounds the grid)
@@ -672,9 +674,8 @@ __global__ __launch_bounds__(kTaps) __attribute__((amdgpu_waves_per_eu(kViews ? 
         gv.lut = va.lut;
         gv.light = va.light;
         gv.lx = va.lx;
-        gv.id = inv3(va.lx);
-        const vec3 bmax = v3(0.4999f, 0.4999f, 0.4999f);
-        gv.tf = bmax * gv.id;
+        gv.id = va.id;
+        gv.tf = va.tf;
         gv.alpha = va.alpha;
         gv.g = va.g;
         // the HG-table factor that is the same for every descriptor of the view:
@@ -1054,6 +1055,8 @@ static void image_args(const Volume* v, const mrp_render_params* p, PixelMap map
     ga->trs16 = v->tr_cells16;
     ga->light = a->light;
     ga->lx = a->lx;
+    ga->id = inv3(ga->lx);
+    ga->tf = v3(0.4999f, 0.4999f, 0.4999f) * ga->id;
     ga->alpha = p->alpha;
     ga->g = p->g;
     ga->views = nullptr;
@@ -1471,6 +1474,8 @@ static int gather_setup(mrp_volume* vol, int n, const float* pos_act_dev, const 
     ga->trs16 = v->tr_cells16;
     ga->light = normalize(V(light_dir));
     ga->lx = normalize(ga->light);
+    ga->id = inv3(ga->lx);
+    ga->tf = v3(0.4999f, 0.4999f, 0.4999f) * ga->id;
     ga->alpha = alpha;
     ga->g = g;
     ga->n_items = (uint32_t)n;
