@@ -19,9 +19,9 @@
 //     advances every live lane by exactly one tracking step, and all lanes
 //     reach the single density-lookup site together, so the eight corner reads
 //     issue as one wide memory instruction stream instead of three divergent ones.
-//   * Random numbers: Philox4x32-10 per (pixel, sample); an 8-word ring is
-//     topped up at one site per iteration, so the cipher code is not
-//     replicated at every draw site.
+//   * Random numbers: Philox4x32-10 per (pixel, sample); each lane's words
+//     sit in a 16-word ring in LDS, topped up at one site per iteration, so
+//     the cipher code is not replicated at every draw site.
 //   * Results go to a per-(sample, pixel) float4 buffer; a second kernel folds
 //     them into the running mean in sample order, which keeps the image
 //     bit-identical to a sequential per-pixel loop (the CPU oracle).
