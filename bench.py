@@ -701,10 +701,13 @@ def run_headline(a, ia, sc):
     sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=1)
     img_mlp = sc.image()
     rmse_mlp = rmse(img_mlp, ref)
-    sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=0)
+    # each network's time from a second call (the first launch of a kernel loads its code)
+    for _ in range(2):
+        sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=0)
     fp32_ms = sc.st.kernel_ms
     rmse_mlp32 = rmse(sc.image(), ref)
-    sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=2)
+    for _ in range(2):
+        sc.render(0, 1, MLP_SEED, rt=ia.MRPNN, precision=2)
     fp8_ms = sc.st.kernel_ms
     rmse_mlp8 = rmse(sc.image(), ref)
     log("MLP 1-spp RMSE: BF16 %.5g, FP32 %.5g, FP8 %.5g (networks: FP32 %.2f ms, FP8 %.2f ms)"
