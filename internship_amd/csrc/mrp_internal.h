@@ -124,8 +124,16 @@ struct NNTimes {
 // With defer_last, the last chunk's count and stage times are left pending
 // (no host synchronisation): after synchronising the stream, the caller adds
 // them with mrpnn_finish.
+// With `fold` set and one sample per pixel (n == 1, every pixel), the combine
+// stage folds each pixel's sample straight into the running mean (and FP64
+// sums) instead of writing it to v->samples.
+struct SampleFold {
+    float* target = nullptr;
+    double* sum64 = nullptr;
+    float rate = 0.0f;  // 1 / (1 + the sample's index)
+};
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
-                  uint64_t* n_active, bool defer_last = false);
+                  uint64_t* n_active, bool defer_last = false, SampleFold fold = SampleFold{});
 int mrpnn_finish(Volume* v, NNTimes* times, uint64_t* n_active);
 // The MRPNN samples [0, spp) of n_views images in one pass of the stages
 // (view k on vols[k] with ps[k], every view of the same size, width * height a

@@ -860,11 +860,12 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
                                                       const float* __restrict__ pred, vec3 light_color,
                                                       vec3 scatter_rate, const NNArgs* __restrict__ views,
                                                       uint32_t view_items, uint32_t n_items,
-                                                      float4* __restrict__ out) {
+                                                      float4* __restrict__ out, SampleFold fold) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_items) return;
     const uint32_t slot = __float_as_uint(fy[i].w);
     if (slot == 0xffffffffu) return;
+    float4 o;
     if (pos_act[i].w != 0.0f) {
         if constexpr (kViews) {
             const NNArgs& va = views[slot / view_items];
@@ -875,10 +876,27 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
         const uint32_t k = slot_of[i];  // the vertex's compacted descriptor
         const vec3 p = v3(pred[3 * k] + single, pred[3 * k + 1] + single, pred[3 * k + 2] + single);
         const vec3 r = p * light_color * scatter_rate;
-        out[slot] = make_float4(r.x, r.y, r.z, fz[i].w);
+        o = make_float4(r.x, r.y, r.z, fz[i].w);
     } else {
         const float4 sky = fx[i];  // the environment radiance stored by the first vertex
-        out[slot] = make_float4(sky.x, sky.y, sky.z, -1.0f);
+        o = make_float4(sky.x, sky.y, sky.z, -1.0f);
+    }
+    if (!kViews && fold.target) {
+        // one sample per pixel: accumulate_kernel's fold of this pixel's one sample
+        const vec3 c = vmax(v3(0, 0, 0), v3(o.x, o.y, o.z));
+        float* t = fold.target + 3 * (size_t)slot;
+        const vec3 tg = lerp(v3(t[0], t[1], t[2]), c, fold.rate);
+        t[0] = tg.x;
+        t[1] = tg.y;
+        t[2] = tg.z;
+        if (fold.sum64) {
+            double* s = fold.sum64 + 3 * (size_t)slot;
+            s[0] += (double)c.x;
+            s[1] += (double)c.y;
+            s[2] += (double)c.z;
+        }
+    } else {
+        out[slot] = o;
     }
 }
 
@@ -1070,7 +1088,7 @@ static void image_args(const Volume* v, const mrp_render_params* p, PixelMap map
 // weights, g, srp) are shared by all of them.
 static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const void* packed, const float* weights,
                       float g, vec3 srp, size_t total, float4* out, NNTimes* times, uint64_t* n_active,
-                      bool defer_last) {
+                      bool defer_last, SampleFold fold = SampleFold{}) {
     const bool views = a.views != nullptr;
     const size_t chunk_items = std::max<size_t>(256, std::min<size_t>((size_t)1 << 22, total));
     static_assert(2 * kTileBytes == (size_t)32 * kFeatures * sizeof(float), "tiled BF16 descriptors fit the FP32 rows");
@@ -1175,11 +1193,11 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
             if (views)
                 combine_kernel<true><<<blocks, 256, 0, v->stream>>>(pos_act, single, fx, fy, fz, slot_of, pred,
                                                                     a.light_color, a.scatter_rate, a.views,
-                                                                    a.view_items, cnt, out);
+                                                                    a.view_items, cnt, out, SampleFold{});
             else
                 combine_kernel<false><<<blocks, 256, 0, v->stream>>>(pos_act, single, fx, fy, fz, slot_of, pred,
                                                                      a.light_color, a.scatter_rate, nullptr, 0u, cnt,
-                                                                     out);
+                                                                     out, fold);
             MRP_HIP(hipGetLastError());
         }
         // the count into pinned host memory (a copy into pageable memory would block)
@@ -1197,7 +1215,7 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
 // First vertex -> gather -> network -> combine for samples [s_begin, s_begin+n)
 // of the pixels `map` selects, in chunks of at most 4M vertices.
 int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_begin, int n, NNTimes* times,
-                  uint64_t* n_active, bool defer_last) {
+                  uint64_t* n_active, bool defer_last, SampleFold fold) {
     const void* packed = nullptr;
     int rc = prepare_volume(v, p, &packed);
     if (rc) return rc;
@@ -1205,8 +1223,10 @@ int mrpnn_samples(Volume* v, const mrp_render_params* p, PixelMap map, int s_beg
     GatherArgs ga;
     image_args(v, p, map, s_begin, &a, &ga);
     const size_t total = (size_t)n * (size_t)a.wl * (size_t)p->height;
+    if (fold.target && (n != 1 || map.checker >= 0 || a.wl != p->width))
+        return fail(MRP_E_INVALID, "a fold into the running mean needs one sample of every pixel");
     return run_stages(v, a, ga, p->mlp_precision, packed, v->nn_weights, p->g, srp_of(a.scatter_rate), total,
-                      (float4*)v->samples, times, n_active, defer_last);
+                      (float4*)v->samples, times, n_active, defer_last, fold);
 }
 
 size_t mrpnn_view_table_bytes(int n_views) { return (size_t)n_views * (sizeof(NNArgs) + sizeof(GatherArgs)); }
@@ -1318,8 +1338,16 @@ int mrpnn_dev(Volume* v, const mrp_render_params* p, float* target, double* sum6
         const bool last = s0 + n >= p->spp_count;
         mrp_render_params q = *p;
         if (env_lights) env_light_of_sample(v, p->spp_begin + s0, p->seed, &q.light_dir, &q.light_color);
-        rc = mrpnn_samples(v, &q, map, p->spp_begin + s0, n, &tm, &n_active, last);
-        if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
+        if (n == 1) {  // the combine stage folds the one sample into the running mean
+            SampleFold fold;
+            fold.target = target;
+            fold.sum64 = sum64;
+            fold.rate = 1.0f / (float)(1 + p->spp_begin + s0);
+            rc = mrpnn_samples(v, &q, map, p->spp_begin + s0, 1, &tm, &n_active, last, fold);
+        } else {
+            rc = mrpnn_samples(v, &q, map, p->spp_begin + s0, n, &tm, &n_active, last);
+            if (rc == MRP_OK) rc = launch_accumulate(v, v->samples, P, p->spp_begin + s0, n, target, sum64);
+        }
     }
     float total_ms = 0.0f;
     unsigned long long cnt4[4] = {0, 0, 0, 0};
