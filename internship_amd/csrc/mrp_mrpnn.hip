@@ -82,6 +82,9 @@ struct NNArgs {
 // The multi-view forms (serving: 64 views of 16 volumes, no brick occupancy)
 // measured their batch at 4 and 4 (MRPNN stages 73.3 ms per 720p batch
 // against 74.0 with 6 and 4 and 74.9 with 3 and 6 or 8 and 4).
+// Shadow rays are taken in kShadowBuckets buckets of their expected length
+// (shadow_bucket); the compaction counts them, the first vertex clears the counts.
+constexpr int kShadowBuckets = 64;
 constexpr int kFVBatch = 3;
 constexpr int kShadowBatch = 6;
 constexpr int kFVBatchViews = 4;
@@ -180,12 +183,14 @@ template <bool kViews>
 __global__ __launch_bounds__(256) void first_vertex_kernel(NNArgs a, float4* __restrict__ pos_act,
                                                            float4* __restrict__ fx, float4* __restrict__ fy,
                                                            float4* __restrict__ fz, uint32_t* __restrict__ rng_pos,
-                                                           uint32_t* __restrict__ count_out) {
+                                                           uint32_t* __restrict__ count_out,
+                                                           uint32_t* __restrict__ buckets) {
     __shared__ unsigned red[256];
     __shared__ uint32_t s_ring[256 * kRingWords];  // the threads' Philox rings (16 KB)
     extern __shared__ uint32_t s_occ[];             // occupancy_words(a.res) words (one image), a.occ_words (views)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0u) *count_out = 0u;
+    if (i < 2u * kShadowBuckets) buckets[i] = 0u;  // the compaction counts, the shadow stage's offsets
     uint32_t lk = 0u;
     if constexpr (kViews) {
         // several volumes: a workgroup whose items all belong to one view (all
@@ -284,7 +289,6 @@ __global__ __launch_bounds__(256) void shadow_kernel(NNArgs a0, const uint32_t* 
 // similar length and the longest start first (0.281 -> 0.261 ms at 1080p,
 // the sort included).  Counting sort in two passes; the order within a
 // bucket depends on scheduling, no result does.
-constexpr int kShadowBuckets = 64;
 template <bool kViews>
 __device__ __forceinline__ int shadow_bucket(const NNArgs& a0, const float4* __restrict__ pos_act,
                                              const float4* __restrict__ fy, uint32_t i) {
@@ -301,25 +305,6 @@ __device__ __forceinline__ int shadow_bucket(const NNArgs& a0, const float4* __r
     return kShadowBuckets - 1 - (b < 0 ? 0 : (b > kShadowBuckets - 1 ? kShadowBuckets - 1 : b));
 }
 
-__global__ void clear_buckets_kernel(uint32_t* __restrict__ c) {
-    if (threadIdx.x < 2 * kShadowBuckets) c[threadIdx.x] = 0u;
-}
-
-template <bool kViews>
-__global__ __launch_bounds__(256) void shadow_count_kernel(NNArgs a0, const uint32_t* __restrict__ list,
-                                                           const uint32_t* __restrict__ n_act_dev,
-                                                           const float4* __restrict__ pos_act,
-                                                           const float4* __restrict__ fy, uint32_t* __restrict__ counts) {
-    __shared__ uint32_t hist[kShadowBuckets];
-    const uint32_t n_act = *n_act_dev;
-    if (blockIdx.x * blockDim.x >= n_act) return;  // uniform over the workgroup
-    if (threadIdx.x < kShadowBuckets) hist[threadIdx.x] = 0u;
-    __syncthreads();
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n_act) atomicAdd(&hist[shadow_bucket<kViews>(a0, pos_act, fy, list[k])], 1u);
-    __syncthreads();
-    if (threadIdx.x < kShadowBuckets && hist[threadIdx.x]) atomicAdd(&counts[threadIdx.x], hist[threadIdx.x]);
-}
 
 template <bool kViews>
 __global__ __launch_bounds__(256) void shadow_scatter_kernel(NNArgs a0, const uint32_t* __restrict__ list,
@@ -903,15 +888,21 @@ __global__ __launch_bounds__(256) void combine_kernel(const float4* __restrict__
 // Active first vertices -> a dense list (wave ballots, one atomic per workgroup).
 // The list order depends on scheduling; every descriptor's result does not.
 constexpr int kCompactBlock = 1024;
-__global__ __launch_bounds__(kCompactBlock) void compact_kernel(const float4* __restrict__ pos_act, uint32_t n,
+template <bool kViews>
+__global__ __launch_bounds__(kCompactBlock) void compact_kernel(NNArgs a0, const float4* __restrict__ pos_act,
+                                                                const float4* __restrict__ fy, uint32_t n,
                                                                 uint32_t* __restrict__ list,
                                                                 uint32_t* __restrict__ slot_of,
-                                                                uint32_t* __restrict__ count) {
+                                                                uint32_t* __restrict__ count,
+                                                                uint32_t* __restrict__ buckets) {
     // one atomic per workgroup (a wave ballot per wave, a scan of the wave
-    // counts in LDS): the counter's atomics serialise
+    // counts in LDS): the counter's atomics serialise.  The listed vertices
+    // are also counted into the shadow stage's length buckets.
     __shared__ uint32_t wave_off[kCompactBlock / 64];
     __shared__ uint32_t block_base;
+    __shared__ uint32_t hist[kShadowBuckets];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (threadIdx.x < kShadowBuckets) hist[threadIdx.x] = 0u;
     const bool act = i < n && pos_act[i].w != 0.0f;
     const unsigned long long m = __ballot(act);
     const uint32_t lane = __lane_id(), wv = threadIdx.x >> 6;
@@ -933,10 +924,13 @@ __global__ __launch_bounds__(kCompactBlock) void compact_kernel(const float4* __
             const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
             list[k] = i;
             slot_of[i] = k;
+            atomicAdd(&hist[shadow_bucket<kViews>(a0, pos_act, fy, i)], 1u);
         } else {
             slot_of[i] = 0xffffffffu;
         }
     }
+    __syncthreads();
+    if (threadIdx.x < kShadowBuckets && hist[threadIdx.x]) atomicAdd(&buckets[threadIdx.x], hist[threadIdx.x]);
 }
 
 
@@ -1121,17 +1115,23 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
         a.item0 = (uint32_t)i0;
         a.n_items = cnt;
         const unsigned blocks = (cnt + 255) / 256;
+        // the shadow stage's bucket counts and offsets (2 * kShadowBuckets words after the items)
+        uint32_t* bucket_counts = (uint32_t*)(sp + chunk_items * kNNItemBytes);
         MRP_HIP(hipEventRecord(t0, v->stream));
         if (views)
-            first_vertex_kernel<true><<<blocks, 256, a.occ_words * sizeof(uint32_t), v->stream>>>(a, pos_act, fx, fy, fz,
-                                                                                                rng_pos, n_active_dev);
+            first_vertex_kernel<true><<<blocks, 256, a.occ_words * sizeof(uint32_t), v->stream>>>(
+                a, pos_act, fx, fy, fz, rng_pos, n_active_dev, bucket_counts);
         else
             first_vertex_kernel<false><<<blocks, 256, occupancy_words(a.res) * sizeof(uint32_t), v->stream>>>(
-                a, pos_act, fx, fy, fz, rng_pos, n_active_dev);
+                a, pos_act, fx, fy, fz, rng_pos, n_active_dev, bucket_counts);
         MRP_HIP(hipGetLastError());
         MRP_HIP(hipEventRecord(tc, v->stream));
-        compact_kernel<<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
-            pos_act, cnt, list, slot_of, n_active_dev);
+        if (views)
+            compact_kernel<true><<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
+                a, pos_act, fy, cnt, list, slot_of, n_active_dev, bucket_counts);
+        else
+            compact_kernel<false><<<(cnt + kCompactBlock - 1) / kCompactBlock, kCompactBlock, 0, v->stream>>>(
+                a, pos_act, fy, cnt, list, slot_of, n_active_dev, bucket_counts);
         MRP_HIP(hipGetLastError());
         // No read-back here: the shadow rays, the gather and the network read the
         // count on the device, with grids sized for the whole chunk.
@@ -1140,17 +1140,13 @@ static int run_stages(Volume* v, NNArgs a, GatherArgs ga, int precision, const v
         // keeps the compaction's pixel-coherent order); the order lives in the
         // descriptor region, which the gather fills only afterwards
         uint32_t* sorder = (uint32_t*)feat;
-        uint32_t* bucket_counts = (uint32_t*)(sp + chunk_items * kNNItemBytes);
-        clear_buckets_kernel<<<1, 2 * kShadowBuckets, 0, v->stream>>>(bucket_counts);
         if (views) {
-            shadow_count_kernel<true><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, bucket_counts);
             shadow_scatter_kernel<true><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy,
                                                                        bucket_counts, bucket_counts + kShadowBuckets,
                                                                        sorder);
             shadow_kernel<true><<<blocks, 256, 0, v->stream>>>(a, sorder, n_active_dev, pos_act, fy, rng_pos, fx,
                                                                single);
         } else {
-            shadow_count_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy, bucket_counts);
             shadow_scatter_kernel<false><<<blocks, 256, 0, v->stream>>>(a, list, n_active_dev, pos_act, fy,
                                                                         bucket_counts, bucket_counts + kShadowBuckets,
                                                                         sorder);
